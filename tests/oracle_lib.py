@@ -1,0 +1,210 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it as the checker.  The product path never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = None
+
+KYBER_SIZES = {}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _LIB = C.CDLL(path)
+    return _LIB
+
+
+class KyberParams(C.Structure):
+    _fields_ = [("k", C.c_int), ("du", C.c_int), ("dv", C.c_int),
+                ("polyvec_bytes", C.c_size_t), ("polyvec_cbytes", C.c_size_t),
+                ("poly_cbytes", C.c_size_t), ("indcpa_pk", C.c_size_t),
+                ("indcpa_sk", C.c_size_t), ("ct", C.c_size_t), ("pk", C.c_size_t),
+                ("sk", C.c_size_t)]
+
+
+def kyber_params(k):
+    if k not in KYBER_SIZES:
+        p = KyberParams()
+        assert lib().oracle_kyber_params_get(k, C.byref(p)) == 0
+        KYBER_SIZES[k] = p
+    return KYBER_SIZES[k]
+
+
+def buf(n):
+    return (C.c_uint8 * n)()
+
+
+def ptr(a):
+    """Pointer to a contiguous numpy array or ctypes buffer."""
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(C.c_void_p)
+    return C.cast(a, C.c_void_p)
+
+
+def drbg(nbytes, seed=0, stream=0, index=0):
+    out = buf(nbytes)
+    lib().oracle_drbg(out, C.c_size_t(nbytes), C.c_uint64(seed), C.c_uint64(stream), C.c_uint64(index))
+    return bytes(out)
+
+
+def drbg_array(n, nbytes, seed=0, stream=0):
+    """n independent nbytes-long DRBG draws: element i uses index i."""
+    out = np.empty((n, nbytes), dtype=np.uint8)
+    for i in range(n):
+        out[i] = np.frombuffer(drbg(nbytes, seed, stream, i), dtype=np.uint8)
+    return out
+
+
+def shake128(data, n):
+    out = buf(n)
+    lib().oracle_shake128(out, C.c_size_t(n), bytes(data), C.c_size_t(len(data)))
+    return bytes(out)
+
+
+def shake256(data, n):
+    out = buf(n)
+    lib().oracle_shake256(out, C.c_size_t(n), bytes(data), C.c_size_t(len(data)))
+    return bytes(out)
+
+
+def sha3_256(data):
+    out = buf(32)
+    lib().oracle_sha3_256(out, bytes(data), C.c_size_t(len(data)))
+    return bytes(out)
+
+
+def sha3_512(data):
+    out = buf(64)
+    lib().oracle_sha3_512(out, bytes(data), C.c_size_t(len(data)))
+    return bytes(out)
+
+
+def keccakf(state):
+    st = (C.c_uint64 * 25)(*state)
+    lib().oracle_keccakf1600(st)
+    return list(st)
+
+
+# ---- Kyber ---------------------------------------------------------------
+def _i16(a):
+    return np.ascontiguousarray(a, dtype=np.int16)
+
+
+def kyber_ntt(a, reduce=True):
+    a = _i16(a).copy()
+    (lib().oracle_kyber_ntt if reduce else lib().oracle_kyber_ntt_raw)(ptr(a))
+    return a
+
+
+def kyber_invntt(a):
+    a = _i16(a).copy()
+    lib().oracle_kyber_invntt(ptr(a))
+    return a
+
+
+def kyber_basemul(a, b):
+    r = np.zeros(256, dtype=np.int16)
+    lib().oracle_kyber_basemul(ptr(r), ptr(_i16(a)), ptr(_i16(b)))
+    return r
+
+
+def kyber_cbd(b128):
+    r = np.zeros(256, dtype=np.int16)
+    lib().oracle_kyber_cbd(ptr(r), bytes(b128))
+    return r
+
+
+def kyber_zetas():
+    z = np.zeros(128, dtype=np.int16)
+    zi = np.zeros(128, dtype=np.int16)
+    lib().oracle_kyber_zetas(ptr(z), ptr(zi))
+    return z, zi
+
+
+def kyber_compress(a, d):
+    out = buf(32 * d)
+    lib().oracle_kyber_poly_compress(out, ptr(_i16(a)), C.c_int(d))
+    return bytes(out)
+
+
+def kyber_decompress(b, d):
+    r = np.zeros(256, dtype=np.int16)
+    lib().oracle_kyber_poly_decompress(ptr(r), bytes(b), C.c_int(d))
+    return r
+
+
+def kyber_rej_uniform(stream, n=256):
+    r = np.zeros(n, dtype=np.int16)
+    cnt = lib().oracle_kyber_rej_uniform(ptr(r), C.c_uint(n), bytes(stream), C.c_uint(len(stream)))
+    return r[:cnt]
+
+
+def kyber_gen_matrix(k, seed, transposed):
+    a = np.zeros((k, k, 256), dtype=np.int16)
+    lib().oracle_kyber_gen_matrix(C.c_int(k), ptr(a), bytes(seed), C.c_int(transposed))
+    return a
+
+
+def kyber_keypair(k, coins64):
+    p = kyber_params(k)
+    pk, sk = buf(p.pk), buf(p.sk)
+    assert lib().oracle_kyber_keypair_derand(C.c_int(k), pk, sk, bytes(coins64)) == 0
+    return bytes(pk), bytes(sk)
+
+
+def kyber_enc(k, pk, coins32):
+    p = kyber_params(k)
+    ct, ss = buf(p.ct), buf(32)
+    assert lib().oracle_kyber_enc_derand(C.c_int(k), ct, ss, bytes(pk), bytes(coins32)) == 0
+    return bytes(ct), bytes(ss)
+
+
+def kyber_dec(k, ct, sk):
+    ss = buf(32)
+    assert lib().oracle_kyber_dec(C.c_int(k), ss, bytes(ct), bytes(sk)) == 0
+    return bytes(ss)
+
+
+def kyber_indcpa_dec(k, ct, sk):
+    m = buf(32)
+    assert lib().oracle_kyber_indcpa_dec(C.c_int(k), m, bytes(ct), bytes(sk)) == 0
+    return bytes(m)
+
+
+def kyber_batch(op, k, arrays, nthreads=8):
+    """Threaded batch over element-contiguous numpy uint8 arrays."""
+    p = kyber_params(k)
+    L = lib()
+    if op == "keypair":
+        coins, = arrays
+        n = coins.shape[0]
+        pk = np.empty((n, p.pk), np.uint8)
+        sk = np.empty((n, p.sk), np.uint8)
+        assert L.oracle_kyber_keypair_batch(k, C.c_size_t(n), ptr(pk), ptr(sk), ptr(coins), nthreads) == 0
+        return pk, sk
+    if op == "enc":
+        pk, coins = arrays
+        n = pk.shape[0]
+        ct = np.empty((n, p.ct), np.uint8)
+        ss = np.empty((n, 32), np.uint8)
+        assert L.oracle_kyber_enc_batch(k, C.c_size_t(n), ptr(ct), ptr(ss), ptr(pk), ptr(coins), nthreads) == 0
+        return ct, ss
+    if op == "dec":
+        ct, sk = arrays
+        n = ct.shape[0]
+        ss = np.empty((n, 32), np.uint8)
+        assert L.oracle_kyber_dec_batch(k, C.c_size_t(n), ptr(ss), ptr(ct), ptr(sk), nthreads) == 0
+        return ss
+    raise ValueError(op)
