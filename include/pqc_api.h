@@ -1,0 +1,136 @@
+/*
+ * pqc_api.h - C ABI of the MI355X batched lattice-KEM engine (libpqc_amd.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns PQC_OK (0) or a
+ * negative PQC_ERR_* code; nothing calls exit() or prints.
+ *
+ * Reference interfaces replaced (paths relative to the nainag/PQC tree):
+ *   crypto_kem_keypair/enc/dec ... Kyber_GPU_Batched/api.h:18-22,
+ *                                  FrodoKEM_GPU/api.h:20-24
+ *   indcpa_keypair/enc/dec ....... Kyber_GPU_Batched/indcpa.h:16-26 (batched,
+ *                                  COUNT + device pointers + stream)
+ *   cpapke_keypair/enc/dec ....... NewHope_NIST_GPU/cpapke.h:14-24
+ *   sha3_256_n / sha3_512_n ...... Kyber_GPU_Batched/fips202.h:19-38
+ *   poly_ntt_n / poly_invntt_n ... Kyber_GPU_Batched/poly.h (poly.cu:159-190)
+ *   poly_set4 scratch ............ Kyber_GPU_Batched/main.h:14-39 (now owned
+ *                                  by a pqc_ctx)
+ */
+#ifndef PQC_API_H
+#define PQC_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define PQC_OK 0
+#define PQC_ERR_INVALID_ARG (-1)     /* null pointer, misaligned buffer, bad op */
+#define PQC_ERR_UNSUPPORTED (-2)     /* algorithm not enabled in this context */
+#define PQC_ERR_BATCH_TOO_LARGE (-3) /* n above the context's max_batch */
+#define PQC_ERR_NO_MEMORY (-4)       /* HBM budget exceeded or allocation failed */
+#define PQC_ERR_DEVICE (-5)          /* HIP runtime error */
+#define PQC_ERR_NO_DEVICE (-6)       /* no gfx950 device visible */
+
+/* ---- algorithms ------------------------------------------------------------- */
+#define PQC_ALG_KYBER512 1        /* round 2, k=2, du=10, dv=3 */
+#define PQC_ALG_KYBER768 2        /* round 2, k=3, du=10, dv=4 */
+#define PQC_ALG_KYBER1024 3       /* round 2, k=4, du=11, dv=5 */
+#define PQC_ALG_NEWHOPE1024_CPA 4 /* NewHope-1024 CPA-PKE */
+#define PQC_ALG_FRODO976 5        /* FrodoKEM-976-SHAKE */
+#define PQC_ALG_MASK(a) (1u << (a))
+
+/* ---- call flags ------------------------------------------------------------- */
+#define PQC_F_HOST_PTRS 0x1u /* buffers are host memory: staged through the
+                                context, and the call returns when done */
+#define PQC_F_PLAIN 0x2u     /* run the plain per-op kernels instead of the
+                                fused persistent kernels (same outputs) */
+
+typedef struct pqc_ctx pqc_ctx;
+
+/* Context: owns device scratch for up to max_batch elements of every enabled
+ * algorithm.  Creation fails with PQC_ERR_NO_MEMORY when the scratch would
+ * exceed hbm_budget_bytes (0 = no budget).  One context per host thread. */
+int pqc_ctx_create(pqc_ctx **ctx, unsigned alg_mask, size_t max_batch, size_t hbm_budget_bytes);
+void pqc_ctx_destroy(pqc_ctx *ctx);
+size_t pqc_ctx_device_bytes(const pqc_ctx *ctx);
+int pqc_ctx_set_tuning(pqc_ctx *ctx, int fused_waves_per_cu, int fused_grid);
+
+/* Byte sizes of one element: public key, secret key, ciphertext, shared secret. */
+int pqc_alg_sizes(int alg, size_t *pk, size_t *sk, size_t *ct, size_t *ss);
+
+/* ---- batched KEM (element-contiguous buffers: element i at i * size) --------
+ * Device pointers unless PQC_F_HOST_PTRS; asynchronous on `stream` (a
+ * hipStream_t, NULL = default stream) for device pointers.
+ * coins: keypair 64 bytes per element (d || z), enc 32 bytes per element.   */
+int crypto_kem_keypair_derand_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *pk, uint8_t *sk,
+                                    const uint8_t *coins, unsigned flags, void *stream);
+int crypto_kem_enc_derand_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *ss,
+                                const uint8_t *pk, const uint8_t *coins, unsigned flags, void *stream);
+int crypto_kem_dec_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ss, const uint8_t *ct,
+                         const uint8_t *sk, unsigned flags, void *stream);
+/* Same, with coins drawn from the library's randomness source. */
+int crypto_kem_keypair_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *pk, uint8_t *sk,
+                             unsigned flags, void *stream);
+int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *ss,
+                         const uint8_t *pk, unsigned flags, void *stream);
+
+/* ---- NIST drop-ins: one instance, host pointers, synchronous -----------------
+ * Unprefixed names are Kyber-768 (Kyber_GPU_Batched/api.h:18-22 with
+ * KYBER_K 3).  Each call uses a library-owned context on the current device. */
+int crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+int pqc_kyber512_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int pqc_kyber512_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int pqc_kyber512_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+int pqc_kyber768_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int pqc_kyber768_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int pqc_kyber768_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+int pqc_kyber1024_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int pqc_kyber1024_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int pqc_kyber1024_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+
+/* ---- randomness for the non-derand entry points ------------------------------- */
+void pqc_randombytes_seed(uint64_t seed); /* deterministic SHAKE256 DRBG */
+void pqc_randombytes_system(void);        /* OS randomness (default) */
+int pqc_randombytes(uint8_t *out, size_t n);
+
+/* ---- device plumbing, so that callers need no other GPU binding -------------- */
+int pqc_device_count(int *count);
+int pqc_set_device(int device);
+int pqc_device_info(char *name, size_t name_len, int *cu_count, size_t *hbm_bytes);
+int pqc_device_malloc(void **ptr, size_t bytes);
+int pqc_device_free(void *ptr);
+int pqc_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int pqc_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int pqc_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int pqc_stream_create(void **stream);
+int pqc_stream_destroy(void *stream);
+int pqc_stream_synchronize(void *stream);
+int pqc_event_create(void **event);
+int pqc_event_destroy(void *event);
+int pqc_event_record(void *event, void *stream);
+int pqc_event_synchronize(void *event);
+int pqc_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---- component kernels (tests and microbenchmarks) ---------------------------
+ * pqc_keccak_batch: n independent fixed-length messages (element i at
+ * in + i*inlen, digest at out + i*outlen); op 0 SHA3-256 (outlen 32),
+ * 1 SHA3-512 (outlen 64), 2 SHAKE128, 3 SHAKE256.  inlen must be a multiple
+ * of 8.  Replaces sha3_256_n / sha3_512_n (fips202.cu:634-717).
+ * pqc_kyber_poly_transform: n polynomials of 256 int16 in place; op 0 =
+ * poly_ntt (ntt + Barrett), 1 = poly_invntt.                                  */
+int pqc_keccak_batch(int op, size_t n, uint8_t *out, size_t outlen, const uint8_t *in, size_t inlen,
+                     void *stream);
+int pqc_kyber_poly_transform(int op, size_t n, int16_t *polys, void *stream);
+
+const char *pqc_version(void);
+const char *pqc_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PQC_API_H */

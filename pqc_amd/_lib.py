@@ -1,0 +1,109 @@
+"""ctypes binding of libpqc_amd.so (the C ABI declared in include/pqc_api.h).
+
+The library is the product: every KEM call goes to its HIP kernels.  There is
+no CPU fallback; loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpqc_amd.so")
+
+PQC_OK = 0
+PQC_ERR_INVALID_ARG = -1
+PQC_ERR_UNSUPPORTED = -2
+PQC_ERR_BATCH_TOO_LARGE = -3
+PQC_ERR_NO_MEMORY = -4
+PQC_ERR_DEVICE = -5
+PQC_ERR_NO_DEVICE = -6
+
+ALG_KYBER512 = 1
+ALG_KYBER768 = 2
+ALG_KYBER1024 = 3
+ALG_NEWHOPE1024_CPA = 4
+ALG_FRODO976 = 5
+
+F_HOST_PTRS = 0x1
+F_PLAIN = 0x2
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+_I = C.c_int
+_U = C.c_uint
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "pqc_ctx_create": (_I, [C.POINTER(_P), _U, _SZ, _SZ]),
+    "pqc_ctx_destroy": (None, [_P]),
+    "pqc_ctx_device_bytes": (_SZ, [_P]),
+    "pqc_ctx_set_tuning": (_I, [_P, _I, _I]),
+    "pqc_alg_sizes": (_I, [_I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ)]),
+    "crypto_kem_keypair_derand_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
+    "crypto_kem_enc_derand_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _P, _U, _P]),
+    "crypto_kem_dec_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
+    "crypto_kem_keypair_batch": (_I, [_P, _I, _SZ, _P, _P, _U, _P]),
+    "crypto_kem_enc_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
+    "crypto_kem_keypair": (_I, [_P, _P]),
+    "crypto_kem_enc": (_I, [_P, _P, _P]),
+    "crypto_kem_dec": (_I, [_P, _P, _P]),
+    "pqc_randombytes_seed": (None, [C.c_uint64]),
+    "pqc_randombytes_system": (None, []),
+    "pqc_randombytes": (_I, [_P, _SZ]),
+    "pqc_device_count": (_I, [C.POINTER(_I)]),
+    "pqc_set_device": (_I, [_I]),
+    "pqc_device_info": (_I, [C.c_char_p, _SZ, C.POINTER(_I), C.POINTER(_SZ)]),
+    "pqc_device_malloc": (_I, [C.POINTER(_P), _SZ]),
+    "pqc_device_free": (_I, [_P]),
+    "pqc_memcpy_htod": (_I, [_P, _P, _SZ, _P]),
+    "pqc_memcpy_dtoh": (_I, [_P, _P, _SZ, _P]),
+    "pqc_memcpy_dtod": (_I, [_P, _P, _SZ, _P]),
+    "pqc_stream_create": (_I, [C.POINTER(_P)]),
+    "pqc_stream_destroy": (_I, [_P]),
+    "pqc_stream_synchronize": (_I, [_P]),
+    "pqc_event_create": (_I, [C.POINTER(_P)]),
+    "pqc_event_destroy": (_I, [_P]),
+    "pqc_event_record": (_I, [_P, _P]),
+    "pqc_event_synchronize": (_I, [_P]),
+    "pqc_event_elapsed_ms": (_I, [C.POINTER(C.c_float), _P, _P]),
+    "pqc_keccak_batch": (_I, [_I, _SZ, _P, _SZ, _P, _SZ, _P]),
+    "pqc_kyber_poly_transform": (_I, [_I, _SZ, _P, _P]),
+    "pqc_version": (C.c_char_p, []),
+    "pqc_strerror": (C.c_char_p, [_I]),
+}
+for _alg in ("kyber512", "kyber768", "kyber1024"):
+    SIGNATURES["pqc_%s_crypto_kem_keypair" % _alg] = (_I, [_P, _P])
+    SIGNATURES["pqc_%s_crypto_kem_enc" % _alg] = (_I, [_P, _P, _P])
+    SIGNATURES["pqc_%s_crypto_kem_dec" % _alg] = (_I, [_P, _P, _P])
+
+_lib = None
+
+
+class PQCError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = "%s failed: %d" % (what, code)
+        if _lib is not None:
+            msg += " (%s)" % _lib.pqc_strerror(code).decode()
+        super().__init__(msg)
+
+
+def lib():
+    """Load the built library; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libpqc_amd.so is not built (run __graft_entry__.build() or "
+                               "make -C pqc_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != PQC_OK:
+        raise PQCError(rc, what)
+    return rc

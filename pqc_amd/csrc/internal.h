@@ -1,0 +1,64 @@
+// Host-side internals shared by the runtime and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/pqc_api.h"
+
+struct pqc_ctx {
+  int device = 0;
+  unsigned alg_mask = 0;
+  size_t max_batch = 0;
+  size_t hbm_budget = 0;
+  size_t ld = 0;  // SoA leading dimension of the plain-path polynomials
+
+  // plain Kyber path: SoA int16 polynomials and per-element byte buffers
+  int16_t* kpolys = nullptr;
+  int kpolys_count = 0;
+  uint8_t* kbytes = nullptr;
+  size_t kbytes_per_elem = 0;
+
+  // fused Kyber path: PRF outputs, (2K+1) x 128 bytes per request
+  uint8_t* kprf = nullptr;
+  size_t kprf_per_elem = 0;
+
+  // staging for host-pointer calls
+  uint8_t* stage = nullptr;
+  size_t stage_bytes = 0;
+
+  uint32_t* status = nullptr;  // device word: bit 0 = a sampler hit its block cap
+
+  size_t allocated = 0;  // bytes of device memory owned by the context
+  int cu_count = 0;
+  int fused_waves_per_cu = 4;
+  int fused_grid_override = 0;
+};
+
+namespace pqc {
+
+inline int hip_err(hipError_t e) { return e == hipSuccess ? PQC_OK : PQC_ERR_DEVICE; }
+
+// plain per-op Kyber path (kyber_plain.hip); all pointers are device pointers
+int kyber_plain_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins,
+                        hipStream_t s);
+int kyber_plain_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk,
+                    const uint8_t* coins, hipStream_t s);
+int kyber_plain_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
+                    hipStream_t s);
+
+// fused persistent Kyber path (kyber_fused.hip)
+int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk,
+                    const uint8_t* coins, hipStream_t s);
+int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
+                    hipStream_t s);
+
+// component kernels exposed for tests (kyber_fused.hip)
+int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);
+int keccak_batch(int op, size_t n, uint8_t* out, size_t outlen, const uint8_t* in, size_t inlen,
+                 hipStream_t s);
+
+size_t kyber_plain_polys(int kmax);
+size_t kyber_plain_bytes_per_elem(int kmax);
+
+}  // namespace pqc

@@ -1,0 +1,468 @@
+// Fused persistent Kyber encapsulation / decapsulation.
+//
+// One workgroup = one wave64; a persistent grid walks tiles of up to 64
+// requests.  Inside a tile the work is split two ways:
+//   * per-request sponge chains (H(ct), G, PRF, KDF, and for encaps H(seed),
+//     H(pk), H(ct)) run one request per lane, the Keccak state in VGPRs;
+//   * the matrix A^T is generated in rounds of up to 64 SHAKE128 entries, one
+//     entry per lane, rejection-sampled straight from the state registers into
+//     LDS, and consumed by the wave-cooperative polynomial arithmetic
+//     (one request at a time, 4 coefficients per lane, NTT butterflies through
+//     register/lane-bit swaps; kyber_wave.cuh).
+// Keccak lanes stay >= 98% busy: every request contributes K*K entries per
+// round and one lane per sequential chain.
+//
+// Data flow per request follows the round-2 FO decapsulation assembled from
+// the reference's pieces: indcpa_dec (Kyber_GPU_Batched/indcpa.cu:691-733),
+// G = sha3_512 (fips202.cu:672-689), indcpa_enc (indcpa.cu:578-679),
+// verify + cmov (verify.cu:15-50, here a wave-wide OR of coefficient
+// mismatches, no branch on it), H = sha3_256 (fips202.cu:615-632) and
+// KDF = SHAKE256 (symmetric.h:33).  Comparing compressed coefficients is
+// comparing ciphertext bytes: every width packs 256 coefficients into whole
+// bytes with no unused bits.
+#include "internal.h"
+#include "kyber_wave.cuh"
+#include "sponge.cuh"
+
+namespace pqc {
+namespace kyber {
+namespace fused {
+
+__constant__ LaneTwiddles c_lane_tw = make_lane_twiddles();
+
+constexpr int kAStride = 262;   // int16 slots per A entry: 256 + guard, odd dword count
+constexpr int kMaxBlocks = 32;  // SHAKE128 blocks per entry before giving up (never reached)
+
+template <int K>
+struct Tile {
+  static constexpr int RA = 64 / (K * K);           // requests per A round: 16, 7, 4
+  static constexpr int NR = (K == 3) ? 9 : 64 / RA;  // rounds per tile: 4, 9, 16
+  static constexpr int T = RA * NR;                  // requests per tile: 64, 63, 64
+  static constexpr int E = RA * K * K;               // A entries per round: 64, 63, 64
+  static constexpr int PRFB = (2 * K + 1) * 128;     // PRF bytes per request
+};
+
+struct Lds {
+  int16_t a[64 * kAStride];
+  uint8_t msg[64][32];
+  uint32_t fail[64];
+};
+
+// ---- wave-cooperative pieces -------------------------------------------------
+
+// indcpa_dec for one request; the message is written to msg (LDS, 32 bytes).
+template <int K>
+__device__ __forceinline__ void coop_indcpa_dec(const uint8_t* ct, const uint8_t* sk, const LaneZ& z,
+                                                uint8_t* msg) {
+  using P = Params<K>;
+  const int l = lane_id();
+  P4 mp = {{0, 0, 0, 0}};
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    uint32_t t[4];
+    unpack_group<P::du>(t, ct + 32 * P::du * i, nat_group(l));
+    P4 u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u.v[r] = decompress_d<P::du>(t[r]);
+    ntt(u, z);
+    P4 s;
+    load_poly12(s, sk + 384 * i, ntt_group(l));
+    basemul_acc(mp, s, u, z.bm);
+  }
+  reduce(mp);
+  invntt(mp, z);
+  uint32_t tv[4];
+  unpack_group<P::dv>(tv, ct + P::polyvec_cbytes, nat_group(l));
+  uint32_t nib = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int32_t x = barrett(decompress_d<P::dv>(tv[r]) - mp.v[r]);
+    nib |= compress_d<1>(csubq(x)) << r;
+  }
+  const uint32_t other = (uint32_t)__shfl_xor((int)nib, 1);
+  if ((l & 1) == 0) msg[l >> 1] = (uint8_t)(nib | (other << 4));
+}
+
+// Pack the lane's 4 D-bit values with its odd neighbour's: even lanes write D bytes.
+template <int D>
+__device__ __forceinline__ void pack_pair(uint8_t* dst, const uint32_t (&t)[4]) {
+  const int l = lane_id();
+  uint64_t mine = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) mine |= (uint64_t)t[r] << (D * r);
+  const uint64_t nb = (uint64_t)__shfl_xor((long long)mine, 1);
+  if ((l & 1) == 0) {
+    // 8 values, 8*D bits: low 4*D bits mine, next 4*D bits the neighbour's
+    const uint64_t lo = mine | (nb << (4 * D));
+    const uint64_t hi = (4 * D + 4 * D > 64) ? (nb >> (64 - 4 * D)) : 0;
+    uint8_t* o = dst + D * (l >> 1);
+#pragma unroll
+    for (int b = 0; b < D; ++b) o[b] = (uint8_t)(b < 8 ? (lo >> (8 * b)) : (hi >> (8 * (b - 8))));
+  }
+}
+
+// Re-encryption of one request (indcpa_enc from sp-sampling on).  DEC: compare
+// with ct and return the lane's mismatch bits.  ENC: write ct.
+template <int K, bool DEC>
+__device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* a_entries, const uint8_t* prf,
+                                                   const uint8_t* pk, const uint8_t* msg, uint8_t* ct,
+                                                   const LaneZ& z) {
+  using P = Params<K>;
+  const int l = lane_id();
+  const int gn = ntt_group(l);
+  const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
+  P4 sp[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    cbd_group(sp[j], prf16[64 * j + l]);
+    ntt(sp[j], z);
+  }
+  uint32_t mismatch = 0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    P4 acc = {{0, 0, 0, 0}};
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t* e32 = reinterpret_cast<const uint32_t*>(a_entries + (i * K + j) * kAStride + 4 * gn);
+      const uint32_t w0 = e32[0], w1 = e32[1];
+      P4 a = {{(int32_t)(w0 & 0xFFFF), (int32_t)(w0 >> 16), (int32_t)(w1 & 0xFFFF), (int32_t)(w1 >> 16)}};
+      basemul_acc(acc, a, sp[j], z.bm);
+    }
+    reduce(acc);
+    invntt(acc, z);
+    P4 e;
+    cbd_group(e, prf16[64 * (K + i) + l]);
+    uint32_t c[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c[r] = compress_d<P::du>(csubq(barrett(acc.v[r] + e.v[r])));
+    uint8_t* cti = ct + 32 * P::du * i;
+    if constexpr (DEC) {
+      uint32_t t[4];
+      unpack_group<P::du>(t, cti, nat_group(l));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
+    } else {
+      pack_pair<P::du>(cti, c);
+    }
+  }
+  {
+    P4 acc = {{0, 0, 0, 0}};
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      P4 t;
+      load_poly12(t, pk + 384 * j, gn);
+      basemul_acc(acc, t, sp[j], z.bm);
+    }
+    reduce(acc);
+    invntt(acc, z);
+    P4 e;
+    cbd_group(e, prf16[64 * (2 * K) + l]);
+    const uint32_t nib = (msg[l >> 1] >> (4 * (l & 1))) & 0xF;
+    uint32_t c[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t kq = (int32_t)((0u - ((nib >> r) & 1)) & ((Q + 1) / 2));
+      c[r] = compress_d<P::dv>(csubq(barrett(acc.v[r] + e.v[r] + kq)));
+    }
+    uint8_t* ctv = ct + P::polyvec_cbytes;
+    if constexpr (DEC) {
+      uint32_t t[4];
+      unpack_group<P::dv>(t, ctv, nat_group(l));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
+    } else {
+      pack_pair<P::dv>(ctv, c);
+    }
+  }
+  return mismatch;
+}
+
+// ---- per-lane sponge pieces ---------------------------------------------------
+
+// SHAKE128(rho || i || j) rejection-sampled into an LDS entry (gen_matrix_n,
+// indcpa.cu:339-403, rej_uniform indcpa.cu:290-322).  Every candidate is
+// written: accepted ones to slot min(ctr, 256), rejected ones to the guard
+// slot 256.  Returns the accept count (>= 256 unless the block cap was hit).
+__device__ __forceinline__ int sample_entry(int16_t* dst, const uint64_t (&rho)[4], int x, int y, bool valid) {
+  uint64_t A[25];
+#pragma unroll
+  for (int t = 0; t < 25; ++t) A[t] = (t < 4) ? rho[t] : 0;
+  A[4] = (uint64_t)x | ((uint64_t)y << 8) | (0x1Full << 16);
+  A[kRateShake128 - 1] ^= 0x8000000000000000ull;
+  int ctr = 0;
+  for (int blk = 0; blk < kMaxBlocks; ++blk) {
+    if (blk >= 4 && !__any(valid && ctr < N)) break;
+    keccakf1600(A);
+#pragma unroll
+    for (int w = 0; w < kRateShake128; ++w) {
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
+        const bool ok = val < 19u * Q;
+        const int32_t red = (int32_t)val - (int32_t)(val >> 12) * Q;
+        const int slot = ok ? min(ctr, N) : N;
+        dst[slot] = (int16_t)red;
+        ctr += ok ? 1 : 0;
+      }
+    }
+  }
+  return ctr;
+}
+
+template <int K, bool DEC>
+__global__ __launch_bounds__(64) void k_fused(int n, int ntiles, uint8_t* __restrict__ ss,
+                                              uint8_t* __restrict__ ct, const uint8_t* __restrict__ key,
+                                              const uint8_t* __restrict__ coins, uint8_t* __restrict__ prf,
+                                              uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  using TL = Tile<K>;
+  __shared__ Lds sh;
+  const int lane = threadIdx.x;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  constexpr int KEYB = DEC ? P::sk_bytes : P::pk_bytes;
+  constexpr int PK_OFF = DEC ? P::sk_cpa_bytes : 0;  // public key inside the key buffer
+
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int base = tile * TL::T;
+    const int r = base + lane;
+    const bool active = lane < TL::T && r < n;
+    const int rc = active ? r : min(base, n - 1);  // clamped request for address math
+    const uint8_t* key_r = key + (size_t)rc * KEYB;
+    uint8_t* prf_r = prf + (size_t)rc * TL::PRFB;
+    uint64_t kbar[4], hct[4];
+
+    if constexpr (DEC) {
+      // ---- H(ct), one request per lane
+      {
+        uint64_t A[25];
+        sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(ct + (size_t)rc * P::ct_bytes),
+                                           P::ct_bytes / 8, 0x06);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) hct[w] = A[w];
+      }
+      // ---- indcpa_dec, wave-cooperative, one request at a time
+      for (int t = 0; t < TL::T; ++t) {
+        const int req = base + t;
+        if (req >= n) break;
+        coop_indcpa_dec<K>(ct + (size_t)req * P::ct_bytes, key + (size_t)req * KEYB, z, sh.msg[t]);
+      }
+      __syncthreads();
+    } else {
+      // ---- m = H(seed) ("don't release system RNG output"), H(pk)
+      uint64_t A[25];
+      uint64_t c4[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) c4[w] = load_u64(coins + (size_t)rc * 32 + 8 * w);
+      absorb_fixed<kRateSha3_256, 4>(A, c4, 0x06);
+      if (active) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) store_u64(sh.msg[lane] + 8 * w, A[w]);
+      }
+      sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(key_r), P::pk_bytes / 8, 0x06);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) hct[w] = A[w];  // holds H(pk) until G
+      __syncthreads();
+    }
+
+    // ---- (Kbar, coins) = G(m || H(pk)); PRF(coins, 0..2K) -> scratch
+    {
+      uint64_t m[8], A[25];
+      const uint8_t* msg = sh.msg[lane < TL::T ? lane : 0];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        m[w] = load_u64(msg + 8 * w);
+        m[4 + w] = DEC ? load_u64(key_r + P::hpk_off + 8 * w) : hct[w];
+      }
+      absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
+      uint64_t rc4[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        kbar[w] = A[w];
+        rc4[w] = A[4 + w];
+      }
+      for (int nonce = 0; nonce < 2 * K + 1; ++nonce) {
+        absorb_fixed<kRateShake256, 4>(A, rc4, (uint64_t)nonce | (0x1Full << 8));
+        if (active) {
+#pragma unroll
+          for (int w = 0; w < 16; ++w) store_u64(prf_r + 128 * nonce + 8 * w, A[w]);
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- A^T rounds + re-encryption
+    for (int g = 0; g < TL::NR; ++g) {
+      {
+        const int e = lane;
+        const int t = g * TL::RA + e / (K * K);
+        const int ij = e % (K * K);
+        const int req = base + t;
+        const bool valid = e < TL::E && req < n;
+        const uint8_t* seed = key + (size_t)(valid ? req : rc) * KEYB + PK_OFF + P::polyvec_bytes;
+        uint64_t rho[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) rho[w] = load_u64(seed + 8 * w);
+        const int got = sample_entry(sh.a + e * kAStride, rho, ij / K, ij % K, valid);
+        if (valid && got < N) atomicOr(status, 1u);
+      }
+      __syncthreads();
+      for (int tt = 0; tt < TL::RA; ++tt) {
+        const int t = g * TL::RA + tt;
+        const int req = base + t;
+        if (req >= n) break;
+        const uint8_t* kr = key + (size_t)req * KEYB;
+        const uint32_t mm = coop_reencrypt<K, DEC>(sh.a + tt * K * K * kAStride, prf + (size_t)req * TL::PRFB,
+                                                   kr + PK_OFF, sh.msg[t], ct + (size_t)req * P::ct_bytes, z);
+        if constexpr (DEC) {
+          const uint32_t f = __any(mm != 0) ? 1u : 0u;
+          if (lane == 0) sh.fail[t] = f;
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- KDF
+    {
+      uint64_t m[8], A[25];
+      if constexpr (DEC) {
+        const uint64_t mask = 0 - (uint64_t)(sh.fail[lane < TL::T ? lane : 0] & 1u);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const uint64_t zz = load_u64(key_r + P::z_off + 8 * w);
+          m[w] = kbar[w] ^ (mask & (zz ^ kbar[w]));
+          m[4 + w] = hct[w];
+        }
+      } else {
+        sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(ct + (size_t)rc * P::ct_bytes),
+                                           P::ct_bytes / 8, 0x06);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          m[w] = kbar[w];
+          m[4 + w] = A[w];
+        }
+      }
+      absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+      if (active) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) store_u64(ss + (size_t)r * 32 + 8 * w, A[w]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- component kernels -------------------------------------------------------
+
+// poly_ntt / poly_invntt on polynomials in coefficient order, one per wave.
+__global__ __launch_bounds__(64) void k_poly_transform(int op, int n, int16_t* polys) {
+  const int l = threadIdx.x;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  for (int p = blockIdx.x; p < n; p += gridDim.x) {
+    int16_t* poly = polys + (size_t)p * N;
+    const int gin = op == 0 ? nat_group(l) : ntt_group(l);
+    const int gout = op == 0 ? ntt_group(l) : nat_group(l);
+    P4 x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x.v[r] = poly[4 * gin + r];
+    if (op == 0)
+      ntt(x, z);
+    else
+      invntt(x, z);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) poly[4 * gout + r] = (int16_t)x.v[r];
+    __syncthreads();
+  }
+}
+
+// n fixed-length messages, one per lane: SHA3-256 / SHA3-512 / SHAKE128 / SHAKE256.
+template <int RATE>
+__device__ void keccak_one(uint8_t* out, size_t outlen, const uint8_t* in, int words, uint8_t domain) {
+  uint64_t A[25];
+  sponge_absorb_words<RATE>(A, reinterpret_cast<const uint64_t*>(in), words, domain);
+  size_t done = 0;
+  while (true) {
+#pragma unroll
+    for (int w = 0; w < RATE; ++w) {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const size_t o = done + 8 * w + b;
+        if (o < outlen) out[o] = (uint8_t)(A[w] >> (8 * b));
+      }
+    }
+    done += 8 * RATE;
+    if (done >= outlen) break;
+    keccakf1600(A);
+  }
+}
+
+__global__ void k_keccak_batch(int op, int n, uint8_t* out, size_t outlen, const uint8_t* in, size_t inlen) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const uint8_t* m = in + (size_t)x * inlen;
+  uint8_t* o = out + (size_t)x * outlen;
+  const int words = (int)(inlen / 8);
+  switch (op) {
+    case 0: keccak_one<kRateSha3_256>(o, outlen, m, words, 0x06); break;
+    case 1: keccak_one<kRateSha3_512>(o, outlen, m, words, 0x06); break;
+    case 2: keccak_one<kRateShake128>(o, outlen, m, words, 0x1F); break;
+    default: keccak_one<kRateShake256>(o, outlen, m, words, 0x1F); break;
+  }
+}
+
+template <int K, bool DEC>
+static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const uint8_t* key, const uint8_t* coins,
+                        hipStream_t s) {
+  if (n == 0) return PQC_OK;
+  const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->fused_waves_per_cu;
+  if (grid > ntiles) grid = ntiles;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_fused<K, DEC>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, ss, ct, key, coins, c->kprf,
+                     c->status);
+  return hip_err(hipGetLastError());
+}
+
+}  // namespace fused
+}  // namespace kyber
+
+int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation
+  switch (k) {
+    case 2: return kyber::fused::launch_fused<2, true>(c, n, ss, ctm, sk, nullptr, s);
+    case 3: return kyber::fused::launch_fused<3, true>(c, n, ss, ctm, sk, nullptr, s);
+    case 4: return kyber::fused::launch_fused<4, true>(c, n, ss, ctm, sk, nullptr, s);
+    default: return PQC_ERR_UNSUPPORTED;
+  }
+}
+
+int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+                    hipStream_t s) {
+  switch (k) {
+    case 2: return kyber::fused::launch_fused<2, false>(c, n, ss, ct, pk, coins, s);
+    case 3: return kyber::fused::launch_fused<3, false>(c, n, ss, ct, pk, coins, s);
+    case 4: return kyber::fused::launch_fused<4, false>(c, n, ss, ct, pk, coins, s);
+    default: return PQC_ERR_UNSUPPORTED;
+  }
+}
+
+int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s) {
+  if (op != 0 && op != 1) return PQC_ERR_INVALID_ARG;
+  if (n == 0) return PQC_OK;
+  const int grid = (int)(n < 4096 ? n : 4096);
+  hipLaunchKernelGGL(kyber::fused::k_poly_transform, dim3(grid), dim3(64), 0, s, op, (int)n, polys);
+  return hip_err(hipGetLastError());
+}
+
+int keccak_batch(int op, size_t n, uint8_t* out, size_t outlen, const uint8_t* in, size_t inlen, hipStream_t s) {
+  if (op < 0 || op > 3 || (inlen % 8) != 0) return PQC_ERR_INVALID_ARG;
+  if ((op == 0 && outlen != 32) || (op == 1 && outlen != 64) || outlen == 0) return PQC_ERR_INVALID_ARG;
+  if (n == 0) return PQC_OK;
+  const int grid = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(kyber::fused::k_keccak_batch, dim3(grid), dim3(256), 0, s, op, (int)n, out, outlen, in, inlen);
+  return hip_err(hipGetLastError());
+}
+
+}  // namespace pqc

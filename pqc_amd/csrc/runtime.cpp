@@ -1,0 +1,532 @@
+// C ABI runtime: contexts, dispatch to the plain / fused kernels, host-pointer
+// staging, the NIST single-instance drop-ins, randomness and device plumbing.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/random.h>
+
+#include <mutex>
+
+#include "internal.h"
+
+namespace {
+
+struct AlgInfo {
+  int kind;  // 1 = Kyber
+  int k;
+  size_t pk, sk, ct, ss;
+};
+
+bool alg_info(int alg, AlgInfo* a) {
+  switch (alg) {
+    case PQC_ALG_KYBER512: *a = {1, 2, 800, 1632, 736, 32}; return true;
+    case PQC_ALG_KYBER768: *a = {1, 3, 1184, 2400, 1088, 32}; return true;
+    case PQC_ALG_KYBER1024: *a = {1, 4, 1568, 3168, 1568, 32}; return true;
+    default: return false;
+  }
+}
+
+int check_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PQC_ERR_NO_DEVICE;
+  return PQC_OK;
+}
+
+int ctx_alloc(pqc_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) {
+    *p = nullptr;
+    return PQC_OK;
+  }
+  if (c->hbm_budget && c->allocated + bytes > c->hbm_budget) return PQC_ERR_NO_MEMORY;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return PQC_ERR_NO_MEMORY;
+  }
+  c->allocated += bytes;
+  return PQC_OK;
+}
+
+bool aligned8(const void* p) { return ((uintptr_t)p & 7u) == 0; }
+
+// ---- randomness ----------------------------------------------------------------
+std::mutex g_rng_mu;
+bool g_rng_det = false;
+uint64_t g_rng_seed = 0, g_rng_index = 0;
+
+// The deterministic DRBG is SHAKE256("pqc-mi355x" || le64 seed || le64 stream
+// || le64 index); it is computed on the host by a small portable Keccak.
+uint64_t rotl(uint64_t x, int n) { return n ? (x << n) | (x >> (64 - n)) : x; }
+
+void host_keccakf(uint64_t a[25]) {
+  static uint64_t rc[24];
+  static int rho[25];
+  static bool init = false;
+  if (!init) {
+    for (int ir = 0; ir < 24; ir++) {
+      uint64_t v = 0;
+      for (int j = 0; j <= 6; j++) {
+        unsigned t = (unsigned)(j + 7 * ir) % 255, r = 1;
+        for (unsigned i = 0; i < t; i++) {
+          r <<= 1;
+          if (r & 0x100) r ^= 0x171;
+        }
+        if (r & 1) v |= 1ull << ((1 << j) - 1);
+      }
+      rc[ir] = v;
+    }
+    rho[0] = 0;
+    unsigned x = 1, y = 0;
+    for (unsigned t = 0; t < 24; t++) {
+      rho[x + 5 * y] = (int)(((t + 1) * (t + 2) / 2) % 64);
+      unsigned nx = y, ny = (2 * x + 3 * y) % 5;
+      x = nx;
+      y = ny;
+    }
+    init = true;
+  }
+  uint64_t b[25], c[5], d[5];
+  for (int ir = 0; ir < 24; ir++) {
+    for (int i = 0; i < 5; i++) c[i] = a[i] ^ a[i + 5] ^ a[i + 10] ^ a[i + 15] ^ a[i + 20];
+    for (int i = 0; i < 5; i++) d[i] = c[(i + 4) % 5] ^ rotl(c[(i + 1) % 5], 1);
+    for (int i = 0; i < 25; i++) a[i] ^= d[i % 5];
+    for (int xx = 0; xx < 5; xx++)
+      for (int yy = 0; yy < 5; yy++) b[yy + 5 * ((2 * xx + 3 * yy) % 5)] = rotl(a[xx + 5 * yy], rho[xx + 5 * yy]);
+    for (int yy = 0; yy < 5; yy++)
+      for (int xx = 0; xx < 5; xx++)
+        a[xx + 5 * yy] = b[xx + 5 * yy] ^ (~b[(xx + 1) % 5 + 5 * yy] & b[(xx + 2) % 5 + 5 * yy]);
+    a[0] ^= rc[ir];
+  }
+}
+
+void drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, uint64_t index) {
+  uint8_t in[136] = {0};
+  memcpy(in, "pqc-mi355x", 10);
+  for (int i = 0; i < 8; i++) {
+    in[10 + i] = (uint8_t)(seed >> (8 * i));
+    in[18 + i] = (uint8_t)(stream >> (8 * i));
+    in[26 + i] = (uint8_t)(index >> (8 * i));
+  }
+  in[34] = 0x1F;
+  in[135] |= 0x80;
+  uint64_t a[25] = {0};
+  for (int w = 0; w < 17; w++)
+    for (int b = 0; b < 8; b++) a[w] |= (uint64_t)in[8 * w + b] << (8 * b);
+  while (len) {
+    host_keccakf(a);
+    for (int w = 0; w < 17 && len; w++)
+      for (int b = 0; b < 8 && len; b++, len--) *out++ = (uint8_t)(a[w] >> (8 * b));
+  }
+}
+
+// ---- library-owned context for the NIST drop-ins ----------------------------------
+std::mutex g_default_mu;
+pqc_ctx* g_default[64] = {nullptr};
+
+pqc_ctx* default_ctx() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default[dev]) {
+    pqc_ctx* c = nullptr;
+    unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024);
+    if (pqc_ctx_create(&c, mask, 64, 0) != PQC_OK) return nullptr;
+    g_default[dev] = c;
+  }
+  return g_default[dev];
+}
+
+int stage_ensure(pqc_ctx* c, size_t bytes) {
+  if (c->stage_bytes >= bytes) return PQC_OK;
+  if (c->stage) {
+    (void)hipFree(c->stage);
+    c->allocated -= c->stage_bytes;
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+  }
+  void* p = nullptr;
+  int rc = ctx_alloc(c, &p, bytes);
+  if (rc) return rc;
+  c->stage = (uint8_t*)p;
+  c->stage_bytes = bytes;
+  return PQC_OK;
+}
+
+enum Op { OP_KEYPAIR, OP_ENC, OP_DEC };
+
+// Common entry: validates, stages host buffers, dispatches.
+int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+             const uint8_t* i1, unsigned flags, void* stream) {
+  if (!c) return PQC_ERR_INVALID_ARG;
+  AlgInfo a;
+  if (!alg_info(alg, &a)) return PQC_ERR_UNSUPPORTED;
+  if (!(c->alg_mask & PQC_ALG_MASK(alg))) return PQC_ERR_UNSUPPORTED;
+  if (n > c->max_batch) return PQC_ERR_BATCH_TOO_LARGE;
+  if (n == 0) return PQC_OK;
+  if (!o0 || !i0 || (op != OP_DEC && (!o1 || !i1)) || (op == OP_DEC && !i1)) return PQC_ERR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
+
+  // sizes of o0, o1, i0, i1 per element
+  size_t so0, so1, si0, si1;
+  switch (op) {
+    case OP_KEYPAIR: so0 = a.pk; so1 = a.sk; si0 = 64; si1 = 0; break;
+    case OP_ENC: so0 = a.ct; so1 = a.ss; si0 = a.pk; si1 = 32; break;
+    default: so0 = a.ss; so1 = 0; si0 = a.ct; si1 = a.sk; break;
+  }
+  uint8_t *d_o0 = o0, *d_o1 = o1;
+  const uint8_t *d_i0 = i0, *d_i1 = i1;
+  const bool host = (flags & PQC_F_HOST_PTRS) != 0;
+  if (host) {
+    const size_t total = n * (so0 + so1 + si0 + si1) + 4 * 256;
+    int rc = stage_ensure(c, total);
+    if (rc) return rc;
+    uint8_t* p = c->stage;
+    auto take = [&](size_t bytes) {
+      uint8_t* q = p;
+      p += (bytes + 255) & ~(size_t)255;
+      return q;
+    };
+    d_o0 = take(n * so0);
+    d_o1 = so1 ? take(n * so1) : nullptr;
+    uint8_t* t0 = take(n * si0);
+    uint8_t* t1 = si1 ? take(n * si1) : nullptr;
+    if (hipMemcpyAsync(t0, i0, n * si0, hipMemcpyHostToDevice, s) != hipSuccess) return PQC_ERR_DEVICE;
+    if (si1 && hipMemcpyAsync(t1, i1, n * si1, hipMemcpyHostToDevice, s) != hipSuccess) return PQC_ERR_DEVICE;
+    d_i0 = t0;
+    d_i1 = t1;
+  } else {
+    if (!aligned8(o0) || (o1 && !aligned8(o1)) || !aligned8(i0) || (i1 && !aligned8(i1)))
+      return PQC_ERR_INVALID_ARG;
+  }
+  if (op == OP_KEYPAIR) {
+    // keypair derand: i0 = coins (64 B/elem); key generation runs on the plain kernels
+    int rc = pqc::kyber_plain_keypair(c, a.k, n, d_o0, d_o1, d_i0, s);
+    if (rc) return rc;
+  } else if (op == OP_ENC) {
+    int rc = plain ? pqc::kyber_plain_enc(c, a.k, n, d_o0, d_o1, d_i0, d_i1, s)
+                   : pqc::kyber_fused_enc(c, a.k, n, d_o0, d_o1, d_i0, d_i1, s);
+    if (rc) return rc;
+  } else {
+    int rc = plain ? pqc::kyber_plain_dec(c, a.k, n, d_o0, d_i0, d_i1, s)
+                   : pqc::kyber_fused_dec(c, a.k, n, d_o0, d_i0, d_i1, s);
+    if (rc) return rc;
+  }
+  if (host) {
+    if (hipMemcpyAsync(o0, d_o0, n * so0, hipMemcpyDeviceToHost, s) != hipSuccess) return PQC_ERR_DEVICE;
+    if (so1 && hipMemcpyAsync(o1, d_o1, n * so1, hipMemcpyDeviceToHost, s) != hipSuccess) return PQC_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return PQC_ERR_DEVICE;
+    uint32_t st = 0;
+    if (hipMemcpy(&st, c->status, 4, hipMemcpyDeviceToHost) != hipSuccess) return PQC_ERR_DEVICE;
+    if (st) return PQC_ERR_DEVICE;
+  }
+  return PQC_OK;
+}
+
+int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1) {
+  pqc_ctx* c = default_ctx();
+  if (!c) return PQC_ERR_NO_DEVICE;
+  uint8_t coins[64];
+  if (op == OP_KEYPAIR) {
+    if (pqc_randombytes(coins, 64)) return PQC_ERR_DEVICE;
+    i0 = coins;
+  } else if (op == OP_ENC) {
+    if (pqc_randombytes(coins, 32)) return PQC_ERR_DEVICE;
+    i1 = coins;
+  }
+  return kem_call(c, alg, op, 1, o0, o1, i0, i1, PQC_F_HOST_PTRS, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hbm_budget) {
+  if (!out || max_batch == 0 || max_batch > (1u << 26)) return PQC_ERR_INVALID_ARG;
+  *out = nullptr;
+  int rc = check_device();
+  if (rc) return rc;
+  const unsigned known = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024);
+  if (alg_mask == 0 || (alg_mask & ~known)) return PQC_ERR_UNSUPPORTED;
+  pqc_ctx* c = new pqc_ctx();
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return PQC_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+    delete c;
+    return PQC_ERR_DEVICE;
+  }
+  c->cu_count = prop.multiProcessorCount;
+  c->alg_mask = alg_mask;
+  c->max_batch = max_batch;
+  c->hbm_budget = hbm_budget;
+  c->ld = (max_batch + 63) & ~(size_t)63;
+  int kmax = 0;
+  for (int alg = PQC_ALG_KYBER512; alg <= PQC_ALG_KYBER1024; ++alg)
+    if (alg_mask & PQC_ALG_MASK(alg)) kmax = alg + 1;
+  void* p = nullptr;
+  if (kmax) {
+    c->kpolys_count = (int)pqc::kyber_plain_polys(kmax);
+    rc = ctx_alloc(c, &p, (size_t)c->kpolys_count * 256 * c->ld * sizeof(int16_t));
+    if (rc) goto fail;
+    c->kpolys = (int16_t*)p;
+    c->kbytes_per_elem = pqc::kyber_plain_bytes_per_elem(kmax);
+    rc = ctx_alloc(c, &p, c->kbytes_per_elem * c->ld);
+    if (rc) goto fail;
+    c->kbytes = (uint8_t*)p;
+    c->kprf_per_elem = (size_t)(2 * kmax + 1) * 128;
+    rc = ctx_alloc(c, &p, c->kprf_per_elem * c->ld);
+    if (rc) goto fail;
+    c->kprf = (uint8_t*)p;
+  }
+  rc = ctx_alloc(c, &p, 256);
+  if (rc) goto fail;
+  c->status = (uint32_t*)p;
+  if (hipMemset(c->status, 0, 256) != hipSuccess) {
+    rc = PQC_ERR_DEVICE;
+    goto fail;
+  }
+  *out = c;
+  return PQC_OK;
+fail:
+  pqc_ctx_destroy(c);
+  return rc;
+}
+
+void pqc_ctx_destroy(pqc_ctx* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  if (c->kpolys) (void)hipFree(c->kpolys);
+  if (c->kbytes) (void)hipFree(c->kbytes);
+  if (c->kprf) (void)hipFree(c->kprf);
+  if (c->stage) (void)hipFree(c->stage);
+  if (c->status) (void)hipFree(c->status);
+  delete c;
+}
+
+size_t pqc_ctx_device_bytes(const pqc_ctx* c) { return c ? c->allocated : 0; }
+
+int pqc_ctx_set_tuning(pqc_ctx* c, int waves_per_cu, int grid) {
+  if (!c || waves_per_cu < 0 || waves_per_cu > 32 || grid < 0) return PQC_ERR_INVALID_ARG;
+  if (waves_per_cu) c->fused_waves_per_cu = waves_per_cu;
+  c->fused_grid_override = grid;
+  return PQC_OK;
+}
+
+int pqc_alg_sizes(int alg, size_t* pk, size_t* sk, size_t* ct, size_t* ss) {
+  AlgInfo a;
+  if (!alg_info(alg, &a)) return PQC_ERR_UNSUPPORTED;
+  if (pk) *pk = a.pk;
+  if (sk) *sk = a.sk;
+  if (ct) *ct = a.ct;
+  if (ss) *ss = a.ss;
+  return PQC_OK;
+}
+
+int crypto_kem_keypair_derand_batch(pqc_ctx* c, int alg, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins,
+                                    unsigned flags, void* stream) {
+  return kem_call(c, alg, OP_KEYPAIR, n, pk, sk, coins, coins, flags, stream);
+}
+
+int crypto_kem_enc_derand_batch(pqc_ctx* c, int alg, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk,
+                                const uint8_t* coins, unsigned flags, void* stream) {
+  return kem_call(c, alg, OP_ENC, n, ct, ss, pk, coins, flags, stream);
+}
+
+int crypto_kem_dec_batch(pqc_ctx* c, int alg, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
+                         unsigned flags, void* stream) {
+  return kem_call(c, alg, OP_DEC, n, ss, nullptr, ct, sk, flags, stream);
+}
+
+int crypto_kem_keypair_batch(pqc_ctx* c, int alg, size_t n, uint8_t* pk, uint8_t* sk, unsigned flags, void* stream) {
+  if (!c || n > c->max_batch) return c ? PQC_ERR_BATCH_TOO_LARGE : PQC_ERR_INVALID_ARG;
+  uint8_t* coins = new (std::nothrow) uint8_t[n * 64 + 1];
+  if (!coins) return PQC_ERR_NO_MEMORY;
+  int rc = pqc_randombytes(coins, n * 64);
+  if (!rc) {
+    if (flags & PQC_F_HOST_PTRS) {
+      rc = kem_call(c, alg, OP_KEYPAIR, n, pk, sk, coins, coins, flags, stream);
+    } else {
+      void* d = nullptr;
+      rc = hipMalloc(&d, n * 64) == hipSuccess ? PQC_OK : PQC_ERR_NO_MEMORY;
+      if (!rc && hipMemcpy(d, coins, n * 64, hipMemcpyHostToDevice) != hipSuccess) rc = PQC_ERR_DEVICE;
+      if (!rc) rc = kem_call(c, alg, OP_KEYPAIR, n, pk, sk, (uint8_t*)d, (uint8_t*)d, flags, stream);
+      if (!rc && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) rc = PQC_ERR_DEVICE;
+      if (d) (void)hipFree(d);
+    }
+  }
+  delete[] coins;
+  return rc;
+}
+
+int crypto_kem_enc_batch(pqc_ctx* c, int alg, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, unsigned flags,
+                         void* stream) {
+  if (!c || n > c->max_batch) return c ? PQC_ERR_BATCH_TOO_LARGE : PQC_ERR_INVALID_ARG;
+  uint8_t* coins = new (std::nothrow) uint8_t[n * 32 + 1];
+  if (!coins) return PQC_ERR_NO_MEMORY;
+  int rc = pqc_randombytes(coins, n * 32);
+  if (!rc) {
+    if (flags & PQC_F_HOST_PTRS) {
+      rc = kem_call(c, alg, OP_ENC, n, ct, ss, pk, coins, flags, stream);
+    } else {
+      void* d = nullptr;
+      rc = hipMalloc(&d, n * 32) == hipSuccess ? PQC_OK : PQC_ERR_NO_MEMORY;
+      if (!rc && hipMemcpy(d, coins, n * 32, hipMemcpyHostToDevice) != hipSuccess) rc = PQC_ERR_DEVICE;
+      if (!rc) rc = kem_call(c, alg, OP_ENC, n, ct, ss, pk, (uint8_t*)d, flags, stream);
+      if (!rc && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) rc = PQC_ERR_DEVICE;
+      if (d) (void)hipFree(d);
+    }
+  }
+  delete[] coins;
+  return rc;
+}
+
+int crypto_kem_keypair(unsigned char* pk, unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_KEYPAIR, pk, sk, nullptr, nullptr); }
+int crypto_kem_enc(unsigned char* ct, unsigned char* ss, const unsigned char* pk) { return single(PQC_ALG_KYBER768, OP_ENC, ct, ss, pk, nullptr); }
+int crypto_kem_dec(unsigned char* ss, const unsigned char* ct, const unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_DEC, ss, nullptr, ct, sk); }
+
+#define PQC_NIST_SET(name, alg)                                                                        \
+  int pqc_##name##_crypto_kem_keypair(unsigned char* pk, unsigned char* sk) {                          \
+    return single(alg, OP_KEYPAIR, pk, sk, nullptr, nullptr);                                          \
+  }                                                                                                    \
+  int pqc_##name##_crypto_kem_enc(unsigned char* ct, unsigned char* ss, const unsigned char* pk) {     \
+    return single(alg, OP_ENC, ct, ss, pk, nullptr);                                                   \
+  }                                                                                                    \
+  int pqc_##name##_crypto_kem_dec(unsigned char* ss, const unsigned char* ct, const unsigned char* sk) { \
+    return single(alg, OP_DEC, ss, nullptr, ct, sk);                                                   \
+  }
+PQC_NIST_SET(kyber512, PQC_ALG_KYBER512)
+PQC_NIST_SET(kyber768, PQC_ALG_KYBER768)
+PQC_NIST_SET(kyber1024, PQC_ALG_KYBER1024)
+
+void pqc_randombytes_seed(uint64_t seed) {
+  std::lock_guard<std::mutex> lk(g_rng_mu);
+  g_rng_det = true;
+  g_rng_seed = seed;
+  g_rng_index = 0;
+}
+
+void pqc_randombytes_system(void) {
+  std::lock_guard<std::mutex> lk(g_rng_mu);
+  g_rng_det = false;
+}
+
+int pqc_randombytes(uint8_t* out, size_t n) {
+  if (!out && n) return PQC_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(g_rng_mu);
+  if (g_rng_det) {
+    drbg(out, n, g_rng_seed, 0xFFFFFFFFull, g_rng_index++);
+    return PQC_OK;
+  }
+  size_t got = 0;
+  while (got < n) {
+    ssize_t r = getrandom(out + got, n - got, 0);
+    if (r <= 0) return PQC_ERR_DEVICE;
+    got += (size_t)r;
+  }
+  return PQC_OK;
+}
+
+int pqc_device_count(int* count) {
+  if (!count) return PQC_ERR_INVALID_ARG;
+  *count = 0;
+  if (hipGetDeviceCount(count) != hipSuccess) {
+    *count = 0;
+    return PQC_ERR_NO_DEVICE;
+  }
+  return PQC_OK;
+}
+
+int pqc_set_device(int device) { return pqc::hip_err(hipSetDevice(device)); }
+
+int pqc_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return PQC_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return PQC_ERR_DEVICE;
+  if (name && name_len) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+  return PQC_OK;
+}
+
+int pqc_device_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return PQC_ERR_INVALID_ARG;
+  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+    *ptr = nullptr;
+    return PQC_ERR_NO_MEMORY;
+  }
+  return PQC_OK;
+}
+
+int pqc_device_free(void* ptr) { return pqc::hip_err(hipFree(ptr)); }
+
+int pqc_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+  if (stream) return pqc::hip_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return pqc::hip_err(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
+int pqc_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+  if (stream) return pqc::hip_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return pqc::hip_err(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+
+int pqc_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+  return pqc::hip_err(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+}
+
+int pqc_stream_create(void** stream) {
+  if (!stream) return PQC_ERR_INVALID_ARG;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PQC_ERR_DEVICE;
+  *stream = s;
+  return PQC_OK;
+}
+
+int pqc_stream_destroy(void* stream) { return pqc::hip_err(hipStreamDestroy((hipStream_t)stream)); }
+int pqc_stream_synchronize(void* stream) { return pqc::hip_err(hipStreamSynchronize((hipStream_t)stream)); }
+
+int pqc_event_create(void** ev) {
+  if (!ev) return PQC_ERR_INVALID_ARG;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return PQC_ERR_DEVICE;
+  *ev = e;
+  return PQC_OK;
+}
+
+int pqc_event_destroy(void* ev) { return pqc::hip_err(hipEventDestroy((hipEvent_t)ev)); }
+int pqc_event_record(void* ev, void* stream) { return pqc::hip_err(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream)); }
+int pqc_event_synchronize(void* ev) { return pqc::hip_err(hipEventSynchronize((hipEvent_t)ev)); }
+int pqc_event_elapsed_ms(float* ms, void* a, void* b) {
+  if (!ms) return PQC_ERR_INVALID_ARG;
+  return pqc::hip_err(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+}
+
+int pqc_keccak_batch(int op, size_t n, uint8_t* out, size_t outlen, const uint8_t* in, size_t inlen, void* stream) {
+  if (!out || !in) return PQC_ERR_INVALID_ARG;
+  return pqc::keccak_batch(op, n, out, outlen, in, inlen, (hipStream_t)stream);
+}
+
+int pqc_kyber_poly_transform(int op, size_t n, int16_t* polys, void* stream) {
+  if (!polys) return PQC_ERR_INVALID_ARG;
+  return pqc::kyber_poly_transform(op, n, polys, (hipStream_t)stream);
+}
+
+const char* pqc_version(void) { return "pqc-amd 0.1 (gfx950)"; }
+
+const char* pqc_strerror(int code) {
+  switch (code) {
+    case PQC_OK: return "ok";
+    case PQC_ERR_INVALID_ARG: return "invalid argument";
+    case PQC_ERR_UNSUPPORTED: return "algorithm not supported by this context";
+    case PQC_ERR_BATCH_TOO_LARGE: return "batch larger than the context's max_batch";
+    case PQC_ERR_NO_MEMORY: return "device memory budget exceeded";
+    case PQC_ERR_DEVICE: return "device error";
+    case PQC_ERR_NO_DEVICE: return "no device";
+    default: return "unknown error";
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,257 @@
+"""Python mirror of the engine's KEM interface.
+
+Names and argument meaning follow the reference's NIST API
+(Kyber_GPU_Batched/api.h:18-22: crypto_kem_keypair(pk, sk),
+crypto_kem_enc(ct, ss, pk), crypto_kem_dec(ss, ct, sk)) and its batched
+internal API (indcpa.h:16-26: a COUNT plus device buffers and a stream).
+Errors raise PQCError carrying the C ABI's status code.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check
+
+ALGS = {
+    "kyber512": L.ALG_KYBER512,
+    "kyber768": L.ALG_KYBER768,
+    "kyber1024": L.ALG_KYBER1024,
+}
+KYBER_K = {L.ALG_KYBER512: 2, L.ALG_KYBER768: 3, L.ALG_KYBER1024: 4}
+
+
+def alg_id(alg):
+    return ALGS[alg] if isinstance(alg, str) else int(alg)
+
+
+def sizes(alg):
+    """(pk, sk, ct, ss) byte sizes of one element."""
+    pk, sk, ct, ss = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_size_t()
+    check(L.lib().pqc_alg_sizes(alg_id(alg), C.byref(pk), C.byref(sk), C.byref(ct), C.byref(ss)), "pqc_alg_sizes")
+    return pk.value, sk.value, ct.value, ss.value
+
+
+def _ptr(a):
+    if isinstance(a, DeviceBuffer):
+        return C.c_void_p(a.ptr)
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(C.c_void_p)
+    if a is None:
+        return None
+    raise TypeError(type(a))
+
+
+class DeviceBuffer:
+    """A device allocation made through the library (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(L.lib().pqc_device_malloc(C.byref(p), nbytes), "pqc_device_malloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr, stream)
+        return b
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(L.lib().pqc_memcpy_htod(self.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes,
+                                      stream.handle if stream else None), "htod")
+
+    def download(self, shape, dtype=np.uint8, stream=None):
+        out = np.empty(shape, dtype=dtype)
+        check(L.lib().pqc_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes,
+                                      stream.handle if stream else None), "dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            L.lib().pqc_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        check(L.lib().pqc_stream_create(C.byref(p)), "pqc_stream_create")
+        self.handle = p.value
+
+    def synchronize(self):
+        check(L.lib().pqc_stream_synchronize(self.handle), "pqc_stream_synchronize")
+
+    def __del__(self):
+        try:
+            L.lib().pqc_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        check(L.lib().pqc_event_create(C.byref(p)), "pqc_event_create")
+        self.handle = p.value
+
+    def record(self, stream=None):
+        check(L.lib().pqc_event_record(self.handle, stream.handle if stream else None), "pqc_event_record")
+
+    def synchronize(self):
+        check(L.lib().pqc_event_synchronize(self.handle), "pqc_event_synchronize")
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(L.lib().pqc_event_elapsed_ms(C.byref(ms), self.handle, end.handle), "pqc_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            L.lib().pqc_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Context:
+    """Owns device scratch for up to max_batch elements (replaces poly_set4)."""
+
+    def __init__(self, algs=("kyber512", "kyber768", "kyber1024"), max_batch=1024, hbm_budget=0):
+        mask = 0
+        for a in algs:
+            mask |= 1 << alg_id(a)
+        p = C.c_void_p()
+        check(L.lib().pqc_ctx_create(C.byref(p), mask, max_batch, hbm_budget), "pqc_ctx_create")
+        self.handle = p.value
+        self.max_batch = max_batch
+
+    def close(self):
+        if self.handle:
+            L.lib().pqc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return L.lib().pqc_ctx_device_bytes(self.handle)
+
+    def set_tuning(self, waves_per_cu=0, grid=0):
+        check(L.lib().pqc_ctx_set_tuning(self.handle, waves_per_cu, grid), "pqc_ctx_set_tuning")
+
+    # ---- batched calls on device buffers (asynchronous on `stream`) ----------
+    def keypair_derand(self, alg, n, pk, sk, coins, plain=False, stream=None):
+        flags = L.F_PLAIN if plain else 0
+        check(L.lib().crypto_kem_keypair_derand_batch(self.handle, alg_id(alg), n, _ptr(pk), _ptr(sk), _ptr(coins),
+                                                      flags, stream.handle if stream else None), "keypair")
+
+    def enc_derand(self, alg, n, ct, ss, pk, coins, plain=False, stream=None):
+        flags = L.F_PLAIN if plain else 0
+        check(L.lib().crypto_kem_enc_derand_batch(self.handle, alg_id(alg), n, _ptr(ct), _ptr(ss), _ptr(pk),
+                                                  _ptr(coins), flags, stream.handle if stream else None), "enc")
+
+    def dec(self, alg, n, ss, ct, sk, plain=False, stream=None):
+        flags = L.F_PLAIN if plain else 0
+        check(L.lib().crypto_kem_dec_batch(self.handle, alg_id(alg), n, _ptr(ss), _ptr(ct), _ptr(sk), flags,
+                                           stream.handle if stream else None), "dec")
+
+    # ---- host numpy arrays (staged, synchronous) -----------------------------
+    def keypair_host(self, alg, coins, plain=False):
+        pkb, skb, _, _ = sizes(alg)
+        coins = np.ascontiguousarray(coins, dtype=np.uint8)
+        n = coins.shape[0]
+        pk = np.empty((n, pkb), np.uint8)
+        sk = np.empty((n, skb), np.uint8)
+        flags = L.F_HOST_PTRS | (L.F_PLAIN if plain else 0)
+        check(L.lib().crypto_kem_keypair_derand_batch(self.handle, alg_id(alg), n, _ptr(pk), _ptr(sk), _ptr(coins),
+                                                      flags, None), "keypair")
+        return pk, sk
+
+    def enc_host(self, alg, pk, coins, plain=False):
+        _, _, ctb, ssb = sizes(alg)
+        pk = np.ascontiguousarray(pk, dtype=np.uint8)
+        coins = np.ascontiguousarray(coins, dtype=np.uint8)
+        n = pk.shape[0]
+        ct = np.empty((n, ctb), np.uint8)
+        ss = np.empty((n, ssb), np.uint8)
+        flags = L.F_HOST_PTRS | (L.F_PLAIN if plain else 0)
+        check(L.lib().crypto_kem_enc_derand_batch(self.handle, alg_id(alg), n, _ptr(ct), _ptr(ss), _ptr(pk),
+                                                  _ptr(coins), flags, None), "enc")
+        return ct, ss
+
+    def dec_host(self, alg, ct, sk, plain=False):
+        _, _, _, ssb = sizes(alg)
+        ct = np.ascontiguousarray(ct, dtype=np.uint8)
+        sk = np.ascontiguousarray(sk, dtype=np.uint8)
+        n = ct.shape[0]
+        ss = np.empty((n, ssb), np.uint8)
+        flags = L.F_HOST_PTRS | (L.F_PLAIN if plain else 0)
+        check(L.lib().crypto_kem_dec_batch(self.handle, alg_id(alg), n, _ptr(ss), _ptr(ct), _ptr(sk), flags, None),
+              "dec")
+        return ss
+
+
+# ---- NIST single-instance drop-ins (host bytes in, bytes out) -------------------
+def _nist(prefix):
+    return prefix + "crypto_kem_"
+
+
+def crypto_kem_keypair(alg="kyber768"):
+    pkb, skb, _, _ = sizes(alg)
+    pk, sk = (C.c_uint8 * pkb)(), (C.c_uint8 * skb)()
+    check(getattr(L.lib(), "pqc_%s_crypto_kem_keypair" % alg)(pk, sk), "crypto_kem_keypair")
+    return bytes(pk), bytes(sk)
+
+
+def crypto_kem_enc(pk, alg="kyber768"):
+    _, _, ctb, ssb = sizes(alg)
+    ct, ss = (C.c_uint8 * ctb)(), (C.c_uint8 * ssb)()
+    check(getattr(L.lib(), "pqc_%s_crypto_kem_enc" % alg)(ct, ss, bytes(pk)), "crypto_kem_enc")
+    return bytes(ct), bytes(ss)
+
+
+def crypto_kem_dec(ct, sk, alg="kyber768"):
+    _, _, _, ssb = sizes(alg)
+    ss = (C.c_uint8 * ssb)()
+    check(getattr(L.lib(), "pqc_%s_crypto_kem_dec" % alg)(ss, bytes(ct), bytes(sk)), "crypto_kem_dec")
+    return bytes(ss)
+
+
+def randombytes_seed(seed):
+    L.lib().pqc_randombytes_seed(seed)
+
+
+def randombytes_system():
+    L.lib().pqc_randombytes_system()
+
+
+def keccak_batch(op, msgs, outlen):
+    """op 0 SHA3-256, 1 SHA3-512, 2 SHAKE128, 3 SHAKE256 over n fixed-length messages (device)."""
+    msgs = np.ascontiguousarray(msgs, dtype=np.uint8)
+    n, inlen = msgs.shape
+    din = DeviceBuffer.from_array(msgs) if msgs.size else DeviceBuffer(8)
+    dout = DeviceBuffer(n * outlen)
+    check(L.lib().pqc_keccak_batch(op, n, dout.ptr, outlen, din.ptr, inlen, None), "pqc_keccak_batch")
+    return dout.download((n, outlen))
+
+
+def kyber_poly_transform(op, polys):
+    """op 0 = poly_ntt, 1 = poly_invntt on an (n, 256) int16 array (device)."""
+    polys = np.ascontiguousarray(polys, dtype=np.int16)
+    d = DeviceBuffer.from_array(polys)
+    check(L.lib().pqc_kyber_poly_transform(op, polys.shape[0], d.ptr, None), "pqc_kyber_poly_transform")
+    return d.download(polys.shape, np.int16)

@@ -1,0 +1,194 @@
+"""GPU parity of the Kyber kernels against the CPU oracle (bit-exact).
+
+Every test prints the case it runs before touching the GPU."""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+ALG = {2: "kyber512", 3: "kyber768", 4: "kyber1024"}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pqc():
+    import pqc_amd
+    return pqc_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(pqc):
+    c = pqc.Context(["kyber512", "kyber768", "kyber1024"], max_batch=4096)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("op,outlen", [(0, 32), (1, 64), (2, 168), (2, 700), (3, 136), (3, 300)])
+@pytest.mark.parametrize("inlen", [0, 32, 64, 136, 168, 1088, 1184])
+def test_keccak_batch_vs_hashlib(pqc, op, outlen, inlen):
+    say("keccak op", op, "inlen", inlen, "outlen", outlen)
+    rng = np.random.default_rng(op * 1000 + inlen)
+    n = 70
+    msgs = rng.integers(0, 256, (n, inlen), dtype=np.uint8)
+    out = pqc.keccak_batch(op, msgs, outlen)
+    fns = [lambda b: hashlib.sha3_256(b).digest(), lambda b: hashlib.sha3_512(b).digest(),
+           lambda b: hashlib.shake_128(b).digest(outlen), lambda b: hashlib.shake_256(b).digest(outlen)]
+    for i in range(n):
+        assert out[i].tobytes() == fns[op](msgs[i].tobytes()), i
+
+
+def test_poly_ntt_invntt_vs_oracle(pqc):
+    rng = np.random.default_rng(7)
+    cases = [rng.integers(-3328, 3329, (40, 256)), rng.integers(-2, 3, (40, 256)), rng.integers(0, 3329, (40, 256))]
+    for ci, polys in enumerate(cases):
+        say("ntt case", ci)
+        polys = polys.astype(np.int16)
+        got = pqc.kyber_poly_transform(0, polys)
+        for i in range(len(polys)):
+            assert (got[i] == O.kyber_ntt(polys[i])).all(), (ci, i)
+        say("invntt case", ci)
+        src = np.stack([O.kyber_ntt(p) for p in polys])
+        got = pqc.kyber_poly_transform(1, src)
+        for i in range(len(polys)):
+            assert (got[i] == O.kyber_invntt(src[i])).all(), (ci, i)
+
+
+def oracle_triples(k, n, seed_stream):
+    coins = O.drbg_array(n, 64, seed=0, stream=seed_stream)
+    ecoins = O.drbg_array(n, 32, seed=0, stream=seed_stream + 1)
+    pk, sk = O.kyber_batch("keypair", k, [coins])
+    ct, ss = O.kyber_batch("enc", k, [pk, ecoins])
+    return coins, ecoins, pk, sk, ct, ss
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("plain", [True, False])
+def test_kem_vs_oracle(ctx, k, plain):
+    n = 130  # two full tiles plus a ragged one
+    say("kem k", k, "plain", plain, "n", n)
+    coins, ecoins, pk, sk, ct, ss = oracle_triples(k, n, 10 * k)
+    gpk, gsk = ctx.keypair_host(ALG[k], coins, plain=plain)
+    assert (gpk == pk).all() and (gsk == sk).all()
+    gct, gss = ctx.enc_host(ALG[k], pk, ecoins, plain=plain)
+    assert (gct == ct).all()
+    assert (gss == ss).all()
+    gss2 = ctx.dec_host(ALG[k], ct, sk, plain=plain)
+    assert (gss2 == ss).all()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("plain", [True, False])
+def test_decaps_implicit_rejection_vs_oracle(ctx, k, plain):
+    n = 96
+    say("reject k", k, "plain", plain)
+    coins, ecoins, pk, sk, ct, ss = oracle_triples(k, n, 100 + 10 * k)
+    rng = np.random.default_rng(k)
+    bad = ct.copy()
+    for i in range(n):
+        if i % 3 == 0:
+            bad[i, rng.integers(0, ct.shape[1])] ^= 1 << rng.integers(0, 8)   # one flipped bit
+        elif i % 3 == 1:
+            bad[i] = rng.integers(0, 256, ct.shape[1], dtype=np.uint8)        # garbage
+    want = O.kyber_batch("dec", k, [bad, sk])
+    got = ctx.dec_host(ALG[k], bad, sk, plain=plain)
+    assert (got == want).all()
+    for i in range(0, n, 3):
+        z = sk[i, -32:].tobytes()
+        assert got[i].tobytes() == hashlib.shake_256(z + hashlib.sha3_256(bad[i].tobytes()).digest()).digest(32)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 1000])
+def test_ragged_batches_kyber768(ctx, n):
+    say("ragged n", n)
+    coins, ecoins, pk, sk, ct, ss = oracle_triples(3, n, 500 + n)
+    gss = ctx.dec_host("kyber768", ct, sk)
+    assert (gss == ss).all()
+    gct, gss2 = ctx.enc_host("kyber768", pk, ecoins)
+    assert (gct == ct).all() and (gss2 == ss).all()
+
+
+def test_batch_of_one_equals_batch_of_n(ctx):
+    say("batch-of-1 vs batch-of-n")
+    coins, ecoins, pk, sk, ct, ss = oracle_triples(3, 40, 900)
+    full = ctx.dec_host("kyber768", ct, sk)
+    for i in range(0, 40, 7):
+        one = ctx.dec_host("kyber768", ct[i:i + 1], sk[i:i + 1])
+        assert (one[0] == full[i]).all()
+
+
+def test_errors(pqc, ctx):
+    from pqc_amd import _lib as L
+    say("error paths")
+    with pytest.raises(pqc.PQCError) as e:
+        ctx.dec_host("kyber768", np.zeros((4097, 1088), np.uint8), np.zeros((4097, 2400), np.uint8))
+    assert e.value.code == L.PQC_ERR_BATCH_TOO_LARGE
+    small = pqc.Context(["kyber512"], max_batch=8)
+    with pytest.raises(pqc.PQCError) as e:
+        small.dec_host("kyber768", np.zeros((1, 1088), np.uint8), np.zeros((1, 2400), np.uint8))
+    assert e.value.code == L.PQC_ERR_UNSUPPORTED
+    small.close()
+    with pytest.raises(pqc.PQCError) as e:
+        pqc.Context(["kyber1024"], max_batch=65536, hbm_budget=1 << 20)
+    assert e.value.code == L.PQC_ERR_NO_MEMORY
+
+
+def test_nist_dropins_deterministic(pqc):
+    say("NIST drop-ins")
+    for k, alg in ALG.items():
+        pqc.randombytes_seed(42)
+        pk, sk = pqc.crypto_kem_keypair(alg)
+        ct, ss = pqc.crypto_kem_enc(pk, alg)
+        assert pqc.crypto_kem_dec(ct, sk, alg) == ss
+        # the same DRBG draws through the oracle
+        import struct
+        d0 = hashlib.shake_256(b"pqc-mi355x" + struct.pack("<QQQ", 42, 0xFFFFFFFF, 0)).digest(64)
+        d1 = hashlib.shake_256(b"pqc-mi355x" + struct.pack("<QQQ", 42, 0xFFFFFFFF, 1)).digest(32)
+        opk, osk = O.kyber_keypair(k, d0)
+        assert (pk, sk) == (opk, osk)
+        oct_, oss = O.kyber_enc(k, opk, d1)
+        assert (ct, ss) == (oct_, oss)
+    pqc.randombytes_system()
+
+
+def test_full_size_fused_equals_plain_kyber768(pqc):
+    """65536 Kyber-768 decaps: fused == plain on every element, round trip holds,
+    and a sample of elements matches the oracle."""
+    n = 65536
+    say("full size", n)
+    ctx = pqc.Context(["kyber768"], max_batch=n)
+    rng = np.random.default_rng(0)
+    coins = rng.integers(0, 256, (n, 64), dtype=np.uint8)
+    ecoins = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    dco = pqc.DeviceBuffer.from_array(coins)
+    dec_ = pqc.DeviceBuffer.from_array(ecoins)
+    pk, sk = pqc.DeviceBuffer(n * 1184), pqc.DeviceBuffer(n * 2400)
+    ct, ss = pqc.DeviceBuffer(n * 1088), pqc.DeviceBuffer(n * 32)
+    ssf, ssp = pqc.DeviceBuffer(n * 32), pqc.DeviceBuffer(n * 32)
+    ctx.keypair_derand("kyber768", n, pk, sk, dco)
+    ctx.enc_derand("kyber768", n, ct, ss, pk, dec_)
+    # tamper 1/8 of the ciphertexts in place
+    h_ct = ct.download((n, 1088))
+    h_ct[::8, 5] ^= 0x10
+    ct.upload(h_ct)
+    ctx.dec("kyber768", n, ssf, ct, sk)
+    ctx.dec("kyber768", n, ssp, ct, sk, plain=True)
+    a, b, e = ssf.download((n, 32)), ssp.download((n, 32)), ss.download((n, 32))
+    assert (a == b).all()
+    good = np.ones(n, bool)
+    good[::8] = False
+    assert (a[good] == e[good]).all()
+    assert (a[~good] != e[~good]).any(axis=1).all()
+    h_sk = sk.download((n, 2400))
+    idx = rng.choice(n, 48, replace=False)
+    want = O.kyber_batch("dec", 3, [np.ascontiguousarray(h_ct[idx]), np.ascontiguousarray(h_sk[idx])])
+    assert (a[idx] == want).all()
+    ctx.close()
