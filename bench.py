@@ -1,0 +1,174 @@
+"""Benchmark: Kyber-768 decapsulations per second at batch 65536 on one MI355X,
+fused persistent kernel vs the plain per-op kernels, with the CPU oracle timed
+on the host cores in the same run.
+
+A step is one decapsulation pass over one batch of 65536 requests (each with
+its own key pair) whose ciphertexts and secret keys are already resident in
+HBM.  The inputs come from seeded coins, expanded by the engine's own key
+generation and encapsulation kernels.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BATCH = 65536
+ALG = "kyber768"
+K = 3
+KECCAK_PER_DECAPS = 54         # G 1 + A^T 36 + PRF 7 + H(ct) 9 + KDF 1
+OPS_PER_KECCAK = 7440          # 24 rounds x 155 64-bit ops = 7440 32-bit ops
+BYTES_PER_DECAPS = 1088 + 2400 + 32  # compulsory: ct in, sk in, ss out
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz int32 ops
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, stream, steps, warmup, pqc):
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    evs = [(pqc.Event(), pqc.Event()) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    per = [a.elapsed_ms(b) for a, b in evs]
+    return wall, per
+
+
+def cpu_baseline(ct_h, sk_h, ss_gpu, threads):
+    from tests import oracle_lib as O
+    n1 = min(4096, len(ct_h))
+    O.kyber_batch("dec", K, [ct_h[:64], sk_h[:64]], nthreads=threads)  # warm
+    t0 = time.perf_counter()
+    ss = O.kyber_batch("dec", K, [np.ascontiguousarray(ct_h[:n1]), np.ascontiguousarray(sk_h[:n1])], nthreads=threads)
+    dt = time.perf_counter() - t0
+    assert (ss == ss_gpu[:n1]).all(), "CPU oracle and GPU disagree"
+    # single core on a smaller sample
+    n2 = min(512, len(ct_h))
+    t0 = time.perf_counter()
+    ss1 = O.kyber_batch("dec", K, [np.ascontiguousarray(ct_h[:n2]), np.ascontiguousarray(sk_h[:n2])], nthreads=1)
+    dt1 = time.perf_counter() - t0
+    assert (ss1 == ss_gpu[:n2]).all()
+    return n1 / dt, n1, n2 / dt1, n2
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine shards by request with no collective, "
+              "so multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import pqc_amd as pqc
+    n = args.batch
+    ctx = pqc.Context([ALG], max_batch=n)
+    stream = pqc.Stream()
+    rng = np.random.default_rng(0)
+    coins = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 64), dtype=np.uint8))
+    ecoins = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8))
+    pk, sk = pqc.DeviceBuffer(n * 1184), pqc.DeviceBuffer(n * 2400)
+    ct, ss_enc = pqc.DeviceBuffer(n * 1088), pqc.DeviceBuffer(n * 32)
+    ss_f, ss_p = pqc.DeviceBuffer(n * 32), pqc.DeviceBuffer(n * 32)
+    ctx.keypair_derand(ALG, n, pk, sk, coins, stream=stream)
+    ctx.enc_derand(ALG, n, ct, ss_enc, pk, ecoins, stream=stream)
+    stream.synchronize()
+    log("inputs ready: %d Kyber-768 key pairs and ciphertexts in HBM" % n)
+
+    fused = lambda: ctx.dec(ALG, n, ss_f, ct, sk, stream=stream)
+    plain = lambda: ctx.dec(ALG, n, ss_p, ct, sk, plain=True, stream=stream)
+    wall_f, per_f = timed(fused, stream, args.steps, args.warmup, pqc)
+    log("fused: %.3f ms/step" % (wall_f * 1e3 / args.steps))
+    wall_p, per_p = timed(plain, stream, max(3, args.steps // 4), 1, pqc)
+    log("plain: %.3f ms/step" % (wall_p * 1e3 / max(3, args.steps // 4)))
+
+    a, b, e = ss_f.download((n, 32)), ss_p.download((n, 32)), ss_enc.download((n, 32))
+    bitexact = bool((a == b).all() and (a == e).all())
+
+    ms_f = wall_f * 1e3 / args.steps
+    ms_p = wall_p * 1e3 / max(3, args.steps // 4)
+    kern_ms = float(np.mean(per_f))
+    rate_f = n / (ms_f / 1e3)
+    rate_p = n / (ms_p / 1e3)
+    ops = n * KECCAK_PER_DECAPS * OPS_PER_KECCAK
+    achieved = ops / (kern_ms / 1e3) / 1e12
+
+    out = {
+        "metric": "Kyber-768 decaps/s at batch 64K (fused vs plain); p99 latency at fixed rate",
+        "value": round(rate_f, 1),
+        "unit": "decaps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_f, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic: seeded coins, keys and ciphertexts generated on the GPU by the engine",
+        "config": {
+            "workload": "kyber768_decaps_fused",
+            "alg": "Kyber-768 round 2 (k=3, eta=2, du=10, dv=4)",
+            "batch": n,
+            "plain_decaps_per_s": round(rate_p, 1),
+            "plain_ms_per_step": round(ms_p, 4),
+            "fused_over_plain": round(rate_f / rate_p, 3),
+            "batch_latency_ms_p50": round(float(np.percentile(per_f, 50)), 4),
+            "batch_latency_ms_p99": round(float(np.percentile(per_f, 99)), 4),
+            "bitexact_fused_plain_encaps": bitexact,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_fused<3,true> (fused Kyber-768 decaps)",
+            "achieved": round(achieved, 3),
+            "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "Tops/s (int32)",
+            "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "work_per_launch": "%d decaps x 54 Keccak-f x 7440 int32 ops" % n,
+            "kernel_ms": round(kern_ms, 4),
+            "hbm_GBps_compulsory": round(n * BYTES_PER_DECAPS / (kern_ms / 1e3) / 1e9, 1),
+            "hbm_frac_compulsory": round(n * BYTES_PER_DECAPS / (kern_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        ct_h = ct.download((n, 1088))
+        sk_h = sk.download((n, 2400))
+        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        r_all, s_all, r_one, s_one = cpu_baseline(ct_h, sk_h, a, thr)
+        out["cpu_baseline"] = {
+            "value": round(r_all, 1),
+            "unit": "decaps/s",
+            "cores": thr,
+            "kind": "port",
+            "sample": "%d Kyber-768 decaps of the same batch on %d threads (and %d on 1 thread: %.1f/s); "
+                      "outputs checked equal to the GPU's" % (s_all, thr, s_one, r_one),
+            "single_core_value": round(r_one, 1),
+        }
+        out["config"]["gpu_over_cpu"] = round(rate_f / r_all, 1)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -58,45 +58,82 @@ __constant__ const uint64_t c_keccak_rc[24] = {
     keccak_rc(12), keccak_rc(13), keccak_rc(14), keccak_rc(15), keccak_rc(16), keccak_rc(17),
     keccak_rc(18), keccak_rc(19), keccak_rc(20), keccak_rc(21), keccak_rc(22), keccak_rc(23)};
 
+// The permutation works on explicit 32-bit halves so that every step maps to
+// one gfx950 instruction per half:
+//   three-input XOR and chi's a ^ (~b & c) ... v_bitop3_b32 (truth tables 0x96
+//     and 0xD2 over operand patterns S0 = 0xF0, S1 = 0xCC, S2 = 0xAA)
+//   64-bit rotation .......................... two v_alignbit_b32
+// giving 180 VALU instructions per round (theta 80, rho 48, chi 50, iota 2).
+struct Lane2 {
+  uint32_t lo, hi;
+};
+
+__device__ __forceinline__ uint32_t xor3_32(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+__device__ __forceinline__ uint32_t chi_32(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);  // a ^ (~b & c)
+}
+
 template <int N>
-__device__ __forceinline__ uint64_t rotl64(uint64_t x) {
+__device__ __forceinline__ Lane2 rotl_2(Lane2 x) {
   if constexpr (N == 0) {
     return x;
+  } else if constexpr (N < 32) {
+    return Lane2{__builtin_amdgcn_alignbit(x.lo, x.hi, 32 - N), __builtin_amdgcn_alignbit(x.hi, x.lo, 32 - N)};
+  } else if constexpr (N == 32) {
+    return Lane2{x.hi, x.lo};
   } else {
-    return (x << N) | (x >> (64 - N));
+    return Lane2{__builtin_amdgcn_alignbit(x.hi, x.lo, 64 - N), __builtin_amdgcn_alignbit(x.lo, x.hi, 64 - N)};
   }
 }
 
 template <int I>
-__device__ __forceinline__ void keccak_rho_pi(uint64_t (&B)[25], const uint64_t (&A)[25],
-                                              const uint64_t (&D)[5]) {
-  B[kKeccakMaps.pi_dst[I]] = rotl64<kKeccakMaps.rho[I]>(A[I] ^ D[I % 5]);
-  if constexpr (I + 1 < 25) keccak_rho_pi<I + 1>(B, A, D);
+__device__ __forceinline__ void keccak_theta_rho_pi(Lane2 (&B)[25], const Lane2 (&A)[25], const Lane2 (&Cm)[5],
+                                                    const Lane2 (&Rp)[5]) {
+  // theta folded into one XOR3: A ^ C[x-1] ^ rot(C[x+1], 1)
+  constexpr int x = I % 5;
+  const Lane2 t{xor3_32(A[I].lo, Cm[x].lo, Rp[x].lo), xor3_32(A[I].hi, Cm[x].hi, Rp[x].hi)};
+  B[kKeccakMaps.pi_dst[I]] = rotl_2<kKeccakMaps.rho[I]>(t);
+  if constexpr (I + 1 < 25) keccak_theta_rho_pi<I + 1>(B, A, Cm, Rp);
 }
 
 // One Keccak-f[1600] permutation of a register-resident state.  The round loop
 // stays rolled (the round constant is a uniform scalar load) so that every
 // call site costs one copy of the round body.
-__device__ __forceinline__ void keccakf1600(uint64_t (&A)[25]) {
+__device__ __forceinline__ void keccakf1600(uint64_t (&A64)[25]) {
+  Lane2 A[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A[i] = Lane2{(uint32_t)A64[i], (uint32_t)(A64[i] >> 32)};
 #pragma unroll 1
   for (int round = 0; round < 24; ++round) {
-    uint64_t C[5], D[5], B[25];
+    Lane2 C[5], Cm[5], Rp[5], B[25];
 #pragma unroll
-    for (int x = 0; x < 5; ++x) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
+    for (int x = 0; x < 5; ++x) {
+      C[x].lo = xor3_32(xor3_32(A[x].lo, A[x + 5].lo, A[x + 10].lo), A[x + 15].lo, A[x + 20].lo);
+      C[x].hi = xor3_32(xor3_32(A[x].hi, A[x + 5].hi, A[x + 10].hi), A[x + 15].hi, A[x + 20].hi);
+    }
 #pragma unroll
-    for (int x = 0; x < 5; ++x) D[x] = C[(x + 4) % 5] ^ rotl64<1>(C[(x + 1) % 5]);
-    keccak_rho_pi<0>(B, A, D);
+    for (int x = 0; x < 5; ++x) {
+      Cm[x] = C[(x + 4) % 5];
+      Rp[x] = rotl_2<1>(C[(x + 1) % 5]);
+    }
+    keccak_theta_rho_pi<0>(B, A, Cm, Rp);
 #pragma unroll
     for (int y = 0; y < 5; ++y) {
 #pragma unroll
       for (int x = 0; x < 5; ++x) {
-        const uint64_t a = B[x + 5 * y], b = B[(x + 1) % 5 + 5 * y], c = B[(x + 2) % 5 + 5 * y];
-        // a ^ (~b & c) written as a bit-select: b ? a : a ^ c  (v_bfi_b32)
-        A[x + 5 * y] = (b & a) | (~b & (a ^ c));
+        const Lane2 a = B[x + 5 * y], b = B[(x + 1) % 5 + 5 * y], c = B[(x + 2) % 5 + 5 * y];
+        A[x + 5 * y] = Lane2{chi_32(a.lo, b.lo, c.lo), chi_32(a.hi, b.hi, c.hi)};
       }
     }
-    A[0] ^= c_keccak_rc[round];
+    const uint64_t rc = c_keccak_rc[round];
+    A[0].lo ^= (uint32_t)rc;
+    A[0].hi ^= (uint32_t)(rc >> 32);
   }
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A64[i] = (uint64_t)A[i].lo | ((uint64_t)A[i].hi << 32);
 }
 
 __device__ __forceinline__ void keccak_zero(uint64_t (&A)[25]) {

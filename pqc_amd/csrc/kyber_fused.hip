@@ -33,17 +33,23 @@ __constant__ LaneTwiddles c_lane_tw = make_lane_twiddles();
 constexpr int kAStride = 262;   // int16 slots per A entry: 256 + guard, odd dword count
 constexpr int kMaxBlocks = 32;  // SHAKE128 blocks per entry before giving up (never reached)
 
+// A tile is 64 requests, one per lane in the per-request phases.  Its 64*K*K
+// matrix entries are generated in K*K rounds of 64 (one per lane).  Entry
+// E = t*K*K + i*K + j lives in ring slot E mod RING; a round's 64 entries plus
+// the at most K*K-1 earlier entries of a request that straddles two rounds are
+// exactly RING consecutive values of E, so no live slot is ever overwritten.
 template <int K>
 struct Tile {
-  static constexpr int RA = 64 / (K * K);           // requests per A round: 16, 7, 4
-  static constexpr int NR = (K == 3) ? 9 : 64 / RA;  // rounds per tile: 4, 9, 16
-  static constexpr int T = RA * NR;                  // requests per tile: 64, 63, 64
-  static constexpr int E = RA * K * K;               // A entries per round: 64, 63, 64
-  static constexpr int PRFB = (2 * K + 1) * 128;     // PRF bytes per request
+  static constexpr int T = 64;                                  // requests per tile
+  static constexpr int KK = K * K;
+  static constexpr int NR = KK;                                 // rounds: 64*K*K / 64
+  static constexpr int RING = (64 % KK == 0) ? 64 : 64 + KK - 1;  // 64, 72, 64 slots
+  static constexpr int PRFB = (2 * K + 1) * 128;                // PRF bytes per request
 };
 
+template <int RING>
 struct Lds {
-  int16_t a[64 * kAStride];
+  int16_t a[RING * kAStride];
   uint8_t msg[64][32];
   uint32_t fail[64];
 };
@@ -104,10 +110,11 @@ __device__ __forceinline__ void pack_pair(uint8_t* dst, const uint32_t (&t)[4]) 
 // Re-encryption of one request (indcpa_enc from sp-sampling on).  DEC: compare
 // with ct and return the lane's mismatch bits.  ENC: write ct.
 template <int K, bool DEC>
-__device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* a_entries, const uint8_t* prf,
+__device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, const uint8_t* prf,
                                                    const uint8_t* pk, const uint8_t* msg, uint8_t* ct,
                                                    const LaneZ& z) {
   using P = Params<K>;
+  constexpr int RING = Tile<K>::RING;
   const int l = lane_id();
   const int gn = ntt_group(l);
   const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
@@ -123,7 +130,8 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* a_entries, con
     P4 acc = {{0, 0, 0, 0}};
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const uint32_t* e32 = reinterpret_cast<const uint32_t*>(a_entries + (i * K + j) * kAStride + 4 * gn);
+      const int slot = (e0 + i * K + j) % RING;
+      const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
       const uint32_t w0 = e32[0], w1 = e32[1];
       P4 a = {{(int32_t)(w0 & 0xFFFF), (int32_t)(w0 >> 16), (int32_t)(w1 & 0xFFFF), (int32_t)(w1 >> 16)}};
       basemul_acc(acc, a, sp[j], z.bm);
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, uint8_t* __rest
                                               uint32_t* __restrict__ status) {
   using P = Params<K>;
   using TL = Tile<K>;
-  __shared__ Lds sh;
+  __shared__ Lds<TL::RING> sh;
   const int lane = threadIdx.x;
   LaneZ z;
   load_lane_z(z, &c_lane_tw);
@@ -292,33 +300,36 @@ __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, uint8_t* __rest
     __syncthreads();
 
     // ---- A^T rounds + re-encryption
+    int t_done = 0;  // requests whose re-encryption is finished
     for (int g = 0; g < TL::NR; ++g) {
       {
-        const int e = lane;
-        const int t = g * TL::RA + e / (K * K);
-        const int ij = e % (K * K);
+        const int E = 64 * g + lane;
+        const int t = E / TL::KK;
+        const int ij = E % TL::KK;
         const int req = base + t;
-        const bool valid = e < TL::E && req < n;
+        const bool valid = req < n;
         const uint8_t* seed = key + (size_t)(valid ? req : rc) * KEYB + PK_OFF + P::polyvec_bytes;
         uint64_t rho[4];
 #pragma unroll
         for (int w = 0; w < 4; ++w) rho[w] = load_u64(seed + 8 * w);
-        const int got = sample_entry(sh.a + e * kAStride, rho, ij / K, ij % K, valid);
+        const int got = sample_entry(sh.a + (E % TL::RING) * kAStride, rho, ij / K, ij % K, valid);
         if (valid && got < N) atomicOr(status, 1u);
       }
       __syncthreads();
-      for (int tt = 0; tt < TL::RA; ++tt) {
-        const int t = g * TL::RA + tt;
+      // requests all of whose entries are now sampled
+      const int t_end = min(TL::T, (64 * (g + 1)) / TL::KK);
+      for (int t = t_done; t < t_end; ++t) {
         const int req = base + t;
         if (req >= n) break;
         const uint8_t* kr = key + (size_t)req * KEYB;
-        const uint32_t mm = coop_reencrypt<K, DEC>(sh.a + tt * K * K * kAStride, prf + (size_t)req * TL::PRFB,
-                                                   kr + PK_OFF, sh.msg[t], ct + (size_t)req * P::ct_bytes, z);
+        const uint32_t mm = coop_reencrypt<K, DEC>(sh.a, t * TL::KK, prf + (size_t)req * TL::PRFB, kr + PK_OFF,
+                                                   sh.msg[t], ct + (size_t)req * P::ct_bytes, z);
         if constexpr (DEC) {
           const uint32_t f = __any(mm != 0) ? 1u : 0u;
           if (lane == 0) sh.fail[t] = f;
         }
       }
+      t_done = t_end;
       __syncthreads();
     }
 

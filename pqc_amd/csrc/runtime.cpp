@@ -230,6 +230,7 @@ int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const ui
   if (op == OP_KEYPAIR) {
     if (pqc_randombytes(coins, 64)) return PQC_ERR_DEVICE;
     i0 = coins;
+    i1 = coins;
   } else if (op == OP_ENC) {
     if (pqc_randombytes(coins, 32)) return PQC_ERR_DEVICE;
     i1 = coins;
