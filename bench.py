@@ -26,6 +26,10 @@ OPS_PER_KECCAK = 7440          # 24 rounds x 155 64-bit ops = 7440 32-bit ops
 BYTES_PER_DECAPS = 1088 + 2400 + 32  # compulsory: ct in, sk in, ss out
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz int32 ops
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per fused launch at batch 65536 from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs, each scaled by the same counter's reading of a 256 MiB
+# device copy (profiles/kyber768_decaps_b65536.md): 454.5 MB read + 60.8 MB written.
+MEASURED_TRAFFIC_BYTES_B65536 = 515.3e6
 
 
 def log(*a):
@@ -73,6 +77,10 @@ def main():
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--only", choices=["fused", "plain"], default=None,
+                    help="time one path only (profiling runs)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards by request with no collective, "
@@ -93,11 +101,24 @@ def main():
     ctx.enc_derand(ALG, n, ct, ss_enc, pk, ecoins, stream=stream)
     stream.synchronize()
     log("inputs ready: %d Kyber-768 key pairs and ciphertexts in HBM" % n)
+    if args.calibrate_copy_mb:
+        nb = args.calibrate_copy_mb << 20
+        src, dst = pqc.DeviceBuffer(nb), pqc.DeviceBuffer(nb)
+        pqc.lib().pqc_memcpy_dtod(dst.ptr, src.ptr, nb, stream.handle)
+        stream.synchronize()
+        log("calibration copy: %d MiB device to device" % args.calibrate_copy_mb)
+        del src, dst
 
     fused = lambda: ctx.dec(ALG, n, ss_f, ct, sk, stream=stream)
     plain = lambda: ctx.dec(ALG, n, ss_p, ct, sk, plain=True, stream=stream)
+    if args.only == "plain":
+        wall_p, per_p = timed(plain, stream, args.steps, args.warmup, pqc)
+        log("plain: %.3f ms/step" % (wall_p * 1e3 / args.steps))
+        return
     wall_f, per_f = timed(fused, stream, args.steps, args.warmup, pqc)
     log("fused: %.3f ms/step" % (wall_f * 1e3 / args.steps))
+    if args.only == "fused":
+        return
     wall_p, per_p = timed(plain, stream, max(3, args.steps // 4), 1, pqc)
     log("plain: %.3f ms/step" % (wall_p * 1e3 / max(3, args.steps // 4)))
 
@@ -147,7 +168,7 @@ def main():
             "kernel_ms": round(kern_ms, 4),
             "hbm_GBps_compulsory": round(n * BYTES_PER_DECAPS / (kern_ms / 1e3) / 1e9, 1),
             "hbm_frac_compulsory": round(n * BYTES_PER_DECAPS / (kern_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_BYTES_B65536 if n == BATCH else None,
         },
         "cpu_baseline": None,
     }
