@@ -93,6 +93,17 @@ int pqc_kyber1024_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
 int pqc_kyber1024_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
 int pqc_kyber1024_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
 
+/* ---- NewHope-1024 CPA-PKE, plain batched kernels ------------------------------
+ * Replaces cpapke_keypair/enc/dec (NewHope_NIST_GPU/cpapke.h:14-24).
+ * Element sizes: seed 32, pk 1824, sk 1792, ct 2176, message 32, coins 32.
+ * Requires PQC_ALG_NEWHOPE1024_CPA in the context's mask.                       */
+int pqc_newhope1024_cpa_keypair_batch(pqc_ctx *ctx, size_t n, uint8_t *pk, uint8_t *sk,
+                                      const uint8_t *seeds, unsigned flags, void *stream);
+int pqc_newhope1024_cpa_enc_batch(pqc_ctx *ctx, size_t n, uint8_t *ct, const uint8_t *msgs,
+                                  const uint8_t *pk, const uint8_t *coins, unsigned flags, void *stream);
+int pqc_newhope1024_cpa_dec_batch(pqc_ctx *ctx, size_t n, uint8_t *msgs, const uint8_t *ct,
+                                  const uint8_t *sk, unsigned flags, void *stream);
+
 /* ---- randomness for the non-derand entry points ------------------------------- */
 void pqc_randombytes_seed(uint64_t seed); /* deterministic SHAKE256 DRBG */
 void pqc_randombytes_system(void);        /* OS randomness (default) */

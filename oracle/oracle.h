@@ -75,6 +75,21 @@ int oracle_kyber_enc_batch(int k, size_t n, uint8_t *ct, uint8_t *ss, const uint
 int oracle_kyber_keypair_batch(int k, size_t n, uint8_t *pk, uint8_t *sk, const uint8_t *coins,
                                int nthreads);
 
+/* ---- NewHope-1024 CPA-PKE (NewHope_NIST_GPU) --------------------------------
+ * pk 1824, sk 1792, ct 2176, message 32 bytes. */
+void oracle_newhope_tables(uint16_t omegas_inv[512], uint16_t gammas[1024], uint16_t gammas_inv[1024]);
+uint16_t oracle_newhope_montgomery_reduce(uint32_t a);
+uint16_t oracle_newhope_barrett_reduce(uint16_t a);
+void oracle_newhope_ntt(uint16_t *r);     /* poly_ntt: mul gammas, ntt */
+void oracle_newhope_invntt(uint16_t *r);  /* poly_invntt */
+void oracle_newhope_poly_uniform(uint16_t *a, const uint8_t seed[32]);
+void oracle_newhope_poly_sample(uint16_t *r, const uint8_t seed[32], uint8_t nonce);
+int oracle_newhope_cpa_keypair(uint8_t *pk, uint8_t *sk, const uint8_t seed[32]);
+int oracle_newhope_cpa_enc(uint8_t *c, const uint8_t m[32], const uint8_t *pk, const uint8_t coin[32]);
+int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk);
+int oracle_newhope_triple_batch(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
+                                const uint8_t *seed, const uint8_t *msg, const uint8_t *coin, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

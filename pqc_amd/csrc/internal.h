@@ -23,6 +23,11 @@ struct pqc_ctx {
   uint8_t* kprf = nullptr;
   size_t kprf_per_elem = 0;
 
+  // NewHope-1024 CPA plain path: 8 SoA uint16 polynomials of 1024 coefficients
+  // and 64 bytes per element (seed expansion)
+  uint16_t* nhpolys = nullptr;
+  uint8_t* nhbytes = nullptr;
+
   // staging for host-pointer calls
   uint8_t* stage = nullptr;
   size_t stage_bytes = 0;
@@ -52,6 +57,12 @@ int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const
                     const uint8_t* coins, hipStream_t s);
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
                     hipStream_t s);
+
+// NewHope-1024 CPA plain batched path (newhope.hip)
+int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s);
+int newhope_enc(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
+                hipStream_t s);
+int newhope_dec(pqc_ctx* c, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
 
 // component kernels exposed for tests (kyber_fused.hip)
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);

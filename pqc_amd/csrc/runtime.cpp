@@ -22,6 +22,7 @@ bool alg_info(int alg, AlgInfo* a) {
     case PQC_ALG_KYBER512: *a = {1, 2, 800, 1632, 736, 32}; return true;
     case PQC_ALG_KYBER768: *a = {1, 3, 1184, 2400, 1088, 32}; return true;
     case PQC_ALG_KYBER1024: *a = {1, 4, 1568, 3168, 1568, 32}; return true;
+    case PQC_ALG_NEWHOPE1024_CPA: *a = {2, 0, 1824, 1792, 2176, 32}; return true;
     default: return false;
   }
 }
@@ -154,73 +155,94 @@ int stage_ensure(pqc_ctx* c, size_t bytes) {
 
 enum Op { OP_KEYPAIR, OP_ENC, OP_DEC };
 
-// Common entry: validates, stages host buffers, dispatches.
-int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
-             const uint8_t* i1, unsigned flags, void* stream) {
-  if (!c) return PQC_ERR_INVALID_ARG;
-  AlgInfo a;
-  if (!alg_info(alg, &a)) return PQC_ERR_UNSUPPORTED;
-  if (!(c->alg_mask & PQC_ALG_MASK(alg))) return PQC_ERR_UNSUPPORTED;
-  if (n > c->max_batch) return PQC_ERR_BATCH_TOO_LARGE;
-  if (n == 0) return PQC_OK;
-  if (!o0 || !i0 || (op != OP_DEC && (!o1 || !i1)) || (op == OP_DEC && !i1)) return PQC_ERR_INVALID_ARG;
-  hipStream_t s = (hipStream_t)stream;
-  const bool plain = (flags & PQC_F_PLAIN) != 0;
+// One buffer argument of a batched call: `per` bytes per element.
+struct Buf {
+  const void* p;
+  size_t per;
+  bool out;
+};
 
-  // sizes of o0, o1, i0, i1 per element
-  size_t so0, so1, si0, si1;
-  switch (op) {
-    case OP_KEYPAIR: so0 = a.pk; so1 = a.sk; si0 = 64; si1 = 0; break;
-    case OP_ENC: so0 = a.ct; so1 = a.ss; si0 = a.pk; si1 = 32; break;
-    default: so0 = a.ss; so1 = 0; si0 = a.ct; si1 = a.sk; break;
+// Validates pointers, stages host buffers through the context when
+// PQC_F_HOST_PTRS is set, runs `body` on device pointers, copies outputs back
+// and synchronises.  Device-pointer calls stay asynchronous on the stream.
+template <size_t NB, typename F>
+int staged(pqc_ctx* c, size_t n, unsigned flags, hipStream_t s, const Buf (&bufs)[NB], F body) {
+  uint8_t* dev[NB];
+  for (size_t i = 0; i < NB; ++i) {
+    if (!bufs[i].p) return PQC_ERR_INVALID_ARG;
+    dev[i] = (uint8_t*)bufs[i].p;
   }
-  uint8_t *d_o0 = o0, *d_o1 = o1;
-  const uint8_t *d_i0 = i0, *d_i1 = i1;
   const bool host = (flags & PQC_F_HOST_PTRS) != 0;
   if (host) {
-    const size_t total = n * (so0 + so1 + si0 + si1) + 4 * 256;
+    size_t total = 0;
+    for (size_t i = 0; i < NB; ++i) total += ((n * bufs[i].per + 255) & ~(size_t)255);
     int rc = stage_ensure(c, total);
     if (rc) return rc;
     uint8_t* p = c->stage;
-    auto take = [&](size_t bytes) {
-      uint8_t* q = p;
-      p += (bytes + 255) & ~(size_t)255;
-      return q;
-    };
-    d_o0 = take(n * so0);
-    d_o1 = so1 ? take(n * so1) : nullptr;
-    uint8_t* t0 = take(n * si0);
-    uint8_t* t1 = si1 ? take(n * si1) : nullptr;
-    if (hipMemcpyAsync(t0, i0, n * si0, hipMemcpyHostToDevice, s) != hipSuccess) return PQC_ERR_DEVICE;
-    if (si1 && hipMemcpyAsync(t1, i1, n * si1, hipMemcpyHostToDevice, s) != hipSuccess) return PQC_ERR_DEVICE;
-    d_i0 = t0;
-    d_i1 = t1;
+    for (size_t i = 0; i < NB; ++i) {
+      dev[i] = p;
+      p += (n * bufs[i].per + 255) & ~(size_t)255;
+      if (!bufs[i].out &&
+          hipMemcpyAsync(dev[i], bufs[i].p, n * bufs[i].per, hipMemcpyHostToDevice, s) != hipSuccess)
+        return PQC_ERR_DEVICE;
+    }
   } else {
-    if (!aligned8(o0) || (o1 && !aligned8(o1)) || !aligned8(i0) || (i1 && !aligned8(i1)))
-      return PQC_ERR_INVALID_ARG;
+    for (size_t i = 0; i < NB; ++i)
+      if (!aligned8(bufs[i].p)) return PQC_ERR_INVALID_ARG;
   }
-  if (op == OP_KEYPAIR) {
-    // keypair derand: i0 = coins (64 B/elem); key generation runs on the plain kernels
-    int rc = pqc::kyber_plain_keypair(c, a.k, n, d_o0, d_o1, d_i0, s);
-    if (rc) return rc;
-  } else if (op == OP_ENC) {
-    int rc = plain ? pqc::kyber_plain_enc(c, a.k, n, d_o0, d_o1, d_i0, d_i1, s)
-                   : pqc::kyber_fused_enc(c, a.k, n, d_o0, d_o1, d_i0, d_i1, s);
-    if (rc) return rc;
-  } else {
-    int rc = plain ? pqc::kyber_plain_dec(c, a.k, n, d_o0, d_i0, d_i1, s)
-                   : pqc::kyber_fused_dec(c, a.k, n, d_o0, d_i0, d_i1, s);
-    if (rc) return rc;
-  }
+  int rc = body(dev);
+  if (rc) return rc;
   if (host) {
-    if (hipMemcpyAsync(o0, d_o0, n * so0, hipMemcpyDeviceToHost, s) != hipSuccess) return PQC_ERR_DEVICE;
-    if (so1 && hipMemcpyAsync(o1, d_o1, n * so1, hipMemcpyDeviceToHost, s) != hipSuccess) return PQC_ERR_DEVICE;
+    for (size_t i = 0; i < NB; ++i)
+      if (bufs[i].out &&
+          hipMemcpyAsync((void*)bufs[i].p, dev[i], n * bufs[i].per, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return PQC_ERR_DEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return PQC_ERR_DEVICE;
     uint32_t st = 0;
     if (hipMemcpy(&st, c->status, 4, hipMemcpyDeviceToHost) != hipSuccess) return PQC_ERR_DEVICE;
     if (st) return PQC_ERR_DEVICE;
   }
   return PQC_OK;
+}
+
+int check_call(pqc_ctx* c, int alg, size_t n) {
+  if (!c) return PQC_ERR_INVALID_ARG;
+  if (!(c->alg_mask & PQC_ALG_MASK(alg))) return PQC_ERR_UNSUPPORTED;
+  if (n > c->max_batch) return PQC_ERR_BATCH_TOO_LARGE;
+  return PQC_OK;
+}
+
+// Kyber batched entry: o0/o1 outputs, i0/i1 inputs (see the switch for sizes).
+int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
+             const uint8_t* i1, unsigned flags, void* stream) {
+  AlgInfo a;
+  if (!alg_info(alg, &a) || a.kind != 1) return PQC_ERR_UNSUPPORTED;
+  int rc = check_call(c, alg, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
+  const int k = a.k;
+  switch (op) {
+    case OP_KEYPAIR: {
+      const Buf b[3] = {{o0, a.pk, true}, {o1, a.sk, true}, {i0, 64, false}};
+      // key generation runs on the plain kernels
+      return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::kyber_plain_keypair(c, k, n, d[0], d[1], d[2], s); });
+    }
+    case OP_ENC: {
+      const Buf b[4] = {{o0, a.ct, true}, {o1, a.ss, true}, {i0, a.pk, false}, {i1, 32, false}};
+      return staged(c, n, flags, s, b, [&](uint8_t** d) {
+        return plain ? pqc::kyber_plain_enc(c, k, n, d[0], d[1], d[2], d[3], s)
+                     : pqc::kyber_fused_enc(c, k, n, d[0], d[1], d[2], d[3], s);
+      });
+    }
+    default: {
+      const Buf b[3] = {{o0, a.ss, true}, {i0, a.ct, false}, {i1, a.sk, false}};
+      return staged(c, n, flags, s, b, [&](uint8_t** d) {
+        return plain ? pqc::kyber_plain_dec(c, k, n, d[0], d[1], d[2], s)
+                     : pqc::kyber_fused_dec(c, k, n, d[0], d[1], d[2], s);
+      });
+    }
+  }
 }
 
 int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1) {
@@ -247,7 +269,8 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
   *out = nullptr;
   int rc = check_device();
   if (rc) return rc;
-  const unsigned known = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024);
+  const unsigned known = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) |
+                         PQC_ALG_MASK(PQC_ALG_KYBER1024) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA);
   if (alg_mask == 0 || (alg_mask & ~known)) return PQC_ERR_UNSUPPORTED;
   pqc_ctx* c = new pqc_ctx();
   if (hipGetDevice(&c->device) != hipSuccess) {
@@ -282,6 +305,14 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     if (rc) goto fail;
     c->kprf = (uint8_t*)p;
   }
+  if (alg_mask & PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA)) {
+    rc = ctx_alloc(c, &p, (size_t)8 * 1024 * c->ld * sizeof(uint16_t));
+    if (rc) goto fail;
+    c->nhpolys = (uint16_t*)p;
+    rc = ctx_alloc(c, &p, (size_t)64 * c->ld);
+    if (rc) goto fail;
+    c->nhbytes = (uint8_t*)p;
+  }
   rc = ctx_alloc(c, &p, 256);
   if (rc) goto fail;
   c->status = (uint32_t*)p;
@@ -302,6 +333,8 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->kpolys) (void)hipFree(c->kpolys);
   if (c->kbytes) (void)hipFree(c->kbytes);
   if (c->kprf) (void)hipFree(c->kprf);
+  if (c->nhpolys) (void)hipFree(c->nhpolys);
+  if (c->nhbytes) (void)hipFree(c->nhbytes);
   if (c->stage) (void)hipFree(c->stage);
   if (c->status) (void)hipFree(c->status);
   delete c;
@@ -382,6 +415,33 @@ int crypto_kem_enc_batch(pqc_ctx* c, int alg, size_t n, uint8_t* ct, uint8_t* ss
   }
   delete[] coins;
   return rc;
+}
+
+int pqc_newhope1024_cpa_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seeds,
+                                      unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_NEWHOPE1024_CPA, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[3] = {{pk, 1824, true}, {sk, 1792, true}, {seeds, 32, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_keypair(c, n, d[0], d[1], d[2], s); });
+}
+
+int pqc_newhope1024_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* msgs, const uint8_t* pk,
+                                  const uint8_t* coins, unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_NEWHOPE1024_CPA, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[4] = {{ct, 2176, true}, {msgs, 32, false}, {pk, 1824, false}, {coins, 32, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_enc(c, n, d[0], d[1], d[2], d[3], s); });
+}
+
+int pqc_newhope1024_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uint8_t* ct, const uint8_t* sk,
+                                  unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_NEWHOPE1024_CPA, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[3] = {{msgs, 32, true}, {ct, 2176, false}, {sk, 1792, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, n, d[0], d[1], d[2], s); });
 }
 
 int crypto_kem_keypair(unsigned char* pk, unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_KEYPAIR, pk, sk, nullptr, nullptr); }

@@ -1,0 +1,44 @@
+"""Writes tests/golden/newhope1024_tables.json: the numeric NTT tables the
+reference tabulates for n = 1024 in NewHope_NIST_GPU/precomp.cu
+(omegas_inv_bitrev_montgomery, gammas_bitrev_montgomery,
+gammas_inv_montgomery; the __device__ arrays at precomp.cu:191-371) and the
+10-bit bit-reversal table of NewHope_NIST_GPU/ntt.cu:46.
+
+Data only: it pins the oracle's closed-form table generator against the
+reference's own tables.  Run from the repository root where the reference is
+mounted:  python tests/golden/make_newhope_tables.py /root/reference
+"""
+import json
+import re
+import sys
+from pathlib import Path
+
+
+def extract(src, name, n, device=True):
+    pat = r"%s\s+%s\s*\[[^\]]*\]\s*=\s*\{([^}]*)\}" % ("__device__\\s+uint16_t" if device else "uint16_t", re.escape(name))
+    m = re.search(pat, src)
+    if not m:
+        raise SystemExit("table %s not found" % name)
+    vals = [int(v) for v in re.findall(r"\d+", m.group(1))]
+    assert len(vals) == n, (name, len(vals))
+    return vals
+
+
+def main():
+    ref = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference") / "NewHope_NIST_GPU"
+    pre = (ref / "precomp.cu").read_text()
+    ntt = (ref / "ntt.cu").read_text()
+    out = {
+        "source": "NewHope_NIST_GPU/precomp.cu:191-371, ntt.cu:46",
+        "omegas_inv_bitrev_montgomery": extract(pre, "omegas_inv_bitrev_montgomery", 512),
+        "gammas_bitrev_montgomery": extract(pre, "gammas_bitrev_montgomery", 1024),
+        "gammas_inv_montgomery": extract(pre, "gammas_inv_montgomery", 1024),
+        "bitrev_table": extract(ntt, "bitrev_table", 1024),
+    }
+    dst = Path(__file__).with_name("newhope1024_tables.json")
+    dst.write_text(json.dumps(out) + "\n")
+    print("wrote", dst)
+
+
+if __name__ == "__main__":
+    main()

@@ -208,3 +208,72 @@ def kyber_batch(op, k, arrays, nthreads=8):
         assert L.oracle_kyber_dec_batch(k, C.c_size_t(n), ptr(ss), ptr(ct), ptr(sk), nthreads) == 0
         return ss
     raise ValueError(op)
+
+
+# ---- NewHope-1024 CPA ------------------------------------------------------------
+NH_PK, NH_SK, NH_CT = 1824, 1792, 2176
+
+
+def _u16(a):
+    return np.ascontiguousarray(a, dtype=np.uint16)
+
+
+def newhope_tables():
+    o = np.zeros(512, np.uint16)
+    g = np.zeros(1024, np.uint16)
+    gi = np.zeros(1024, np.uint16)
+    lib().oracle_newhope_tables(ptr(o), ptr(g), ptr(gi))
+    return o, g, gi
+
+
+def newhope_ntt(a):
+    a = _u16(a).copy()
+    lib().oracle_newhope_ntt(ptr(a))
+    return a
+
+
+def newhope_invntt(a):
+    a = _u16(a).copy()
+    lib().oracle_newhope_invntt(ptr(a))
+    return a
+
+
+def newhope_uniform(seed):
+    a = np.zeros(1024, np.uint16)
+    lib().oracle_newhope_poly_uniform(ptr(a), bytes(seed))
+    return a
+
+
+def newhope_sample(seed, nonce):
+    a = np.zeros(1024, np.uint16)
+    lib().oracle_newhope_poly_sample(ptr(a), bytes(seed), C.c_uint8(nonce))
+    return a
+
+
+def newhope_keypair(seed):
+    pk, sk = buf(NH_PK), buf(NH_SK)
+    assert lib().oracle_newhope_cpa_keypair(pk, sk, bytes(seed)) == 0
+    return bytes(pk), bytes(sk)
+
+
+def newhope_enc(m, pk, coin):
+    c = buf(NH_CT)
+    assert lib().oracle_newhope_cpa_enc(c, bytes(m), bytes(pk), bytes(coin)) == 0
+    return bytes(c)
+
+
+def newhope_dec(c, sk):
+    m = buf(32)
+    assert lib().oracle_newhope_cpa_dec(m, bytes(c), bytes(sk)) == 0
+    return bytes(m)
+
+
+def newhope_triples(seed, msg, coin, nthreads=8):
+    n = seed.shape[0]
+    pk = np.empty((n, NH_PK), np.uint8)
+    sk = np.empty((n, NH_SK), np.uint8)
+    ct = np.empty((n, NH_CT), np.uint8)
+    m2 = np.empty((n, 32), np.uint8)
+    lib().oracle_newhope_triple_batch(C.c_size_t(n), ptr(pk), ptr(sk), ptr(ct), ptr(m2), ptr(seed), ptr(msg),
+                                      ptr(coin), nthreads)
+    return pk, sk, ct, m2

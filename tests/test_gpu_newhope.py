@@ -1,0 +1,72 @@
+"""GPU parity of the NewHope-1024 CPA batched kernels against the CPU oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import pqc_amd
+    c = pqc_amd.Context(["newhope1024cpa"], max_batch=16384)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n", [1, 63, 130])
+def test_triples_vs_oracle(ctx, n):
+    say("newhope triples n", n)
+    seed = O.drbg_array(n, 32, 0, 700 + n)
+    msg = O.drbg_array(n, 32, 0, 701 + n)
+    coin = O.drbg_array(n, 32, 0, 702 + n)
+    pk, sk, ct, m2 = ctx.newhope_triples_host(seed, msg, coin)
+    opk, osk, oct_, om2 = O.newhope_triples(seed, msg, coin)
+    assert (pk == opk).all()
+    assert (sk == osk).all()
+    assert (ct == oct_).all()
+    assert (m2 == om2).all() and (m2 == msg).all()
+
+
+def test_decrypt_garbage_ciphertexts_vs_oracle(ctx):
+    say("newhope garbage ct")
+    n = 64
+    seed = O.drbg_array(n, 32, 0, 800)
+    rng = np.random.default_rng(3)
+    pk, sk, _, _ = O.newhope_triples(seed, O.drbg_array(n, 32, 0, 801), O.drbg_array(n, 32, 0, 802))
+    ct = rng.integers(0, 256, (n, 2176), dtype=np.uint8)
+    m = np.empty((n, 32), np.uint8)
+    ctx.newhope_dec(n, m, ct, sk, host=True)
+    for i in range(n):
+        assert m[i].tobytes() == O.newhope_dec(ct[i].tobytes(), sk[i].tobytes())
+
+
+def test_full_size_16384_roundtrip_and_sample(ctx):
+    import pqc_amd
+    n = 16384
+    say("newhope full size", n)
+    rng = np.random.default_rng(11)
+    seed = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    msg = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    coin = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    d_seed, d_msg, d_coin = (pqc_amd.DeviceBuffer.from_array(x) for x in (seed, msg, coin))
+    d_pk, d_sk, d_ct, d_m2 = (pqc_amd.DeviceBuffer(n * b) for b in (1824, 1792, 2176, 32))
+    ctx.newhope_keypair(n, d_pk, d_sk, d_seed)
+    ctx.newhope_enc(n, d_ct, d_msg, d_pk, d_coin)
+    ctx.newhope_dec(n, d_m2, d_ct, d_sk)
+    m2 = d_m2.download((n, 32))
+    assert (m2 == msg).all()
+    idx = rng.choice(n, 24, replace=False)
+    ct = d_ct.download((n, 2176))
+    pk = d_pk.download((n, 1824))
+    opk, osk, oct_, _ = O.newhope_triples(np.ascontiguousarray(seed[idx]), np.ascontiguousarray(msg[idx]),
+                                         np.ascontiguousarray(coin[idx]))
+    assert (pk[idx] == opk).all() and (ct[idx] == oct_).all()
