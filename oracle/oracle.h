@@ -90,6 +90,22 @@ int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk);
 int oracle_newhope_triple_batch(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
                                 const uint8_t *seed, const uint8_t *msg, const uint8_t *coin, int nthreads);
 
+/* ---- FrodoKEM-976-SHAKE (FrodoKEM_GPU) --------------------------------------
+ * pk 15632, sk 31296, ct 15744, ss 24; n = 976, nbar = 8, q = 2^16. */
+void oracle_frodo_sample_n(uint16_t *s, size_t n);
+void oracle_frodo_pack(uint8_t *out, size_t outlen, const uint16_t *in, size_t inlen, unsigned lsb);
+void oracle_frodo_unpack(uint16_t *out, size_t outlen, const uint8_t *in, size_t inlen, unsigned lsb);
+void oracle_frodo_gen_a(uint16_t *A, const uint8_t seedA[16]);
+void oracle_frodo_mul_add_sa_plus_e(uint16_t *out, const uint16_t *A, const uint16_t *S, const uint16_t *E);
+int oracle_frodo_keypair_derand(uint8_t *pk, uint8_t *sk, const uint8_t randomness[64]);
+int oracle_frodo_enc_derand(uint8_t *ct, uint8_t ss[24], const uint8_t *pk, const uint8_t mu[24]);
+int oracle_frodo_enc_derand_a(uint8_t *ct, uint8_t ss[24], const uint8_t *pk, const uint8_t mu[24],
+                              const uint16_t *A);
+int oracle_frodo_dec(uint8_t ss[24], const uint8_t *ct, const uint8_t *sk);
+int oracle_frodo_dec_a(uint8_t ss[24], const uint8_t *ct, const uint8_t *sk, const uint16_t *A);
+int oracle_frodo_enc_batch(size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *pk, const uint8_t *mu, int nthreads);
+int oracle_frodo_dec_batch(size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

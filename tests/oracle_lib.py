@@ -277,3 +277,75 @@ def newhope_triples(seed, msg, coin, nthreads=8):
     lib().oracle_newhope_triple_batch(C.c_size_t(n), ptr(pk), ptr(sk), ptr(ct), ptr(m2), ptr(seed), ptr(msg),
                                       ptr(coin), nthreads)
     return pk, sk, ct, m2
+
+
+# ---- FrodoKEM-976-SHAKE ------------------------------------------------------------
+FR_PK, FR_SK, FR_CT, FR_SS = 15632, 31296, 15744, 24
+
+
+def frodo_sample(raw_u16):
+    s = np.ascontiguousarray(raw_u16, dtype=np.uint16).copy()
+    lib().oracle_frodo_sample_n(ptr(s), C.c_size_t(s.size))
+    return s
+
+
+def frodo_pack(vals, lsb):
+    vals = np.ascontiguousarray(vals, dtype=np.uint16)
+    outlen = (vals.size * lsb + 7) // 8
+    out = buf(outlen)
+    lib().oracle_frodo_pack(out, C.c_size_t(outlen), ptr(vals), C.c_size_t(vals.size), C.c_uint(lsb))
+    return bytes(out)
+
+
+def frodo_unpack(data, n, lsb):
+    out = np.zeros(n, np.uint16)
+    lib().oracle_frodo_unpack(ptr(out), C.c_size_t(n), bytes(data), C.c_size_t(len(data)), C.c_uint(lsb))
+    return out
+
+
+def frodo_gen_a(seedA):
+    A = np.zeros((976, 976), np.uint16)
+    lib().oracle_frodo_gen_a(ptr(A), bytes(seedA))
+    return A
+
+
+def frodo_sa_plus_e(A, S, E):
+    out = np.zeros((8, 976), np.uint16)
+    lib().oracle_frodo_mul_add_sa_plus_e(ptr(out), ptr(np.ascontiguousarray(A, np.uint16)),
+                                         ptr(np.ascontiguousarray(S, np.uint16)), ptr(np.ascontiguousarray(E, np.uint16)))
+    return out
+
+
+def frodo_keypair(randomness64):
+    pk, sk = buf(FR_PK), buf(FR_SK)
+    assert lib().oracle_frodo_keypair_derand(pk, sk, bytes(randomness64)) == 0
+    return bytes(pk), bytes(sk)
+
+
+def frodo_enc(pk, mu):
+    ct, ss = buf(FR_CT), buf(FR_SS)
+    assert lib().oracle_frodo_enc_derand(ct, ss, bytes(pk), bytes(mu)) == 0
+    return bytes(ct), bytes(ss)
+
+
+def frodo_dec(ct, sk):
+    ss = buf(FR_SS)
+    assert lib().oracle_frodo_dec(ss, bytes(ct), bytes(sk)) == 0
+    return bytes(ss)
+
+
+def frodo_enc_batch(pk, mu, nthreads=8):
+    mu = np.ascontiguousarray(mu, np.uint8)
+    n = mu.shape[0]
+    ct = np.empty((n, FR_CT), np.uint8)
+    ss = np.empty((n, FR_SS), np.uint8)
+    assert lib().oracle_frodo_enc_batch(C.c_size_t(n), ptr(ct), ptr(ss), bytes(pk), ptr(mu), nthreads) == 0
+    return ct, ss
+
+
+def frodo_dec_batch(ct, sk, nthreads=8):
+    ct = np.ascontiguousarray(ct, np.uint8)
+    n = ct.shape[0]
+    ss = np.empty((n, FR_SS), np.uint8)
+    assert lib().oracle_frodo_dec_batch(C.c_size_t(n), ptr(ss), ptr(ct), bytes(sk), nthreads) == 0
+    return ss
