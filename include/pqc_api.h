@@ -104,6 +104,24 @@ int pqc_newhope1024_cpa_enc_batch(pqc_ctx *ctx, size_t n, uint8_t *ct, const uin
 int pqc_newhope1024_cpa_dec_batch(pqc_ctx *ctx, size_t n, uint8_t *msgs, const uint8_t *ct,
                                   const uint8_t *sk, unsigned flags, void *stream);
 
+/* ---- FrodoKEM-976-SHAKE: batches under one device-resident key ------------------
+ * Replaces crypto_kem_keypair_gpu/enc/dec (FrodoKEM_GPU/api.h:20-24,
+ * frodo976.cu:87-690).  Element sizes: randomness 64 (s || seedSE || z),
+ * pk 15632, sk 31296, ct 15744, ss 24, mu 24.
+ * pqc_frodo976_load_key expands A once (is_sk = 1: a secret key, needed for
+ * decapsulation; 0: a public key); enc/dec batches then share it, and their
+ * S'.A and S'.B products run as one int8-MFMA GEMM.  pqc_frodo976_products
+ * copies the last batch's B'' (8 x 976 uint16 per element) and V (8 x 8).  */
+int pqc_frodo976_keypair_batch(pqc_ctx *ctx, size_t n, uint8_t *pk, uint8_t *sk,
+                               const uint8_t *randomness, unsigned flags, void *stream);
+int pqc_frodo976_load_key(pqc_ctx *ctx, const uint8_t *key, int is_sk, unsigned flags, void *stream);
+int pqc_frodo976_enc_batch(pqc_ctx *ctx, size_t n, uint8_t *ct, uint8_t *ss, const uint8_t *mu,
+                           unsigned flags, void *stream);
+int pqc_frodo976_dec_batch(pqc_ctx *ctx, size_t n, uint8_t *ss, const uint8_t *ct, unsigned flags,
+                           void *stream);
+int pqc_frodo976_products(pqc_ctx *ctx, size_t n, uint16_t *bpp, uint16_t *v, unsigned flags,
+                          void *stream);
+
 /* ---- randomness for the non-derand entry points ------------------------------- */
 void pqc_randombytes_seed(uint64_t seed); /* deterministic SHAKE256 DRBG */
 void pqc_randombytes_system(void);        /* OS randomness (default) */

@@ -70,6 +70,11 @@ SIGNATURES = {
     "pqc_newhope1024_cpa_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_dec_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
+    "pqc_frodo976_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
+    "pqc_frodo976_load_key": (_I, [_P, _P, _I, _U, _P]),
+    "pqc_frodo976_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
+    "pqc_frodo976_dec_batch": (_I, [_P, _SZ, _P, _P, _U, _P]),
+    "pqc_frodo976_products": (_I, [_P, _SZ, _P, _P, _U, _P]),
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }

@@ -28,6 +28,18 @@ struct pqc_ctx {
   uint16_t* nhpolys = nullptr;
   uint8_t* nhbytes = nullptr;
 
+  // FrodoKEM-976: device-resident key, expanded A (uint16 and MFMA digit
+  // fragments), per-element scratch (S' fragments, E', E'', GEMM output, mu, k,
+  // F sponge states, fail flags)
+  uint16_t* fr_a16 = nullptr;
+  int8_t* fr_frag = nullptr;
+  int8_t* fr_sfrag = nullptr;
+  uint8_t* fr_sk = nullptr;
+  uint8_t* fr_pk = nullptr;
+  uint8_t* fr_pkh = nullptr;
+  uint8_t* fr_elem = nullptr;
+  int fr_loaded = 0, fr_has_sk = 0;
+
   // staging for host-pointer calls
   uint8_t* stage = nullptr;
   size_t stage_bytes = 0;
@@ -63,6 +75,14 @@ int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_
 int newhope_enc(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
                 hipStream_t s);
 int newhope_dec(pqc_ctx* c, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
+
+// FrodoKEM-976 (frodo.hip)
+size_t frodo_bytes_per_elem();
+int frodo_load_key(pqc_ctx* c, const uint8_t* key, int is_sk, hipStream_t s);
+int frodo_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* rnd, hipStream_t s);
+int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, hipStream_t s);
+int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s);
+int frodo_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, hipStream_t s);
 
 // component kernels exposed for tests (kyber_fused.hip)
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);

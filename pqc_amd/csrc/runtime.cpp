@@ -23,6 +23,7 @@ bool alg_info(int alg, AlgInfo* a) {
     case PQC_ALG_KYBER768: *a = {1, 3, 1184, 2400, 1088, 32}; return true;
     case PQC_ALG_KYBER1024: *a = {1, 4, 1568, 3168, 1568, 32}; return true;
     case PQC_ALG_NEWHOPE1024_CPA: *a = {2, 0, 1824, 1792, 2176, 32}; return true;
+    case PQC_ALG_FRODO976: *a = {3, 0, 15632, 31296, 15744, 24}; return true;
     default: return false;
   }
 }
@@ -270,7 +271,8 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
   int rc = check_device();
   if (rc) return rc;
   const unsigned known = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) |
-                         PQC_ALG_MASK(PQC_ALG_KYBER1024) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA);
+                         PQC_ALG_MASK(PQC_ALG_KYBER1024) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA) |
+                         PQC_ALG_MASK(PQC_ALG_FRODO976);
   if (alg_mask == 0 || (alg_mask & ~known)) return PQC_ERR_UNSUPPORTED;
   pqc_ctx* c = new pqc_ctx();
   if (hipGetDevice(&c->device) != hipSuccess) {
@@ -313,6 +315,29 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     if (rc) goto fail;
     c->nhbytes = (uint8_t*)p;
   }
+  if (alg_mask & PQC_ALG_MASK(PQC_ALG_FRODO976)) {
+    rc = ctx_alloc(c, &p, (size_t)976 * 976 * 2);
+    if (rc) goto fail;
+    c->fr_a16 = (uint16_t*)p;
+    rc = ctx_alloc(c, &p, (size_t)2 * 64 * 16 * 64 * 16);
+    if (rc) goto fail;
+    c->fr_frag = (int8_t*)p;
+    rc = ctx_alloc(c, &p, (size_t)8192 * c->ld);
+    if (rc) goto fail;
+    c->fr_sfrag = (int8_t*)p;
+    if (hipMemset(c->fr_sfrag, 0, (size_t)8192 * c->ld) != hipSuccess) {
+      rc = PQC_ERR_DEVICE;
+      goto fail;
+    }
+    rc = ctx_alloc(c, &p, 31296 + 15632 + 64);
+    if (rc) goto fail;
+    c->fr_sk = (uint8_t*)p;
+    c->fr_pk = c->fr_sk + 31296;
+    c->fr_pkh = c->fr_pk + 15632;
+    rc = ctx_alloc(c, &p, pqc::frodo_bytes_per_elem() * c->ld);
+    if (rc) goto fail;
+    c->fr_elem = (uint8_t*)p;
+  }
   rc = ctx_alloc(c, &p, 256);
   if (rc) goto fail;
   c->status = (uint32_t*)p;
@@ -333,6 +358,11 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->kpolys) (void)hipFree(c->kpolys);
   if (c->kbytes) (void)hipFree(c->kbytes);
   if (c->kprf) (void)hipFree(c->kprf);
+  if (c->fr_a16) (void)hipFree(c->fr_a16);
+  if (c->fr_frag) (void)hipFree(c->fr_frag);
+  if (c->fr_sfrag) (void)hipFree(c->fr_sfrag);
+  if (c->fr_sk) (void)hipFree(c->fr_sk);
+  if (c->fr_elem) (void)hipFree(c->fr_elem);
   if (c->nhpolys) (void)hipFree(c->nhpolys);
   if (c->nhbytes) (void)hipFree(c->nhbytes);
   if (c->stage) (void)hipFree(c->stage);
@@ -442,6 +472,50 @@ int pqc_newhope1024_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uin
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{msgs, 32, true}, {ct, 2176, false}, {sk, 1792, false}};
   return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, n, d[0], d[1], d[2], s); });
+}
+
+int pqc_frodo976_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* randomness,
+                               unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_FRODO976, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[3] = {{pk, 15632, true}, {sk, 31296, true}, {randomness, 64, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::frodo_keypair(c, n, d[0], d[1], d[2], s); });
+}
+
+int pqc_frodo976_load_key(pqc_ctx* c, const uint8_t* key, int is_sk, unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_FRODO976, 1);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[1] = {{key, (size_t)(is_sk ? 31296 : 15632), false}};
+  return staged(c, 1, flags, s, b, [&](uint8_t** d) { return pqc::frodo_load_key(c, d[0], is_sk, s); });
+}
+
+int pqc_frodo976_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, unsigned flags,
+                           void* stream) {
+  int rc = check_call(c, PQC_ALG_FRODO976, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[3] = {{ct, 15744, true}, {ss, 24, true}, {mu, 24, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::frodo_enc(c, n, d[0], d[1], d[2], s); });
+}
+
+int pqc_frodo976_dec_batch(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_FRODO976, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[2] = {{ss, 24, true}, {ct, 15744, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::frodo_dec(c, n, d[0], d[1], s); });
+}
+
+int pqc_frodo976_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_FRODO976, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[2] = {{bpp, 976 * 8 * 2, true}, {v, 64 * 2, true}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) {
+    return pqc::frodo_products(c, n, (uint16_t*)d[0], (uint16_t*)d[1], s);
+  });
 }
 
 int crypto_kem_keypair(unsigned char* pk, unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_KEYPAIR, pk, sk, nullptr, nullptr); }

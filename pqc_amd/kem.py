@@ -18,6 +18,7 @@ ALGS = {
     "kyber768": L.ALG_KYBER768,
     "kyber1024": L.ALG_KYBER1024,
     "newhope1024cpa": L.ALG_NEWHOPE1024_CPA,
+    "frodo976": L.ALG_FRODO976,
 }
 KYBER_K = {L.ALG_KYBER512: 2, L.ALG_KYBER768: 3, L.ALG_KYBER1024: 4}
 
@@ -234,6 +235,65 @@ class Context:
         self.newhope_enc(n, ct, msgs, pk, coins, host=True)
         self.newhope_dec(n, m2, ct, sk, host=True)
         return pk, sk, ct, m2
+
+    # ---- FrodoKEM-976: batches under one device-resident key ----------------------
+    def frodo_keypair(self, n, pk, sk, randomness, stream=None, host=False):
+        flags = L.F_HOST_PTRS if host else 0
+        check(L.lib().pqc_frodo976_keypair_batch(self.handle, n, _ptr(pk), _ptr(sk), _ptr(randomness), flags,
+                                                 stream.handle if stream else None), "frodo keypair")
+
+    def frodo_load_key(self, key, is_sk, stream=None, host=False):
+        flags = L.F_HOST_PTRS if host else 0
+        check(L.lib().pqc_frodo976_load_key(self.handle, _ptr(key), 1 if is_sk else 0, flags,
+                                            stream.handle if stream else None), "frodo load_key")
+
+    def frodo_enc(self, n, ct, ss, mu, stream=None, host=False):
+        flags = L.F_HOST_PTRS if host else 0
+        check(L.lib().pqc_frodo976_enc_batch(self.handle, n, _ptr(ct), _ptr(ss), _ptr(mu), flags,
+                                             stream.handle if stream else None), "frodo enc")
+
+    def frodo_dec(self, n, ss, ct, stream=None, host=False):
+        flags = L.F_HOST_PTRS if host else 0
+        check(L.lib().pqc_frodo976_dec_batch(self.handle, n, _ptr(ss), _ptr(ct), flags,
+                                             stream.handle if stream else None), "frodo dec")
+
+    def frodo_products(self, n, bpp=None, v=None, stream=None, host=False):
+        flags = L.F_HOST_PTRS if host else 0
+        check(L.lib().pqc_frodo976_products(self.handle, n, _ptr(bpp), _ptr(v), flags,
+                                            stream.handle if stream else None), "frodo products")
+
+    def frodo_keypair_host(self, randomness):
+        randomness = np.ascontiguousarray(randomness, dtype=np.uint8)
+        n = randomness.shape[0]
+        pk = np.empty((n, 15632), np.uint8)
+        sk = np.empty((n, 31296), np.uint8)
+        self.frodo_keypair(n, pk, sk, randomness, host=True)
+        return pk, sk
+
+    def frodo_enc_host(self, pk, mu):
+        """Encapsulate every row of mu under one public key: (ct, ss, B'', V)."""
+        mu = np.ascontiguousarray(mu, dtype=np.uint8)
+        n = mu.shape[0]
+        self.frodo_load_key(np.frombuffer(bytes(pk), np.uint8).copy(), False, host=True)
+        ct = np.empty((n, 15744), np.uint8)
+        ss = np.empty((n, 24), np.uint8)
+        self.frodo_enc(n, ct, ss, mu, host=True)
+        return ct, ss
+
+    def frodo_dec_host(self, sk, ct):
+        ct = np.ascontiguousarray(ct, dtype=np.uint8)
+        n = ct.shape[0]
+        self.frodo_load_key(np.frombuffer(bytes(sk), np.uint8).copy(), True, host=True)
+        ss = np.empty((n, 24), np.uint8)
+        self.frodo_dec(n, ss, ct, host=True)
+        return ss
+
+    def frodo_products_host(self, n):
+        """B'' (n x 8 x 976) and V (n x 8 x 8) of the last Frodo batch, uint16."""
+        bpp = np.empty((n, 8, 976), np.uint16)
+        v = np.empty((n, 8, 8), np.uint16)
+        self.frodo_products(n, bpp, v, host=True)
+        return bpp, v
 
 
 # ---- NIST single-instance drop-ins (host bytes in, bytes out) -------------------

@@ -1,0 +1,124 @@
+"""GPU parity of the FrodoKEM-976-SHAKE engine (shared-key batches, int8-MFMA
+products) against the CPU oracle, bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+N, NB, CT, SK, PK = 976, 8, 15744, 31296, 15632
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import pqc_amd
+    c = pqc_amd.Context(["frodo976"], max_batch=4096)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def keys():
+    pk, sk = O.frodo_keypair(O.drbg(64, seed=0, stream=900, index=0))
+    return pk, sk
+
+
+def test_keypair_vs_oracle(ctx):
+    say("frodo keypair n=3")
+    rnd = O.drbg_array(3, 64, 0, 901)
+    pk, sk = ctx.frodo_keypair_host(rnd)
+    for i in range(3):
+        opk, osk = O.frodo_keypair(rnd[i].tobytes())
+        assert pk[i].tobytes() == opk
+        assert sk[i].tobytes() == osk
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 130])
+def test_enc_vs_oracle(ctx, keys, n):
+    say("frodo enc n", n)
+    pk, _ = keys
+    mu = O.drbg_array(n, 24, 0, 910 + n)
+    ct, ss = ctx.frodo_enc_host(pk, mu)
+    oct_, oss = O.frodo_enc_batch(pk, mu, nthreads=8)
+    assert (ct == oct_).all()
+    assert (ss == oss).all()
+    # B'' of the batch is the B' the ciphertext carries (big-endian in ct)
+    bpp, v = ctx.frodo_products_host(n)
+    assert (bpp == ct[:, : 2 * N * NB].view(">u2").reshape(n, NB, N)).all()
+
+
+def test_dec_vs_oracle_with_tampering(ctx, keys):
+    n = 64
+    say("frodo dec n", n)
+    pk, sk = keys
+    mu = O.drbg_array(n, 24, 0, 920)
+    ct, ss = O.frodo_enc_batch(pk, mu, nthreads=8)
+    ct = ct.copy()
+    # tamper in B' (early and late), in C, and in the last F block's bytes
+    for i, pos in ((3, 0), (9, 7000), (17, 15615), (30, 15616), (41, 15743), (50, 15650)):
+        ct[i, pos] ^= 0x01
+    got = ctx.frodo_dec_host(sk, ct)
+    want = O.frodo_dec_batch(ct, sk, nthreads=8)
+    assert (got == want).all()
+    tampered = {3, 9, 17, 30, 41, 50}
+    for i in range(n):
+        assert (got[i] == ss[i]).all() == (i not in tampered)
+
+
+def test_dec_random_ciphertexts_vs_oracle(ctx, keys):
+    say("frodo dec random ct")
+    _, sk = keys
+    rng = np.random.default_rng(5)
+    ct = rng.integers(0, 256, (5, CT), dtype=np.uint8)
+    got = ctx.frodo_dec_host(sk, ct)
+    assert (got == O.frodo_dec_batch(ct, sk, nthreads=5)).all()
+
+
+def test_key_switch_and_error_paths(ctx, keys):
+    import pqc_amd
+    say("frodo key switch / errors")
+    pk, sk = keys
+    pk2, sk2 = O.frodo_keypair(O.drbg(64, seed=0, stream=930, index=0))
+    mu = O.drbg_array(4, 24, 0, 931)
+    ct2, ss2 = ctx.frodo_enc_host(pk2, mu)
+    assert (ct2 == O.frodo_enc_batch(pk2, mu, nthreads=4)[0]).all()
+    # a public key alone cannot decapsulate
+    ss = np.empty((4, 24), np.uint8)
+    with pytest.raises(pqc_amd.PQCError):
+        ctx.frodo_dec(4, ss, ct2, host=True)
+    assert (ctx.frodo_dec_host(sk2, ct2) == ss2).all()
+    with pytest.raises(pqc_amd.PQCError):
+        big = np.zeros((4097, 24), np.uint8)
+        ctx.frodo_enc(4097, np.zeros((4097, CT), np.uint8), np.zeros((4097, 24), np.uint8), big, host=True)
+
+
+def test_full_size_4096_roundtrip_and_sample(ctx, keys):
+    import pqc_amd
+    n = 4096
+    say("frodo full size", n)
+    pk, sk = keys
+    rng = np.random.default_rng(17)
+    mu = rng.integers(0, 256, (n, 24), dtype=np.uint8)
+    ct, ss = ctx.frodo_enc_host(pk, mu)
+    bad = ct.copy()
+    bad[::16, 1234] ^= 0x80
+    d_ct = pqc_amd.DeviceBuffer.from_array(bad)
+    d_ss = pqc_amd.DeviceBuffer(n * 24)
+    ctx.frodo_load_key(np.frombuffer(sk, np.uint8).copy(), True, host=True)
+    ctx.frodo_dec(n, d_ss, d_ct)
+    ss2 = d_ss.download((n, 24))
+    same = (ss2 == ss).all(axis=1)
+    assert (~same[::16]).all()
+    assert same[np.arange(n) % 16 != 0].all()
+    idx = np.array([0, 1, 15, 16, 2047, 2048, 4094, 4095])
+    oct_, oss = O.frodo_enc_batch(pk, mu[idx], nthreads=8)
+    assert (ct[idx] == oct_).all() and (ss[idx] == oss).all()
+    assert (ss2[idx] == O.frodo_dec_batch(bad[idx], sk, nthreads=8)).all()
