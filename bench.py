@@ -6,6 +6,11 @@ A step is one decapsulation pass over one batch of 65536 requests (each with
 its own key pair) whose ciphertexts and secret keys are already resident in
 HBM.  The inputs come from seeded coins, expanded by the engine's own key
 generation and encapsulation kernels.  Prints one JSON line.
+
+--workload frodo976 measures FrodoKEM-976 decapsulation of 4096 ciphertexts
+under one device-resident key, with the shared-A int8-MFMA product timed on
+its own; --workload newhope1024 measures 16384 NewHope-1024 CPA
+keygen+encaps+decaps triples.  Both print one JSON line of the same shape.
 """
 import argparse
 import json
@@ -81,11 +86,16 @@ def main():
                     help="time one path only (profiling runs)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--workload", choices=["kyber768", "frodo976", "newhope1024"], default="kyber768")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards by request with no collective, "
               "so multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "frodo976":
+        return bench_frodo(args)
+    if args.workload == "newhope1024":
+        return bench_newhope(args)
 
     import pqc_amd as pqc
     n = args.batch
@@ -187,6 +197,154 @@ def main():
             "single_core_value": round(r_one, 1),
         }
         out["config"]["gpu_over_cpu"] = round(rate_f / r_all, 1)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+# ---- FrodoKEM-976: 4096 decapsulations under one device-resident key -------------
+FRODO_BATCH = 4096
+FRODO_N, FRODO_NBAR = 976, 8
+MFMA_I8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
+
+
+def bench_frodo(args):
+    import pqc_amd as pqc
+    n = FRODO_BATCH if args.batch == BATCH else args.batch
+    ctx = pqc.Context(["frodo976"], max_batch=n)
+    stream = pqc.Stream()
+    rng = np.random.default_rng(0)
+    rnd = rng.integers(0, 256, (1, 64), dtype=np.uint8)
+    pk, sk = ctx.frodo_keypair_host(rnd)
+    mu = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 24), dtype=np.uint8))
+    ct, ss_enc, ss = pqc.DeviceBuffer(n * 15744), pqc.DeviceBuffer(n * 24), pqc.DeviceBuffer(n * 24)
+    ctx.frodo_load_key(pk[0], False, host=True)
+    ctx.frodo_enc(n, ct, ss_enc, mu, stream=stream)
+    ctx.frodo_load_key(sk[0], True, host=True)
+    stream.synchronize()
+    log("inputs ready: one FrodoKEM-976 key pair, %d ciphertexts in HBM" % n)
+
+    dec = lambda: ctx.frodo_dec(n, ss, ct, stream=stream)
+    wall, per = timed(dec, stream, args.steps, args.warmup, pqc)
+    ms = wall * 1e3 / args.steps
+    log("frodo decaps: %.3f ms/step" % ms)
+    ok = bool((ss.download((n, 24)) == ss_enc.download((n, 24))).all())
+    # the shared-A product alone (its decapsulation form, comparing against ct),
+    # on the same stream and the same batch
+    gemm = lambda: ctx.frodo_gemm(n, ct, stream=stream)
+    _, per_g = timed(gemm, stream, max(args.steps, 20), args.warmup, pqc)
+    gemm_ms = float(np.mean(per_g))
+    rows = FRODO_NBAR * n
+    int8_ops = 2 * 2 * rows * FRODO_N * FRODO_N          # hi and lo digit products, unpadded
+    modq_ops = 2 * rows * FRODO_N * FRODO_N              # S'.A mod 2^16
+    achieved = int8_ops / (gemm_ms / 1e3) / 1e12
+    out = {
+        "metric": "FrodoKEM-976 decaps/s at batch 4096 under one key (shared-A S'.A on int8 MFMA)",
+        "value": round(n / (ms / 1e3), 1),
+        "unit": "decaps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 MFMA (hi/lo digits of uint16), uint16 mod 2^16",
+        "data": "synthetic: one seeded key pair and %d seeded encapsulations made on the GPU" % n,
+        "config": {
+            "workload": "frodo976_decaps_shared_a",
+            "batch": n,
+            "gemm": "(%dx976).(976x976) S'.A plus S'.B, mod 2^16" % rows,
+            "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
+            "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
+            "decaps_equal_encaps": ok,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "k_gemm<decaps> (shared-A product + compare, v_mfma_i32_16x16x64_i8)",
+            "achieved": round(achieved, 2),
+            "peak": MFMA_I8_PEAK_TOPS,
+            "unit": "TOPS (int8)",
+            "frac": round(achieved / MFMA_I8_PEAK_TOPS, 4),
+            "work_per_launch": "%d rows x 976 x 976 x 2 digits x 2 ops = %.2f G int8 ops "
+                               "(%.2f G mod-2^16 ops)" % (rows, int8_ops / 1e9, modq_ops / 1e9),
+            "kernel_ms": round(gemm_ms, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from tests import oracle_lib as O
+        ct_h = ct.download((n, 15744))
+        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        m = min(n, 8 * thr)
+        t0 = time.perf_counter()
+        ss_cpu = O.frodo_dec_batch(np.ascontiguousarray(ct_h[:m]), sk[0].tobytes(), nthreads=thr)
+        dt = time.perf_counter() - t0
+        assert (ss_cpu == ss.download((n, 24))[:m]).all(), "CPU oracle and GPU disagree"
+        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "decaps/s", "cores": thr, "kind": "port",
+                               "sample": "%d FrodoKEM-976 decaps of the same batch on %d threads, sharing A; "
+                                         "outputs checked equal to the GPU's" % (m, thr)}
+        out["config"]["gpu_over_cpu"] = round(out["value"] / (m / dt), 1)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+# ---- NewHope-1024 CPA: 16384 keygen + encaps + decaps triples ----------------------
+NEWHOPE_BATCH = 16384
+
+
+def bench_newhope(args):
+    import pqc_amd as pqc
+    n = NEWHOPE_BATCH if args.batch == BATCH else args.batch
+    ctx = pqc.Context(["newhope1024cpa"], max_batch=n)
+    stream = pqc.Stream()
+    rng = np.random.default_rng(0)
+    seed, msg, coin = (pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8)) for _ in range(3))
+    pk, sk, ct, m2 = pqc.DeviceBuffer(n * 1824), pqc.DeviceBuffer(n * 1792), pqc.DeviceBuffer(n * 2176), \
+        pqc.DeviceBuffer(n * 32)
+
+    def triple():
+        ctx.newhope_keypair(n, pk, sk, seed, stream=stream)
+        ctx.newhope_enc(n, ct, msg, pk, coin, stream=stream)
+        ctx.newhope_dec(n, m2, ct, sk, stream=stream)
+
+    wall, per = timed(triple, stream, args.steps, args.warmup, pqc)
+    ms = wall * 1e3 / args.steps
+    log("newhope triples: %.3f ms/step" % ms)
+    ok = bool((m2.download((n, 32)) == msg.download((n, 32))).all())
+    out = {
+        "metric": "NewHope-1024 CPA keygen+encaps+decaps triples/s at batch 16384 (plain batched kernels)",
+        "value": round(n / (ms / 1e3), 1),
+        "unit": "triples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "uint16",
+        "data": "synthetic: seeded 32-byte seeds, messages and coins",
+        "config": {"workload": "newhope1024_cpa_triples", "batch": n,
+                   "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
+                   "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
+                   "messages_recovered": ok},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from tests import oracle_lib as O
+        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        m = min(n, 256 * thr)
+        s_h, m_h, c_h = seed.download((n, 32))[:m], msg.download((n, 32))[:m], coin.download((n, 32))[:m]
+        t0 = time.perf_counter()
+        opk, osk, oct_, om2 = O.newhope_triples(s_h, m_h, c_h, nthreads=thr)
+        dt = time.perf_counter() - t0
+        assert (oct_ == ct.download((n, 2176))[:m]).all(), "CPU oracle and GPU disagree"
+        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "triples/s", "cores": thr, "kind": "port",
+                               "sample": "%d NewHope-1024 triples of the same batch on %d threads; "
+                                         "ciphertexts checked equal to the GPU's" % (m, thr)}
+        out["config"]["gpu_over_cpu"] = round(out["value"] / (m / dt), 1)
     print(json.dumps(out), flush=True)
     ctx.close()
 

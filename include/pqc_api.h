@@ -151,10 +151,18 @@ int pqc_event_elapsed_ms(float *ms, void *start, void *stop);
  * 1 SHA3-512 (outlen 64), 2 SHAKE128, 3 SHAKE256.  inlen must be a multiple
  * of 8.  Replaces sha3_256_n / sha3_512_n (fips202.cu:634-717).
  * pqc_kyber_poly_transform: n polynomials of 256 int16 in place; op 0 =
- * poly_ntt (ntt + Barrett), 1 = poly_invntt.                                  */
+ * poly_ntt (ntt + Barrett), 1 = poly_invntt.
+ * pqc_frodo976_gemm: re-runs only the shared-A product
+ * [B'' | V] = S'.[A | B] + [E' | E''] of the last Frodo-976 batch of n >= this n
+ * elements (frodo_mul_add_sa_plus_e_gpu + frodo_mul_add_sb_plus_e_kernel,
+ * frodo_macrify_reference.cu:215-234,261-281), asynchronously on the stream.
+ * With ct (device pointer, the last decapsulation batch's ciphertexts) it runs
+ * the decapsulation form that compares against ct; with ct = NULL the form
+ * that writes B'' and V to the context (read them with pqc_frodo976_products). */
 int pqc_keccak_batch(int op, size_t n, uint8_t *out, size_t outlen, const uint8_t *in, size_t inlen,
                      void *stream);
 int pqc_kyber_poly_transform(int op, size_t n, int16_t *polys, void *stream);
+int pqc_frodo976_gemm(pqc_ctx *ctx, size_t n, const uint8_t *ct, void *stream);
 
 const char *pqc_version(void);
 const char *pqc_strerror(int code);

@@ -44,6 +44,28 @@ __device__ __forceinline__ uint32_t sample(uint32_t r) {
 
 __device__ __forceinline__ uint32_t bswap16(uint32_t v) { return ((v & 0xFF) << 8) | ((v >> 8) & 0xFF); }
 
+// frodo_sample_n on the two 16-bit halves of raw at once, in packed 16-bit
+// arithmetic (v_pk_sub_u16 / v_pk_lshrrev_b16 / v_pk_add_u16), the same
+// (CDF[j] - prnd) >> 15 count and sign fold as the scalar form above.
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+constexpr uint16_t kCdf[10] = {5638, 15915, 23689, 28571, 31116, 32217, 32613, 32731, 32760, 32766};
+
+__device__ __forceinline__ uint32_t sample2(uint32_t raw) {
+  const u16x2 r = __builtin_bit_cast(u16x2, raw);
+  const u16x2 one = {1, 1}, fifteen = {15, 15}, zero = {0, 0};
+  const u16x2 prnd = r >> one, sign = r & one;
+  u16x2 s = zero;
+#pragma unroll
+  for (int j = 0; j < 10; ++j) {
+    const u16x2 c = {kCdf[j], kCdf[j]};
+    s += (u16x2)(c - prnd) >> fifteen;
+  }
+  return __builtin_bit_cast(uint32_t, ((zero - sign) ^ s) + sign);
+}
+
+// the low bytes of two 16-bit samples, i.e. two int8 values in [-10, 10]
+__device__ __forceinline__ uint32_t bytes2(uint32_t s2) { return __builtin_amdgcn_perm(0u, s2, 0x0C0C0200u); }
+
 // ---- key expansion -----------------------------------------------------------
 
 // generateMatA2 (frodo_macrify_reference.cu:164-188): row i =
@@ -126,167 +148,100 @@ __global__ __launch_bounds__(64) void k_w_decode(int n, const uint8_t* ct, const
   }
 }
 
-// ---- G2 + the S'/E'/E'' SHAKE256 stream (one lane per request) ---------------------
-// Lanes [0, n): (seedSE || k) = SHAKE256(pkh || mu, 48); stream
-// SHAKE256(0x96 || seedSE) sampled into S' (int8, MFMA A-operand fragments),
-// E' ([8n][976] uint16) and E'' ([n][64] uint16).
-// Decapsulation adds lanes [npad, npad + n): the first 115 SHAKE256 blocks of
-// F(ct || key), which cover only ciphertext bytes, saved as a sponge state.
-__global__ void k_shake(int n, int npad, int with_f, const uint8_t* pkh, const uint8_t* mu, int8_t* sfrag,
-                        uint16_t* e1, uint16_t* e2, uint8_t* kout, const uint8_t* ct, uint64_t* fstate) {
+// ---- G2 + the S'/E'/E'' SHAKE256 stream (one lane pair per request) ----------------
+// Pairs [0, npad): (seedSE || k) = SHAKE256(pkh || mu, 48), then the stream
+// SHAKE256(0x96 || seedSE, 31360 B) sampled into S' (int8 MFMA operand
+// fragments) and E' | E'' (int8, in the GEMM output order: see eoff) (kem.c:87-98,153-166).
+// Each lane of a pair samples the two 16-bit values of its half of every word.
+// Decapsulation adds pairs [npad, npad + n): the first 115 SHAKE256 blocks of
+// F(ct || key), which hold only ciphertext bytes, saved as a sponge state.
+__device__ __forceinline__ void absorb_ct_pair(uint32_t (&S)[25], const uint8_t* c, int half) {
+#pragma unroll
+  for (int w = 0; w < 25; ++w) S[w] = 0;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(c) + half;
+  for (int blk = 0; blk < (CT + SS) / 136; ++blk) {  // 115 full blocks
+#pragma unroll
+    for (int w = 0; w < kRateShake256; ++w) S[w] ^= p[2 * w];
+    keccakf1600_pair(S, (uint32_t)half);
+    p += 2 * kRateShake256;
+  }
+}
+
+// Byte offset of [E' | E''] entry (m, j) (row m of the 8n stacked rows, column j
+// < 984, E'' at j = 976 + i) in the layout the GEMM epilogue reads: per 16-row
+// m-tile and 64-column block, 64 lanes x 16 bytes; lane (m & 15) + 16 ((j >> 2) & 3)
+// holds columns 16 ni + 4 (lane >> 4) + r at byte 4 ni + r -- the four n-tiles of
+// the block, four consecutive columns each, as the swapped-operand MFMA output has them.
+__device__ __forceinline__ size_t eoff(int m, int j) {
+  const int lane = (m & 15) + 16 * ((j >> 2) & 3);
+  return ((((size_t)(m >> 4) * 16 + (j >> 6)) * 64 + lane) * 4 + ((j >> 4) & 3)) * 4 + (j & 3);
+}
+
+__global__ __launch_bounds__(64) void k_shake(int n, int npad, int with_f, const uint8_t* pkh, const uint8_t* mu,
+                                              int mu_stride, int8_t* sfrag, int8_t* efrag, uint8_t* kout,
+                                              const uint8_t* ct, uint32_t* fstate) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < npad) {
-    const int r = t;
+  const int pr = t >> 1, half = t & 1;
+  uint32_t S[25];
+  if (pr < npad) {
+    const int r = pr;
     if (r >= n) return;
-    uint64_t S[25], g[6];
+#pragma unroll
+    for (int w = 0; w < 25; ++w) S[w] = 0;
+    const uint32_t* ph = reinterpret_cast<const uint32_t*>(pkh);
+    const uint32_t* pm = reinterpret_cast<const uint32_t*>(mu + (size_t)r * mu_stride);
 #pragma unroll
     for (int w = 0; w < 3; ++w) {
-      g[w] = load_u64(pkh + 8 * w);
-      g[3 + w] = load_u64(mu + (size_t)r * 32 + 8 * w);
+      S[w] = ph[2 * w + half];
+      S[3 + w] = pm[2 * w + half];
     }
-    absorb_fixed<kRateShake256, 6>(S, g, 0x1F);
-    const uint64_t s0 = S[0], s1 = S[1], s2 = S[2];
+    S[6] ^= half ? 0u : 0x1Fu;
+    S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;
+    keccakf1600_pair(S, (uint32_t)half);
+    uint32_t* pk = reinterpret_cast<uint32_t*>(kout + (size_t)r * 32);
 #pragma unroll
-    for (int w = 0; w < 3; ++w) store_u64(kout + (size_t)r * 32 + 8 * w, S[3 + w]);
-    keccak_zero(S);
-    S[0] = 0x96 | (s0 << 8);
-    S[1] = (s0 >> 56) | (s1 << 8);
-    S[2] = (s1 >> 56) | (s2 << 8);
-    S[3] = (s2 >> 56) | (0x1Full << 8);
-    S[kRateShake256 - 1] ^= 0x8000000000000000ull;
-    for (int blk = 0; blk * 68 < NSAMP; ++blk) {
-      keccakf1600(S);
+    for (int w = 0; w < 3; ++w) pk[2 * w + half] = S[3 + w];
+    // seedSE as whole 64-bit words in both lanes, shifted behind the 0x96 byte
+    uint64_t sd[3];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      const uint32_t other = pair_swap(S[w]);
+      sd[w] = half ? ((uint64_t)other | ((uint64_t)S[w] << 32)) : ((uint64_t)S[w] | ((uint64_t)other << 32));
+    }
+    const uint64_t init[4] = {0x96 | (sd[0] << 8), (sd[0] >> 56) | (sd[1] << 8), (sd[1] >> 56) | (sd[2] << 8),
+                              (sd[2] >> 56) | (0x1Full << 8)};
+#pragma unroll
+    for (int w = 0; w < 25; ++w) S[w] = w < 4 ? (uint32_t)(init[w < 4 ? w : 0] >> (32 * half)) : 0u;
+    S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;
+    for (int blk = 0; blk * 68 < NSAMP; ++blk) {  // 231 blocks
+      keccakf1600_pair(S, (uint32_t)half);
 #pragma unroll
       for (int w = 0; w < kRateShake256; ++w) {
-        const int g0 = blk * 68 + 4 * w;  // first sample of this group of 4
-        if (g0 >= NSAMP) break;
-        const uint64_t raw = S[w];
-        uint32_t s4[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s4[q] = sample((uint32_t)(raw >> (16 * q)));
-        if (g0 < N * NB) {
-          const int k = g0 / N, j = g0 % N, m = 8 * r + k;
+        if (blk * 68 + 4 * w >= NSAMP) break;
+        const int g = blk * 68 + 4 * w + 2 * half;  // this lane's first sample
+        const uint16_t b2 = (uint16_t)bytes2(sample2(S[w]));
+        if (g < N * NB) {
+          const int k = g / N, j = g % N, m = 8 * r + k;
           const int lane = (m & 15) + 16 * ((j >> 4) & 3);
           const size_t off = ((((size_t)(m >> 4) * KSTEPS + (j >> 6)) * 64 + lane) * 16) + (j & 15);
-          const uint32_t packed = (s4[0] & 0xFF) | ((s4[1] & 0xFF) << 8) | ((s4[2] & 0xFF) << 16) | ((s4[3] & 0xFF) << 24);
-          *reinterpret_cast<uint32_t*>(sfrag + off) = packed;
+          *reinterpret_cast<uint16_t*>(sfrag + off) = b2;
+        } else if (g < 2 * N * NB) {
+          const int g1 = g - N * NB;
+          *reinterpret_cast<uint16_t*>(efrag + eoff(8 * r + g1 / N, g1 % N)) = b2;
         } else {
-          const uint64_t v = (uint64_t)s4[0] | ((uint64_t)s4[1] << 16) | ((uint64_t)s4[2] << 32) | ((uint64_t)s4[3] << 48);
-          if (g0 < 2 * N * NB) {
-            const int g1 = g0 - N * NB, k = g1 / N, j = g1 % N;
-            *reinterpret_cast<uint64_t*>(e1 + ((size_t)(8 * r + k) * N + j)) = v;
-          } else {
-            *reinterpret_cast<uint64_t*>(e2 + ((size_t)r * 64 + (g0 - 2 * N * NB))) = v;
-          }
+          const int g2 = g - 2 * N * NB;
+          *reinterpret_cast<uint16_t*>(efrag + eoff(8 * r + (g2 >> 3), N + (g2 & 7))) = b2;
         }
       }
     }
   } else if (with_f) {
-    const int r = t - npad;
+    const int r = pr - npad;
     if (r >= n) return;
-    uint64_t S[25];
-    keccak_zero(S);
-    const uint64_t* p = reinterpret_cast<const uint64_t*>(ct + (size_t)r * CT);
-    for (int blk = 0; blk < (CT + SS) / 136; ++blk) {  // 115 full blocks of ciphertext bytes
+    absorb_ct_pair(S, ct + (size_t)r * CT, half);
 #pragma unroll
-      for (int w = 0; w < kRateShake256; ++w) S[w] ^= p[w];
-      keccakf1600(S);
-      p += kRateShake256;
-    }
-#pragma unroll
-    for (int w = 0; w < 25; ++w) fstate[(size_t)r * 25 + w] = S[w];
+    for (int w = 0; w < 25; ++w) fstate[(size_t)r * 50 + 2 * w + half] = S[w];
   }
 }
-
-// ---- the shared-A GEMM on int8 MFMA ----------------------------------------------
-// Workgroup = 4 waves: 256 rows (16 m-tiles, 64 per wave) x 64 columns (4 n-tiles),
-// both digits.  Per 64-deep k-step the 8 B-operand fragments go through LDS
-// (double-buffered, each wave loads 2), the A-operand fragments straight to
-// registers.  Workgroups are ordered so that the 16 column blocks of one row
-// block run on the same XCD and share its S' rows in that XCD's L2.
-constexpr int GEMM_NB = NT / 4;  // 16 column blocks
-
-__global__ __launch_bounds__(256) void k_gemm(int mtiles, int mblocks, int nrows, const v4i* __restrict__ sfrag,
-                                              const v4i* __restrict__ bfrag, const uint16_t* __restrict__ e1,
-                                              const uint16_t* __restrict__ e2, uint16_t* __restrict__ out) {
-  __shared__ v4i lds[2][8][64];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // XCD-aware order: dispatch slot s on XCD x -> row block 8 (s / 16) + x, column block s % 16
-  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int mb = 8 * (slot / GEMM_NB) + xcd, nb = slot % GEMM_NB;
-  if (mb >= mblocks) return;
-  const int mt0 = mb * 16 + wave * 4;
-  v4i acc[4][4][2];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int d = 0; d < 2; ++d) acc[a][b][d] = v4i{0, 0, 0, 0};
-
-  // this wave's two B fragments: f = 2 wave + q -> (ni = f >> 1, d = f & 1)
-  auto bidx = [&](int f, int ks) {
-    const int ni = f >> 1, d = f & 1;
-    return (((size_t)(d * NT + nb * 4 + ni)) * KSTEPS + ks) * 64 + lane;
-  };
-  auto aidx = [&](int mi, int ks) { return (((size_t)(mt0 + mi)) * KSTEPS + ks) * 64 + lane; };
-  const v4i zero = {0, 0, 0, 0};
-
-  v4i bn[2], ac[4], an[4];
-#pragma unroll
-  for (int q = 0; q < 2; ++q) bn[q] = bfrag[bidx(2 * wave + q, 0)];
-#pragma unroll
-  for (int mi = 0; mi < 4; ++mi) ac[mi] = (mt0 + mi < mtiles) ? sfrag[aidx(mi, 0)] : zero;
-#pragma unroll
-  for (int q = 0; q < 2; ++q) lds[0][2 * wave + q][lane] = bn[q];
-  __syncthreads();
-
-  for (int ks = 0; ks < KSTEPS; ++ks) {
-    const int cur = ks & 1;
-    if (ks + 1 < KSTEPS) {
-#pragma unroll
-      for (int q = 0; q < 2; ++q) bn[q] = bfrag[bidx(2 * wave + q, ks + 1)];
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi) an[mi] = (mt0 + mi < mtiles) ? sfrag[aidx(mi, ks + 1)] : zero;
-    }
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-#pragma unroll
-      for (int d = 0; d < 2; ++d) {
-        const v4i b = lds[cur][2 * ni + d][lane];
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi) acc[mi][ni][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ac[mi], b, acc[mi][ni][d], 0, 0, 0);
-      }
-    }
-    if (ks + 1 < KSTEPS) {
-#pragma unroll
-      for (int q = 0; q < 2; ++q) lds[cur ^ 1][2 * wave + q][lane] = bn[q];
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi) ac[mi] = an[mi];
-    }
-    __syncthreads();
-  }
-
-  // epilogue: out = lo + 256 hi + E (mod 2^16); lane holds rows 4 (l >> 4) + r, column l & 15
-#pragma unroll
-  for (int mi = 0; mi < 4; ++mi) {
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-      const int col = 16 * (nb * 4 + ni) + (lane & 15);
-      if (col >= N + NB) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * (mt0 + mi) + 4 * (lane >> 4) + r;
-        if (row >= nrows) continue;
-        uint32_t v = (uint32_t)acc[mi][ni][0][r] + ((uint32_t)acc[mi][ni][1][r] << 8);
-        v += (col < N) ? e1[(size_t)row * N + col] : e2[(size_t)row * NB + (col - N)];
-        out[(size_t)row * OUTC + col] = (uint16_t)v;
-      }
-    }
-  }
-}
-
-// ---- decapsulation: compare, then F's last block --------------------------------
 
 // key_encode of mu (frodo_key_encode_kernel) for coefficient (k, i)
 __device__ __forceinline__ uint32_t encode_mu(const uint8_t* mu, int k, int i) {
@@ -294,33 +249,228 @@ __device__ __forceinline__ uint32_t encode_mu(const uint8_t* mu, int k, int i) {
   return ((t >> (3 * i)) & 7) << 13;
 }
 
-// One wave per request: fail = (B'' != B') or (V + encode(mu') != C), kem.c:191-201
-// without the branch.
-__global__ __launch_bounds__(64) void k_compare(int n, const uint8_t* ct, const uint16_t* out, const uint8_t* mu,
-                                                uint32_t* fail) {
-  const int r = blockIdx.x;
-  if (r >= n) return;
-  const int l = threadIdx.x;
-  const uint8_t* c = ct + (size_t)r * CT;
-  uint32_t mism = 0;
-  for (int idx = l; idx < N * NB; idx += 64) {
-    const int k = idx / N, j = idx % N;
-    const uint32_t bpp = out[(size_t)(8 * r + k) * OUTC + j];
-    mism |= bpp ^ bswap16(*reinterpret_cast<const uint16_t*>(c + 2 * idx));
-  }
-  {
-    const int k = l >> 3, i = l & 7;
-    const uint32_t v = out[(size_t)(8 * r + k) * OUTC + N + i];
-    const uint32_t cc = (v + encode_mu(mu + (size_t)r * 32, k, i)) & 0xFFFF;
-    mism |= cc ^ bswap16(*reinterpret_cast<const uint16_t*>(c + C1 + 2 * (8 * k + i)));
-  }
-  const uint32_t f = __any(mism != 0) ? 1u : 0u;
-  if (l == 0) fail[r] = f;
+// ---- the shared-A GEMM on int8 MFMA ----------------------------------------------
+// Workgroup = 4 waves, tile = 256 rows (16 m-tiles, 4 per wave) x 64 columns
+// (4 n-tiles) x 2 digits.  Each 64-deep k-step's 16 S' and 8 [A | B] fragments
+// go global -> LDS by global_load_lds_dwordx4 (each wave moves 4 + 2) into a
+// 3-slot ring, two steps ahead of the MFMAs; every slot is its own __shared__
+// object so the compiler's LDS-DMA tracking never makes an LDS read wait for an
+// unrelated slot.  Per step: this wave's DMAs of the step done (explicit vmcnt),
+// LDS reads of the previous step returned (lgkmcnt 0), one s_barrier.
+//
+// The MFMA takes [A | B] as its first operand, so the output is the transposed
+// tile: lane l holds row 16 mt + (l & 15), columns 16 nt + 4 (l >> 4) + r,
+// i.e. four consecutive uint16 of a ciphertext row -- the epilogue reads and
+// writes 8-byte words.  Its loads (E' | E'' 16 bytes per m-tile, and for
+// decapsulation the ciphertext words) are issued at step 13, right after the
+// last DMA, and land during the last MFMA steps.
+//
+// Epilogue per mode:
+//   products: [B'' | V] (uint16) to the output buffer;
+//   encaps:   B'' big-endian into ct, C = V + encode(mu) likewise
+//             (frodo_pack, util.cu:18-64);
+//   decaps:   compare with ct's B' and C (kem.c:191-201) and store one flag
+//             byte per (request, column block) -- no atomics, no branch on data.
+//
+// Workgroups are ordered so that the 16 column blocks of one row block run on
+// one XCD and share its S' rows in that XCD's L2.  S' and E' are allocated in
+// whole 256-row blocks, so no load needs a bounds check; rows past 8n are never
+// stored.  N-tiles 62 and 63 (columns 992..1023) hold only padding: skipped.
+constexpr int GEMM_NB = NT / 4;  // 16 column blocks
+constexpr int GEMM_NTILES = (N + NB + 15) / 16;  // 62 n-tiles carry data
+enum { kProducts = 0, kEncaps = 1, kDecaps = 2 };
+
+__shared__ v4i g_sa0[16][64], g_sa1[16][64], g_sa2[16][64];  // S' fragments of one k-step
+__shared__ v4i g_sb0[8][64], g_sb1[8][64], g_sb2[8][64];     // [A | B] fragments of one k-step
+
+template <int ST>
+__device__ __forceinline__ v4i (*ring_a())[64] {
+  if constexpr (ST == 0) return g_sa0;
+  else if constexpr (ST == 1) return g_sa1;
+  else return g_sa2;
+}
+template <int ST>
+__device__ __forceinline__ v4i (*ring_b())[64] {
+  if constexpr (ST == 0) return g_sb0;
+  else if constexpr (ST == 1) return g_sb1;
+  else return g_sb2;
 }
 
-// One lane per request: absorb ct[15640:15744] || key (k' or s by mask) into
-// the saved sponge state, pad, permute: ss = F(ct || key) (kem.c:195-202).
-__global__ void k_f_final(int n, uint8_t* ss, const uint8_t* ct, const uint64_t* fstate, const uint32_t* fail,
+struct GemmArgs {
+  const v4i* sfrag;
+  const v4i* bfrag;
+  int mt0, nb, wave, lane;
+};
+
+template <int KS>
+__device__ __forceinline__ void gemm_issue(const GemmArgs& g) {
+#if defined(__HIP_DEVICE_COMPILE__)  // the LDS-DMA builtin exists only in the device pass
+  constexpr int st = KS % 3;
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+    __builtin_amdgcn_global_load_lds(g.sfrag + ((size_t)(g.mt0 + mi) * KSTEPS + KS) * 64 + g.lane,
+                                     &ring_a<st>()[g.wave * 4 + mi][0], 16, 0, 0);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int f = 2 * g.wave + q, ni = f >> 1, d = f & 1;
+    __builtin_amdgcn_global_load_lds(g.bfrag + ((size_t)(d * NT + g.nb * 4 + ni) * KSTEPS + KS) * 64 + g.lane,
+                                     &ring_b<st>()[f][0], 16, 0, 0);
+  }
+#endif
+}
+
+// s_waitcnt immediates: vmcnt(v), expcnt(7), lgkmcnt(0)
+template <int V>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+  static_assert(V < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | 0x0070);
+}
+
+constexpr int DMA_PER_STEP = 6;  // global_load_lds issued per wave per k-step
+
+template <int MODE>
+struct EpiRegs {
+  uint4 e[4];                              // E' | E'' bytes of the 4 m-tiles
+  uint2 c[MODE == kDecaps ? 4 : 1][4];     // ciphertext words (decapsulation)
+  static constexpr int kLoads = MODE == kDecaps ? 20 : 4;
+};
+
+template <int MODE>
+__device__ __forceinline__ uint8_t* ct_word(uint8_t* ct, int row, int n0) {
+  const int req = row >> 3, k = row & 7;
+  uint8_t* c = ct + (size_t)req * CT;
+  if (n0 < N) return c + 2 * (k * N + n0);
+  if (n0 < N + NB) return c + C1 + 2 * (8 * k + (n0 - N));
+  return c;  // padding columns: a valid address, never compared or stored
+}
+
+template <int MODE>
+__device__ __forceinline__ void epi_issue(EpiRegs<MODE>& ep, const GemmArgs& g, const int8_t* efrag, uint8_t* ct,
+                                          int nrows) {
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+    ep.e[mi] = *reinterpret_cast<const uint4*>(efrag + (((size_t)(g.mt0 + mi) * 16 + g.nb) * 64 + g.lane) * 16);
+  if constexpr (MODE == kDecaps) {
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const int row = min(16 * (g.mt0 + mi) + (g.lane & 15), nrows - 1);
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        const int n0 = 16 * (4 * g.nb + ni) + 4 * (g.lane >> 4);
+        ep.c[mi][ni] = *reinterpret_cast<const uint2*>(ct_word<MODE>(ct, row, n0));
+      }
+    }
+  }
+}
+
+template <int MODE, int KS>
+__device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g, EpiRegs<MODE>& ep,
+                                          const int8_t* efrag, uint8_t* ct, int nrows) {
+  // this wave's DMAs of step KS done; what may stay in flight: the DMAs of step
+  // KS + 1 (issued before) and, from step 14 on, the epilogue loads (issued at 13)
+  constexpr int E = EpiRegs<MODE>::kLoads;
+  if constexpr (KS <= 13) wait_vm_lgkm0<DMA_PER_STEP>();
+  else if constexpr (KS == 14) wait_vm_lgkm0<DMA_PER_STEP + E>();
+  else wait_vm_lgkm0<E>();
+  asm volatile("s_barrier" ::: "memory");  // a workgroup barrier no memory access moves across
+  if constexpr (KS + 2 < KSTEPS) gemm_issue<KS + 2>(g);
+  if constexpr (KS == 13) {
+    // the explicit vmcnt values above count the epilogue loads as the newest:
+    // keep the scheduler from moving them ahead of this step's DMAs
+    __builtin_amdgcn_sched_barrier(0);
+    epi_issue<MODE>(ep, g, efrag, ct, nrows);
+  }
+  constexpr int st = KS % 3;
+  v4i a[4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi) a[mi] = ring_a<st>()[g.wave * 4 + mi][g.lane];
+#pragma unroll
+  for (int ni = 0; ni < 4; ++ni) {
+    if (4 * g.nb + ni >= GEMM_NTILES) continue;  // uniform: padding n-tiles
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const v4i b = ring_b<st>()[2 * ni + d][g.lane];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+        acc[mi][ni][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b, a[mi], acc[mi][ni][d], 0, 0, 0);
+    }
+  }
+  if constexpr (KS + 1 < KSTEPS) gemm_step<MODE, KS + 1>(acc, g, ep, efrag, ct, nrows);
+}
+
+__device__ __forceinline__ uint32_t bswap16x2(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x02030001u); }
+__device__ __forceinline__ uint32_t sx8(uint32_t w, int i) { return (uint32_t)(((int32_t)(w << (24 - 8 * i))) >> 24); }
+__device__ __forceinline__ uint32_t word_of(const uint4& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* __restrict__ sfrag,
+                                                 const v4i* __restrict__ bfrag, const int8_t* __restrict__ efrag,
+                                                 uint16_t* __restrict__ out, uint8_t* __restrict__ ct,
+                                                 const uint8_t* __restrict__ mu, int mu_stride,
+                                                 uint8_t* __restrict__ flags) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // XCD-aware order: dispatch slot s on XCD x -> row block 8 (s / 16) + x, column block s % 16
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int mb = 8 * (slot / GEMM_NB) + xcd, nb = slot % GEMM_NB;
+  if (mb >= mblocks) return;
+  const GemmArgs g{sfrag, bfrag, mb * 16 + wave * 4, nb, wave, lane};
+  const int nrows = 8 * n;
+  v4i acc[4][4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) acc[a][b][d] = v4i{0, 0, 0, 0};
+  EpiRegs<MODE> ep;
+  gemm_issue<0>(g);
+  gemm_issue<1>(g);
+  gemm_step<MODE, 0>(acc, g, ep, efrag, ct, nrows);
+
+  // epilogue: value = lo + 256 hi + E (mod 2^16); lane holds row 16 mt + (l & 15),
+  // columns n0 .. n0 + 3 with n0 = 16 nt + 4 (l >> 4)
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi) {
+    const int row = 16 * (g.mt0 + mi) + (lane & 15), req = row >> 3, k = row & 7;
+    uint32_t mism = 0;
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int n0 = 16 * (4 * nb + ni) + 4 * (lane >> 4);
+      if (row >= nrows || n0 >= N + NB) continue;
+      const uint32_t ew = word_of(ep.e[mi], ni);
+      uint32_t v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        v[r] = (uint32_t)acc[mi][ni][0][r] + ((uint32_t)acc[mi][ni][1][r] << 8) + sx8(ew, r);
+        if constexpr (MODE != kProducts)
+          if (n0 >= N) v[r] += encode_mu(mu + (size_t)req * mu_stride, k, n0 - N + r);
+      }
+      const uint32_t p0 = (v[0] & 0xFFFF) | (v[1] << 16), p1 = (v[2] & 0xFFFF) | (v[3] << 16);
+      if constexpr (MODE == kProducts) {
+        *reinterpret_cast<uint2*>(out + (size_t)row * OUTC + n0) = make_uint2(p0, p1);
+      } else if constexpr (MODE == kEncaps) {
+        *reinterpret_cast<uint2*>(ct_word<MODE>(ct, row, n0)) = make_uint2(bswap16x2(p0), bswap16x2(p1));
+      } else {
+        mism |= (p0 ^ bswap16x2(ep.c[mi][ni].x)) | (p1 ^ bswap16x2(ep.c[mi][ni].y));
+      }
+    }
+    if constexpr (MODE == kDecaps) {
+      // lanes with (l & 15) < 8 hold request 2 mt, the others 2 mt + 1
+      const uint64_t bal = __ballot(mism != 0);
+      const int req0 = 2 * (g.mt0 + mi);
+      if (lane == 0) {
+        if (req0 < n) flags[(size_t)req0 * GEMM_NB + nb] = (uint8_t)((bal & 0x00FF00FF00FF00FFull) != 0);
+        if (req0 + 1 < n) flags[(size_t)(req0 + 1) * GEMM_NB + nb] = (uint8_t)((bal & 0xFF00FF00FF00FF00ull) != 0);
+      }
+    }
+  }
+}
+
+// ---- decapsulation: F's last block ----------------------------------------------
+// One lane per request: absorb ct[15640:15744] || key (k' or s, by the OR of the
+// request's 16 compare flags, as a mask) into the saved sponge state, pad,
+// permute: ss = F(ct || key) (kem.c:195-202).
+__global__ void k_f_final(int n, uint8_t* ss, const uint8_t* ct, const uint64_t* fstate, const uint8_t* flags,
                           const uint8_t* kp, const uint8_t* s) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -330,7 +480,8 @@ __global__ void k_f_final(int n, uint8_t* ss, const uint8_t* ct, const uint64_t*
   const uint64_t* p = reinterpret_cast<const uint64_t*>(ct + (size_t)r * CT + 115 * 136);
 #pragma unroll
   for (int w = 0; w < 13; ++w) S[w] ^= p[w];
-  const uint64_t mask = 0 - (uint64_t)(fail[r] & 1);
+  const uint4 f = *reinterpret_cast<const uint4*>(flags + (size_t)r * GEMM_NB);
+  const uint64_t mask = 0 - (uint64_t)((f.x | f.y | f.z | f.w) != 0);
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
     const uint64_t a = load_u64(kp + (size_t)r * 32 + 8 * w), b = load_u64(s + 8 * w);
@@ -341,47 +492,24 @@ __global__ void k_f_final(int n, uint8_t* ss, const uint8_t* ct, const uint64_t*
   for (int w = 0; w < 3; ++w) store_u64(ss + (size_t)r * 24 + 8 * w, S[w]);
 }
 
-// ---- encapsulation: pack ct, then F(ct || k) --------------------------------------
-__global__ __launch_bounds__(64) void k_enc_pack(int n, uint8_t* ct, const uint16_t* out, const uint8_t* mu) {
-  const int r = blockIdx.x;
-  if (r >= n) return;
-  const int l = threadIdx.x;
-  uint8_t* c = ct + (size_t)r * CT;
-  for (int idx = l; idx < N * NB; idx += 64) {
-    const int k = idx / N, j = idx % N;
-    *reinterpret_cast<uint16_t*>(c + 2 * idx) = (uint16_t)bswap16(out[(size_t)(8 * r + k) * OUTC + j]);
-  }
-  const int k = l >> 3, i = l & 7;
-  const uint32_t v = (out[(size_t)(8 * r + k) * OUTC + N + i] + encode_mu(mu + (size_t)r * MU, k, i)) & 0xFFFF;
-  *reinterpret_cast<uint16_t*>(c + C1 + 2 * (8 * k + i)) = (uint16_t)bswap16(v);
-}
-
-__global__ void k_f_full(int n, uint8_t* ss, const uint8_t* ct, const uint8_t* kp) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  uint64_t S[25];
-  keccak_zero(S);
-  const uint64_t* p = reinterpret_cast<const uint64_t*>(ct + (size_t)r * CT);
-  for (int blk = 0; blk < 115; ++blk) {
-#pragma unroll
-    for (int w = 0; w < kRateShake256; ++w) S[w] ^= p[w];
-    keccakf1600(S);
-    p += kRateShake256;
-  }
-#pragma unroll
-  for (int w = 0; w < 13; ++w) S[w] ^= p[w];
-#pragma unroll
-  for (int w = 0; w < 3; ++w) S[13 + w] ^= load_u64(kp + (size_t)r * 32 + 8 * w);
-  S[16] ^= 0x1Full | 0x8000000000000000ull;
-  keccakf1600(S);
-  for (int w = 0; w < 3; ++w) store_u64(ss + (size_t)r * 24 + 8 * w, S[w]);
-}
-
-// copy n mu values (24 bytes each) to 32-byte strided scratch
-__global__ void k_stride_mu(int n, uint8_t* dst, const uint8_t* src) {
+// ---- encapsulation: ss = F(ct || k), one lane pair per request ---------------------
+__global__ __launch_bounds__(64) void k_f_full(int n, uint8_t* ss, const uint8_t* ct, const uint8_t* kp) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * MU) return;
-  dst[(size_t)(t / MU) * 32 + t % MU] = src[t];
+  const int r = t >> 1, half = t & 1;
+  if (r >= n) return;
+  uint32_t S[25];
+  absorb_ct_pair(S, ct + (size_t)r * CT, half);
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(ct + (size_t)r * CT + 115 * 136) + half;
+#pragma unroll
+  for (int w = 0; w < 13; ++w) S[w] ^= p[2 * w];
+  const uint32_t* k = reinterpret_cast<const uint32_t*>(kp + (size_t)r * 32) + half;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) S[13 + w] ^= k[2 * w];
+  S[16] ^= half ? 0x80000000u : 0x1Fu;
+  keccakf1600_pair(S, (uint32_t)half);
+  uint32_t* o = reinterpret_cast<uint32_t*>(ss + (size_t)r * 24) + half;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) o[2 * w] = S[w];
 }
 
 // pkh = SHAKE256(pk, 24) for a loaded public key (lane 0)
@@ -466,10 +594,10 @@ __global__ void k_kp_pkh(int r, uint8_t* sk, const uint8_t* pk) {
 using namespace frodo;
 
 size_t frodo_bytes_per_elem() {
-  // E' 15616, E'' 128, GEMM output 16384, mu 32, k 32, F state 200, fail 4
-  // (key generation reuses E'..output for its 31232-byte S||E samples and mu
-  // for seedA); the S' fragments live in their own 8 KB-per-element buffer
-  return 15616 + 128 + 16384 + 32 + 32 + 200 + 4;
+  // 31232: key generation's S || E samples (uint16), or [E' | E''] int8 fragments
+  // 8192 + GEMM output uint16 16384; then mu 32, k 32, F state 200, compare flags 16.
+  // The S' fragments live in their own 8 KB-per-element buffer.
+  return 31232 + 32 + 32 + 200 + 16;
 }
 
 static int frodo_need(pqc_ctx* c) { return (c->fr_frag && c->fr_sfrag) ? PQC_OK : PQC_ERR_UNSUPPORTED; }
@@ -495,43 +623,44 @@ int frodo_load_key(pqc_ctx* c, const uint8_t* key, int is_sk, hipStream_t s) {
 }
 
 struct FrScratch {
-  int8_t* sfrag;
-  uint16_t *e1, *e2, *out;
-  uint8_t *mu, *kp;
-  uint64_t* fstate;
-  uint32_t* fail;
+  int8_t *sfrag, *efrag;
+  uint16_t *out, *se;
+  uint8_t *mu, *kp, *flags;
+  uint32_t* fstate;
 };
 
 static FrScratch fr_scratch(pqc_ctx* c) {
-  const size_t ld = c->ld;
+  const size_t ld = c->ld;  // a multiple of 64, so every region below is 16-byte aligned
   uint8_t* p = c->fr_elem;
   FrScratch f;
-  f.e1 = (uint16_t*)p;
-  p += (size_t)8 * N * 2 * ld;
-  f.e2 = (uint16_t*)p;
-  p += (size_t)128 * ld;
-  f.out = (uint16_t*)p;
-  p += (size_t)8 * OUTC * 2 * ld;
+  f.se = (uint16_t*)p;
+  f.efrag = (int8_t*)p;                              // 8 rows x 1024 bytes per element
+  f.out = (uint16_t*)(f.efrag + (size_t)8192 * ld);  // 8 rows x 1024 uint16 per element
+  p += (size_t)31232 * ld;
   f.mu = p;
   p += (size_t)32 * ld;
   f.kp = p;
   p += (size_t)32 * ld;
-  f.fstate = (uint64_t*)p;
+  f.fstate = (uint32_t*)p;
   p += (size_t)200 * ld;
-  f.fail = (uint32_t*)p;
+  f.flags = p;
   f.sfrag = c->fr_sfrag;
   return f;
 }
 
-static int launch_gemm(pqc_ctx* c, size_t n, const FrScratch& f, hipStream_t s) {
+template <int MODE>
+static int launch_gemm(pqc_ctx* c, size_t n, const FrScratch& f, uint8_t* ct, const uint8_t* mu, int mu_stride,
+                       hipStream_t s) {
   const int mtiles = (int)((8 * n + 15) / 16);
-  const int mblocks = (mtiles + 15) / 16;
+  const int mblocks = (mtiles + 15) / 16;  // whole 256-row blocks fit the S' buffer (ld is a multiple of 64)
   const int mblocks8 = (mblocks + 7) / 8 * 8;
-  hipLaunchKernelGGL(k_gemm, dim3(mblocks8 * GEMM_NB), dim3(256), 0, s, mtiles, mblocks, (int)(8 * n),
+  hipLaunchKernelGGL(k_gemm<MODE>, dim3(mblocks8 * GEMM_NB), dim3(256), 0, s, mblocks, (int)n,
                      reinterpret_cast<const v4i*>(f.sfrag), reinterpret_cast<const v4i*>(c->fr_frag),
-                     (const uint16_t*)f.e1, (const uint16_t*)f.e2, f.out);
+                     (const int8_t*)f.efrag, f.out, ct, mu, mu_stride, f.flags);
   return hip_err(hipGetLastError());
 }
+
+static inline int pairs_pad(int n) { return (n + 31) / 32 * 32; }
 
 int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s) {
   int rc = frodo_need(c);
@@ -539,15 +668,14 @@ int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t 
   if (!c->fr_loaded || !c->fr_has_sk) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
-  const int ni = (int)n, npad = (ni + 63) / 64 * 64;
+  const int ni = (int)n, npad = pairs_pad(ni);
   hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, (const uint8_t*)c->fr_sk, f.mu);
-  hipLaunchKernelGGL(k_shake, dim3((npad + ni + 63) / 64), dim3(64), 0, s, ni, npad, 1, (const uint8_t*)c->fr_pkh,
-                     (const uint8_t*)f.mu, f.sfrag, f.e1, f.e2, f.kp, ct, f.fstate);
-  rc = launch_gemm(c, n, f, s);
+  hipLaunchKernelGGL(k_shake, dim3((2 * (npad + ni) + 63) / 64), dim3(64), 0, s, ni, npad, 1,
+                     (const uint8_t*)c->fr_pkh, (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
+  rc = launch_gemm<kDecaps>(c, n, f, const_cast<uint8_t*>(ct), f.mu, 32, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_compare, dim3(ni), dim3(64), 0, s, ni, ct, (const uint16_t*)f.out, (const uint8_t*)f.mu, f.fail);
   hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, ct, (const uint64_t*)f.fstate,
-                     (const uint32_t*)f.fail, (const uint8_t*)f.kp, (const uint8_t*)c->fr_sk);
+                     (const uint8_t*)f.flags, (const uint8_t*)f.kp, (const uint8_t*)c->fr_sk);
   return hip_err(hipGetLastError());
 }
 
@@ -557,14 +685,13 @@ int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu,
   if (!c->fr_loaded) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
-  const int ni = (int)n, npad = (ni + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_stride_mu, dim3((ni * MU + 255) / 256), dim3(256), 0, s, ni, f.mu, mu);
-  hipLaunchKernelGGL(k_shake, dim3(npad / 64), dim3(64), 0, s, ni, npad, 0, (const uint8_t*)c->fr_pkh,
-                     (const uint8_t*)f.mu, f.sfrag, f.e1, f.e2, f.kp, (const uint8_t*)nullptr, (uint64_t*)nullptr);
-  rc = launch_gemm(c, n, f, s);
+  const int ni = (int)n, npad = pairs_pad(ni);
+  hipLaunchKernelGGL(k_shake, dim3(2 * npad / 64), dim3(64), 0, s, ni, npad, 0, (const uint8_t*)c->fr_pkh, mu, MU,
+                     f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
+  rc = launch_gemm<kEncaps>(c, n, f, ct, mu, MU, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_enc_pack, dim3(ni), dim3(64), 0, s, ni, ct, (const uint16_t*)f.out, mu);
-  hipLaunchKernelGGL(k_f_full, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct, (const uint8_t*)f.kp);
+  hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct,
+                     (const uint8_t*)f.kp);
   return hip_err(hipGetLastError());
 }
 
@@ -572,26 +699,36 @@ int frodo_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t*
   int rc = frodo_need(c);
   if (rc) return rc;
   if (n == 0) return PQC_OK;
-  // per-key S||E samples and seeds go through the E' region of the scratch
   FrScratch f = fr_scratch(c);
-  uint16_t* se = f.e1;
   uint8_t* seedA = f.mu;
   const int ni = (int)n;
-  hipLaunchKernelGGL(k_kp_stream, dim3((ni + 63) / 64), dim3(64), 0, s, ni, rnd, seedA, se);
+  hipLaunchKernelGGL(k_kp_stream, dim3((ni + 63) / 64), dim3(64), 0, s, ni, rnd, seedA, f.se);
   for (int r = 0; r < ni; ++r) {
     hipLaunchKernelGGL(k_gen_a, dim3((N + 63) / 64), dim3(64), 0, s, c->fr_a16, (const uint8_t*)seedA + (size_t)r * 16);
     hipLaunchKernelGGL(k_kp_finish, dim3((N * NB + 255) / 256), dim3(256), 0, s, r, pk, sk, (const uint16_t*)c->fr_a16,
-                       (const uint16_t*)se, (const uint8_t*)seedA, rnd);
+                       (const uint16_t*)f.se, (const uint8_t*)seedA, rnd);
     hipLaunchKernelGGL(k_kp_pkh, dim3(1), dim3(64), 0, s, r, sk, (const uint8_t*)pk);
   }
   c->fr_loaded = 0;  // the A buffer now holds the last generated key's A
   return hip_err(hipGetLastError());
 }
 
+int frodo_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, hipStream_t s) {
+  int rc = frodo_need(c);
+  if (rc) return rc;
+  if (!c->fr_loaded) return PQC_ERR_INVALID_ARG;
+  FrScratch f = fr_scratch(c);
+  if (ct) return launch_gemm<kDecaps>(c, n, f, const_cast<uint8_t*>(ct), f.mu, 32, s);
+  return launch_gemm<kProducts>(c, n, f, nullptr, nullptr, 0, s);
+}
+
 int frodo_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, hipStream_t s) {
   int rc = frodo_need(c);
   if (rc) return rc;
+  if (!c->fr_loaded) return PQC_ERR_INVALID_ARG;
   FrScratch f = fr_scratch(c);
+  rc = launch_gemm<kProducts>(c, n, f, nullptr, nullptr, 0, s);
+  if (rc) return rc;
   if (bpp && hipMemcpy2DAsync(bpp, N * 2, f.out, OUTC * 2, N * 2, 8 * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return PQC_ERR_DEVICE;
   if (v && hipMemcpy2DAsync(v, NB * 2, f.out + N, OUTC * 2, NB * 2, 8 * n, hipMemcpyDeviceToDevice, s) != hipSuccess)

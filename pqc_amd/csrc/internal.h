@@ -83,6 +83,7 @@ int frodo_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t*
 int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, hipStream_t s);
 int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s);
 int frodo_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, hipStream_t s);
+int frodo_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, hipStream_t s);
 
 // component kernels exposed for tests (kyber_fused.hip)
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);

@@ -518,6 +518,13 @@ int pqc_frodo976_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, unsi
   });
 }
 
+int pqc_frodo976_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, void* stream) {
+  int rc = check_call(c, PQC_ALG_FRODO976, n);
+  if (rc || n == 0) return rc;
+  if (ct && !aligned8(ct)) return PQC_ERR_INVALID_ARG;
+  return pqc::frodo_gemm(c, n, ct, (hipStream_t)stream);
+}
+
 int crypto_kem_keypair(unsigned char* pk, unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_KEYPAIR, pk, sk, nullptr, nullptr); }
 int crypto_kem_enc(unsigned char* ct, unsigned char* ss, const unsigned char* pk) { return single(PQC_ALG_KYBER768, OP_ENC, ct, ss, pk, nullptr); }
 int crypto_kem_dec(unsigned char* ss, const unsigned char* ct, const unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_DEC, ss, nullptr, ct, sk); }

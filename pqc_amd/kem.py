@@ -262,6 +262,11 @@ class Context:
         check(L.lib().pqc_frodo976_products(self.handle, n, _ptr(bpp), _ptr(v), flags,
                                             stream.handle if stream else None), "frodo products")
 
+    def frodo_gemm(self, n, ct=None, stream=None):
+        """Re-run only the shared-A product of the last batch (measurement).  With
+        the last decapsulation batch's device ciphertexts, the decapsulation form."""
+        check(L.lib().pqc_frodo976_gemm(self.handle, n, _ptr(ct), stream.handle if stream else None), "frodo gemm")
+
     def frodo_keypair_host(self, randomness):
         randomness = np.ascontiguousarray(randomness, dtype=np.uint8)
         n = randomness.shape[0]
