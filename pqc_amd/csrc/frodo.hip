@@ -53,7 +53,12 @@ constexpr uint16_t kCdf[10] = {5638, 15915, 23689, 28571, 31116, 32217, 32613, 3
 __device__ __forceinline__ uint32_t sample2(uint32_t raw) {
   const u16x2 r = __builtin_bit_cast(u16x2, raw);
   const u16x2 one = {1, 1}, fifteen = {15, 15}, zero = {0, 0};
-  const u16x2 prnd = r >> one, sign = r & one;
+  const u16x2 sign = r & one;
+  // hide prnd's range (< 2^15) from the compiler, which would otherwise turn the
+  // sign-bit count into a compare + select per half
+  uint32_t pr = __builtin_bit_cast(uint32_t, (u16x2)(r >> one));
+  asm volatile("" : "+v"(pr));
+  const u16x2 prnd = __builtin_bit_cast(u16x2, pr);
   u16x2 s = zero;
 #pragma unroll
   for (int j = 0; j < 10; ++j) {

@@ -144,7 +144,8 @@ __device__ __forceinline__ void keccakf1600(uint64_t (&A64)[25]) {
 // instructions per round per lane (vs 180 for a whole state in one lane), so a
 // long sequential sponge (Frodo's 231-block noise stream) finishes ~1.5x sooner.
 __device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm(1,0,3,2)
+  // quad_perm(1,0,3,2); every source lane is valid, so no "old" value is needed
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
 }
 
 template <int N>
