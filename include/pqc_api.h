@@ -77,6 +77,45 @@ int crypto_kem_keypair_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *pk, uint8
 int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *ss,
                          const uint8_t *pk, unsigned flags, void *stream);
 
+/* ---- mixed Kyber-512/768/1024 batches -----------------------------------------
+ * One descriptor per request (the reference has no counterpart: its batches are
+ * one parameter set at a time, indcpa.h:16-26).  Buffers are named by byte
+ * offsets (multiples of 8) into four device arenas whose sizes are given for
+ * bounds checks.  The requests are bucketed by parameter set on the device and
+ * run through the fused persistent kernels; each result lands at its
+ * descriptor's offsets, identical to the single-parameter-set calls.  A
+ * descriptor that fails validation (algorithm not in the context, unaligned or
+ * out-of-arena offset) is skipped and reported by pqc_ctx_take_status.
+ * Device pointers only, asynchronous on the stream, and capturable into a graph
+ * (no allocation, no host synchronisation).                                     */
+typedef struct pqc_kem_req {
+  int32_t alg;        /* PQC_ALG_KYBER512 / 768 / 1024 */
+  uint32_t reserved;  /* 0 */
+  uint64_t key_off;   /* pk (encapsulation) or sk (decapsulation) in the key arena */
+  uint64_t ct_off;    /* ciphertext: written by encapsulation, read by decapsulation */
+  uint64_t ss_off;    /* 32-byte shared secret */
+  uint64_t coins_off; /* 32 coin bytes (encapsulation only) */
+} pqc_kem_req;
+int pqc_kem_enc_mixed(pqc_ctx *ctx, size_t n, const pqc_kem_req *reqs, uint8_t *ct_arena, size_t ct_bytes,
+                      const uint8_t *pk_arena, size_t pk_bytes, uint8_t *ss_arena, size_t ss_bytes,
+                      const uint8_t *coins_arena, size_t coins_bytes, void *stream);
+int pqc_kem_dec_mixed(pqc_ctx *ctx, size_t n, const pqc_kem_req *reqs, const uint8_t *ct_arena, size_t ct_bytes,
+                      const uint8_t *sk_arena, size_t sk_bytes, uint8_t *ss_arena, size_t ss_bytes,
+                      void *stream);
+/* Waits for the stream, then reads and clears the context's device status word:
+ * bit 0 = a matrix sampler hit its block cap, bit 1 = a mixed-batch descriptor
+ * was rejected.  Host-pointer calls check it themselves.                        */
+int pqc_ctx_take_status(pqc_ctx *ctx, void *stream, uint32_t *status);
+
+/* ---- graph capture of device-pointer batch calls --------------------------------
+ * Between begin and end, device-pointer calls on `stream` (non-NULL) are recorded
+ * instead of run; the graph replays them, on the same buffers, with one launch. */
+typedef struct pqc_graph pqc_graph;
+int pqc_graph_capture_begin(void *stream);
+int pqc_graph_capture_end(void *stream, pqc_graph **graph);
+int pqc_graph_launch(pqc_graph *graph, void *stream);
+void pqc_graph_destroy(pqc_graph *graph);
+
 /* ---- NIST drop-ins: one instance, host pointers, synchronous -----------------
  * Unprefixed names are Kyber-768 (Kyber_GPU_Batched/api.h:18-22 with
  * KYBER_K 3).  Each call uses a library-owned context on the current device. */

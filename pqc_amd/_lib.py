@@ -76,6 +76,13 @@ SIGNATURES = {
     "pqc_frodo976_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_frodo976_dec_batch": (_I, [_P, _SZ, _P, _P, _U, _P]),
     "pqc_frodo976_products": (_I, [_P, _SZ, _P, _P, _U, _P]),
+    "pqc_kem_enc_mixed": (_I, [_P, _SZ, _P, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _P]),
+    "pqc_kem_dec_mixed": (_I, [_P, _SZ, _P, _P, _SZ, _P, _SZ, _P, _SZ, _P]),
+    "pqc_ctx_take_status": (_I, [_P, _P, C.POINTER(C.c_uint32)]),
+    "pqc_graph_capture_begin": (_I, [_P]),
+    "pqc_graph_capture_end": (_I, [_P, C.POINTER(_P)]),
+    "pqc_graph_launch": (_I, [_P, _P]),
+    "pqc_graph_destroy": (None, [_P]),
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }

@@ -19,9 +19,13 @@ struct pqc_ctx {
   uint8_t* kbytes = nullptr;
   size_t kbytes_per_elem = 0;
 
-  // fused Kyber path: PRF outputs, (2K+1) x 128 bytes per request
+  // fused Kyber path: PRF outputs, (2K+1) x 128 bytes per request (largest K held)
   uint8_t* kprf = nullptr;
   size_t kprf_per_elem = 0;
+  // mixed Kyber batches: request indices bucketed by K (3 x max_batch) and the
+  // three bucket sizes
+  uint32_t* kmix_order = nullptr;
+  uint32_t* kmix_counts = nullptr;
 
   // NewHope-1024 CPA plain path: 8 SoA uint16 polynomials of 1024 coefficients
   // and 64 bytes per element (seed expansion)
@@ -69,6 +73,11 @@ int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const
                     const uint8_t* coins, hipStream_t s);
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
                     hipStream_t s);
+
+// mixed Kyber-512/768/1024 batch through the fused kernels (kyber_fused.hip)
+int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, uint8_t* ct, size_t ct_bytes,
+                      const uint8_t* key, size_t key_bytes, uint8_t* ss, size_t ss_bytes, const uint8_t* coins,
+                      size_t coins_bytes, hipStream_t s);
 
 // NewHope-1024 CPA plain batched path (newhope.hip)
 int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s);

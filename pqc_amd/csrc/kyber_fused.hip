@@ -217,149 +217,266 @@ __device__ __forceinline__ int sample_entry(int16_t* dst, const uint64_t (&rho)[
   return ctr;
 }
 
-template <int K, bool DEC>
-__global__ __launch_bounds__(64) void k_fused(int n, int ntiles, uint8_t* __restrict__ ss,
-                                              uint8_t* __restrict__ ct, const uint8_t* __restrict__ key,
-                                              const uint8_t* __restrict__ coins, uint8_t* __restrict__ prf,
-                                              uint32_t* __restrict__ status) {
+// Where the requests of a tile live.  DenseSrc: element-contiguous arrays of
+// one parameter set (element r at r * size).  MixedSrc: requests of mixed
+// parameter sets named by descriptors (pqc_kem_req: alg + byte offsets into
+// four arenas), bucketed by K into `order` by k_mixed_classify; index r of the
+// K bucket is request order[(K - 2) * cap + r].
+struct ReqP {
+  uint8_t* ss;
+  uint8_t* ct;
+  const uint8_t* key;
+  const uint8_t* coins;
+  uint8_t* prf;
+};
+
+struct DenseSrc {
+  uint8_t* ss;
+  uint8_t* ct;
+  const uint8_t* key;
+  const uint8_t* coins;
+  uint8_t* prf;
+  template <int K, bool DEC>
+  __device__ __forceinline__ ReqP req(int r) const {
+    using P = Params<K>;
+    return ReqP{ss + (size_t)r * 32, ct + (size_t)r * P::ct_bytes,
+                key + (size_t)r * (DEC ? P::sk_bytes : P::pk_bytes), coins ? coins + (size_t)r * 32 : nullptr,
+                prf + (size_t)r * Tile<K>::PRFB};
+  }
+};
+
+struct MixedSrc {
+  const pqc_kem_req* reqs;
+  const uint32_t* order;
+  int cap;
+  uint8_t* ss;
+  uint8_t* ct;
+  const uint8_t* key;
+  const uint8_t* coins;
+  uint8_t* prf;       // prf_stride bytes per request index
+  size_t prf_stride;  // >= Tile<K>::PRFB of every K the context holds
+  template <int K, bool DEC>
+  __device__ __forceinline__ ReqP req(int r) const {
+    const uint32_t idx = order[(size_t)(K - 2) * cap + r];
+    const pqc_kem_req d = reqs[idx];
+    return ReqP{ss + d.ss_off, ct + d.ct_off, key + d.key_off, coins ? coins + d.coins_off : nullptr,
+                prf + (size_t)idx * prf_stride};
+  }
+};
+
+// One tile of up to 64 requests (indices base .. base + 63 of the source, n of
+// them valid) through the whole FO encapsulation (DEC false) or decapsulation.
+template <int K, bool DEC, class Src>
+__device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<Tile<K>::RING>& sh, const LaneZ& z,
+                                           uint32_t* __restrict__ status) {
   using P = Params<K>;
   using TL = Tile<K>;
-  __shared__ Lds<TL::RING> sh;
+  constexpr int PK_OFF = DEC ? P::sk_cpa_bytes : 0;  // public key inside the key buffer
   const int lane = threadIdx.x;
+  const int r = base + lane;
+  const bool active = lane < TL::T && r < n;
+  const int rc = active ? r : min(base, n - 1);  // clamped request for address math
+  const ReqP me = src.template req<K, DEC>(rc);
+  uint64_t kbar[4], hct[4];
+
+  if constexpr (DEC) {
+    // ---- H(ct), one request per lane
+    {
+      uint64_t A[25];
+      sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) hct[w] = A[w];
+    }
+    // ---- indcpa_dec, wave-cooperative, one request at a time
+    for (int t = 0; t < TL::T; ++t) {
+      if (base + t >= n) break;
+      const ReqP q = src.template req<K, DEC>(base + t);
+      coop_indcpa_dec<K>(q.ct, q.key, z, sh.msg[t]);
+    }
+    __syncthreads();
+  } else {
+    // ---- m = H(seed) ("don't release system RNG output"), H(pk)
+    uint64_t A[25];
+    uint64_t c4[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) c4[w] = load_u64(me.coins + 8 * w);
+    absorb_fixed<kRateSha3_256, 4>(A, c4, 0x06);
+    if (active) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) store_u64(sh.msg[lane] + 8 * w, A[w]);
+    }
+    sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.key), P::pk_bytes / 8, 0x06);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) hct[w] = A[w];  // holds H(pk) until G
+    __syncthreads();
+  }
+
+  // ---- (Kbar, coins) = G(m || H(pk)); PRF(coins, 0..2K) -> scratch
+  {
+    uint64_t m[8], A[25];
+    const uint8_t* msg = sh.msg[lane < TL::T ? lane : 0];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      m[w] = load_u64(msg + 8 * w);
+      m[4 + w] = DEC ? load_u64(me.key + P::hpk_off + 8 * w) : hct[w];
+    }
+    absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
+    uint64_t rc4[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      kbar[w] = A[w];
+      rc4[w] = A[4 + w];
+    }
+    for (int nonce = 0; nonce < 2 * K + 1; ++nonce) {
+      absorb_fixed<kRateShake256, 4>(A, rc4, (uint64_t)nonce | (0x1Full << 8));
+      if (active) {
+#pragma unroll
+        for (int w = 0; w < 16; ++w) store_u64(me.prf + 128 * nonce + 8 * w, A[w]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- A^T rounds + re-encryption
+  int t_done = 0;  // requests whose re-encryption is finished
+  for (int g = 0; g < TL::NR; ++g) {
+    {
+      const int E = 64 * g + lane;
+      const int t = E / TL::KK;
+      const int ij = E % TL::KK;
+      const bool valid = base + t < n;
+      const uint8_t* seed = src.template req<K, DEC>(valid ? base + t : rc).key + PK_OFF + P::polyvec_bytes;
+      uint64_t rho[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) rho[w] = load_u64(seed + 8 * w);
+      const int got = sample_entry(sh.a + (E % TL::RING) * kAStride, rho, ij / K, ij % K, valid);
+      if (valid && got < N) atomicOr(status, 1u);
+    }
+    __syncthreads();
+    // requests all of whose entries are now sampled
+    const int t_end = min(TL::T, (64 * (g + 1)) / TL::KK);
+    for (int t = t_done; t < t_end; ++t) {
+      if (base + t >= n) break;
+      const ReqP q = src.template req<K, DEC>(base + t);
+      const uint32_t mm = coop_reencrypt<K, DEC>(sh.a, t * TL::KK, q.prf, q.key + PK_OFF, sh.msg[t], q.ct, z);
+      if constexpr (DEC) {
+        const uint32_t f = __any(mm != 0) ? 1u : 0u;
+        if (lane == 0) sh.fail[t] = f;
+      }
+    }
+    t_done = t_end;
+    __syncthreads();
+  }
+
+  // ---- KDF
+  {
+    uint64_t m[8], A[25];
+    if constexpr (DEC) {
+      const uint64_t mask = 0 - (uint64_t)(sh.fail[lane < TL::T ? lane : 0] & 1u);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const uint64_t zz = load_u64(me.key + P::z_off + 8 * w);
+        m[w] = kbar[w] ^ (mask & (zz ^ kbar[w]));
+        m[4 + w] = hct[w];
+      }
+    } else {
+      sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        m[w] = kbar[w];
+        m[4 + w] = A[w];
+      }
+    }
+    absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+    if (active) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) store_u64(me.ss + 8 * w, A[w]);
+    }
+  }
+  __syncthreads();
+}
+
+template <int K, bool DEC>
+__global__ __launch_bounds__(64) void k_fused(int n, int ntiles, DenseSrc src, uint32_t* __restrict__ status) {
+  __shared__ Lds<Tile<K>::RING> sh;
   LaneZ z;
   load_lane_z(z, &c_lane_tw);
-  constexpr int KEYB = DEC ? P::sk_bytes : P::pk_bytes;
-  constexpr int PK_OFF = DEC ? P::sk_cpa_bytes : 0;  // public key inside the key buffer
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) fused_tile<K, DEC>(src, tile * 64, n, sh, z, status);
+}
 
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int base = tile * TL::T;
-    const int r = base + lane;
-    const bool active = lane < TL::T && r < n;
-    const int rc = active ? r : min(base, n - 1);  // clamped request for address math
-    const uint8_t* key_r = key + (size_t)rc * KEYB;
-    uint8_t* prf_r = prf + (size_t)rc * TL::PRFB;
-    uint64_t kbar[4], hct[4];
+// ---- mixed parameter sets ------------------------------------------------------
+// Bucket n descriptors by K (Kyber-512/768/1024 -> K 2/3/4) into order[(K-2) * cap
+// + slot], counts[K-2] = bucket sizes.  One wave-aggregated global atomic per
+// bucket per wave; the order inside a bucket is whatever the atomics give (the
+// results do not depend on it).  Descriptors with another algorithm id, an
+// unaligned offset, a buffer outside its arena or a parameter set the context
+// was not created for are skipped and set status bit 1.
+__global__ void k_mixed_zero(uint32_t* counts) {
+  if (threadIdx.x < 3) counts[threadIdx.x] = 0;
+}
 
-    if constexpr (DEC) {
-      // ---- H(ct), one request per lane
-      {
-        uint64_t A[25];
-        sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(ct + (size_t)rc * P::ct_bytes),
-                                           P::ct_bytes / 8, 0x06);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) hct[w] = A[w];
-      }
-      // ---- indcpa_dec, wave-cooperative, one request at a time
-      for (int t = 0; t < TL::T; ++t) {
-        const int req = base + t;
-        if (req >= n) break;
-        coop_indcpa_dec<K>(ct + (size_t)req * P::ct_bytes, key + (size_t)req * KEYB, z, sh.msg[t]);
-      }
-      __syncthreads();
-    } else {
-      // ---- m = H(seed) ("don't release system RNG output"), H(pk)
-      uint64_t A[25];
-      uint64_t c4[4];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) c4[w] = load_u64(coins + (size_t)rc * 32 + 8 * w);
-      absorb_fixed<kRateSha3_256, 4>(A, c4, 0x06);
-      if (active) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) store_u64(sh.msg[lane] + 8 * w, A[w]);
-      }
-      sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(key_r), P::pk_bytes / 8, 0x06);
-#pragma unroll
-      for (int w = 0; w < 4; ++w) hct[w] = A[w];  // holds H(pk) until G
-      __syncthreads();
+__global__ void k_mixed_classify(int n, int cap, int dec, unsigned alg_mask, const pqc_kem_req* __restrict__ reqs,
+                                 uint64_t ct_bytes,
+                                 uint64_t key_bytes, uint64_t ss_bytes, uint64_t coins_bytes,
+                                 uint32_t* __restrict__ order, uint32_t* __restrict__ counts,
+                                 uint32_t* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = 0;
+  if (i < n) {
+    const pqc_kem_req d = reqs[i];
+    const bool known = (d.alg == PQC_ALG_KYBER512 || d.alg == PQC_ALG_KYBER768 || d.alg == PQC_ALG_KYBER1024) &&
+                       (alg_mask & PQC_ALG_MASK(d.alg));
+    const int kk = known ? d.alg + 1 : 0;
+    if (kk) {
+      // per-K sizes: ct, sk, pk
+      const uint64_t ctb = kk == 2 ? 768 : kk == 3 ? 1088 : 1568;
+      const uint64_t keyb = dec ? (kk == 2 ? 1632 : kk == 3 ? 2400 : 3168) : (kk == 2 ? 800 : kk == 3 ? 1184 : 1568);
+      const bool ok = ((d.ct_off | d.key_off | d.ss_off | d.coins_off) & 7) == 0 && d.ct_off + ctb <= ct_bytes &&
+                      d.key_off + keyb <= key_bytes && d.ss_off + 32 <= ss_bytes &&
+                      (dec || d.coins_off + 32 <= coins_bytes);
+      k = ok ? kk : 0;
     }
-
-    // ---- (Kbar, coins) = G(m || H(pk)); PRF(coins, 0..2K) -> scratch
-    {
-      uint64_t m[8], A[25];
-      const uint8_t* msg = sh.msg[lane < TL::T ? lane : 0];
+    if (!k) atomicOr(status, 2u);
+  }
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        m[w] = load_u64(msg + 8 * w);
-        m[4 + w] = DEC ? load_u64(key_r + P::hpk_off + 8 * w) : hct[w];
-      }
-      absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
-      uint64_t rc4[4];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        kbar[w] = A[w];
-        rc4[w] = A[4 + w];
-      }
-      for (int nonce = 0; nonce < 2 * K + 1; ++nonce) {
-        absorb_fixed<kRateShake256, 4>(A, rc4, (uint64_t)nonce | (0x1Full << 8));
-        if (active) {
-#pragma unroll
-          for (int w = 0; w < 16; ++w) store_u64(prf_r + 128 * nonce + 8 * w, A[w]);
-        }
-      }
+  for (int kk = 2; kk <= 4; ++kk) {
+    const uint64_t mask = __ballot(k == kk);
+    if (!mask) continue;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counts[kk - 2], (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (k == kk) {
+      const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+      order[(size_t)(kk - 2) * cap + base + below] = (uint32_t)i;
     }
-    __syncthreads();
+  }
+}
 
-    // ---- A^T rounds + re-encryption
-    int t_done = 0;  // requests whose re-encryption is finished
-    for (int g = 0; g < TL::NR; ++g) {
-      {
-        const int E = 64 * g + lane;
-        const int t = E / TL::KK;
-        const int ij = E % TL::KK;
-        const int req = base + t;
-        const bool valid = req < n;
-        const uint8_t* seed = key + (size_t)(valid ? req : rc) * KEYB + PK_OFF + P::polyvec_bytes;
-        uint64_t rho[4];
-#pragma unroll
-        for (int w = 0; w < 4; ++w) rho[w] = load_u64(seed + 8 * w);
-        const int got = sample_entry(sh.a + (E % TL::RING) * kAStride, rho, ij / K, ij % K, valid);
-        if (valid && got < N) atomicOr(status, 1u);
-      }
-      __syncthreads();
-      // requests all of whose entries are now sampled
-      const int t_end = min(TL::T, (64 * (g + 1)) / TL::KK);
-      for (int t = t_done; t < t_end; ++t) {
-        const int req = base + t;
-        if (req >= n) break;
-        const uint8_t* kr = key + (size_t)req * KEYB;
-        const uint32_t mm = coop_reencrypt<K, DEC>(sh.a, t * TL::KK, prf + (size_t)req * TL::PRFB, kr + PK_OFF,
-                                                   sh.msg[t], ct + (size_t)req * P::ct_bytes, z);
-        if constexpr (DEC) {
-          const uint32_t f = __any(mm != 0) ? 1u : 0u;
-          if (lane == 0) sh.fail[t] = f;
-        }
-      }
-      t_done = t_end;
-      __syncthreads();
-    }
+// Persistent walk over the tiles of the three buckets, K = 4 first (longest
+// tiles first keeps the tail short).  LDS is sized for the largest ring (K = 3).
+union LdsMixed {
+  Lds<Tile<2>::RING> k2;
+  Lds<Tile<3>::RING> k3;
+  Lds<Tile<4>::RING> k4;
+};
 
-    // ---- KDF
-    {
-      uint64_t m[8], A[25];
-      if constexpr (DEC) {
-        const uint64_t mask = 0 - (uint64_t)(sh.fail[lane < TL::T ? lane : 0] & 1u);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          const uint64_t zz = load_u64(key_r + P::z_off + 8 * w);
-          m[w] = kbar[w] ^ (mask & (zz ^ kbar[w]));
-          m[4 + w] = hct[w];
-        }
-      } else {
-        sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(ct + (size_t)rc * P::ct_bytes),
-                                           P::ct_bytes / 8, 0x06);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          m[w] = kbar[w];
-          m[4 + w] = A[w];
-        }
-      }
-      absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
-      if (active) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) store_u64(ss + (size_t)r * 32 + 8 * w, A[w]);
-      }
-    }
-    __syncthreads();
+template <bool DEC>
+__global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t* __restrict__ counts,
+                                                    uint32_t* __restrict__ status) {
+  __shared__ LdsMixed sh;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  const int n2 = (int)counts[0], n3 = (int)counts[1], n4 = (int)counts[2];
+  const int t4 = (n4 + 63) / 64, t3 = (n3 + 63) / 64, t2 = (n2 + 63) / 64;
+  for (int tile = blockIdx.x; tile < t4 + t3 + t2; tile += gridDim.x) {
+    if (tile < t4)
+      fused_tile<4, DEC>(src, tile * 64, n4, sh.k4, z, status);
+    else if (tile < t4 + t3)
+      fused_tile<3, DEC>(src, (tile - t4) * 64, n3, sh.k3, z, status);
+    else
+      fused_tile<2, DEC>(src, (tile - t4 - t3) * 64, n2, sh.k2, z, status);
   }
 }
 
@@ -431,8 +548,8 @@ static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
   int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->fused_waves_per_cu;
   if (grid > ntiles) grid = ntiles;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_fused<K, DEC>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, ss, ct, key, coins, c->kprf,
-                     c->status);
+  const DenseSrc src{ss, ct, key, coins, c->kprf};
+  hipLaunchKernelGGL((k_fused<K, DEC>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, src, c->status);
   return hip_err(hipGetLastError());
 }
 
@@ -457,6 +574,35 @@ int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const
     case 4: return kyber::fused::launch_fused<4, false>(c, n, ss, ct, pk, coins, s);
     default: return PQC_ERR_UNSUPPORTED;
   }
+}
+
+// Mixed Kyber-512/768/1024 batch: classify into buckets, then one persistent
+// launch.  Every launch has a fixed shape (the bucket sizes are read on the
+// device), so the sequence can be captured into a graph and replayed.
+int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, uint8_t* ct, size_t ct_bytes,
+                      const uint8_t* key, size_t key_bytes, uint8_t* ss, size_t ss_bytes, const uint8_t* coins,
+                      size_t coins_bytes, hipStream_t s) {
+  using namespace kyber::fused;
+  if (n == 0) return PQC_OK;
+  if (!c->kmix_order || !c->kprf) return PQC_ERR_UNSUPPORTED;
+  const int cap = (int)c->max_batch;
+  hipLaunchKernelGGL(k_mixed_zero, dim3(1), dim3(64), 0, s, c->kmix_counts);
+  hipLaunchKernelGGL(k_mixed_classify, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, cap, dec,
+                     c->alg_mask, reqs,
+                     (uint64_t)ct_bytes, (uint64_t)key_bytes, (uint64_t)ss_bytes, (uint64_t)coins_bytes,
+                     c->kmix_order, c->kmix_counts, c->status);
+  const MixedSrc src{reqs, c->kmix_order, cap, ss, ct, key, coins, c->kprf, c->kprf_per_elem};
+  const int max_tiles = (int)((n + 63) / 64) + 2;  // buckets round up separately
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->fused_waves_per_cu;
+  if (grid > max_tiles) grid = max_tiles;
+  if (grid < 1) grid = 1;
+  if (dec)
+    hipLaunchKernelGGL(k_fused_mixed<true>, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
+                       c->status);
+  else
+    hipLaunchKernelGGL(k_fused_mixed<false>, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
+                       c->status);
+  return hip_err(hipGetLastError());
 }
 
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s) {

@@ -6,6 +6,7 @@
 #include <sys/random.h>
 
 #include <mutex>
+#include <new>
 
 #include "internal.h"
 
@@ -306,6 +307,10 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     rc = ctx_alloc(c, &p, c->kprf_per_elem * c->ld);
     if (rc) goto fail;
     c->kprf = (uint8_t*)p;
+    rc = ctx_alloc(c, &p, (size_t)3 * c->ld * sizeof(uint32_t) + 256);
+    if (rc) goto fail;
+    c->kmix_order = (uint32_t*)p;
+    c->kmix_counts = (uint32_t*)((uint8_t*)p + (size_t)3 * c->ld * sizeof(uint32_t));
   }
   if (alg_mask & PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA)) {
     rc = ctx_alloc(c, &p, (size_t)8 * 1024 * c->ld * sizeof(uint16_t));
@@ -358,6 +363,7 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->kpolys) (void)hipFree(c->kpolys);
   if (c->kbytes) (void)hipFree(c->kbytes);
   if (c->kprf) (void)hipFree(c->kprf);
+  if (c->kmix_order) (void)hipFree(c->kmix_order);
   if (c->fr_a16) (void)hipFree(c->fr_a16);
   if (c->fr_frag) (void)hipFree(c->fr_frag);
   if (c->fr_sfrag) (void)hipFree(c->fr_sfrag);
@@ -523,6 +529,83 @@ int pqc_frodo976_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, void* stream) {
   if (rc || n == 0) return rc;
   if (ct && !aligned8(ct)) return PQC_ERR_INVALID_ARG;
   return pqc::frodo_gemm(c, n, ct, (hipStream_t)stream);
+}
+
+int pqc_kem_enc_mixed(pqc_ctx* c, size_t n, const pqc_kem_req* reqs, uint8_t* ct, size_t ct_bytes, const uint8_t* pk,
+                      size_t pk_bytes, uint8_t* ss, size_t ss_bytes, const uint8_t* coins, size_t coins_bytes,
+                      void* stream) {
+  if (!c) return PQC_ERR_INVALID_ARG;
+  if (!c->kmix_order) return PQC_ERR_UNSUPPORTED;
+  if (n > c->max_batch) return PQC_ERR_BATCH_TOO_LARGE;
+  if (n == 0) return PQC_OK;
+  if (!reqs || !ct || !pk || !ss || !coins || !aligned8(reqs) || !aligned8(ct) || !aligned8(pk) || !aligned8(ss) ||
+      !aligned8(coins))
+    return PQC_ERR_INVALID_ARG;
+  return pqc::kyber_fused_mixed(c, 0, n, reqs, ct, ct_bytes, pk, pk_bytes, ss, ss_bytes, coins, coins_bytes,
+                                (hipStream_t)stream);
+}
+
+int pqc_kem_dec_mixed(pqc_ctx* c, size_t n, const pqc_kem_req* reqs, const uint8_t* ct, size_t ct_bytes,
+                      const uint8_t* sk, size_t sk_bytes, uint8_t* ss, size_t ss_bytes, void* stream) {
+  if (!c) return PQC_ERR_INVALID_ARG;
+  if (!c->kmix_order) return PQC_ERR_UNSUPPORTED;
+  if (n > c->max_batch) return PQC_ERR_BATCH_TOO_LARGE;
+  if (n == 0) return PQC_OK;
+  if (!reqs || !ct || !sk || !ss || !aligned8(reqs) || !aligned8(ct) || !aligned8(sk) || !aligned8(ss))
+    return PQC_ERR_INVALID_ARG;
+  return pqc::kyber_fused_mixed(c, 1, n, reqs, const_cast<uint8_t*>(ct), ct_bytes, sk, sk_bytes, ss, ss_bytes,
+                                nullptr, 0, (hipStream_t)stream);
+}
+
+int pqc_ctx_take_status(pqc_ctx* c, void* stream, uint32_t* status) {
+  if (!c || !status) return PQC_ERR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t zero = 0;
+  if (hipStreamSynchronize(s) != hipSuccess) return PQC_ERR_DEVICE;
+  if (hipMemcpy(status, c->status, 4, hipMemcpyDeviceToHost) != hipSuccess) return PQC_ERR_DEVICE;
+  if (hipMemcpy(c->status, &zero, 4, hipMemcpyHostToDevice) != hipSuccess) return PQC_ERR_DEVICE;
+  return PQC_OK;
+}
+
+struct pqc_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int pqc_graph_capture_begin(void* stream) {
+  if (!stream) return PQC_ERR_INVALID_ARG;  // the legacy default stream cannot be captured
+  return hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal) == hipSuccess ? PQC_OK
+                                                                                                     : PQC_ERR_DEVICE;
+}
+
+int pqc_graph_capture_end(void* stream, pqc_graph** out) {
+  if (!stream || !out) return PQC_ERR_INVALID_ARG;
+  *out = nullptr;
+  auto* g = new (std::nothrow) pqc_graph;
+  if (!g) return PQC_ERR_NO_MEMORY;
+  if (hipStreamEndCapture((hipStream_t)stream, &g->graph) != hipSuccess || !g->graph) {
+    delete g;
+    return PQC_ERR_DEVICE;
+  }
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    return PQC_ERR_DEVICE;
+  }
+  *out = g;
+  return PQC_OK;
+}
+
+int pqc_graph_launch(pqc_graph* g, void* stream) {
+  if (!g || !g->exec) return PQC_ERR_INVALID_ARG;
+  return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? PQC_OK : PQC_ERR_DEVICE;
+}
+
+void pqc_graph_destroy(pqc_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
 }
 
 int crypto_kem_keypair(unsigned char* pk, unsigned char* sk) { return single(PQC_ALG_KYBER768, OP_KEYPAIR, pk, sk, nullptr, nullptr); }

@@ -35,7 +35,7 @@ def sizes(alg):
 
 
 def _ptr(a):
-    if isinstance(a, DeviceBuffer):
+    if isinstance(a, DeviceBuffer) or hasattr(a, "ptr"):  # a buffer or a slice of one
         return C.c_void_p(a.ptr)
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"]
@@ -121,6 +121,44 @@ class Event:
     def __del__(self):
         try:
             L.lib().pqc_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+# descriptor of one request in a mixed Kyber batch (pqc_kem_req, 40 bytes)
+KEM_REQ_DTYPE = np.dtype([("alg", "<i4"), ("reserved", "<u4"), ("key_off", "<u8"), ("ct_off", "<u8"),
+                          ("ss_off", "<u8"), ("coins_off", "<u8")])
+assert KEM_REQ_DTYPE.itemsize == 40
+
+
+class Graph:
+    """Device-pointer batch calls recorded on a stream and replayed with one launch."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @staticmethod
+    def capture(stream, fn):
+        check(L.lib().pqc_graph_capture_begin(stream.handle), "pqc_graph_capture_begin")
+        try:
+            fn()
+        finally:
+            h = C.c_void_p()
+            rc = L.lib().pqc_graph_capture_end(stream.handle, C.byref(h))
+        check(rc, "pqc_graph_capture_end")
+        return Graph(h.value)
+
+    def launch(self, stream):
+        check(L.lib().pqc_graph_launch(self.handle, stream.handle), "pqc_graph_launch")
+
+    def destroy(self):
+        if self.handle:
+            L.lib().pqc_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
         except Exception:
             pass
 
@@ -235,6 +273,22 @@ class Context:
         self.newhope_enc(n, ct, msgs, pk, coins, host=True)
         self.newhope_dec(n, m2, ct, sk, host=True)
         return pk, sk, ct, m2
+
+    # ---- mixed Kyber-512/768/1024 batches (device arenas + descriptors) ----------
+    def kem_enc_mixed(self, n, reqs, ct, pk, ss, coins, stream=None):
+        check(L.lib().pqc_kem_enc_mixed(self.handle, n, _ptr(reqs), _ptr(ct), ct.nbytes, _ptr(pk), pk.nbytes,
+                                        _ptr(ss), ss.nbytes, _ptr(coins), coins.nbytes,
+                                        stream.handle if stream else None), "pqc_kem_enc_mixed")
+
+    def kem_dec_mixed(self, n, reqs, ct, sk, ss, stream=None):
+        check(L.lib().pqc_kem_dec_mixed(self.handle, n, _ptr(reqs), _ptr(ct), ct.nbytes, _ptr(sk), sk.nbytes,
+                                        _ptr(ss), ss.nbytes, stream.handle if stream else None), "pqc_kem_dec_mixed")
+
+    def take_status(self, stream=None):
+        st = C.c_uint32()
+        check(L.lib().pqc_ctx_take_status(self.handle, stream.handle if stream else None, C.byref(st)),
+              "pqc_ctx_take_status")
+        return st.value
 
     # ---- FrodoKEM-976: batches under one device-resident key ----------------------
     def frodo_keypair(self, n, pk, sk, randomness, stream=None, host=False):
