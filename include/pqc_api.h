@@ -58,6 +58,20 @@ void pqc_ctx_destroy(pqc_ctx *ctx);
 size_t pqc_ctx_device_bytes(const pqc_ctx *ctx);
 int pqc_ctx_set_tuning(pqc_ctx *ctx, int fused_waves_per_cu, int fused_grid);
 
+/* Launch parameters the autotuner chooses (replaces the reference's per-GPU
+ * block-size tables and its ANALYSIS_MODE sweep, indcpa.cu:27-70,499-562).
+ * Every value only changes how the work is spread, never a result.          */
+typedef struct pqc_tuning {
+  int32_t plain_block;        /* threads per workgroup, plain per-op Kyber kernels: 64..1024, multiple of 64 */
+  int32_t newhope_block;      /* same for the NewHope-1024 kernels */
+  int32_t fused_waves_per_cu; /* fused persistent grid = CUs x this: 1..4 (LDS holds 4) */
+  int32_t mixed_order;        /* mixed-batch tile order: 0 = K 4,3,2; 1 = K 2,3,4; 2 = round-robin over K */
+  int32_t frodo_stream_block; /* threads per workgroup of the Frodo noise-stream kernel: 64, 128 or 256 */
+  int32_t reserved[3];        /* 0 */
+} pqc_tuning;
+int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
+int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */
+
 /* Byte sizes of one element: public key, secret key, ciphertext, shared secret. */
 int pqc_alg_sizes(int alg, size_t *pk, size_t *sk, size_t *ct, size_t *ss);
 
@@ -76,6 +90,65 @@ int crypto_kem_keypair_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *pk, uint8
                              unsigned flags, void *stream);
 int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *ss,
                          const uint8_t *pk, unsigned flags, void *stream);
+
+/* ---- autotuner ----------------------------------------------------------------
+ * Times the candidate configurations of one workload family (the knobs that
+ * family uses) with HIP events and keeps every measurement as a sample
+ * (13 features, log ms) in the context.  `run` launches the workload once,
+ * asynchronously, on the given stream and returns 0 or a PQC_ERR code.
+ * use_model = 0: candidates in a fixed order (the configuration in use first)
+ * up to `budget` (0 = all).  use_model = 1: once the context's learned cost
+ * model has been fitted (after the first use_model call), it ranks all
+ * candidates and only the best-ranked `budget` are measured; the model is then
+ * refitted on the GPU with every sample.  The best measured configuration is
+ * installed in the context and returned.                                       */
+#define PQC_TUNE_PLAIN_KYBER 0 /* plain_block */
+#define PQC_TUNE_FUSED_KYBER 1 /* fused_waves_per_cu */
+#define PQC_TUNE_MIXED_KYBER 2 /* fused_waves_per_cu x mixed_order */
+#define PQC_TUNE_FRODO 3       /* frodo_stream_block */
+#define PQC_TUNE_NEWHOPE 4     /* newhope_block */
+typedef int (*pqc_workload_fn)(void *user, void *stream);
+typedef struct pqc_tune_request {
+  int32_t family;    /* PQC_TUNE_* */
+  int32_t k;         /* Kyber K (2, 3, 4), else 0 */
+  int32_t op;        /* 0 key generation, 1 encapsulation, 2 decapsulation */
+  int32_t budget;    /* measurements allowed, 0 = every candidate */
+  uint64_t n;        /* batch size */
+  int32_t use_model; /* 1 = let the learned cost model choose what to measure */
+  int32_t reps;      /* timed runs per measurement (the median counts) */
+} pqc_tune_request;
+typedef struct pqc_tune_result {
+  pqc_tuning best;
+  float best_ms;      /* measured */
+  float predicted_ms; /* the model's prediction for `best` (0 without the model) */
+  int32_t measured;   /* configurations timed */
+  int32_t reserved;
+} pqc_tune_result;
+int pqc_autotune(pqc_ctx *ctx, const pqc_tune_request *req, pqc_workload_fn run, void *user, void *stream,
+                 pqc_tune_result *out);
+/* features of (request, configuration) as the cost model sees them; returns 13 */
+int pqc_tune_features(const pqc_tune_request *req, const pqc_tuning *t, float *out, int count);
+/* copies up to cap samples (13 features each, log ms) and returns the count */
+int pqc_tune_samples(pqc_ctx *ctx, float *x, float *y, int cap);
+
+/* ---- the learned cost model (an MLP trained on the GPU) -------------------------
+ * in_dim -> 32 -> 32 -> 1 with ReLU, fp32, loss = mean squared error.  Forward,
+ * backward and Adam run in the library's own HIP kernels.  Host arrays in/out;
+ * every call synchronises.  pqc_ctx_costmodel returns the context's model
+ * (created on first use, owned by the context).                                */
+typedef struct pqc_costmodel pqc_costmodel;
+int pqc_costmodel_create(pqc_costmodel **m, int in_dim, int max_samples, uint64_t seed);
+void pqc_costmodel_destroy(pqc_costmodel *m);
+int pqc_costmodel_num_params(const pqc_costmodel *m);
+int pqc_costmodel_trained_steps(const pqc_costmodel *m); /* Adam steps taken so far */
+int pqc_costmodel_get_params(const pqc_costmodel *m, float *out, size_t count);
+int pqc_costmodel_set_params(pqc_costmodel *m, const float *in, size_t count);
+int pqc_costmodel_predict(pqc_costmodel *m, int n, const float *x, float *y);
+int pqc_costmodel_grad(pqc_costmodel *m, int n, const float *x, const float *t, float *loss, float *grad,
+                       size_t count);
+int pqc_costmodel_fit(pqc_costmodel *m, int n, const float *x, const float *t, int steps, float lr,
+                      float *final_loss);
+pqc_costmodel *pqc_ctx_costmodel(pqc_ctx *ctx);
 
 /* ---- mixed Kyber-512/768/1024 batches -----------------------------------------
  * One descriptor per request (the reference has no counterpart: its batches are

@@ -83,6 +83,21 @@ SIGNATURES = {
     "pqc_graph_capture_end": (_I, [_P, C.POINTER(_P)]),
     "pqc_graph_launch": (_I, [_P, _P]),
     "pqc_graph_destroy": (None, [_P]),
+    "pqc_ctx_get_tuning": (_I, [_P, _P]),
+    "pqc_ctx_put_tuning": (_I, [_P, _P]),
+    "pqc_autotune": (_I, [_P, _P, _P, _P, _P, _P]),
+    "pqc_tune_features": (_I, [_P, _P, _P, _I]),
+    "pqc_tune_samples": (_I, [_P, _P, _P, _I]),
+    "pqc_costmodel_create": (_I, [C.POINTER(_P), _I, _I, C.c_uint64]),
+    "pqc_costmodel_destroy": (None, [_P]),
+    "pqc_costmodel_num_params": (_I, [_P]),
+    "pqc_costmodel_trained_steps": (_I, [_P]),
+    "pqc_costmodel_get_params": (_I, [_P, _P, _SZ]),
+    "pqc_costmodel_set_params": (_I, [_P, _P, _SZ]),
+    "pqc_costmodel_predict": (_I, [_P, _I, _P, _P]),
+    "pqc_costmodel_grad": (_I, [_P, _I, _P, _P, C.POINTER(C.c_float), _P, _SZ]),
+    "pqc_costmodel_fit": (_I, [_P, _I, _P, _P, _I, C.c_float, C.POINTER(C.c_float)]),
+    "pqc_ctx_costmodel": (_P, [_P]),
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }

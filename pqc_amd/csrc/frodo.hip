@@ -182,7 +182,7 @@ __device__ __forceinline__ size_t eoff(int m, int j) {
   return ((((size_t)(m >> 4) * 16 + (j >> 6)) * 64 + lane) * 4 + ((j >> 4) & 3)) * 4 + (j & 3);
 }
 
-__global__ __launch_bounds__(64) void k_shake(int n, int npad, int with_f, const uint8_t* pkh, const uint8_t* mu,
+__global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, const uint8_t* pkh, const uint8_t* mu,
                                               int mu_stride, int8_t* sfrag, int8_t* efrag, uint8_t* kout,
                                               const uint8_t* ct, uint32_t* fstate) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -675,7 +675,8 @@ int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t 
   FrScratch f = fr_scratch(c);
   const int ni = (int)n, npad = pairs_pad(ni);
   hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, (const uint8_t*)c->fr_sk, f.mu);
-  hipLaunchKernelGGL(k_shake, dim3((2 * (npad + ni) + 63) / 64), dim3(64), 0, s, ni, npad, 1,
+  const int sb = c->tune.frodo_stream_block;
+  hipLaunchKernelGGL(k_shake, dim3((2 * (npad + ni) + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 1,
                      (const uint8_t*)c->fr_pkh, (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
   rc = launch_gemm<kDecaps>(c, n, f, const_cast<uint8_t*>(ct), f.mu, 32, s);
   if (rc) return rc;
@@ -691,7 +692,8 @@ int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu,
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
   const int ni = (int)n, npad = pairs_pad(ni);
-  hipLaunchKernelGGL(k_shake, dim3(2 * npad / 64), dim3(64), 0, s, ni, npad, 0, (const uint8_t*)c->fr_pkh, mu, MU,
+  const int sb = c->tune.frodo_stream_block;
+  hipLaunchKernelGGL(k_shake, dim3((2 * npad + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 0, (const uint8_t*)c->fr_pkh, mu, MU,
                      f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
   rc = launch_gemm<kEncaps>(c, n, f, ct, mu, MU, s);
   if (rc) return rc;

@@ -52,8 +52,9 @@ struct pqc_ctx {
 
   size_t allocated = 0;  // bytes of device memory owned by the context
   int cu_count = 0;
-  int fused_waves_per_cu = 4;
   int fused_grid_override = 0;
+  pqc_tuning tune = {256, 128, 4, 0, 64, {0, 0, 0}};
+  void* tuner = nullptr;  // autotuner samples and the learned cost model (tuner.cpp)
 };
 
 namespace pqc {
@@ -98,6 +99,8 @@ int frodo_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, hipStream_t s);
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);
 int keccak_batch(int op, size_t n, uint8_t* out, size_t outlen, const uint8_t* in, size_t inlen,
                  hipStream_t s);
+
+void pqc_tuner_free(pqc_ctx* c);
 
 size_t kyber_plain_polys(int kmax);
 size_t kyber_plain_bytes_per_elem(int kmax);

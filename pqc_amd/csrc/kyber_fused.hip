@@ -454,8 +454,52 @@ __global__ void k_mixed_classify(int n, int cap, int dec, unsigned alg_mask, con
   }
 }
 
-// Persistent walk over the tiles of the three buckets, K = 4 first (longest
-// tiles first keeps the tail short).  LDS is sized for the largest ring (K = 3).
+// Tile i of a mixed batch -> (K, tile index inside the K bucket).  order 0:
+// K = 4, 3, 2 (longest tiles first keeps the tail short); 1: K = 2, 3, 4;
+// 2: round-robin over the buckets that still have tiles, in the order 4, 3, 2.
+struct TilePick {
+  int k, idx;
+};
+
+__device__ __forceinline__ TilePick pick_tile(int i, int order, int t4, int t3, int t2) {
+  if (order == 0) {
+    if (i < t4) return {4, i};
+    if (i < t4 + t3) return {3, i - t4};
+    return {2, i - t4 - t3};
+  }
+  if (order == 1) {
+    if (i < t2) return {2, i};
+    if (i < t2 + t3) return {3, i - t2};
+    return {4, i - t2 - t3};
+  }
+  // round-robin: phases in which 3, 2, then 1 buckets remain
+  int ks[3] = {4, 3, 2}, ts[3] = {t4, t3, t2};
+  int done = 0;  // tiles per bucket consumed by earlier phases
+  int live = (t4 > 0) + (t3 > 0) + (t2 > 0);
+  while (live > 0) {
+    int m = 1 << 30;  // rounds in this phase: the smallest remaining bucket
+    for (int b = 0; b < 3; ++b)
+      if (ts[b] > done && ts[b] - done < m) m = ts[b] - done;
+    if (m == (1 << 30)) break;
+    if (i < live * m) {
+      const int round = i / live, pos = i % live;
+      int seen = 0;
+      for (int b = 0; b < 3; ++b)
+        if (ts[b] > done) {
+          if (seen == pos) return {ks[b], done + round};
+          ++seen;
+        }
+    }
+    i -= live * m;
+    done += m;
+    live = 0;
+    for (int b = 0; b < 3; ++b) live += ts[b] > done ? 1 : 0;
+  }
+  return {0, 0};  // not reached for i below t4 + t3 + t2
+}
+
+// Persistent walk over the tiles of the three buckets.  LDS is sized for the
+// largest ring (K = 3).
 union LdsMixed {
   Lds<Tile<2>::RING> k2;
   Lds<Tile<3>::RING> k3;
@@ -463,7 +507,7 @@ union LdsMixed {
 };
 
 template <bool DEC>
-__global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t* __restrict__ counts, int order,
                                                     uint32_t* __restrict__ status) {
   __shared__ LdsMixed sh;
   LaneZ z;
@@ -471,12 +515,13 @@ __global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t
   const int n2 = (int)counts[0], n3 = (int)counts[1], n4 = (int)counts[2];
   const int t4 = (n4 + 63) / 64, t3 = (n3 + 63) / 64, t2 = (n2 + 63) / 64;
   for (int tile = blockIdx.x; tile < t4 + t3 + t2; tile += gridDim.x) {
-    if (tile < t4)
-      fused_tile<4, DEC>(src, tile * 64, n4, sh.k4, z, status);
-    else if (tile < t4 + t3)
-      fused_tile<3, DEC>(src, (tile - t4) * 64, n3, sh.k3, z, status);
-    else
-      fused_tile<2, DEC>(src, (tile - t4 - t3) * 64, n2, sh.k2, z, status);
+    const TilePick p = pick_tile(tile, order, t4, t3, t2);
+    if (p.k == 4)
+      fused_tile<4, DEC>(src, p.idx * 64, n4, sh.k4, z, status);
+    else if (p.k == 3)
+      fused_tile<3, DEC>(src, p.idx * 64, n3, sh.k3, z, status);
+    else if (p.k == 2)
+      fused_tile<2, DEC>(src, p.idx * 64, n2, sh.k2, z, status);
   }
 }
 
@@ -545,7 +590,7 @@ static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
                         hipStream_t s) {
   if (n == 0) return PQC_OK;
   const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
-  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->fused_waves_per_cu;
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;
   if (grid > ntiles) grid = ntiles;
   if (grid < 1) grid = 1;
   const DenseSrc src{ss, ct, key, coins, c->kprf};
@@ -593,15 +638,15 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
                      c->kmix_order, c->kmix_counts, c->status);
   const MixedSrc src{reqs, c->kmix_order, cap, ss, ct, key, coins, c->kprf, c->kprf_per_elem};
   const int max_tiles = (int)((n + 63) / 64) + 2;  // buckets round up separately
-  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->fused_waves_per_cu;
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;
   if (grid > max_tiles) grid = max_tiles;
   if (grid < 1) grid = 1;
   if (dec)
     hipLaunchKernelGGL(k_fused_mixed<true>, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
-                       c->status);
+                       c->tune.mixed_order, c->status);
   else
     hipLaunchKernelGGL(k_fused_mixed<false>, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
-                       c->status);
+                       c->tune.mixed_order, c->status);
   return hip_err(hipGetLastError());
 }
 

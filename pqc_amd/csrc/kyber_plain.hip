@@ -15,7 +15,6 @@ namespace pqc {
 namespace kyber {
 namespace plain {
 
-constexpr int kBlock = 256;
 
 __constant__ int16_t c_zetas[128];
 __constant__ int16_t c_zetas_inv[128];
@@ -415,6 +414,7 @@ __global__ void k_sk_tail(int n, uint8_t* sk, const uint8_t* pk, const uint8_t* 
 struct Scratch {
   pqc_ctx* c;
   size_t ps;  // elements per poly = 256 * ld
+  int block;  // threads per workgroup (tuning)
   Pv poly(int i) const { return Pv{c->kpolys + (size_t)i * ps, (int)c->ld}; }
   // field region `off` holds that field for every element; ld is a multiple of
   // 64, so every region start stays 8-byte aligned
@@ -437,14 +437,14 @@ static int ensure_tables(int dev) {
   return PQC_OK;
 }
 
-#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, __VA_ARGS__)
+#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(S.block), 0, s, __VA_ARGS__)
 
 // indcpa_enc (indcpa.cu:578-679) for n elements: m (32 B/elem at msg),
 // pk (stride pk_stride), coins (32 B at coins + x*coins_stride) -> ct.
 template <int K>
 static void indcpa_enc(const Scratch& S, int n, uint8_t* ct, const uint8_t* msg, const uint8_t* pk,
                        size_t pk_stride, const uint8_t* coins, size_t coins_stride, hipStream_t s) {
-  const int grid = (n + kBlock - 1) / kBlock;
+  const int grid = (n + S.block - 1) / S.block;
   const size_t ps = S.ps;
   // poly slots: at[0..K*K), sp, ep, epp, pkpv, kpoly, bp, v
   const int AT = 0, SP = K * K, EP = SP + K, EPP = EP + K, PKPV = EPP + 1, KP = PKPV + K, BP = KP + 1,
@@ -473,8 +473,8 @@ template <int K>
 static int plain_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   using P = Params<K>;
   const int n = (int)n_;
-  const int grid = (n + kBlock - 1) / kBlock;
-  Scratch S{c, (size_t)N * c->ld};
+  Scratch S{c, (size_t)N * c->ld, c->tune.plain_block};
+  const int grid = (n + S.block - 1) / S.block;
   const size_t ps = S.ps;
   // decryption slots live after the encryption slots
   const int base = K * K + 5 * K + 4;
@@ -503,8 +503,8 @@ static int plain_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint
                      hipStream_t s) {
   using P = Params<K>;
   const int n = (int)n_;
-  const int grid = (n + kBlock - 1) / kBlock;
-  Scratch S{c, (size_t)N * c->ld};
+  Scratch S{c, (size_t)N * c->ld, c->tune.plain_block};
+  const int grid = (n + S.block - 1) / S.block;
   // buf64 = m || H(pk); m = H(coins) ("don't release system RNG output")
   LAUNCH(k_hash_h, n, S.bytes(kBuf64), (size_t)64, coins, (size_t)32, 4);
   LAUNCH(k_hash_h, n, S.bytes(kBuf64) + 32, (size_t)64, pk, (size_t)P::pk_bytes, P::pk_bytes / 8);
@@ -523,8 +523,8 @@ template <int K>
 static int plain_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
   using P = Params<K>;
   const int n = (int)n_;
-  const int grid = (n + kBlock - 1) / kBlock;
-  Scratch S{c, (size_t)N * c->ld};
+  Scratch S{c, (size_t)N * c->ld, c->tune.plain_block};
+  const int grid = (n + S.block - 1) / S.block;
   const size_t ps = S.ps;
   const int A = 0, SKPV = K * K, E = SKPV + K, PKPV = E + K;
   // (publicseed, noiseseed) = G(d)

@@ -21,7 +21,6 @@ constexpr int CBYTES = 3 * N / 8;      // 384
 constexpr int PK = POLYBYTES + SYM;    // 1824
 constexpr int SK = POLYBYTES;          // 1792
 constexpr int CT = POLYBYTES + CBYTES; // 2176
-constexpr int kBlock = 128;
 
 // ---- tables: the Pari/GP definitions of precomp.cu:5-72 (n = 1024, g = 7,
 // mont = 2^18), evaluated at compile time -------------------------------------
@@ -352,13 +351,14 @@ struct Scratch {
   Pv poly(int i) const { return Pv{c->nhpolys + (size_t)i * N * c->ld, (int)c->ld}; }
 };
 
-#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, __VA_ARGS__)
+#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__)
 
 }  // namespace newhope
 
 int newhope_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
   using namespace newhope;
-  const int n = (int)n_, grid = (n + kBlock - 1) / kBlock;
+  const int block = c->tune.newhope_block;
+  const int n = (int)n_, grid = (n + block - 1) / block;
   Scratch S{c};
   uint8_t* z = c->nhbytes;  // 64 bytes per element: publicseed || noiseseed
   const int AHAT = 0, EHAT = 1, AS = 2, BHAT = 3, SHAT = 4;
@@ -378,7 +378,8 @@ int newhope_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8
 int newhope_enc(pqc_ctx* c, size_t n_, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
                 hipStream_t s) {
   using namespace newhope;
-  const int n = (int)n_, grid = (n + kBlock - 1) / kBlock;
+  const int block = c->tune.newhope_block;
+  const int n = (int)n_, grid = (n + block - 1) / block;
   Scratch S{c};
   uint8_t* seed = c->nhbytes;  // publicseed copy, 32 bytes per element (stride 64)
   const int SP = 0, EP = 1, VP = 2, AHAT = 3, BHAT = 4, EPP = 5, UHAT = 6, V = 7;
@@ -403,7 +404,8 @@ int newhope_enc(pqc_ctx* c, size_t n_, uint8_t* ct, const uint8_t* m, const uint
 
 int newhope_dec(pqc_ctx* c, size_t n_, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   using namespace newhope;
-  const int n = (int)n_, grid = (n + kBlock - 1) / kBlock;
+  const int block = c->tune.newhope_block;
+  const int n = (int)n_, grid = (n + block - 1) / block;
   Scratch S{c};
   const int VP = 0, UHAT = 1, TMP = 2, SHAT = 3;
   LAUNCH(k_frombytes, n, S.poly(SHAT), sk, (size_t)SK, (uint8_t*)nullptr);

@@ -360,6 +360,7 @@ fail:
 void pqc_ctx_destroy(pqc_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
+  pqc::pqc_tuner_free(c);
   if (c->kpolys) (void)hipFree(c->kpolys);
   if (c->kbytes) (void)hipFree(c->kbytes);
   if (c->kprf) (void)hipFree(c->kprf);
@@ -380,8 +381,25 @@ size_t pqc_ctx_device_bytes(const pqc_ctx* c) { return c ? c->allocated : 0; }
 
 int pqc_ctx_set_tuning(pqc_ctx* c, int waves_per_cu, int grid) {
   if (!c || waves_per_cu < 0 || waves_per_cu > 32 || grid < 0) return PQC_ERR_INVALID_ARG;
-  if (waves_per_cu) c->fused_waves_per_cu = waves_per_cu;
+  if (waves_per_cu) c->tune.fused_waves_per_cu = waves_per_cu;
   c->fused_grid_override = grid;
+  return PQC_OK;
+}
+
+int pqc_ctx_get_tuning(const pqc_ctx* c, pqc_tuning* t) {
+  if (!c || !t) return PQC_ERR_INVALID_ARG;
+  *t = c->tune;
+  return PQC_OK;
+}
+
+int pqc_ctx_put_tuning(pqc_ctx* c, const pqc_tuning* t) {
+  if (!c || !t) return PQC_ERR_INVALID_ARG;
+  auto block_ok = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
+  if (!block_ok(t->plain_block) || !block_ok(t->newhope_block) || t->fused_waves_per_cu < 1 ||
+      t->fused_waves_per_cu > 4 || t->mixed_order < 0 || t->mixed_order > 2 ||
+      (t->frodo_stream_block != 64 && t->frodo_stream_block != 128 && t->frodo_stream_block != 256))
+    return PQC_ERR_INVALID_ARG;
+  c->tune = *t;
   return PQC_OK;
 }
 

@@ -17,7 +17,9 @@ def declared_functions():
             continue
         src = open(os.path.join(ROOT, "include", hdr)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", src, flags=re.M):
+        # function declarations; typedefs (function-pointer types) are not symbols
+        for m in re.finditer(r"^\s*(?!typedef\b)(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", src,
+                             flags=re.M):
             name = m.group(1)
             if name not in ("if", "while", "for", "sizeof", "return"):
                 names.append(name)

@@ -1,0 +1,143 @@
+"""The learned cost model and the autotuner on the GPU.
+
+* gradients of the HIP backward kernel against the float64 CPU restatement
+  (oracle/costmodel_ref.py, itself pinned by finite differences);
+* GPU Adam training against CPU Adam on the same data;
+* the gain: the model, trained on the GPU with measured timings of other batch
+  sizes, picks a launch configuration for an unseen batch size without timing
+  anything; its pick is scored against that batch size's measured table, next to
+  the configuration used without the model (the library default) and a blind
+  pick.  Tuning never changes a result: the test also checks the outputs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import costmodel_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pqc():
+    import pqc_amd
+    return pqc_amd
+
+
+def test_gradients_match_cpu_restatement(pqc):
+    from pqc_amd.tuner import CostModel
+    say("cost model gradients")
+    rng = np.random.default_rng(5)
+    d, n = 13, 300  # three workgroups of the gradient kernel
+    m = CostModel(in_dim=d, max_samples=512, seed=3)
+    p = rng.normal(scale=0.25, size=m.num_params).astype(np.float32)
+    m.set_params(p)
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    t = rng.normal(size=n).astype(np.float32)
+    loss, g = m.grad(x, t)
+    rl, rg = R.grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    # fp32 sums over 300 samples vs float64: relative 1e-3 of the largest entry
+    scale = np.abs(rg).max()
+    assert abs(loss - rl) <= 1e-4 * max(1.0, abs(rl))
+    assert np.abs(g - rg).max() <= 1e-3 * scale, np.abs(g - rg).max() / scale
+    yp = m.predict(x)
+    ry, _ = R.forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.abs(yp - ry).max() <= 1e-4 * max(1.0, np.abs(ry).max())
+    m.close()
+
+
+def test_gpu_adam_matches_cpu_adam(pqc):
+    from pqc_amd.tuner import CostModel
+    say("cost model Adam")
+    rng = np.random.default_rng(6)
+    d, n = 13, 200
+    x = rng.uniform(-1, 1, size=(n, d)).astype(np.float32)
+    t = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2] + 0.3 * x[:, 7]).astype(np.float32)
+    m = CostModel(in_dim=d, max_samples=512, seed=4)
+    p0 = m.get_params()
+    l0 = R.loss(p0, x, t)
+    gpu_loss = m.fit(x, t, steps=300, lr=3e-3)
+    cpu_p = R.adam_fit(p0.astype(np.float64), x.astype(np.float64), t.astype(np.float64), steps=300, lr=3e-3)
+    cpu_loss = R.loss(cpu_p, x, t)
+    say(" loss start %.4f gpu %.5f cpu %.5f" % (l0, gpu_loss, cpu_loss))
+    assert gpu_loss < 0.2 * l0
+    assert abs(gpu_loss - cpu_loss) <= 0.05 * cpu_loss + 1e-4
+    assert abs(R.loss(m.get_params(), x, t) - gpu_loss) <= 1e-3 * max(gpu_loss, 1e-3)
+    m.close()
+
+
+def test_trained_model_beats_untuned_and_blind_choice(pqc):
+    from pqc_amd import tuner as T
+    N = 65536
+    ctx = pqc.Context(["kyber768", "newhope1024cpa"], max_batch=N)
+    st = pqc.Stream()
+    rng = np.random.default_rng(0)
+    coins = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (N, 64), dtype=np.uint8))
+    ec = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (N, 32), dtype=np.uint8))
+    pk, sk = pqc.DeviceBuffer(N * 1184), pqc.DeviceBuffer(N * 2400)
+    ct, ss, ss2 = pqc.DeviceBuffer(N * 1088), pqc.DeviceBuffer(N * 32), pqc.DeviceBuffer(N * 32)
+    ctx.keypair_derand("kyber768", N, pk, sk, coins, stream=st)
+    ctx.enc_derand("kyber768", N, ct, ss, pk, ec, stream=st)
+    st.synchronize()
+    default = T.get_tuning(ctx)
+
+    def plain(n):
+        return lambda s: ctx.dec("kyber768", n, ss2, ct, sk, plain=True, stream=s)
+
+    def fused(n):
+        return lambda s: ctx.dec("kyber768", n, ss2, ct, sk, stream=s)
+
+    def table(fam, n, run):
+        T.put_tuning(ctx, **{k: v for k, v in default.as_dict().items()})
+        before = len(T.samples(ctx)[1])
+        req = T.request(fam, n, k=3, op=2, budget=0, reps=3)
+        T.autotune(ctx, req, run, st)
+        x, y = T.samples(ctx)
+        return req, x[before:], np.exp(y[before:])
+
+    # training tables: other batch sizes, both Kyber paths
+    for n in (1024, 4096, 65536):
+        say("table plain", n)
+        table("plain_kyber", n, plain(n))
+    for n in (1024, 4096, 32768, 65536):
+        say("table fused", n)
+        table("fused_kyber", n, fused(n))
+    x, y = T.samples(ctx)
+    model = T.CostModel.of_context(ctx)
+    loss = model.fit(x, y, steps=3000, lr=3e-3)
+    say("model trained on %d samples, loss %.5f" % (len(y), loss))
+    assert loss < 0.02
+
+    # held-out batch size: the measured table, then the model's pick without timing
+    n = 16384
+    req, fx, ms = table("plain_kyber", n, plain(n))
+    pred = model.predict(fx)
+    best = ms.min()
+    pick = int(np.argmin(pred))
+    # the default configuration is the first row of the table (measured first)
+    regret_model = ms[pick] / best
+    regret_default = ms[0] / best
+    regret_blind = ms.mean() / best
+    say("held-out n=%d: best %.3f ms; model pick %.3f ms (regret %.3f), default %.3f ms (%.3f), blind %.3f" %
+        (n, best, ms[pick], regret_model, ms[0], regret_default, regret_blind))
+    assert regret_model <= 1.06
+    assert regret_model < regret_blind
+    assert regret_model <= regret_default + 0.01
+
+    # end to end: pqc_autotune with the model and a budget of one measurement
+    T.put_tuning(ctx, **{k: v for k, v in default.as_dict().items()})
+    res = T.autotune(ctx, T.request("plain_kyber", n, k=3, op=2, budget=1, use_model=True, reps=3), plain(n), st)
+    say("model-guided, 1 measurement: %s %.3f ms (predicted %.3f)" % (res.best.as_dict(), res.best_ms,
+                                                                        res.predicted_ms))
+    assert res.measured == 1 and res.best_ms <= 1.08 * best
+    # tuning changes no result
+    ctx.dec("kyber768", n, ss2, ct, sk, plain=True, stream=st)
+    st.synchronize()
+    assert (ss2.download((n, 32)) == ss.download((n, 32))).all()
+    ctx.close()
