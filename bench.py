@@ -35,6 +35,9 @@ HBM_PEAK_GBPS = 8000.0
 # WRITE_SIZE runs, each scaled by the same counter's reading of a 256 MiB
 # device copy (profiles/kyber768_decaps_b65536.md): 454.5 MB read + 60.8 MB written.
 MEASURED_TRAFFIC_BYTES_B65536 = 515.3e6
+# HBM bytes per Frodo-976 decapsulation GEMM launch at batch 4096, same method
+# (profiles/frodo976_decaps_b4096.md): 197.7 MB read + 0.07 MB written.
+MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 197.8e6
 
 
 def log(*a):
@@ -111,7 +114,7 @@ def main():
                     help="time one path only (profiling runs)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["kyber768", "frodo976", "newhope1024"], default="kyber768")
+    ap.add_argument("--workload", choices=["kyber768", "frodo976", "newhope1024", "kyber_mixed"], default="kyber768")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards by request with no collective, "
@@ -121,6 +124,8 @@ def main():
         return bench_frodo(args)
     if args.workload == "newhope1024":
         return bench_newhope(args)
+    if args.workload == "kyber_mixed":
+        return bench_mixed(args)
 
     import pqc_amd as pqc
     n = args.batch
@@ -287,7 +292,7 @@ def bench_frodo(args):
             "work_per_launch": "%d rows x 976 x 976 x 2 digits x 2 ops = %.2f G int8 ops "
                                "(%.2f G mod-2^16 ops)" % (rows, int8_ops / 1e9, modq_ops / 1e9),
             "kernel_ms": round(gemm_ms, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_FRODO_GEMM_B4096 if n == FRODO_BATCH else None,
         },
         "cpu_baseline": None,
     }
@@ -364,6 +369,89 @@ def bench_newhope(args):
                                "sample": "%d NewHope-1024 triples of the same batch on %d threads; "
                                          "ciphertexts checked equal to the GPU's" % (m, thr)}
         out["config"]["gpu_over_cpu"] = round(out["value"] / (m / dt), 1)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+# ---- mixed Kyber-512/768/1024 (1:2:1), 65536 encaps + decaps, fused in a graph ---------
+def bench_mixed(args):
+    import pqc_amd as pqc
+    from pqc_amd import tuner as T
+    n = args.batch
+    ctx = pqc.Context(["kyber512", "kyber768", "kyber1024"], max_batch=n)
+    stream = pqc.Stream()
+    base = np.array([1, 2, 2, 3] * ((n + 3) // 4), dtype=np.int32)[:n]
+    algs = np.random.default_rng(0).permutation(base)
+    lay = pqc.MixedLayout(algs, np.random.default_rng(1))
+    ar = pqc.DeviceArenas(lay)
+    rng = np.random.default_rng(2)
+    ar.buf["kcoins"].upload(rng.integers(0, 256, (lay.nbytes["kcoins"],), dtype=np.uint8))
+    ar.buf["coins"].upload(rng.integers(0, 256, (lay.nbytes["coins"],), dtype=np.uint8))
+    for a in (1, 2, 3):
+        ctx.keypair_derand(a, lay.count[a], ar.region_ptr("pk", a), ar.region_ptr("sk", a),
+                           ar.region_ptr("kcoins", a), stream=stream)
+    ss_dec = pqc.DeviceBuffer(lay.nbytes["ss"])
+    stream.synchronize()
+    log("inputs ready: %d requests (Kyber-512/768/1024 = %d/%d/%d), one key pair each" %
+        (n, lay.count[1], lay.count[2], lay.count[3]))
+
+    def enc_dec(s):
+        ctx.kem_enc_mixed(n, ar.enc_reqs, ar.buf["ct"], ar.buf["pk"], ar.buf["ss"], ar.buf["coins"], stream=s)
+        ctx.kem_dec_mixed(n, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ss_dec, stream=s)
+
+    # the autotuner picks grid waves x tile order for this batch
+    tuned = T.autotune(ctx, T.request("mixed_kyber", n, k=3, op=2, budget=0, reps=2), enc_dec, stream)
+    log("tuned: %s %.3f ms" % (tuned.best.as_dict(), tuned.best_ms))
+    graph = pqc.Graph.capture(stream, lambda: enc_dec(stream))
+    wall, per = timed(lambda: graph.launch(stream), stream, args.steps, args.warmup, pqc)
+    ms = wall * 1e3 / args.steps
+    ss_e, ss_d = ar.download("ss"), ss_dec.download((lay.nbytes["ss"],))
+    ct_f = ar.download("ct")
+    ok_fused = bool((ss_e == ss_d).all())
+
+    def plain_step():
+        for a in (1, 2, 3):
+            ctx.enc_derand(a, lay.count[a], ar.region_ptr("ct", a), ar.region_ptr("ss", a), ar.region_ptr("pk", a),
+                           ar.region_ptr("coins", a), plain=True, stream=stream)
+        for a in (1, 2, 3):
+            ctx.dec(a, lay.count[a], ss_region(a), ar.region_ptr("ct", a), ar.region_ptr("sk", a), plain=True,
+                    stream=stream)
+
+    def ss_region(a):
+        from pqc_amd.mixed import _Slice
+        off, cnt = lay.region["ss"][a]
+        return _Slice(ss_dec, off, cnt * 32)
+
+    psteps = max(3, args.steps // 4)
+    wall_p, _ = timed(plain_step, stream, psteps, 1, pqc)
+    ms_p = wall_p * 1e3 / psteps
+    bitexact = bool(ok_fused and (ar.download("ct") == ct_f).all() and (ar.download("ss") == ss_e).all() and
+                    (ss_dec.download((lay.nbytes["ss"],)) == ss_d).all())
+    graph.destroy()
+    out = {
+        "metric": "mixed Kyber-512/768/1024 (1:2:1) encaps+decaps requests/s at batch 64K, fused graph vs plain",
+        "value": round(n / (ms / 1e3), 1),
+        "unit": "requests/s (encaps + decaps each)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic: seeded key pairs (one per request) and coins, made on the GPU",
+        "config": {"workload": "kyber_mixed_enc_dec_graph", "batch": n,
+                   "counts_512_768_1024": [lay.count[1], lay.count[2], lay.count[3]],
+                   "tuning": tuned.best.as_dict(),
+                   "plain_ms_per_step": round(ms_p, 4),
+                   "fused_over_plain": round(ms_p / ms, 3),
+                   "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
+                   "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
+                   "bitexact_fused_plain": bitexact},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
     ctx.close()
 
