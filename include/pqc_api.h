@@ -78,7 +78,11 @@ int pqc_alg_sizes(int alg, size_t *pk, size_t *sk, size_t *ct, size_t *ss);
 /* ---- batched KEM (element-contiguous buffers: element i at i * size) --------
  * Device pointers unless PQC_F_HOST_PTRS; asynchronous on `stream` (a
  * hipStream_t, NULL = default stream) for device pointers.
- * coins: keypair 64 bytes per element (d || z), enc 32 bytes per element.   */
+ * coins: keypair 64 bytes per element (Kyber d || z; Frodo s || seedSE || z),
+ * enc 32 bytes per element (Kyber) or 24 (Frodo: mu).
+ * FrodoKEM-976 here has one key per element (frodo976.cu:87-690 per call): A is
+ * expanded per element, 128 at a time; batches under one key are far faster
+ * through pqc_frodo976_load_key + pqc_frodo976_enc/dec_batch.                  */
 int crypto_kem_keypair_derand_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *pk, uint8_t *sk,
                                     const uint8_t *coins, unsigned flags, void *stream);
 int crypto_kem_enc_derand_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *ss,
@@ -204,6 +208,10 @@ int pqc_kyber768_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, cons
 int pqc_kyber1024_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
 int pqc_kyber1024_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
 int pqc_kyber1024_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+/* FrodoKEM-976-SHAKE: FrodoKEM_GPU/api.h:20-24 (crypto_kem_keypair / enc / dec) */
+int pqc_frodo976_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int pqc_frodo976_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int pqc_frodo976_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
 
 /* ---- NewHope-1024 CPA-PKE, plain batched kernels ------------------------------
  * Replaces cpapke_keypair/enc/dec (NewHope_NIST_GPU/cpapke.h:14-24).

@@ -101,7 +101,7 @@ SIGNATURES = {
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }
-for _alg in ("kyber512", "kyber768", "kyber1024"):
+for _alg in ("kyber512", "kyber768", "kyber1024", "frodo976"):
     SIGNATURES["pqc_%s_crypto_kem_keypair" % _alg] = (_I, [_P, _P])
     SIGNATURES["pqc_%s_crypto_kem_enc" % _alg] = (_I, [_P, _P, _P])
     SIGNATURES["pqc_%s_crypto_kem_dec" % _alg] = (_I, [_P, _P, _P])

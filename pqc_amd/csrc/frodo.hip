@@ -71,6 +71,26 @@ __device__ __forceinline__ uint32_t sample2(uint32_t raw) {
 // the low bytes of two 16-bit samples, i.e. two int8 values in [-10, 10]
 __device__ __forceinline__ uint32_t bytes2(uint32_t s2) { return __builtin_amdgcn_perm(0u, s2, 0x0C0C0200u); }
 
+// Where each request's key lives: one key for the whole batch (strides 0), one
+// per request (kid null, strides = element sizes), or a key store (kid[r] =
+// the key index of request r).
+struct FrKeys {
+  const uint8_t* sk;  // secret keys (s || pk || S^T || pkh), decapsulation
+  size_t sk_stride;
+  const uint8_t* pkh;  // H(pk)
+  size_t pkh_stride;
+  const uint8_t* pk;  // public keys (seedA || B)
+  size_t pk_stride;
+  const uint16_t* A;  // expanded A, 976 x 976 row-major per key (many-key products)
+  size_t a_stride;    // uint16 elements
+  const uint32_t* kid;
+  __device__ __forceinline__ size_t idx(int r) const { return kid ? (size_t)kid[r] : (size_t)r; }
+  __device__ __forceinline__ const uint8_t* sk_of(int r) const { return sk + idx(r) * sk_stride; }
+  __device__ __forceinline__ const uint8_t* pkh_of(int r) const { return pkh + idx(r) * pkh_stride; }
+  __device__ __forceinline__ const uint8_t* pk_of(int r) const { return pk + idx(r) * pk_stride; }
+  __device__ __forceinline__ const uint16_t* a_of(int r) const { return A + idx(r) * a_stride; }
+};
+
 // ---- key expansion -----------------------------------------------------------
 
 // generateMatA2 (frodo_macrify_reference.cu:164-188): row i =
@@ -91,6 +111,35 @@ __global__ void k_gen_a(uint16_t* A, const uint8_t* seedA) {
 #pragma unroll
     for (int w = 0; w < kRateShake128; ++w) {
       const int idx = blk * kRateShake128 + w;  // 64-bit word = 4 columns
+      if (idx < N / 4) row[idx] = S[w];
+    }
+  }
+}
+
+// The same for n matrices (request r's seedA at seeds + r * seed_stride), one
+// thread per (matrix, row); matrix r at A + r * 976 * 976.
+__global__ void k_gen_a_many(int n, uint16_t* A, const uint8_t* seeds, size_t seed_stride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * N) return;
+  const int r = t / N, i = t % N;
+  const uint8_t* seedA = seeds + (size_t)r * seed_stride;
+  uint64_t a0 = 0, a1 = 0;
+  for (int b = 0; b < 8; ++b) {  // byte loads: seeds sit at any offset inside a key
+    a0 |= (uint64_t)seedA[b] << (8 * b);
+    a1 |= (uint64_t)seedA[8 + b] << (8 * b);
+  }
+  uint64_t S[25];
+  keccak_zero(S);
+  S[0] = (uint64_t)i | (a0 << 16);
+  S[1] = (a0 >> 48) | (a1 << 16);
+  S[2] = (a1 >> 48) | (0x1Full << 16);
+  S[kRateShake128 - 1] ^= 0x8000000000000000ull;
+  uint64_t* row = reinterpret_cast<uint64_t*>(A + ((size_t)r * N + i) * N);
+  for (int blk = 0; blk < 12; ++blk) {
+    keccakf1600(S);
+#pragma unroll
+    for (int w = 0; w < kRateShake128; ++w) {
+      const int idx = blk * kRateShake128 + w;
       if (idx < N / 4) row[idx] = S[w];
     }
   }
@@ -127,12 +176,12 @@ __global__ void k_prep_frags(int8_t* frag, const uint16_t* A, const uint8_t* pk)
 // ---- decapsulation: W = B'S, mu' = decode(C - W) -------------------------------
 // One wave per request; lane (i, j) computes W[i][j] (frodo_mul_bs_kernel), the
 // 8 lanes of row i assemble 24 bits of mu' (frodo_key_decode_kernel).
-__global__ __launch_bounds__(64) void k_w_decode(int n, const uint8_t* ct, const uint8_t* sk, uint8_t* mu) {
+__global__ __launch_bounds__(64) void k_w_decode(int n, const uint8_t* ct, FrKeys keys, uint8_t* mu) {
   const int r = blockIdx.x;
   if (r >= n) return;
   const int l = threadIdx.x, i = l >> 3, j = l & 7;
   const uint8_t* bp = ct + (size_t)r * CT + 2 * N * i;
-  const uint8_t* st = sk + SK_S + 2 * N * j;
+  const uint8_t* st = keys.sk_of(r) + SK_S + 2 * N * j;
   uint32_t acc = 0;
   for (int k = 0; k < N; k += 4) {
     const uint64_t bw = load_u64(bp + 2 * k), sw = load_u64(st + 2 * k);
@@ -182,7 +231,11 @@ __device__ __forceinline__ size_t eoff(int m, int j) {
   return ((((size_t)(m >> 4) * 16 + (j >> 6)) * 64 + lane) * 4 + ((j >> 4) & 3)) * 4 + (j & 3);
 }
 
-__global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, const uint8_t* pkh, const uint8_t* mu,
+// ROWS = false: S' into the MFMA fragments, E' | E'' at eoff (shared-A GEMM).
+// ROWS = true: S' and E' as plain int8 rows [8n][976] in sfrag / efrag, E'' after
+// E' as [n][64] (the many-key products).
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKeys keys, const uint8_t* mu,
                                               int mu_stride, int8_t* sfrag, int8_t* efrag, uint8_t* kout,
                                               const uint8_t* ct, uint32_t* fstate) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,7 +246,7 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, cons
     if (r >= n) return;
 #pragma unroll
     for (int w = 0; w < 25; ++w) S[w] = 0;
-    const uint32_t* ph = reinterpret_cast<const uint32_t*>(pkh);
+    const uint32_t* ph = reinterpret_cast<const uint32_t*>(keys.pkh_of(r));
     const uint32_t* pm = reinterpret_cast<const uint32_t*>(mu + (size_t)r * mu_stride);
 #pragma unroll
     for (int w = 0; w < 3; ++w) {
@@ -225,7 +278,14 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, cons
         if (blk * 68 + 4 * w >= NSAMP) break;
         const int g = blk * 68 + 4 * w + 2 * half;  // this lane's first sample
         const uint16_t b2 = (uint16_t)bytes2(sample2(S[w]));
-        if (g < N * NB) {
+        if constexpr (ROWS) {
+          if (g < N * NB)
+            *reinterpret_cast<uint16_t*>(sfrag + (size_t)r * N * NB + g) = b2;
+          else if (g < 2 * N * NB)
+            *reinterpret_cast<uint16_t*>(efrag + (size_t)r * N * NB + (g - N * NB)) = b2;
+          else
+            *reinterpret_cast<uint16_t*>(efrag + (size_t)n * N * NB + (size_t)r * 64 + (g - 2 * N * NB)) = b2;
+        } else if (g < N * NB) {
           const int k = g / N, j = g % N, m = 8 * r + k;
           const int lane = (m & 15) + 16 * ((j >> 4) & 3);
           const size_t off = ((((size_t)(m >> 4) * KSTEPS + (j >> 6)) * 64 + lane) * 16) + (j & 15);
@@ -476,7 +536,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* 
 // request's 16 compare flags, as a mask) into the saved sponge state, pad,
 // permute: ss = F(ct || key) (kem.c:195-202).
 __global__ void k_f_final(int n, uint8_t* ss, const uint8_t* ct, const uint64_t* fstate, const uint8_t* flags,
-                          const uint8_t* kp, const uint8_t* s) {
+                          const uint8_t* kp, FrKeys keys) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   uint64_t S[25];
@@ -487,6 +547,7 @@ __global__ void k_f_final(int n, uint8_t* ss, const uint8_t* ct, const uint64_t*
   for (int w = 0; w < 13; ++w) S[w] ^= p[w];
   const uint4 f = *reinterpret_cast<const uint4*>(flags + (size_t)r * GEMM_NB);
   const uint64_t mask = 0 - (uint64_t)((f.x | f.y | f.z | f.w) != 0);
+  const uint8_t* s = keys.sk_of(r);
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
     const uint64_t a = load_u64(kp + (size_t)r * 32 + 8 * w), b = load_u64(s + 8 * w);
@@ -513,6 +574,122 @@ __global__ __launch_bounds__(64) void k_f_full(int n, uint8_t* ss, const uint8_t
   S[16] ^= half ? 0x80000000u : 0x1Fu;
   keccakf1600_pair(S, (uint32_t)half);
   uint32_t* o = reinterpret_cast<uint32_t*>(ss + (size_t)r * 24) + half;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) o[2 * w] = S[w];
+}
+
+// ---- many keys: products on packed 16-bit VALU ---------------------------------------
+// [B'' | V] = S'.[A | B] + [E' | E''] (mod 2^16) for one request per workgroup
+// pair: workgroup (r, y) computes columns 512 y + 2 t, +1 for thread t with
+// v_pk_mad_u16 (two 16-bit multiply-adds mod 2^16 per lane), reading row k of
+// the request's own A (key cache or per-request expansion) and S'[.][k] as a
+// broadcast from LDS.  Then, as k_gemm's epilogue, compare with ct (decaps,
+// one flag byte per workgroup among the request's 16) or pack into ct (encaps).
+// A request's key has its own A, so there is no shared operand for MFMA here.
+typedef uint16_t u16x2v __attribute__((ext_vector_type(2)));
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mk_product(int n, const int8_t* __restrict__ srow,
+                                                    const int8_t* __restrict__ erow, FrKeys keys,
+                                                    uint8_t* __restrict__ ct, const uint8_t* __restrict__ mu,
+                                                    int mu_stride, uint8_t* __restrict__ flags) {
+  __shared__ uint4 st[N];  // S'[0..7][k] as 8 uint16 per k
+  __shared__ uint32_t wmis[4];
+  const int r = blockIdx.x, y = blockIdx.y, tid = threadIdx.x;
+  if (r >= n) return;
+  for (int k = tid; k < N; k += blockDim.x) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t lo = (uint32_t)(int32_t)srow[((size_t)r * NB + 2 * q) * N + k] & 0xFFFF;
+      const uint32_t hi = (uint32_t)(int32_t)srow[((size_t)r * NB + 2 * q + 1) * N + k] & 0xFFFF;
+      w[q] = lo | (hi << 16);
+    }
+    st[k] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  __syncthreads();
+  const int c0 = 512 * y + 2 * tid;  // this thread's columns c0, c0 + 1
+  const uint16_t* A = keys.a_of(r);
+  const uint8_t* pk = keys.pk_of(r);
+  u16x2v acc[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) acc[i] = u16x2v{0, 0};
+  if (c0 < N + NB) {
+#pragma unroll 4
+    for (int k = 0; k < N; ++k) {
+      uint32_t a2;
+      if (c0 < N)
+        a2 = *reinterpret_cast<const uint32_t*>(A + (size_t)k * N + c0);
+      else  // B columns from the public key, big-endian uint16
+        a2 = __builtin_amdgcn_perm(0u, *reinterpret_cast<const uint32_t*>(pk + SEEDA + 2 * (NB * k + (c0 - N))),
+                                   0x02030001u);
+      const u16x2v av = __builtin_bit_cast(u16x2v, a2);
+      const uint4 sv = st[k];
+      const uint32_t sw[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint16_t s0 = (uint16_t)sw[q], s1 = (uint16_t)(sw[q] >> 16);
+        acc[2 * q] += av * u16x2v{s0, s0};
+        acc[2 * q + 1] += av * u16x2v{s1, s1};
+      }
+    }
+  }
+  // epilogue: + E' | E'', then compare / pack
+  uint32_t mism = 0;
+  if (c0 < N + NB) {
+    const int8_t* e1 = erow + (size_t)r * N * NB;
+    const int8_t* e2 = erow + (size_t)n * N * NB + (size_t)r * 64;
+    uint8_t* c = ct + (size_t)r * CT;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      uint32_t v0 = acc[i][0], v1 = acc[i][1];
+      if (c0 < N) {
+        v0 += (uint32_t)(int32_t)e1[i * N + c0];
+        v1 += (uint32_t)(int32_t)e1[i * N + c0 + 1];
+      } else {
+        const int j = c0 - N;
+        v0 += (uint32_t)(int32_t)e2[8 * i + j] + encode_mu(mu + (size_t)r * mu_stride, i, j);
+        v1 += (uint32_t)(int32_t)e2[8 * i + j + 1] + encode_mu(mu + (size_t)r * mu_stride, i, j + 1);
+      }
+      const uint32_t p = (v0 & 0xFFFF) | (v1 << 16);
+      uint32_t* cw = reinterpret_cast<uint32_t*>(c0 < N ? c + 2 * (i * N + c0) : c + C1 + 2 * (8 * i + (c0 - N)));
+      if constexpr (MODE == kEncaps) *cw = bswap16x2(p);
+      else mism |= p ^ bswap16x2(*cw);
+    }
+  }
+  if constexpr (MODE == kDecaps) {
+    const uint64_t bal = __ballot(mism != 0);
+    if ((tid & 63) == 0) wmis[tid >> 6] = bal != 0;
+    __syncthreads();
+    if (tid < 8) {  // bytes 8 y .. 8 y + 7 of the request's 16 flag bytes
+      const uint8_t f = (uint8_t)(tid == 0 ? (wmis[0] | wmis[1] | wmis[2] | wmis[3]) : 0);
+      flags[(size_t)r * GEMM_NB + 8 * y + tid] = f;
+    }
+  }
+}
+
+// pkh = SHAKE256(pk, 24) for n public keys, one lane pair per key
+__global__ __launch_bounds__(64) void k_pkh_many(int n, uint8_t* pkh, const uint8_t* pk, size_t pk_stride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = t >> 1, half = t & 1;
+  if (r >= n) return;
+  uint32_t S[25];
+#pragma unroll
+  for (int w = 0; w < 25; ++w) S[w] = 0;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(pk + (size_t)r * pk_stride) + half;
+  constexpr int W = PK / 8;  // 1954 words: 114 full blocks + 16 words
+  for (int blk = 0; blk < W / kRateShake256; ++blk) {
+#pragma unroll
+    for (int w = 0; w < kRateShake256; ++w) S[w] ^= p[2 * w];
+    keccakf1600_pair(S, (uint32_t)half);
+    p += 2 * kRateShake256;
+  }
+#pragma unroll
+  for (int w = 0; w < W % kRateShake256; ++w) S[w] ^= p[2 * w];
+  S[W % kRateShake256] ^= half ? 0u : 0x1Fu;
+  S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;
+  keccakf1600_pair(S, (uint32_t)half);
+  uint32_t* o = reinterpret_cast<uint32_t*>(pkh + (size_t)r * 24) + half;
 #pragma unroll
   for (int w = 0; w < 3; ++w) o[2 * w] = S[w];
 }
@@ -667,22 +844,97 @@ static int launch_gemm(pqc_ctx* c, size_t n, const FrScratch& f, uint8_t* ct, co
 
 static inline int pairs_pad(int n) { return (n + 31) / 32 * 32; }
 
+constexpr int kFrChunk = 128;  // per-element-key requests per A expansion (A buffer: 128 x 1.9 MB)
+
+// the loaded key as a key source for every request (strides 0)
+static FrKeys loaded_key(pqc_ctx* c) {
+  return FrKeys{c->fr_sk, 0, c->fr_pkh, 0, c->fr_pk, 0, c->fr_a16, 0, nullptr};
+}
+
 int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s) {
   int rc = frodo_need(c);
   if (rc) return rc;
   if (!c->fr_loaded || !c->fr_has_sk) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
+  const FrKeys keys = loaded_key(c);
   const int ni = (int)n, npad = pairs_pad(ni);
-  hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, (const uint8_t*)c->fr_sk, f.mu);
+  hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, keys, f.mu);
   const int sb = c->tune.frodo_stream_block;
-  hipLaunchKernelGGL(k_shake, dim3((2 * (npad + ni) + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 1,
-                     (const uint8_t*)c->fr_pkh, (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
+  hipLaunchKernelGGL(k_shake<false>, dim3((2 * (npad + ni) + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 1, keys,
+                     (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
   rc = launch_gemm<kDecaps>(c, n, f, const_cast<uint8_t*>(ct), f.mu, 32, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, ct, (const uint64_t*)f.fstate,
-                     (const uint8_t*)f.flags, (const uint8_t*)f.kp, (const uint8_t*)c->fr_sk);
+                     (const uint8_t*)f.flags, (const uint8_t*)f.kp, keys);
   return hip_err(hipGetLastError());
+}
+
+// ---- many keys ------------------------------------------------------------------------
+// Decapsulation / encapsulation of n requests whose keys (and expanded A) come
+// from `keys`; the products run on k_mk_product.
+int frodo_dec_keyed(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const FrKeys& keys, hipStream_t s) {
+  if (!c->fr_elem || !c->fr_sfrag) return PQC_ERR_UNSUPPORTED;
+  if (n == 0) return PQC_OK;
+  FrScratch f = fr_scratch(c);
+  const int ni = (int)n, npad = pairs_pad(ni);
+  hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, keys, f.mu);
+  const int sb = c->tune.frodo_stream_block;
+  hipLaunchKernelGGL(k_shake<true>, dim3((2 * (npad + ni) + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 1, keys,
+                     (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
+  hipLaunchKernelGGL(k_mk_product<kDecaps>, dim3(ni, 2), dim3(256), 0, s, ni, (const int8_t*)f.sfrag,
+                     (const int8_t*)f.efrag, keys, const_cast<uint8_t*>(ct), (const uint8_t*)f.mu, 32, f.flags);
+  hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, ct, (const uint64_t*)f.fstate,
+                     (const uint8_t*)f.flags, (const uint8_t*)f.kp, keys);
+  return hip_err(hipGetLastError());
+}
+
+int frodo_enc_keyed(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, const FrKeys& keys,
+                    hipStream_t s) {
+  if (!c->fr_elem || !c->fr_sfrag) return PQC_ERR_UNSUPPORTED;
+  if (n == 0) return PQC_OK;
+  FrScratch f = fr_scratch(c);
+  const int ni = (int)n, npad = pairs_pad(ni);
+  const int sb = c->tune.frodo_stream_block;
+  hipLaunchKernelGGL(k_shake<true>, dim3((2 * npad + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 0, keys, mu, MU,
+                     f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_mk_product<kEncaps>, dim3(ni, 2), dim3(256), 0, s, ni, (const int8_t*)f.sfrag,
+                     (const int8_t*)f.efrag, keys, ct, mu, MU, (uint8_t*)nullptr);
+  hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct,
+                     (const uint8_t*)f.kp);
+  return hip_err(hipGetLastError());
+}
+
+// Per-element keys (crypto_kem_enc/dec_batch): A is expanded per request into
+// the context's A buffer, kFrChunk requests at a time.
+int frodo_dec_batch(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  if (!c->fr_abuf) return PQC_ERR_UNSUPPORTED;
+  for (size_t off = 0; off < n; off += kFrChunk) {
+    const int m = (int)(n - off < (size_t)kFrChunk ? n - off : (size_t)kFrChunk);
+    const uint8_t* skc = sk + off * SK;
+    hipLaunchKernelGGL(k_gen_a_many, dim3((m * N + 255) / 256), dim3(256), 0, s, m, c->fr_abuf, skc + SS,
+                       (size_t)SK);
+    const FrKeys keys{skc, SK, skc + SK_PKH, SK, skc + SS, SK, c->fr_abuf, (size_t)N * N, nullptr};
+    int rc = frodo_dec_keyed(c, m, ss + off * SS, ct + off * CT, keys, s);
+    if (rc) return rc;
+  }
+  return PQC_OK;
+}
+
+int frodo_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* mu,
+                    hipStream_t s) {
+  if (!c->fr_abuf) return PQC_ERR_UNSUPPORTED;
+  uint8_t* pkh = (uint8_t*)fr_scratch(c).out;  // 24 bytes per request, the products' output is unused here
+  for (size_t off = 0; off < n; off += kFrChunk) {
+    const int m = (int)(n - off < (size_t)kFrChunk ? n - off : (size_t)kFrChunk);
+    const uint8_t* pkc = pk + off * PK;
+    hipLaunchKernelGGL(k_gen_a_many, dim3((m * N + 255) / 256), dim3(256), 0, s, m, c->fr_abuf, pkc, (size_t)PK);
+    hipLaunchKernelGGL(k_pkh_many, dim3((2 * m + 63) / 64), dim3(64), 0, s, m, pkh, pkc, (size_t)PK);
+    const FrKeys keys{nullptr, 0, pkh, 24, pkc, PK, c->fr_abuf, (size_t)N * N, nullptr};
+    int rc = frodo_enc_keyed(c, m, ct + off * CT, ss + off * SS, mu + off * MU, keys, s);
+    if (rc) return rc;
+  }
+  return PQC_OK;
 }
 
 int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, hipStream_t s) {
@@ -693,8 +945,8 @@ int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu,
   FrScratch f = fr_scratch(c);
   const int ni = (int)n, npad = pairs_pad(ni);
   const int sb = c->tune.frodo_stream_block;
-  hipLaunchKernelGGL(k_shake, dim3((2 * npad + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 0, (const uint8_t*)c->fr_pkh, mu, MU,
-                     f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_shake<false>, dim3((2 * npad + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 0, loaded_key(c), mu,
+                     MU, f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
   rc = launch_gemm<kEncaps>(c, n, f, ct, mu, MU, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct,

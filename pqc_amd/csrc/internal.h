@@ -36,6 +36,7 @@ struct pqc_ctx {
   // fragments), per-element scratch (S' fragments, E', E'', GEMM output, mu, k,
   // F sponge states, fail flags)
   uint16_t* fr_a16 = nullptr;
+  uint16_t* fr_abuf = nullptr;  // A of up to 128 per-element keys (crypto_kem_*_batch)
   int8_t* fr_frag = nullptr;
   int8_t* fr_sfrag = nullptr;
   uint8_t* fr_sk = nullptr;
@@ -94,6 +95,9 @@ int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu,
 int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s);
 int frodo_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, hipStream_t s);
 int frodo_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, hipStream_t s);
+int frodo_dec_batch(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
+int frodo_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* mu,
+                    hipStream_t s);
 
 // component kernels exposed for tests (kyber_fused.hip)
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);

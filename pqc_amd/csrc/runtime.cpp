@@ -5,6 +5,7 @@
 #include <string.h>
 #include <sys/random.h>
 
+#include <algorithm>
 #include <mutex>
 #include <new>
 
@@ -132,7 +133,8 @@ pqc_ctx* default_ctx() {
   std::lock_guard<std::mutex> lk(g_default_mu);
   if (!g_default[dev]) {
     pqc_ctx* c = nullptr;
-    unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024);
+    unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024) |
+                    PQC_ALG_MASK(PQC_ALG_FRODO976);
     if (pqc_ctx_create(&c, mask, 64, 0) != PQC_OK) return nullptr;
     g_default[dev] = c;
   }
@@ -218,10 +220,27 @@ int check_call(pqc_ctx* c, int alg, size_t n) {
 int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
              const uint8_t* i1, unsigned flags, void* stream) {
   AlgInfo a;
-  if (!alg_info(alg, &a) || a.kind != 1) return PQC_ERR_UNSUPPORTED;
+  if (!alg_info(alg, &a) || (a.kind != 1 && a.kind != 3)) return PQC_ERR_UNSUPPORTED;
   int rc = check_call(c, alg, n);
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (a.kind == 3) {  // FrodoKEM-976 with one key per element
+    switch (op) {
+      case OP_KEYPAIR: {
+        const Buf b[3] = {{o0, a.pk, true}, {o1, a.sk, true}, {i0, 64, false}};
+        return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::frodo_keypair(c, n, d[0], d[1], d[2], s); });
+      }
+      case OP_ENC: {
+        const Buf b[4] = {{o0, a.ct, true}, {o1, a.ss, true}, {i0, a.pk, false}, {i1, 24, false}};
+        return staged(c, n, flags, s, b,
+                      [&](uint8_t** d) { return pqc::frodo_enc_batch(c, n, d[0], d[1], d[2], d[3], s); });
+      }
+      default: {
+        const Buf b[3] = {{o0, a.ss, true}, {i0, a.ct, false}, {i1, a.sk, false}};
+        return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::frodo_dec_batch(c, n, d[0], d[1], d[2], s); });
+      }
+    }
+  }
   const bool plain = (flags & PQC_F_PLAIN) != 0;
   const int k = a.k;
   switch (op) {
@@ -342,6 +361,9 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     rc = ctx_alloc(c, &p, pqc::frodo_bytes_per_elem() * c->ld);
     if (rc) goto fail;
     c->fr_elem = (uint8_t*)p;
+    rc = ctx_alloc(c, &p, (size_t)std::min<size_t>(c->ld, 128) * 976 * 976 * 2);
+    if (rc) goto fail;
+    c->fr_abuf = (uint16_t*)p;
   }
   rc = ctx_alloc(c, &p, 256);
   if (rc) goto fail;
@@ -366,6 +388,7 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->kprf) (void)hipFree(c->kprf);
   if (c->kmix_order) (void)hipFree(c->kmix_order);
   if (c->fr_a16) (void)hipFree(c->fr_a16);
+  if (c->fr_abuf) (void)hipFree(c->fr_abuf);
   if (c->fr_frag) (void)hipFree(c->fr_frag);
   if (c->fr_sfrag) (void)hipFree(c->fr_sfrag);
   if (c->fr_sk) (void)hipFree(c->fr_sk);
@@ -643,6 +666,7 @@ int crypto_kem_dec(unsigned char* ss, const unsigned char* ct, const unsigned ch
 PQC_NIST_SET(kyber512, PQC_ALG_KYBER512)
 PQC_NIST_SET(kyber768, PQC_ALG_KYBER768)
 PQC_NIST_SET(kyber1024, PQC_ALG_KYBER1024)
+PQC_NIST_SET(frodo976, PQC_ALG_FRODO976)
 
 void pqc_randombytes_seed(uint64_t seed) {
   std::lock_guard<std::mutex> lk(g_rng_mu);

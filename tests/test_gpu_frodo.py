@@ -122,3 +122,38 @@ def test_full_size_4096_roundtrip_and_sample(ctx, keys):
     oct_, oss = O.frodo_enc_batch(pk, mu[idx], nthreads=8)
     assert (ct[idx] == oct_).all() and (ss[idx] == oss).all()
     assert (ss2[idx] == O.frodo_dec_batch(bad[idx], sk, nthreads=8)).all()
+
+
+def test_per_element_keys_batch_api_vs_oracle(ctx):
+    """crypto_kem_*_batch with alg FRODO976: one key per element, A expanded
+    per element (n = 130 crosses the 128-element chunk)."""
+    n = 130
+    say("frodo per-element keys n", n)
+    rnd = O.drbg_array(n, 64, 0, 950)
+    pk, sk = ctx.keypair_host("frodo976", rnd)
+    for i in (0, 1, 129):
+        opk, osk = O.frodo_keypair(rnd[i].tobytes())
+        assert pk[i].tobytes() == opk and sk[i].tobytes() == osk
+    mu = O.drbg_array(n, 24, 0, 951)
+    ct, ss = ctx.enc_host("frodo976", pk, mu)
+    for i in range(n):
+        oct_, oss = O.frodo_enc(pk[i].tobytes(), mu[i].tobytes())
+        assert ct[i].tobytes() == oct_ and ss[i].tobytes() == oss, i
+    bad = ct.copy()
+    bad[::5, 321] ^= 0x02
+    bad[3, 15700] ^= 0x80  # inside C
+    ss2 = ctx.dec_host("frodo976", bad, sk)
+    for i in range(n):
+        assert ss2[i].tobytes() == O.frodo_dec(bad[i].tobytes(), sk[i].tobytes()), i
+        assert (ss2[i] == ss[i]).all() == (i % 5 != 0 and i != 3), i
+
+
+def test_nist_dropins_frodo976(ctx):
+    import pqc_amd
+    say("frodo NIST drop-ins")
+    pqc_amd.randombytes_seed(77)
+    pk, sk = pqc_amd.crypto_kem_keypair("frodo976")
+    ct, ss = pqc_amd.crypto_kem_enc(pk, "frodo976")
+    assert pqc_amd.crypto_kem_dec(ct, sk, "frodo976") == ss
+    assert O.frodo_dec(ct, sk) == ss
+    pqc_amd.randombytes_system()
