@@ -193,6 +193,74 @@ int pqc_graph_capture_end(void *stream, pqc_graph **graph);
 int pqc_graph_launch(pqc_graph *graph, void *stream);
 void pqc_graph_destroy(pqc_graph *graph);
 
+/* ---- serving loop ---------------------------------------------------------------
+ * A server holds keys_per_alg device-resident key pairs per parameter set (key
+ * pair i of set a from coins = the library's generator with key_seed, stream
+ * 0x6B6579 + a, index i) and runs a scheduler thread: requests are admitted in
+ * arrival order into up to `slots` batches in flight at once (continuous
+ * batching: arrivals join the next free slot while others run), each batch at
+ * most max_batch requests and arena_bytes of inputs / outputs, and the bytes of
+ * all requests in flight at most hbm_budget (0 = no limit).  A batch runs the
+ * fused mixed-parameter-set Kyber kernels and the many-key Frodo path, or with
+ * PQC_SERVER_PLAIN the plain per-op kernels.  Results are the same bytes as the
+ * batch calls give for the request alone.  The reference has no counterpart.  */
+#define PQC_SERVER_PLAIN 1u
+typedef struct pqc_server pqc_server;
+typedef struct pqc_server_config {
+  uint32_t alg_mask;     /* Kyber sets and/or FrodoKEM-976 */
+  uint32_t keys_per_alg; /* device-resident key pairs per parameter set */
+  uint64_t key_seed;
+  uint32_t max_batch;    /* requests per batch */
+  uint32_t slots;        /* batches in flight, 1..8 */
+  uint64_t hbm_budget;   /* bytes of in-flight request inputs + outputs, 0 = no limit */
+  uint64_t arena_bytes;  /* per-slot input and output arena (pinned + device), >= 1 MiB */
+  uint32_t max_wait_us;  /* while other batches run, a batch waits at most this long to fill */
+  uint32_t flags;        /* PQC_SERVER_PLAIN */
+  pqc_tuning tuning;     /* launch parameters for the slots' contexts (all zero: defaults) */
+} pqc_server_config;
+typedef struct pqc_server_request {
+  int32_t alg;          /* PQC_ALG_* served */
+  int32_t op;           /* 1 encapsulate, 2 decapsulate */
+  uint32_t key_id;      /* < keys_per_alg */
+  uint32_t reserved;
+  const uint8_t *in;    /* host: encaps coins (Kyber 32 B, Frodo mu 24 B) or decaps ciphertext */
+  uint8_t *out;         /* host: encaps ct || ss, decaps ss; written before the completion */
+  uint64_t tag;         /* returned in the completion */
+} pqc_server_request;
+typedef struct pqc_server_completion {
+  uint64_t tag;
+  int32_t status;       /* PQC_OK or PQC_ERR_* */
+  uint32_t batch;       /* requests in the batch that carried it */
+  double submit_us;     /* server clock */
+  double done_us;
+} pqc_server_completion;
+typedef struct pqc_server_stats {
+  uint64_t submitted, completed, batches;
+  double mean_batch;
+  uint64_t inflight_bytes;
+} pqc_server_stats;
+typedef struct pqc_trace_stats {
+  uint64_t completed, errors;
+  double duration_s, ops_per_s;
+  double p50_us, p90_us, p99_us, p999_us, max_us; /* arrival to completion */
+  double mean_batch;
+} pqc_trace_stats;
+int pqc_server_create(pqc_server **srv, const pqc_server_config *cfg);
+void pqc_server_destroy(pqc_server *srv);
+/* host copies of key pair key_id (either pointer may be NULL) */
+int pqc_server_key(pqc_server *srv, int alg, uint32_t key_id, uint8_t *pk, uint8_t *sk);
+/* thread-safe, non-blocking; in / out must stay valid until the completion */
+int pqc_server_submit(pqc_server *srv, const pqc_server_request *reqs, size_t n);
+/* up to cap completions; waits up to wait_us for the first */
+int pqc_server_poll(pqc_server *srv, pqc_server_completion *out, size_t cap, size_t *got, uint32_t wait_us);
+int pqc_server_drain(pqc_server *srv); /* until every submitted request completed */
+int pqc_server_get_stats(pqc_server *srv, pqc_server_stats *st);
+double pqc_server_now_us(pqc_server *srv);
+/* replays n requests at arrival_us[i] (microseconds from the call) from the calling
+ * thread; latency_us[i] (optional) = completion - arrival; tags are overwritten */
+int pqc_server_run_trace(pqc_server *srv, size_t n, const double *arrival_us, const pqc_server_request *reqs,
+                         double *latency_us, pqc_trace_stats *out);
+
 /* ---- NIST drop-ins: one instance, host pointers, synchronous -----------------
  * Unprefixed names are Kyber-768 (Kyber_GPU_Batched/api.h:18-22 with
  * KYBER_K 3).  Each call uses a library-owned context on the current device. */

@@ -98,6 +98,15 @@ SIGNATURES = {
     "pqc_costmodel_grad": (_I, [_P, _I, _P, _P, C.POINTER(C.c_float), _P, _SZ]),
     "pqc_costmodel_fit": (_I, [_P, _I, _P, _P, _I, C.c_float, C.POINTER(C.c_float)]),
     "pqc_ctx_costmodel": (_P, [_P]),
+    "pqc_server_create": (_I, [C.POINTER(_P), _P]),
+    "pqc_server_destroy": (None, [_P]),
+    "pqc_server_key": (_I, [_P, _I, C.c_uint32, _P, _P]),
+    "pqc_server_submit": (_I, [_P, _P, _SZ]),
+    "pqc_server_poll": (_I, [_P, _P, _SZ, C.POINTER(_SZ), C.c_uint32]),
+    "pqc_server_drain": (_I, [_P]),
+    "pqc_server_get_stats": (_I, [_P, _P]),
+    "pqc_server_now_us": (C.c_double, [_P]),
+    "pqc_server_run_trace": (_I, [_P, _SZ, _P, _P, _P, _P]),
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }

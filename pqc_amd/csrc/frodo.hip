@@ -668,8 +668,10 @@ __global__ __launch_bounds__(256) void k_mk_product(int n, const int8_t* __restr
   }
 }
 
-// pkh = SHAKE256(pk, 24) for n public keys, one lane pair per key
-__global__ __launch_bounds__(64) void k_pkh_many(int n, uint8_t* pkh, const uint8_t* pk, size_t pk_stride) {
+// pkh = SHAKE256(pk, 24) for n public keys, one lane pair per key; key r's
+// hash at pkh + r * out_stride
+__global__ __launch_bounds__(64) void k_pkh_many(int n, uint8_t* pkh, size_t out_stride, const uint8_t* pk,
+                                                 size_t pk_stride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = t >> 1, half = t & 1;
   if (r >= n) return;
@@ -689,7 +691,7 @@ __global__ __launch_bounds__(64) void k_pkh_many(int n, uint8_t* pkh, const uint
   S[W % kRateShake256] ^= half ? 0u : 0x1Fu;
   S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;
   keccakf1600_pair(S, (uint32_t)half);
-  uint32_t* o = reinterpret_cast<uint32_t*>(pkh + (size_t)r * 24) + half;
+  uint32_t* o = reinterpret_cast<uint32_t*>(pkh + (size_t)r * out_stride) + half;
 #pragma unroll
   for (int w = 0; w < 3; ++w) o[2 * w] = S[w];
 }
@@ -760,14 +762,6 @@ __global__ void k_kp_finish(int r, uint8_t* pk, uint8_t* sk, const uint16_t* A, 
   if (t < SS) s[t] = rnd[(size_t)r * 64 + t];
 }
 
-// sk tail: H(pk) for key r
-__global__ void k_kp_pkh(int r, uint8_t* sk, const uint8_t* pk) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t S[25];
-  sponge_absorb_words<kRateShake256>(S, reinterpret_cast<const uint64_t*>(pk + (size_t)r * PK), PK / 8, 0x1F);
-  uint8_t* s = sk + (size_t)r * SK + SK_PKH;
-  for (int w = 0; w < 3; ++w) store_u64(s + 8 * w, S[w]);
-}
 
 }  // namespace frodo
 
@@ -923,14 +917,20 @@ int frodo_dec_batch(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const 
 
 int frodo_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* mu,
                     hipStream_t s) {
+  return frodo_enc_batch_strided(c, n, ct, ss, pk, PK, mu, s);
+}
+
+// public key of element r at pk + r * pk_stride (8-byte aligned)
+int frodo_enc_batch_strided(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, size_t pk_stride,
+                            const uint8_t* mu, hipStream_t s) {
   if (!c->fr_abuf) return PQC_ERR_UNSUPPORTED;
   uint8_t* pkh = (uint8_t*)fr_scratch(c).out;  // 24 bytes per request, the products' output is unused here
   for (size_t off = 0; off < n; off += kFrChunk) {
     const int m = (int)(n - off < (size_t)kFrChunk ? n - off : (size_t)kFrChunk);
-    const uint8_t* pkc = pk + off * PK;
-    hipLaunchKernelGGL(k_gen_a_many, dim3((m * N + 255) / 256), dim3(256), 0, s, m, c->fr_abuf, pkc, (size_t)PK);
-    hipLaunchKernelGGL(k_pkh_many, dim3((2 * m + 63) / 64), dim3(64), 0, s, m, pkh, pkc, (size_t)PK);
-    const FrKeys keys{nullptr, 0, pkh, 24, pkc, PK, c->fr_abuf, (size_t)N * N, nullptr};
+    const uint8_t* pkc = pk + off * pk_stride;
+    hipLaunchKernelGGL(k_gen_a_many, dim3((m * N + 255) / 256), dim3(256), 0, s, m, c->fr_abuf, pkc, pk_stride);
+    hipLaunchKernelGGL(k_pkh_many, dim3((2 * m + 63) / 64), dim3(64), 0, s, m, pkh, (size_t)24, pkc, pk_stride);
+    const FrKeys keys{nullptr, 0, pkh, 24, pkc, pk_stride, c->fr_abuf, (size_t)N * N, nullptr};
     int rc = frodo_enc_keyed(c, m, ct + off * CT, ss + off * SS, mu + off * MU, keys, s);
     if (rc) return rc;
   }
@@ -966,10 +966,33 @@ int frodo_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t*
     hipLaunchKernelGGL(k_gen_a, dim3((N + 63) / 64), dim3(64), 0, s, c->fr_a16, (const uint8_t*)seedA + (size_t)r * 16);
     hipLaunchKernelGGL(k_kp_finish, dim3((N * NB + 255) / 256), dim3(256), 0, s, r, pk, sk, (const uint16_t*)c->fr_a16,
                        (const uint16_t*)f.se, (const uint8_t*)seedA, rnd);
-    hipLaunchKernelGGL(k_kp_pkh, dim3(1), dim3(64), 0, s, r, sk, (const uint8_t*)pk);
   }
+  // sk tail: H(pk) of every key in one launch
+  hipLaunchKernelGGL(k_pkh_many, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, sk + SK_PKH, (size_t)SK,
+                     (const uint8_t*)pk, (size_t)PK);
   c->fr_loaded = 0;  // the A buffer now holds the last generated key's A
   return hip_err(hipGetLastError());
+}
+
+// Key store: n_keys secret keys (pk and H(pk) inside) and their expanded A.
+int frodo_expand_a(size_t n_keys, uint16_t* a_out, const uint8_t* sk_store, hipStream_t s) {
+  if (n_keys == 0) return PQC_OK;
+  const int m = (int)n_keys;
+  hipLaunchKernelGGL(k_gen_a_many, dim3((unsigned)(((size_t)m * N + 255) / 256)), dim3(256), 0, s, m, a_out,
+                     sk_store + SS, (size_t)SK);
+  return hip_err(hipGetLastError());
+}
+
+int frodo_dec_store(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk_store,
+                    const uint16_t* a_store, const uint32_t* kid, hipStream_t s) {
+  const FrKeys keys{sk_store, SK, sk_store + SK_PKH, SK, sk_store + SS, SK, a_store, (size_t)N * N, kid};
+  return frodo_dec_keyed(c, n, ss, ct, keys, s);
+}
+
+int frodo_enc_store(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, const uint8_t* sk_store,
+                    const uint16_t* a_store, const uint32_t* kid, hipStream_t s) {
+  const FrKeys keys{sk_store, SK, sk_store + SK_PKH, SK, sk_store + SS, SK, a_store, (size_t)N * N, kid};
+  return frodo_enc_keyed(c, n, ct, ss, mu, keys, s);
 }
 
 int frodo_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, hipStream_t s) {

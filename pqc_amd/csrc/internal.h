@@ -96,8 +96,21 @@ int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t 
 int frodo_products(pqc_ctx* c, size_t n, uint16_t* bpp, uint16_t* v, hipStream_t s);
 int frodo_gemm(pqc_ctx* c, size_t n, const uint8_t* ct, hipStream_t s);
 int frodo_dec_batch(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
+// key store: secret keys (pk and H(pk) inside) + expanded A; kid = key index per request
+int frodo_expand_a(size_t n_keys, uint16_t* a_out, const uint8_t* sk_store, hipStream_t s);
+int frodo_dec_store(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk_store,
+                    const uint16_t* a_store, const uint32_t* kid, hipStream_t s);
+int frodo_enc_store(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, const uint8_t* sk_store,
+                    const uint16_t* a_store, const uint32_t* kid, hipStream_t s);
+// dst[r] = src[idx[r]] for n records of `bytes` (multiple of 8) bytes (gather.hip)
+int gather_records(size_t n, uint8_t* dst, size_t bytes, const uint8_t* src, size_t src_stride, const uint32_t* idx,
+                   hipStream_t s);
+// the library's deterministic generator: SHAKE256("pqc-mi355x" || seed || stream || index)
+void host_drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, uint64_t index);
 int frodo_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* mu,
                     hipStream_t s);
+int frodo_enc_batch_strided(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, size_t pk_stride,
+                            const uint8_t* mu, hipStream_t s);
 
 // component kernels exposed for tests (kyber_fused.hip)
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s);

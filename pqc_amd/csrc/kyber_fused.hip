@@ -428,8 +428,8 @@ __global__ void k_mixed_classify(int n, int cap, int dec, unsigned alg_mask, con
                        (alg_mask & PQC_ALG_MASK(d.alg));
     const int kk = known ? d.alg + 1 : 0;
     if (kk) {
-      // per-K sizes: ct, sk, pk
-      const uint64_t ctb = kk == 2 ? 768 : kk == 3 ? 1088 : 1568;
+      // per-K sizes (round 2): ct, sk, pk
+      const uint64_t ctb = kk == 2 ? 736 : kk == 3 ? 1088 : 1568;
       const uint64_t keyb = dec ? (kk == 2 ? 1632 : kk == 3 ? 2400 : 3168) : (kk == 2 ? 800 : kk == 3 ? 1184 : 1568);
       const bool ok = ((d.ct_off | d.key_off | d.ss_off | d.coins_off) & 7) == 0 && d.ct_off + ctb <= ct_bytes &&
                       d.key_off + keyb <= key_bytes && d.ss_off + 32 <= ss_bytes &&

@@ -127,6 +127,14 @@ void drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, uint64_t ind
 std::mutex g_default_mu;
 pqc_ctx* g_default[64] = {nullptr};
 
+}  // namespace
+
+void pqc::host_drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, uint64_t index) {
+  drbg(out, len, seed, stream, index);
+}
+
+namespace {
+
 pqc_ctx* default_ctx() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;

@@ -1,0 +1,617 @@
+// KEM serving loop: device-resident key pairs, continuous batching over several
+// in-flight slots, admission by HBM budget, latency accounting.
+//
+// The reference has no server; its benchmark loops whole batches synchronously
+// (Kyber_GPU_Batched/main.cu:376-428).  Here a scheduler thread keeps up to
+// `slots` batches on the GPU at once: requests that arrive while batches are
+// running are admitted into the next free slot as soon as it and the budget
+// allow, so their transfers and kernels overlap the running ones.  Each slot
+// owns a context (scratch), a stream, pinned staging and device arenas.
+//
+// A batch is laid out as dense blocks per (operation, parameter set):
+//   Kyber encaps: coins -> ct, ss      Kyber decaps: ct -> ss
+//   Frodo encaps: mu -> ct, ss         Frodo decaps: ct -> ss
+// Fused mode runs all Kyber requests of the batch through the mixed-batch
+// kernels (descriptors name each request's key in the key store) and the Frodo
+// requests through the many-key path with the cached A of each key.  Plain mode
+// gathers each request's key into dense arrays and runs the plain per-op Kyber
+// kernels per parameter set, and Frodo with A expanded per request.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Set {
+  int alg;
+  size_t pk, sk, ct, ss;
+};
+
+bool set_of(int alg, Set* s) {
+  if (alg != PQC_ALG_KYBER512 && alg != PQC_ALG_KYBER768 && alg != PQC_ALG_KYBER1024 && alg != PQC_ALG_FRODO976)
+    return false;
+  s->alg = alg;
+  return pqc_alg_sizes(alg, &s->pk, &s->sk, &s->ct, &s->ss) == PQC_OK;
+}
+
+// groups of a batch: 0..2 Kyber encaps K 2..4, 3..5 Kyber decaps, 6 Frodo encaps, 7 Frodo decaps
+constexpr int kGroups = 8;
+int group_of(int alg, int op) {
+  const bool enc = op == 1;
+  if (alg == PQC_ALG_FRODO976) return enc ? 6 : 7;
+  return (enc ? 0 : 3) + (alg - PQC_ALG_KYBER512);
+}
+size_t in_bytes(const Set& s, int op) { return op == 1 ? (s.alg == PQC_ALG_FRODO976 ? 24 : 32) : s.ct; }
+size_t out_bytes(const Set& s, int op) { return op == 1 ? s.ct + s.ss : s.ss; }
+size_t align8(size_t v) { return (v + 7) & ~(size_t)7; }
+
+struct Pending {
+  pqc_server_request r;
+  Set set;
+  double t_submit;
+};
+
+struct Slot {
+  pqc_ctx* ctx = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  uint8_t *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr, *d_keys = nullptr;
+  pqc_kem_req *h_desc = nullptr, *d_desc = nullptr;  // Kyber encaps first, then decaps
+  uint32_t *h_kid = nullptr, *d_kid = nullptr;       // key index per request, in batch order
+  std::vector<Pending> reqs;
+  std::vector<size_t> out_a, out_b, len_a, len_b;  // where each request's output lies in h_out
+  size_t bytes = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct pqc_server {
+  pqc_server_config cfg{};
+  unsigned kyber_mask = 0;
+  // key store: per parameter set, keys_per_alg key pairs
+  uint8_t* d_pk = nullptr;  // all Kyber public keys, sets back to back
+  uint8_t* d_sk = nullptr;  // all Kyber secret keys
+  size_t pk_base[4] = {0}, sk_base[4] = {0}, pk_bytes = 0, sk_bytes = 0;
+  uint8_t* d_fsk = nullptr;  // Frodo secret keys
+  uint16_t* d_fa = nullptr;  // Frodo expanded A per key
+  std::vector<uint8_t> h_pk[6], h_sk[6];  // host copies (for clients and tests)
+  std::vector<Slot> slots;
+  size_t arena = 0, key_arena = 0;
+  // queues
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Pending> submitted;
+  std::deque<Pending> pending;
+  std::deque<pqc_server_completion> completions;
+  size_t inflight_bytes = 0;
+  uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_batched_reqs = 0;
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  clk::time_point t0;
+  int device = 0;
+
+  double now_us() const { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); }
+};
+
+namespace {
+
+void free_slot(Slot& sl) {
+  if (sl.ctx) pqc_ctx_destroy(sl.ctx);
+  if (sl.s) (void)hipStreamDestroy(sl.s);
+  if (sl.done) (void)hipEventDestroy(sl.done);
+  for (void* p : {(void*)sl.h_in, (void*)sl.h_out, (void*)sl.h_desc, (void*)sl.h_kid})
+    if (p) (void)hipHostFree(p);
+  for (void* p : {(void*)sl.d_in, (void*)sl.d_out, (void*)sl.d_desc, (void*)sl.d_kid, (void*)sl.d_keys})
+    if (p) (void)hipFree(p);
+}
+
+void free_server(pqc_server* v) {
+  for (auto& sl : v->slots) free_slot(sl);
+  for (void* p : {(void*)v->d_pk, (void*)v->d_sk, (void*)v->d_fsk, (void*)v->d_fa})
+    if (p) (void)hipFree(p);
+  delete v;
+}
+
+// Launch one slot's batch: H2D inputs, kernels, D2H outputs, event.
+int launch(pqc_server* v, Slot& sl) {
+  const bool plain = (v->cfg.flags & PQC_SERVER_PLAIN) != 0;
+  // block layout
+  size_t cnt[kGroups] = {0};
+  for (auto& p : sl.reqs) cnt[group_of(p.r.alg, p.r.op)]++;
+  size_t in_base[kGroups], out_base[kGroups], out2_base[kGroups], io = 0, oo = 0;
+  size_t grp_first[kGroups], pos = 0;
+  int alg_of_group[kGroups];
+  Set gs[kGroups];
+  for (int g = 0; g < kGroups; ++g) {
+    alg_of_group[g] = g < 6 ? PQC_ALG_KYBER512 + g % 3 : PQC_ALG_FRODO976;
+    set_of(alg_of_group[g], &gs[g]);
+    const int op = (g < 3 || g == 6) ? 1 : 2;
+    grp_first[g] = pos;
+    pos += cnt[g];
+    in_base[g] = io;
+    io += align8(cnt[g] * in_bytes(gs[g], op));
+    out_base[g] = oo;
+    if (op == 1) {  // ct block, then ss block
+      oo += align8(cnt[g] * gs[g].ct);
+      out2_base[g] = oo;
+      oo += align8(cnt[g] * gs[g].ss);
+    } else {
+      out2_base[g] = oo;
+      oo += align8(cnt[g] * gs[g].ss);
+    }
+  }
+  // place every request: inputs into h_in, output spans, key ids, descriptors
+  size_t fill[kGroups] = {0};
+  sl.out_a.assign(sl.reqs.size(), 0);
+  sl.out_b.assign(sl.reqs.size(), 0);
+  sl.len_a.assign(sl.reqs.size(), 0);
+  sl.len_b.assign(sl.reqs.size(), 0);
+  // descriptor slots: Kyber encaps (K = 2, 3, 4) first, then decaps
+  size_t desc_first[6], dpos = 0;
+  for (int g = 0; g < 6; ++g) {
+    desc_first[g] = dpos;
+    dpos += cnt[g];
+  }
+  const size_t n_enc = cnt[0] + cnt[1] + cnt[2];
+  for (size_t i = 0; i < sl.reqs.size(); ++i) {
+    const Pending& p = sl.reqs[i];
+    const int g = group_of(p.r.alg, p.r.op);
+    const size_t j = fill[g]++;
+    const Set& s = gs[g];
+    const size_t ib = in_bytes(s, p.r.op);
+    memcpy(sl.h_in + in_base[g] + j * ib, p.r.in, ib);
+    sl.h_kid[grp_first[g] + j] = p.r.key_id;
+    if (p.r.op == 1) {
+      sl.out_a[i] = out_base[g] + j * s.ct;
+      sl.len_a[i] = s.ct;
+      sl.out_b[i] = out2_base[g] + j * s.ss;
+      sl.len_b[i] = s.ss;
+    } else {
+      sl.out_a[i] = out2_base[g] + j * s.ss;
+      sl.len_a[i] = s.ss;
+    }
+    if (g < 6) {  // Kyber descriptor
+      const int kk = g % 3;
+      pqc_kem_req& d = sl.h_desc[desc_first[g] + j];
+      d.alg = p.r.alg;
+      d.reserved = 0;
+      d.key_off = g < 3 ? v->pk_base[kk] + (size_t)p.r.key_id * s.pk : v->sk_base[kk] + (size_t)p.r.key_id * s.sk;
+      if (g < 3) {
+        d.coins_off = in_base[g] + j * 32;
+        d.ct_off = out_base[g] + j * s.ct;
+        d.ss_off = out2_base[g] + j * s.ss;
+      } else {
+        d.coins_off = 0;
+        d.ct_off = in_base[g] + j * s.ct;
+        d.ss_off = out2_base[g] + j * s.ss;
+      }
+    }
+  }
+  hipStream_t st = sl.s;
+  const size_t n = sl.reqs.size();
+  const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
+  if (hipMemcpyAsync(sl.d_in, sl.h_in, io, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
+  if (hipMemcpyAsync(sl.d_kid, sl.h_kid, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
+  int rc = PQC_OK;
+  if (!plain) {
+    if (n_kyber &&
+        hipMemcpyAsync(sl.d_desc, sl.h_desc, n_kyber * sizeof(pqc_kem_req), hipMemcpyHostToDevice, st) != hipSuccess)
+      return PQC_ERR_DEVICE;
+    if (n_enc)
+      rc = pqc_kem_enc_mixed(sl.ctx, n_enc, sl.d_desc, sl.d_out, v->arena, v->d_pk, v->pk_bytes, sl.d_out, v->arena,
+                             sl.d_in, v->arena, st);
+    if (!rc && n_kyber > n_enc)
+      rc = pqc_kem_dec_mixed(sl.ctx, n_kyber - n_enc, sl.d_desc + n_enc, sl.d_in, v->arena, v->d_sk, v->sk_bytes,
+                             sl.d_out, v->arena, st);
+  } else {
+    for (int g = 0; g < 6 && !rc; ++g) {
+      if (!cnt[g]) continue;
+      const int kk = g % 3, K = kk + 2;
+      const Set& s = gs[g];
+      if (g < 3) {
+        rc = pqc::gather_records(cnt[g], sl.d_keys, s.pk, v->d_pk + v->pk_base[kk], s.pk, sl.d_kid + grp_first[g], st);
+        if (!rc)
+          rc = pqc::kyber_plain_enc(sl.ctx, K, cnt[g], sl.d_out + out_base[g], sl.d_out + out2_base[g], sl.d_keys,
+                                    sl.d_in + in_base[g], st);
+      } else {
+        rc = pqc::gather_records(cnt[g], sl.d_keys, s.sk, v->d_sk + v->sk_base[kk], s.sk, sl.d_kid + grp_first[g], st);
+        if (!rc)
+          rc = pqc::kyber_plain_dec(sl.ctx, K, cnt[g], sl.d_out + out2_base[g], sl.d_in + in_base[g], sl.d_keys, st);
+      }
+    }
+  }
+  // Frodo groups
+  if (!rc && cnt[6]) {
+    if (!plain) {
+      rc = pqc::frodo_enc_store(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6], sl.d_in + in_base[6],
+                                v->d_fsk, v->d_fa, sl.d_kid + grp_first[6], st);
+    } else {
+      // public keys gathered out of the secret keys (pk at byte 24 of sk); records
+      // start 8 bytes early to keep 8-byte alignment, so the pk sits at +8.  A is
+      // expanded per request.
+      rc = pqc::gather_records(cnt[6], sl.d_keys, 15632 + 8, v->d_fsk + 24 - 8, 31296, sl.d_kid + grp_first[6], st);
+      if (!rc)
+        rc = pqc::frodo_enc_batch_strided(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6],
+                                          sl.d_keys + 8, 15632 + 8, sl.d_in + in_base[6], st);
+    }
+  }
+  if (!rc && cnt[7]) {
+    if (!plain) {
+      rc = pqc::frodo_dec_store(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], v->d_fsk, v->d_fa,
+                                sl.d_kid + grp_first[7], st);
+    } else {
+      rc = pqc::gather_records(cnt[7], sl.d_keys, 31296, v->d_fsk, 31296, sl.d_kid + grp_first[7], st);
+      if (!rc) rc = pqc::frodo_dec_batch(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], sl.d_keys, st);
+    }
+  }
+  if (rc) return rc;
+  if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
+  if (hipEventRecord(sl.done, st) != hipSuccess) return PQC_ERR_DEVICE;
+  return PQC_OK;
+}
+
+size_t req_bytes(const Pending& p) { return in_bytes(p.set, p.r.op) + out_bytes(p.set, p.r.op); }
+
+// slot key-arena use of one request in plain mode (its gathered key)
+size_t key_bytes(const Pending& p) {
+  if (p.set.alg == PQC_ALG_FRODO976) return p.r.op == 1 ? 15632 + 8 : 31296;
+  return p.r.op == 1 ? p.set.pk : p.set.sk;
+}
+
+void worker_loop(pqc_server* v) {
+  (void)hipSetDevice(v->device);
+  const bool plain = (v->cfg.flags & PQC_SERVER_PLAIN) != 0;
+  std::unique_lock<std::mutex> lk(v->mu);
+  while (!v->stop) {
+    // 1. move submissions in
+    while (!v->submitted.empty()) {
+      v->pending.push_back(v->submitted.front());
+      v->submitted.pop_front();
+    }
+    bool progress = false;
+    // 2. retire finished slots
+    for (auto& sl : v->slots) {
+      if (!sl.busy) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      const int status = q == hipSuccess ? PQC_OK : PQC_ERR_DEVICE;
+      const double t = v->now_us();
+      for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        const Pending& p = sl.reqs[i];
+        if (status == PQC_OK) {
+          memcpy(p.r.out, sl.h_out + sl.out_a[i], sl.len_a[i]);
+          if (sl.len_b[i]) memcpy(p.r.out + sl.len_a[i], sl.h_out + sl.out_b[i], sl.len_b[i]);
+        }
+        v->completions.push_back(pqc_server_completion{p.r.tag, status, (uint32_t)sl.reqs.size(), p.t_submit, t});
+      }
+      v->n_completed += sl.reqs.size();
+      v->inflight_bytes -= sl.bytes;
+      sl.reqs.clear();
+      sl.bytes = 0;
+      sl.busy = false;
+      progress = true;
+      v->cv_done.notify_all();
+    }
+    // 3. launch into free slots
+    for (auto& sl : v->slots) {
+      if (sl.busy || v->pending.empty()) continue;
+      const double waited = v->now_us() - v->pending.front().t_submit;
+      bool any_busy = false;
+      for (auto& o : v->slots) any_busy |= o.busy;
+      if (any_busy && v->pending.size() < v->cfg.max_batch && waited < v->cfg.max_wait_us) break;
+      size_t in_sz = 0, out_sz = 0, key_sz = 0;
+      while (!v->pending.empty() && sl.reqs.size() < v->cfg.max_batch) {
+        const Pending& p = v->pending.front();
+        const size_t ib = align8(in_bytes(p.set, p.r.op)) + 8, ob = align8(out_bytes(p.set, p.r.op)) + 16;
+        const size_t rb = req_bytes(p);
+        if (in_sz + ib > v->arena || out_sz + ob > v->arena) break;
+        if (plain && key_sz + key_bytes(p) > v->key_arena) break;
+        if (v->cfg.hbm_budget && v->inflight_bytes + sl.bytes + rb > v->cfg.hbm_budget && !sl.reqs.empty()) break;
+        if (v->cfg.hbm_budget && v->inflight_bytes + rb > v->cfg.hbm_budget && any_busy) break;
+        in_sz += ib;
+        out_sz += ob;
+        key_sz += key_bytes(p);
+        sl.bytes += rb;
+        sl.reqs.push_back(p);
+        v->pending.pop_front();
+      }
+      if (sl.reqs.empty()) break;
+      v->inflight_bytes += sl.bytes;
+      lk.unlock();
+      const int rc = launch(v, sl);
+      lk.lock();
+      v->n_batches++;
+      v->n_batched_reqs += sl.reqs.size();
+      if (rc) {  // complete the batch with the error
+        const double t = v->now_us();
+        for (auto& p : sl.reqs)
+          v->completions.push_back(pqc_server_completion{p.r.tag, rc, (uint32_t)sl.reqs.size(), p.t_submit, t});
+        v->n_completed += sl.reqs.size();
+        v->inflight_bytes -= sl.bytes;
+        sl.reqs.clear();
+        sl.bytes = 0;
+        v->cv_done.notify_all();
+      } else {
+        sl.busy = true;
+      }
+      progress = true;
+    }
+    if (!progress) {
+      bool any_busy = false;
+      for (auto& o : v->slots) any_busy |= o.busy;
+      // poll the GPU every few microseconds while batches run, else sleep until work
+      if (any_busy || !v->pending.empty())
+        v->cv_work.wait_for(lk, std::chrono::microseconds(5));
+      else
+        v->cv_work.wait_for(lk, std::chrono::milliseconds(2));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
+  if (!out || !cfg) return PQC_ERR_INVALID_ARG;
+  *out = nullptr;
+  const unsigned kyb = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024);
+  if (!cfg->alg_mask || (cfg->alg_mask & ~(kyb | PQC_ALG_MASK(PQC_ALG_FRODO976))) || cfg->keys_per_alg == 0 ||
+      cfg->max_batch == 0 || cfg->slots == 0 || cfg->slots > 8 || cfg->arena_bytes < (1u << 20))
+    return PQC_ERR_INVALID_ARG;
+  auto* v = new (std::nothrow) pqc_server;
+  if (!v) return PQC_ERR_NO_MEMORY;
+  v->cfg = *cfg;
+  v->t0 = clk::now();
+  v->kyber_mask = cfg->alg_mask & kyb;
+  v->arena = cfg->arena_bytes;
+  v->key_arena = cfg->arena_bytes;
+  if (hipGetDevice(&v->device) != hipSuccess) {
+    delete v;
+    return PQC_ERR_NO_DEVICE;
+  }
+  int rc = PQC_OK;
+  const size_t K = cfg->keys_per_alg;
+  hipStream_t s0 = nullptr;
+  if (hipStreamCreate(&s0) != hipSuccess) {
+    delete v;
+    return PQC_ERR_DEVICE;
+  }
+  // ---- key store (key pair i of set a from coins DRBG(seed, 0x6B6579 + a, i))
+  {
+    unsigned gen_mask = cfg->alg_mask;
+    pqc_ctx* gctx = nullptr;
+    rc = pqc_ctx_create(&gctx, gen_mask, K, 0);
+    std::vector<uint8_t> coins(K * 64);
+    for (int a = PQC_ALG_KYBER512; !rc && a <= PQC_ALG_KYBER1024; ++a) {
+      Set s;
+      set_of(a, &s);
+      v->pk_base[a - 1] = v->pk_bytes;
+      v->sk_base[a - 1] = v->sk_bytes;
+      if (!(v->kyber_mask & PQC_ALG_MASK(a))) continue;
+      v->pk_bytes += K * s.pk;
+      v->sk_bytes += K * s.sk;
+    }
+    if (!rc && v->kyber_mask) {
+      if (hipMalloc(&v->d_pk, v->pk_bytes) != hipSuccess || hipMalloc(&v->d_sk, v->sk_bytes) != hipSuccess)
+        rc = PQC_ERR_NO_MEMORY;
+    }
+    for (int a = PQC_ALG_KYBER512; !rc && a <= PQC_ALG_FRODO976; ++a) {
+      if (a == PQC_ALG_NEWHOPE1024_CPA || !(cfg->alg_mask & PQC_ALG_MASK(a))) continue;
+      Set s;
+      set_of(a, &s);
+      for (size_t i = 0; i < K; ++i) pqc::host_drbg(&coins[i * 64], 64, cfg->key_seed, 0x6B6579 + a, i);
+      v->h_pk[a].resize(K * s.pk);
+      v->h_sk[a].resize(K * s.sk);
+      if (a == PQC_ALG_FRODO976) {
+        if (hipMalloc(&v->d_fsk, K * s.sk) != hipSuccess || hipMalloc(&v->d_fa, K * 976 * 976 * 2) != hipSuccess) {
+          rc = PQC_ERR_NO_MEMORY;
+          break;
+        }
+        rc = crypto_kem_keypair_derand_batch(gctx, a, K, v->h_pk[a].data(), v->h_sk[a].data(), coins.data(),
+                                             PQC_F_HOST_PTRS, s0);
+        if (!rc && hipMemcpy(v->d_fsk, v->h_sk[a].data(), K * s.sk, hipMemcpyHostToDevice) != hipSuccess)
+          rc = PQC_ERR_DEVICE;
+        if (!rc) rc = pqc::frodo_expand_a(K, v->d_fa, v->d_fsk, s0);
+      } else {
+        rc = crypto_kem_keypair_derand_batch(gctx, a, K, v->h_pk[a].data(), v->h_sk[a].data(), coins.data(),
+                                             PQC_F_HOST_PTRS, s0);
+        if (!rc && (hipMemcpy(v->d_pk + v->pk_base[a - 1], v->h_pk[a].data(), K * s.pk, hipMemcpyHostToDevice) !=
+                        hipSuccess ||
+                    hipMemcpy(v->d_sk + v->sk_base[a - 1], v->h_sk[a].data(), K * s.sk, hipMemcpyHostToDevice) !=
+                        hipSuccess))
+          rc = PQC_ERR_DEVICE;
+      }
+    }
+    if (!rc && hipStreamSynchronize(s0) != hipSuccess) rc = PQC_ERR_DEVICE;
+    if (gctx) pqc_ctx_destroy(gctx);
+  }
+  (void)hipStreamDestroy(s0);
+  // ---- slots
+  v->slots.resize(cfg->slots);
+  for (auto& sl : v->slots) {
+    if (rc) break;
+    rc = pqc_ctx_create(&sl.ctx, cfg->alg_mask, cfg->max_batch, 0);
+    if (rc) break;
+    if (hipStreamCreate(&sl.s) != hipSuccess || hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+      rc = PQC_ERR_DEVICE;
+    const size_t nb = cfg->max_batch;
+    if (!rc && (hipHostMalloc(&sl.h_in, v->arena) != hipSuccess || hipHostMalloc(&sl.h_out, v->arena) != hipSuccess ||
+                hipHostMalloc(&sl.h_desc, nb * sizeof(pqc_kem_req)) != hipSuccess ||
+                hipHostMalloc(&sl.h_kid, nb * 4) != hipSuccess))
+      rc = PQC_ERR_NO_MEMORY;
+    if (!rc && (hipMalloc(&sl.d_in, v->arena) != hipSuccess || hipMalloc(&sl.d_out, v->arena) != hipSuccess ||
+                hipMalloc(&sl.d_desc, nb * sizeof(pqc_kem_req)) != hipSuccess ||
+                hipMalloc(&sl.d_kid, nb * 4) != hipSuccess))
+      rc = PQC_ERR_NO_MEMORY;
+    if (!rc && (cfg->flags & PQC_SERVER_PLAIN) && hipMalloc(&sl.d_keys, v->key_arena) != hipSuccess)
+      rc = PQC_ERR_NO_MEMORY;
+    if (!rc) (void)pqc_ctx_put_tuning(sl.ctx, &cfg->tuning);  // an all-zero tuning is rejected: defaults stay
+  }
+  if (rc) {
+    free_server(v);
+    return rc;
+  }
+  v->worker = std::thread(worker_loop, v);
+  *out = v;
+  return PQC_OK;
+}
+
+void pqc_server_destroy(pqc_server* v) {
+  if (!v) return;
+  {
+    std::lock_guard<std::mutex> lk(v->mu);
+    v->stop = true;
+  }
+  v->cv_work.notify_all();
+  if (v->worker.joinable()) v->worker.join();
+  (void)hipDeviceSynchronize();
+  free_server(v);
+}
+
+int pqc_server_key(pqc_server* v, int alg, uint32_t key_id, uint8_t* pk, uint8_t* sk) {
+  Set s;
+  if (!v || !set_of(alg, &s) || !(v->cfg.alg_mask & PQC_ALG_MASK(alg))) return PQC_ERR_INVALID_ARG;
+  if (key_id >= v->cfg.keys_per_alg) return PQC_ERR_INVALID_ARG;
+  if (pk) memcpy(pk, v->h_pk[alg].data() + (size_t)key_id * s.pk, s.pk);
+  if (sk) memcpy(sk, v->h_sk[alg].data() + (size_t)key_id * s.sk, s.sk);
+  return PQC_OK;
+}
+
+int pqc_server_submit(pqc_server* v, const pqc_server_request* reqs, size_t n) {
+  if (!v || (!reqs && n)) return PQC_ERR_INVALID_ARG;
+  const double t = v->now_us();
+  for (size_t i = 0; i < n; ++i) {
+    Set s;
+    const pqc_server_request& r = reqs[i];
+    if (!set_of(r.alg, &s) || !(v->cfg.alg_mask & PQC_ALG_MASK(r.alg)) || (r.op != 1 && r.op != 2) ||
+        r.key_id >= v->cfg.keys_per_alg || !r.in || !r.out)
+      return PQC_ERR_INVALID_ARG;
+    if (v->cfg.hbm_budget && in_bytes(s, r.op) + out_bytes(s, r.op) > v->cfg.hbm_budget) return PQC_ERR_NO_MEMORY;
+  }
+  {
+    std::lock_guard<std::mutex> lk(v->mu);
+    for (size_t i = 0; i < n; ++i) {
+      Pending p{reqs[i], {}, t};
+      set_of(reqs[i].alg, &p.set);
+      v->submitted.push_back(p);
+    }
+    v->n_submitted += n;
+  }
+  v->cv_work.notify_one();
+  return PQC_OK;
+}
+
+int pqc_server_poll(pqc_server* v, pqc_server_completion* out, size_t cap, size_t* got, uint32_t wait_us) {
+  if (!v || !got || (!out && cap)) return PQC_ERR_INVALID_ARG;
+  std::unique_lock<std::mutex> lk(v->mu);
+  if (v->completions.empty() && wait_us)
+    v->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !v->completions.empty(); });
+  size_t k = 0;
+  while (k < cap && !v->completions.empty()) {
+    out[k++] = v->completions.front();
+    v->completions.pop_front();
+  }
+  *got = k;
+  return PQC_OK;
+}
+
+int pqc_server_drain(pqc_server* v) {
+  if (!v) return PQC_ERR_INVALID_ARG;
+  std::unique_lock<std::mutex> lk(v->mu);
+  // completions keep coming while the GPU works; 30 s without one is a failure
+  uint64_t seen = v->n_completed;
+  while (v->n_completed != v->n_submitted) {
+    if (!v->cv_done.wait_for(lk, std::chrono::seconds(30), [&] { return v->n_completed != seen; }))
+      return PQC_ERR_DEVICE;
+    seen = v->n_completed;
+  }
+  return PQC_OK;
+}
+
+int pqc_server_get_stats(pqc_server* v, pqc_server_stats* st) {
+  if (!v || !st) return PQC_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(v->mu);
+  st->submitted = v->n_submitted;
+  st->completed = v->n_completed;
+  st->batches = v->n_batches;
+  st->mean_batch = v->n_batches ? (double)v->n_batched_reqs / (double)v->n_batches : 0.0;
+  st->inflight_bytes = v->inflight_bytes;
+  return PQC_OK;
+}
+
+double pqc_server_now_us(pqc_server* v) { return v ? v->now_us() : 0.0; }
+
+// Replay an arrival trace from the calling thread: request i is submitted at
+// arrival_us[i] (server clock, offset to "now" at the call), completions are
+// drained as they come.  latency_us[i] (optional) = completion - arrival.
+int pqc_server_run_trace(pqc_server* v, size_t n, const double* arrival_us, const pqc_server_request* reqs,
+                         double* latency_us, pqc_trace_stats* out) {
+  if (!v || !out || (n && (!arrival_us || !reqs))) return PQC_ERR_INVALID_ARG;
+  std::vector<double> lat(n, 0.0);
+  std::vector<pqc_server_request> rq(reqs, reqs + n);
+  for (size_t i = 0; i < n; ++i) rq[i].tag = i;  // the trace index
+  const double base = v->now_us() + 50.0;
+  size_t next = 0, done = 0;
+  uint64_t errors = 0;
+  std::vector<pqc_server_completion> comp(4096);
+  double last = base, progress = v->now_us();
+  while (done < n) {
+    if (v->now_us() - progress > 30e6) return PQC_ERR_DEVICE;  // 30 s without a completion
+    const double now = v->now_us();
+    size_t end = next;
+    while (end < n && base + arrival_us[end] <= now) ++end;
+    if (end > next) {
+      // one submission for everything that has arrived, each stamped now
+      int rc = pqc_server_submit(v, rq.data() + next, end - next);
+      if (rc) return rc;
+      next = end;
+    }
+    size_t got = 0;
+    pqc_server_poll(v, comp.data(), comp.size(), &got, next < n ? 0 : 200);
+    if (got) progress = v->now_us();
+    for (size_t k = 0; k < got; ++k) {
+      const size_t i = (size_t)comp[k].tag;
+      lat[i] = comp[k].done_us - (base + arrival_us[i]);
+      errors += comp[k].status != PQC_OK;
+      last = std::max(last, comp[k].done_us);
+    }
+    done += got;
+    if (!got && next < n) {
+      // spin toward the next arrival without burning the scheduler's core
+      const double wait = base + arrival_us[next] - v->now_us();
+      if (wait > 20) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait - 10)));
+    }
+  }
+  std::vector<double> sorted = lat;
+  std::sort(sorted.begin(), sorted.end());
+  auto pct = [&](double q) { return n ? sorted[std::min(n - 1, (size_t)(q * (double)(n - 1) + 0.5))] : 0.0; };
+  out->completed = n;
+  out->errors = errors;
+  out->duration_s = (last - base) / 1e6;
+  out->ops_per_s = out->duration_s > 0 ? (double)n / out->duration_s : 0.0;
+  out->p50_us = pct(0.50);
+  out->p90_us = pct(0.90);
+  out->p99_us = pct(0.99);
+  out->p999_us = pct(0.999);
+  out->max_us = n ? sorted.back() : 0.0;
+  pqc_server_stats st;
+  pqc_server_get_stats(v, &st);
+  out->mean_batch = st.mean_batch;
+  if (latency_us) std::copy(lat.begin(), lat.end(), latency_us);
+  return PQC_OK;
+}
+
+}  // extern "C"
