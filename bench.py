@@ -114,7 +114,10 @@ def main():
                     help="time one path only (profiling runs)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["kyber768", "frodo976", "newhope1024", "kyber_mixed"], default="kyber768")
+    ap.add_argument("--workload", choices=["kyber768", "frodo976", "newhope1024", "kyber_mixed", "serving"],
+                    default="kyber768")
+    ap.add_argument("--rate", type=float, default=2e6, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--duration", type=float, default=0.25, help="serving: trace length, seconds")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards by request with no collective, "
@@ -126,6 +129,8 @@ def main():
         return bench_newhope(args)
     if args.workload == "kyber_mixed":
         return bench_mixed(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import pqc_amd as pqc
     n = args.batch
@@ -454,6 +459,66 @@ def bench_mixed(args):
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+# ---- serving loop: Poisson arrivals, mixed Kyber / Frodo, 1024 keys per set ------------
+def serving_trace(n, rate, seed, keys):
+    """Arrival times (us), algorithm ids (90 % Kyber at 512:768:1024 = 1:2:1,
+    10 % FrodoKEM-976), operations (encaps / decaps 50 / 50, an assumption), keys."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1e6 / rate, size=n))
+    kyber = rng.choice(np.array([1, 2, 2, 3], np.int32), size=n)
+    algs = np.where(rng.random(n) < 0.9, kyber, 5).astype(np.int32)
+    ops = rng.choice(np.array([1, 2], np.int32), size=n)
+    kid = rng.integers(0, keys, size=n).astype(np.uint32)
+    return arrivals, algs, ops, kid, rng
+
+
+def bench_serving(args):
+    from pqc_amd.server import RequestSet, Server
+    n = int(args.rate * args.duration)
+    keys = 1024
+    arrivals, algs, ops, kid, rng = serving_trace(n, args.rate, 0, keys)
+    algs_names = ["kyber512", "kyber768", "kyber1024", "frodo976"]
+    results = {}
+    for plain in (False, True):
+        # the plain path cannot hold the offered rate: replay a prefix of the trace
+        m = n if not plain else min(n, int(0.2 * n))
+        t0 = time.perf_counter()
+        srv = Server(algs_names, keys_per_alg=keys, key_seed=0, max_batch=16384, slots=6,
+                     arena_bytes=96 << 20, max_wait_us=200, plain=plain)
+        log("server (plain=%s) up in %.1f s: %d key pairs per set" % (plain, time.perf_counter() - t0, keys))
+        rs = RequestSet.random(algs[:m], ops[:m], kid[:m], rng)
+        st, lat = srv.run_trace(arrivals[:m], rs.reqs)
+        results[plain] = (st, m)
+        log("plain=%s: %d requests, %.0f ops/s, p50 %.0f us, p99 %.0f us, mean batch %.0f, errors %d" %
+            (plain, m, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch, st.errors))
+        srv.close()
+    st, m = results[False]
+    sp, mp = results[True]
+    out = {
+        "metric": "KEM serving loop at Poisson 2M req/s (90% Kyber 1:2:1 / 10% Frodo-976, 1024 keys per set): "
+                  "ops/s and p50/p99 latency, fused vs plain",
+        "value": round(st.ops_per_s, 1),
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(st.duration_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16 / uint16",
+        "data": "synthetic: seeded Poisson trace, random coins and ciphertexts, seeded key pairs",
+        "config": {"workload": "serving_poisson", "offered_rate": args.rate, "requests": m,
+                   "p50_us": round(st.p50_us, 1), "p90_us": round(st.p90_us, 1), "p99_us": round(st.p99_us, 1),
+                   "p999_us": round(st.p999_us, 1), "mean_batch": round(st.mean_batch, 1), "errors": int(st.errors),
+                   "plain": {"requests": mp, "ops_per_s": round(sp.ops_per_s, 1), "p50_us": round(sp.p50_us, 1),
+                             "p99_us": round(sp.p99_us, 1), "mean_batch": round(sp.mean_batch, 1)}},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

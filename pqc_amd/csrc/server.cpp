@@ -64,13 +64,15 @@ struct Pending {
 
 struct Slot {
   pqc_ctx* ctx = nullptr;
-  hipStream_t s = nullptr;
-  hipEvent_t done = nullptr;
+  hipStream_t s = nullptr;   // transfers and the Kyber kernels
+  hipStream_t s2 = nullptr;  // the Frodo kernels (fused mode), overlapping the Kyber ones
+  hipEvent_t done = nullptr, in_ready = nullptr, frodo_done = nullptr;
   uint8_t *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr, *d_keys = nullptr;
   pqc_kem_req *h_desc = nullptr, *d_desc = nullptr;  // Kyber encaps first, then decaps
   uint32_t *h_kid = nullptr, *d_kid = nullptr;       // key index per request, in batch order
   std::vector<Pending> reqs;
   std::vector<size_t> out_a, out_b, len_a, len_b;  // where each request's output lies in h_out
+  std::vector<size_t> in_at;                       // where each request's input goes in h_in
   size_t bytes = 0;
   bool busy = false;
 };
@@ -107,10 +109,34 @@ struct pqc_server {
 
 namespace {
 
+// Run fn(begin, end) over [0, n) on up to `threads` threads (the caller is one of
+// them); small ranges stay on the caller.
+template <typename F>
+void parallel_for(size_t n, int threads, F fn) {
+  const size_t per = 2048;
+  int t = (int)std::min<size_t>((size_t)threads, (n + per - 1) / per);
+  if (t <= 1) {
+    fn((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const size_t chunk = (n + t - 1) / t;
+  for (int i = 1; i < t; ++i) {
+    const size_t b = i * chunk, e = std::min(n, b + chunk);
+    if (b < e) pool.emplace_back(fn, b, e);
+  }
+  fn((size_t)0, std::min(n, chunk));
+  for (auto& th : pool) th.join();
+}
+
+constexpr int kCopyThreads = 8;  // host threads that pack / unpack a batch
+
 void free_slot(Slot& sl) {
   if (sl.ctx) pqc_ctx_destroy(sl.ctx);
   if (sl.s) (void)hipStreamDestroy(sl.s);
-  if (sl.done) (void)hipEventDestroy(sl.done);
+  if (sl.s2) (void)hipStreamDestroy(sl.s2);
+  for (hipEvent_t e : {sl.done, sl.in_ready, sl.frodo_done})
+    if (e) (void)hipEventDestroy(e);
   for (void* p : {(void*)sl.h_in, (void*)sl.h_out, (void*)sl.h_desc, (void*)sl.h_kid})
     if (p) (void)hipHostFree(p);
   for (void* p : {(void*)sl.d_in, (void*)sl.d_out, (void*)sl.d_desc, (void*)sl.d_kid, (void*)sl.d_keys})
@@ -165,13 +191,14 @@ int launch(pqc_server* v, Slot& sl) {
     dpos += cnt[g];
   }
   const size_t n_enc = cnt[0] + cnt[1] + cnt[2];
+  sl.in_at.assign(sl.reqs.size(), 0);
   for (size_t i = 0; i < sl.reqs.size(); ++i) {
     const Pending& p = sl.reqs[i];
     const int g = group_of(p.r.alg, p.r.op);
     const size_t j = fill[g]++;
     const Set& s = gs[g];
     const size_t ib = in_bytes(s, p.r.op);
-    memcpy(sl.h_in + in_base[g] + j * ib, p.r.in, ib);
+    sl.in_at[i] = in_base[g] + j * ib;
     sl.h_kid[grp_first[g] + j] = p.r.key_id;
     if (p.r.op == 1) {
       sl.out_a[i] = out_base[g] + j * s.ct;
@@ -199,11 +226,22 @@ int launch(pqc_server* v, Slot& sl) {
       }
     }
   }
+  // the inputs themselves, copied by several threads
+  parallel_for(sl.reqs.size(), kCopyThreads, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) memcpy(sl.h_in + sl.in_at[i], sl.reqs[i].r.in, in_bytes(sl.reqs[i].set, sl.reqs[i].r.op));
+  });
   hipStream_t st = sl.s;
   const size_t n = sl.reqs.size();
   const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
   if (hipMemcpyAsync(sl.d_in, sl.h_in, io, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
   if (hipMemcpyAsync(sl.d_kid, sl.h_kid, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
+  // fused mode: the Frodo groups run on the second stream once the inputs are in
+  hipStream_t sf = plain ? st : sl.s2;
+  const bool frodo = cnt[6] || cnt[7];
+  if (!plain && frodo) {
+    if (hipEventRecord(sl.in_ready, st) != hipSuccess || hipStreamWaitEvent(sf, sl.in_ready, 0) != hipSuccess)
+      return PQC_ERR_DEVICE;
+  }
   int rc = PQC_OK;
   if (!plain) {
     if (n_kyber &&
@@ -236,7 +274,7 @@ int launch(pqc_server* v, Slot& sl) {
   if (!rc && cnt[6]) {
     if (!plain) {
       rc = pqc::frodo_enc_store(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6], sl.d_in + in_base[6],
-                                v->d_fsk, v->d_fa, sl.d_kid + grp_first[6], st);
+                                v->d_fsk, v->d_fa, sl.d_kid + grp_first[6], sf);
     } else {
       // public keys gathered out of the secret keys (pk at byte 24 of sk); records
       // start 8 bytes early to keep 8-byte alignment, so the pk sits at +8.  A is
@@ -250,11 +288,15 @@ int launch(pqc_server* v, Slot& sl) {
   if (!rc && cnt[7]) {
     if (!plain) {
       rc = pqc::frodo_dec_store(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], v->d_fsk, v->d_fa,
-                                sl.d_kid + grp_first[7], st);
+                                sl.d_kid + grp_first[7], sf);
     } else {
       rc = pqc::gather_records(cnt[7], sl.d_keys, 31296, v->d_fsk, 31296, sl.d_kid + grp_first[7], st);
       if (!rc) rc = pqc::frodo_dec_batch(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], sl.d_keys, st);
     }
+  }
+  if (!rc && !plain && frodo) {
+    if (hipEventRecord(sl.frodo_done, sf) != hipSuccess || hipStreamWaitEvent(st, sl.frodo_done, 0) != hipSuccess)
+      rc = PQC_ERR_DEVICE;
   }
   if (rc) return rc;
   if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
@@ -287,15 +329,20 @@ void worker_loop(pqc_server* v) {
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
       const int status = q == hipSuccess ? PQC_OK : PQC_ERR_DEVICE;
-      const double t = v->now_us();
-      for (size_t i = 0; i < sl.reqs.size(); ++i) {
-        const Pending& p = sl.reqs[i];
-        if (status == PQC_OK) {
-          memcpy(p.r.out, sl.h_out + sl.out_a[i], sl.len_a[i]);
-          if (sl.len_b[i]) memcpy(p.r.out + sl.len_a[i], sl.h_out + sl.out_b[i], sl.len_b[i]);
-        }
-        v->completions.push_back(pqc_server_completion{p.r.tag, status, (uint32_t)sl.reqs.size(), p.t_submit, t});
+      if (status == PQC_OK) {  // the outputs to the callers' buffers, without holding the lock
+        lk.unlock();
+        parallel_for(sl.reqs.size(), kCopyThreads, [&](size_t b, size_t e) {
+          for (size_t i = b; i < e; ++i) {
+            const Pending& p = sl.reqs[i];
+            memcpy(p.r.out, sl.h_out + sl.out_a[i], sl.len_a[i]);
+            if (sl.len_b[i]) memcpy(p.r.out + sl.len_a[i], sl.h_out + sl.out_b[i], sl.len_b[i]);
+          }
+        });
+        lk.lock();
       }
+      const double t = v->now_us();
+      for (const Pending& p : sl.reqs)
+        v->completions.push_back(pqc_server_completion{p.r.tag, status, (uint32_t)sl.reqs.size(), p.t_submit, t});
       v->n_completed += sl.reqs.size();
       v->inflight_bytes -= sl.bytes;
       sl.reqs.clear();
@@ -445,7 +492,10 @@ int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
     if (rc) break;
     rc = pqc_ctx_create(&sl.ctx, cfg->alg_mask, cfg->max_batch, 0);
     if (rc) break;
-    if (hipStreamCreate(&sl.s) != hipSuccess || hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreate(&sl.s) != hipSuccess || hipStreamCreate(&sl.s2) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.in_ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.frodo_done, hipEventDisableTiming) != hipSuccess)
       rc = PQC_ERR_DEVICE;
     const size_t nb = cfg->max_batch;
     if (!rc && (hipHostMalloc(&sl.h_in, v->arena) != hipSuccess || hipHostMalloc(&sl.h_out, v->arena) != hipSuccess ||

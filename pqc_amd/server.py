@@ -145,5 +145,37 @@ class RequestSet:
         self.reqs["out"] = self.outbuf.ctypes.data + self.out_off[:-1]
         self.reqs["tag"] = np.arange(n)
 
+    @classmethod
+    def random(cls, algs, ops, key_ids, rng):
+        """Requests with random inputs (coins, or ciphertexts: decapsulating any
+        bytes does the full work, implicit rejection included), built without a
+        per-request Python loop."""
+        self = cls.__new__(cls)
+        algs = np.asarray(algs, np.int32)
+        ops = np.asarray(ops, np.int32)
+        n = len(algs)
+        self.n = n
+        ct = np.zeros(8, np.int64)
+        ss = np.zeros(8, np.int64)
+        for a in (L.ALG_KYBER512, L.ALG_KYBER768, L.ALG_KYBER1024, L.ALG_FRODO976):
+            _, _, ct[a], ss[a] = sizes(a)
+        coins = np.where(algs == L.ALG_FRODO976, 24, 32)
+        isz = np.where(ops == 1, coins, ct[algs])
+        osz = np.where(ops == 1, ct[algs] + ss[algs], ss[algs])
+        self.in_off = np.zeros(n + 1, np.int64)
+        self.out_off = np.zeros(n + 1, np.int64)
+        self.in_off[1:] = np.cumsum(isz)
+        self.out_off[1:] = np.cumsum(osz)
+        self.inbuf = rng.integers(0, 256, int(self.in_off[-1]) + 8, dtype=np.uint8)
+        self.outbuf = np.zeros(int(self.out_off[-1]) + 8, np.uint8)
+        self.reqs = np.zeros(n, REQ_DTYPE)
+        self.reqs["alg"] = algs
+        self.reqs["op"] = ops
+        self.reqs["key_id"] = key_ids
+        self.reqs["in"] = self.inbuf.ctypes.data + self.in_off[:-1]
+        self.reqs["out"] = self.outbuf.ctypes.data + self.out_off[:-1]
+        self.reqs["tag"] = np.arange(n)
+        return self
+
     def out(self, i):
         return self.outbuf[self.out_off[i]:self.out_off[i + 1]].tobytes()
