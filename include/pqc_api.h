@@ -211,7 +211,10 @@ typedef struct pqc_server_config {
   uint32_t keys_per_alg; /* device-resident key pairs per parameter set */
   uint64_t key_seed;
   uint32_t max_batch;    /* requests per batch */
-  uint32_t slots;        /* batches in flight, 1..8 */
+  uint32_t slots;        /* batches in flight, 1..8; a fused slot drives 3 streams (transfers +
+                          * Kyber, Frodo encaps, Frodo decaps): streams beyond the process's
+                          * hardware queues (HIP's GPU_MAX_HW_QUEUES, default 4) share queues
+                          * and serialise, so keep 3 * slots within it */
   uint64_t hbm_budget;   /* bytes of in-flight request inputs + outputs, 0 = no limit */
   uint64_t arena_bytes;  /* per-slot input and output arena (pinned + device), >= 1 MiB */
   uint32_t max_wait_us;  /* while other batches run, a batch waits at most this long to fill */
@@ -238,6 +241,10 @@ typedef struct pqc_server_stats {
   uint64_t submitted, completed, batches;
   double mean_batch;
   uint64_t inflight_bytes;
+  /* per batch, means over the batches so far: host time packing inputs and
+   * launching, device time from the first copy to the last, host time handing
+   * the outputs back */
+  double pack_us, device_us, retire_us;
 } pqc_server_stats;
 typedef struct pqc_trace_stats {
   uint64_t completed, errors;

@@ -64,9 +64,12 @@ struct Pending {
 
 struct Slot {
   pqc_ctx* ctx = nullptr;
-  hipStream_t s = nullptr;   // transfers and the Kyber kernels
-  hipStream_t s2 = nullptr;  // the Frodo kernels (fused mode), overlapping the Kyber ones
-  hipEvent_t done = nullptr, in_ready = nullptr, frodo_done = nullptr;
+  pqc_ctx* ctx_fd = nullptr;  // fused mode: Frodo decapsulation, with scratch of its own
+  hipStream_t s = nullptr;    // transfers and the Kyber kernels
+  hipStream_t s2 = nullptr;   // fused mode: Frodo encapsulation, overlapping the Kyber kernels
+  hipStream_t s3 = nullptr;   // fused mode: Frodo decapsulation, overlapping both
+  hipEvent_t done = nullptr, in_ready = nullptr, frodo_done = nullptr, fd_done = nullptr;
+  hipEvent_t start = nullptr;  // timing pair with `done`
   uint8_t *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr, *d_keys = nullptr;
   pqc_kem_req *h_desc = nullptr, *d_desc = nullptr;  // Kyber encaps first, then decaps
   uint32_t *h_kid = nullptr, *d_kid = nullptr;       // key index per request, in batch order
@@ -99,6 +102,7 @@ struct pqc_server {
   std::deque<pqc_server_completion> completions;
   size_t inflight_bytes = 0;
   uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_batched_reqs = 0;
+  double pack_us = 0, device_us = 0, retire_us = 0;  // totals over the batches
   std::atomic<bool> stop{false};
   std::thread worker;
   clk::time_point t0;
@@ -133,9 +137,10 @@ constexpr int kCopyThreads = 8;  // host threads that pack / unpack a batch
 
 void free_slot(Slot& sl) {
   if (sl.ctx) pqc_ctx_destroy(sl.ctx);
-  if (sl.s) (void)hipStreamDestroy(sl.s);
-  if (sl.s2) (void)hipStreamDestroy(sl.s2);
-  for (hipEvent_t e : {sl.done, sl.in_ready, sl.frodo_done})
+  if (sl.ctx_fd) pqc_ctx_destroy(sl.ctx_fd);
+  for (hipStream_t t : {sl.s, sl.s2, sl.s3})
+    if (t) (void)hipStreamDestroy(t);
+  for (hipEvent_t e : {sl.done, sl.in_ready, sl.frodo_done, sl.fd_done, sl.start})
     if (e) (void)hipEventDestroy(e);
   for (void* p : {(void*)sl.h_in, (void*)sl.h_out, (void*)sl.h_desc, (void*)sl.h_kid})
     if (p) (void)hipHostFree(p);
@@ -233,14 +238,16 @@ int launch(pqc_server* v, Slot& sl) {
   hipStream_t st = sl.s;
   const size_t n = sl.reqs.size();
   const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
+  if (hipEventRecord(sl.start, st) != hipSuccess) return PQC_ERR_DEVICE;
   if (hipMemcpyAsync(sl.d_in, sl.h_in, io, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
   if (hipMemcpyAsync(sl.d_kid, sl.h_kid, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
-  // fused mode: the Frodo groups run on the second stream once the inputs are in
-  hipStream_t sf = plain ? st : sl.s2;
-  const bool frodo = cnt[6] || cnt[7];
-  if (!plain && frodo) {
-    if (hipEventRecord(sl.in_ready, st) != hipSuccess || hipStreamWaitEvent(sf, sl.in_ready, 0) != hipSuccess)
-      return PQC_ERR_DEVICE;
+  // fused mode: Frodo encapsulation and decapsulation each run on a stream of their
+  // own once the inputs are in
+  hipStream_t sf = plain ? st : sl.s2, sd = plain ? st : sl.s3;
+  if (!plain && (cnt[6] || cnt[7])) {
+    if (hipEventRecord(sl.in_ready, st) != hipSuccess) return PQC_ERR_DEVICE;
+    if (cnt[6] && hipStreamWaitEvent(sf, sl.in_ready, 0) != hipSuccess) return PQC_ERR_DEVICE;
+    if (cnt[7] && hipStreamWaitEvent(sd, sl.in_ready, 0) != hipSuccess) return PQC_ERR_DEVICE;
   }
   int rc = PQC_OK;
   if (!plain) {
@@ -287,17 +294,19 @@ int launch(pqc_server* v, Slot& sl) {
   }
   if (!rc && cnt[7]) {
     if (!plain) {
-      rc = pqc::frodo_dec_store(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], v->d_fsk, v->d_fa,
-                                sl.d_kid + grp_first[7], sf);
+      rc = pqc::frodo_dec_store(sl.ctx_fd, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], v->d_fsk,
+                                v->d_fa, sl.d_kid + grp_first[7], sd);
     } else {
       rc = pqc::gather_records(cnt[7], sl.d_keys, 31296, v->d_fsk, 31296, sl.d_kid + grp_first[7], st);
       if (!rc) rc = pqc::frodo_dec_batch(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], sl.d_keys, st);
     }
   }
-  if (!rc && !plain && frodo) {
-    if (hipEventRecord(sl.frodo_done, sf) != hipSuccess || hipStreamWaitEvent(st, sl.frodo_done, 0) != hipSuccess)
-      rc = PQC_ERR_DEVICE;
-  }
+  if (!rc && !plain && cnt[6] &&
+      (hipEventRecord(sl.frodo_done, sf) != hipSuccess || hipStreamWaitEvent(st, sl.frodo_done, 0) != hipSuccess))
+    rc = PQC_ERR_DEVICE;
+  if (!rc && !plain && cnt[7] &&
+      (hipEventRecord(sl.fd_done, sd) != hipSuccess || hipStreamWaitEvent(st, sl.fd_done, 0) != hipSuccess))
+    rc = PQC_ERR_DEVICE;
   if (rc) return rc;
   if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
   if (hipEventRecord(sl.done, st) != hipSuccess) return PQC_ERR_DEVICE;
@@ -330,6 +339,9 @@ void worker_loop(pqc_server* v) {
       if (q == hipErrorNotReady) continue;
       const int status = q == hipSuccess ? PQC_OK : PQC_ERR_DEVICE;
       if (status == PQC_OK) {  // the outputs to the callers' buffers, without holding the lock
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, sl.start, sl.done) == hipSuccess) v->device_us += 1e3 * ms;
+        const double r0 = v->now_us();
         lk.unlock();
         parallel_for(sl.reqs.size(), kCopyThreads, [&](size_t b, size_t e) {
           for (size_t i = b; i < e; ++i) {
@@ -339,6 +351,7 @@ void worker_loop(pqc_server* v) {
           }
         });
         lk.lock();
+        v->retire_us += v->now_us() - r0;
       }
       const double t = v->now_us();
       for (const Pending& p : sl.reqs)
@@ -377,8 +390,11 @@ void worker_loop(pqc_server* v) {
       if (sl.reqs.empty()) break;
       v->inflight_bytes += sl.bytes;
       lk.unlock();
+      const double l0 = v->now_us();
       const int rc = launch(v, sl);
+      const double l1 = v->now_us();
       lk.lock();
+      v->pack_us += l1 - l0;
       v->n_batches++;
       v->n_batched_reqs += sl.reqs.size();
       if (rc) {  // complete the batch with the error
@@ -491,11 +507,15 @@ int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
   for (auto& sl : v->slots) {
     if (rc) break;
     rc = pqc_ctx_create(&sl.ctx, cfg->alg_mask, cfg->max_batch, 0);
+    if (!rc && !(cfg->flags & PQC_SERVER_PLAIN) && (cfg->alg_mask & PQC_ALG_MASK(PQC_ALG_FRODO976)))
+      rc = pqc_ctx_create(&sl.ctx_fd, PQC_ALG_MASK(PQC_ALG_FRODO976), cfg->max_batch, 0);
     if (rc) break;
     if (hipStreamCreate(&sl.s) != hipSuccess || hipStreamCreate(&sl.s2) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreate(&sl.s3) != hipSuccess || hipEventCreate(&sl.done) != hipSuccess ||
+        hipEventCreate(&sl.start) != hipSuccess ||
         hipEventCreateWithFlags(&sl.in_ready, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.frodo_done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&sl.frodo_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.fd_done, hipEventDisableTiming) != hipSuccess)
       rc = PQC_ERR_DEVICE;
     const size_t nb = cfg->max_batch;
     if (!rc && (hipHostMalloc(&sl.h_in, v->arena) != hipSuccess || hipHostMalloc(&sl.h_out, v->arena) != hipSuccess ||
@@ -509,6 +529,7 @@ int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
     if (!rc && (cfg->flags & PQC_SERVER_PLAIN) && hipMalloc(&sl.d_keys, v->key_arena) != hipSuccess)
       rc = PQC_ERR_NO_MEMORY;
     if (!rc) (void)pqc_ctx_put_tuning(sl.ctx, &cfg->tuning);  // an all-zero tuning is rejected: defaults stay
+    if (!rc && sl.ctx_fd) (void)pqc_ctx_put_tuning(sl.ctx_fd, &cfg->tuning);
   }
   if (rc) {
     free_server(v);
@@ -599,6 +620,10 @@ int pqc_server_get_stats(pqc_server* v, pqc_server_stats* st) {
   st->batches = v->n_batches;
   st->mean_batch = v->n_batches ? (double)v->n_batched_reqs / (double)v->n_batches : 0.0;
   st->inflight_bytes = v->inflight_bytes;
+  const double nb = v->n_batches ? (double)v->n_batches : 1.0;
+  st->pack_us = v->pack_us / nb;
+  st->device_us = v->device_us / nb;
+  st->retire_us = v->retire_us / nb;
   return PQC_OK;
 }
 

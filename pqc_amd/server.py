@@ -30,7 +30,11 @@ class Completion(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
-                ("mean_batch", C.c_double), ("inflight_bytes", C.c_uint64)]
+                ("mean_batch", C.c_double), ("inflight_bytes", C.c_uint64), ("pack_us", C.c_double),
+                ("device_us", C.c_double), ("retire_us", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class TraceStats(C.Structure):
