@@ -231,13 +231,16 @@ __device__ __forceinline__ size_t eoff(int m, int j) {
   return ((((size_t)(m >> 4) * 16 + (j >> 6)) * 64 + lane) * 4 + ((j >> 4) & 3)) * 4 + (j & 3);
 }
 
-// ROWS = false: S' into the MFMA fragments, E' | E'' at eoff (shared-A GEMM).
-// ROWS = true: S' and E' as plain int8 rows [8n][976] in sfrag / efrag, E'' after
-// E' as [n][64] (the many-key products).
-template <bool ROWS>
+// The stream is written raw (RAWB bytes per request, the SHAKE256 output in
+// order) and sampled by k_sample: the serial sponge chain of a lane pair is then
+// one permutation plus 17 dword stores per block, with no sampling or index
+// arithmetic on it.
+constexpr int RAWB = 31424;  // 231 blocks x 136 bytes, rounded up to 16
+static_assert(231 * 136 <= RAWB && 231 * 68 >= NSAMP && RAWB % 16 == 0, "raw stream size");
+
 __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKeys keys, const uint8_t* mu,
-                                              int mu_stride, int8_t* sfrag, int8_t* efrag, uint8_t* kout,
-                                              const uint8_t* ct, uint32_t* fstate) {
+                                              int mu_stride, uint8_t* raw, uint8_t* kout, const uint8_t* ct,
+                                              uint32_t* fstate) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int pr = t >> 1, half = t & 1;
   uint32_t S[25];
@@ -271,33 +274,13 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKe
 #pragma unroll
     for (int w = 0; w < 25; ++w) S[w] = w < 4 ? (uint32_t)(init[w < 4 ? w : 0] >> (32 * half)) : 0u;
     S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;
-    for (int blk = 0; blk * 68 < NSAMP; ++blk) {  // 231 blocks
+    // lane `half` of word w of block b is stream bytes 136 b + 8 w + 4 half
+    uint32_t* out = reinterpret_cast<uint32_t*>(raw + (size_t)r * RAWB) + half;
+    for (int blk = 0; blk < 231; ++blk) {
       keccakf1600_pair(S, (uint32_t)half);
 #pragma unroll
-      for (int w = 0; w < kRateShake256; ++w) {
-        if (blk * 68 + 4 * w >= NSAMP) break;
-        const int g = blk * 68 + 4 * w + 2 * half;  // this lane's first sample
-        const uint16_t b2 = (uint16_t)bytes2(sample2(S[w]));
-        if constexpr (ROWS) {
-          if (g < N * NB)
-            *reinterpret_cast<uint16_t*>(sfrag + (size_t)r * N * NB + g) = b2;
-          else if (g < 2 * N * NB)
-            *reinterpret_cast<uint16_t*>(efrag + (size_t)r * N * NB + (g - N * NB)) = b2;
-          else
-            *reinterpret_cast<uint16_t*>(efrag + (size_t)n * N * NB + (size_t)r * 64 + (g - 2 * N * NB)) = b2;
-        } else if (g < N * NB) {
-          const int k = g / N, j = g % N, m = 8 * r + k;
-          const int lane = (m & 15) + 16 * ((j >> 4) & 3);
-          const size_t off = ((((size_t)(m >> 4) * KSTEPS + (j >> 6)) * 64 + lane) * 16) + (j & 15);
-          *reinterpret_cast<uint16_t*>(sfrag + off) = b2;
-        } else if (g < 2 * N * NB) {
-          const int g1 = g - N * NB;
-          *reinterpret_cast<uint16_t*>(efrag + eoff(8 * r + g1 / N, g1 % N)) = b2;
-        } else {
-          const int g2 = g - 2 * N * NB;
-          *reinterpret_cast<uint16_t*>(efrag + eoff(8 * r + (g2 >> 3), N + (g2 & 7))) = b2;
-        }
-      }
+      for (int w = 0; w < kRateShake256; ++w) out[2 * w] = S[w];
+      out += 2 * kRateShake256;
     }
   } else if (with_f) {
     const int r = pr - npad;
@@ -305,6 +288,50 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKe
     absorb_ct_pair(S, ct + (size_t)r * CT, half);
 #pragma unroll
     for (int w = 0; w < 25; ++w) fstate[(size_t)r * 50 + 2 * w + half] = S[w];
+  }
+}
+
+// CDF sampling of the raw stream (frodo_sample_n, frodo976.cu:693-715; kem.c:94-98):
+// one thread per 8 consecutive samples (16 stream bytes).  Sample g of request r
+// is S' row g / 976, column g % 976 for g < 7808, then E' likewise, then E''
+// (8 x 8).  976 is a multiple of 8, so the 8 samples of a thread share a row.
+// ROWS = false: S' into the MFMA fragments, E' | E'' at eoff (shared-A GEMM).
+// ROWS = true: S' and E' as plain int8 rows [8n][976] in sfrag / efrag, E'' after
+// E' as [n][64] (the many-key products).
+constexpr int kSampleThreads = NSAMP / 8;  // per request
+static_assert(NSAMP % 8 == 0 && (N * NB) % 8 == 0, "sample groups");
+
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_sample(int n, const uint8_t* raw, int8_t* sfrag, int8_t* efrag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * kSampleThreads) return;
+  const int r = t / kSampleThreads, g = 8 * (t - r * kSampleThreads);
+  const uint4 v = *reinterpret_cast<const uint4*>(raw + (size_t)r * RAWB + 2 * g);
+  const uint32_t lo = bytes2(sample2(v.x)) | (bytes2(sample2(v.y)) << 16);
+  const uint32_t hi = bytes2(sample2(v.z)) | (bytes2(sample2(v.w)) << 16);
+  if constexpr (ROWS) {
+    int8_t* dst = g < N * NB       ? sfrag + (size_t)r * N * NB + g
+                  : g < 2 * N * NB ? efrag + (size_t)r * N * NB + (g - N * NB)
+                                   : efrag + (size_t)n * N * NB + (size_t)r * 64 + (g - 2 * N * NB);
+    *reinterpret_cast<uint2*>(dst) = make_uint2(lo, hi);
+  } else if (g < N * NB) {
+    const int k = g / N, j = g % N, m = 8 * r + k;
+    const int lane = (m & 15) + 16 * ((j >> 4) & 3);
+    const size_t off = ((((size_t)(m >> 4) * KSTEPS + (j >> 6)) * 64 + lane) * 16) + (j & 15);
+    *reinterpret_cast<uint2*>(sfrag + off) = make_uint2(lo, hi);
+  } else {
+    int m, j;  // the group's first entry; entries j .. j+3 and j+4 .. j+7 each sit in one lane's word
+    if (g < 2 * N * NB) {
+      const int g1 = g - N * NB;
+      m = 8 * r + g1 / N;
+      j = g1 % N;
+    } else {
+      const int g2 = g - 2 * N * NB;
+      m = 8 * r + (g2 >> 3);
+      j = N + (g2 & 7);
+    }
+    *reinterpret_cast<uint32_t*>(efrag + eoff(m, j)) = lo;
+    *reinterpret_cast<uint32_t*>(efrag + eoff(m, j + 4)) = hi;
   }
 }
 
@@ -773,7 +800,8 @@ size_t frodo_bytes_per_elem() {
   // 31232: key generation's S || E samples (uint16), or [E' | E''] int8 fragments
   // 8192 + GEMM output uint16 16384; then mu 32, k 32, F state 200, compare flags 16.
   // The S' fragments live in their own 8 KB-per-element buffer.
-  return 31232 + 32 + 32 + 200 + 16;
+  // Then the raw S'/E'/E'' SHAKE256 stream, RAWB bytes.
+  return 31232 + 32 + 32 + 200 + 16 + RAWB;
 }
 
 static int frodo_need(pqc_ctx* c) { return (c->fr_frag && c->fr_sfrag) ? PQC_OK : PQC_ERR_UNSUPPORTED; }
@@ -801,7 +829,7 @@ int frodo_load_key(pqc_ctx* c, const uint8_t* key, int is_sk, hipStream_t s) {
 struct FrScratch {
   int8_t *sfrag, *efrag;
   uint16_t *out, *se;
-  uint8_t *mu, *kp, *flags;
+  uint8_t *mu, *kp, *flags, *raw;
   uint32_t* fstate;
 };
 
@@ -820,6 +848,8 @@ static FrScratch fr_scratch(pqc_ctx* c) {
   f.fstate = (uint32_t*)p;
   p += (size_t)200 * ld;
   f.flags = p;
+  p += (size_t)16 * ld;
+  f.raw = p;
   f.sfrag = c->fr_sfrag;
   return f;
 }
@@ -845,6 +875,20 @@ static FrKeys loaded_key(pqc_ctx* c) {
   return FrKeys{c->fr_sk, 0, c->fr_pkh, 0, c->fr_pk, 0, c->fr_a16, 0, nullptr};
 }
 
+// G2 and the S'/E'/E'' stream of n requests (lane pairs; with_f adds the F prefix
+// of each ciphertext), then its sampling into the ROWS or fragment layouts.
+template <bool ROWS>
+static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, const FrKeys& keys, const uint8_t* mu,
+                          int mu_stride, const uint8_t* ct, hipStream_t s) {
+  const int npad = pairs_pad(ni), sb = c->tune.frodo_stream_block;
+  const int pairs = npad + (with_f ? ni : 0);
+  hipLaunchKernelGGL(k_shake, dim3((2 * pairs + sb - 1) / sb), dim3(sb), 0, s, ni, npad, with_f, keys, mu, mu_stride,
+                     f.raw, f.kp, ct, with_f ? f.fstate : (uint32_t*)nullptr);
+  const int nt = ni * kSampleThreads;
+  hipLaunchKernelGGL(k_sample<ROWS>, dim3((nt + 255) / 256), dim3(256), 0, s, ni, (const uint8_t*)f.raw, f.sfrag,
+                     f.efrag);
+}
+
 int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s) {
   int rc = frodo_need(c);
   if (rc) return rc;
@@ -852,11 +896,9 @@ int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t 
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
   const FrKeys keys = loaded_key(c);
-  const int ni = (int)n, npad = pairs_pad(ni);
+  const int ni = (int)n;
   hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, keys, f.mu);
-  const int sb = c->tune.frodo_stream_block;
-  hipLaunchKernelGGL(k_shake<false>, dim3((2 * (npad + ni) + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 1, keys,
-                     (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
+  launch_stream<false>(c, f, ni, 1, keys, f.mu, 32, ct, s);
   rc = launch_gemm<kDecaps>(c, n, f, const_cast<uint8_t*>(ct), f.mu, 32, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, ct, (const uint64_t*)f.fstate,
@@ -871,11 +913,9 @@ int frodo_dec_keyed(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const 
   if (!c->fr_elem || !c->fr_sfrag) return PQC_ERR_UNSUPPORTED;
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
-  const int ni = (int)n, npad = pairs_pad(ni);
+  const int ni = (int)n;
   hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, keys, f.mu);
-  const int sb = c->tune.frodo_stream_block;
-  hipLaunchKernelGGL(k_shake<true>, dim3((2 * (npad + ni) + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 1, keys,
-                     (const uint8_t*)f.mu, 32, f.sfrag, f.efrag, f.kp, ct, f.fstate);
+  launch_stream<true>(c, f, ni, 1, keys, f.mu, 32, ct, s);
   hipLaunchKernelGGL(k_mk_product<kDecaps>, dim3(ni, 2), dim3(256), 0, s, ni, (const int8_t*)f.sfrag,
                      (const int8_t*)f.efrag, keys, const_cast<uint8_t*>(ct), (const uint8_t*)f.mu, 32, f.flags);
   hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, ct, (const uint64_t*)f.fstate,
@@ -888,10 +928,8 @@ int frodo_enc_keyed(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_
   if (!c->fr_elem || !c->fr_sfrag) return PQC_ERR_UNSUPPORTED;
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
-  const int ni = (int)n, npad = pairs_pad(ni);
-  const int sb = c->tune.frodo_stream_block;
-  hipLaunchKernelGGL(k_shake<true>, dim3((2 * npad + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 0, keys, mu, MU,
-                     f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
+  const int ni = (int)n;
+  launch_stream<true>(c, f, ni, 0, keys, mu, MU, nullptr, s);
   hipLaunchKernelGGL(k_mk_product<kEncaps>, dim3(ni, 2), dim3(256), 0, s, ni, (const int8_t*)f.sfrag,
                      (const int8_t*)f.efrag, keys, ct, mu, MU, (uint8_t*)nullptr);
   hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct,
@@ -943,10 +981,8 @@ int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu,
   if (!c->fr_loaded) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
-  const int ni = (int)n, npad = pairs_pad(ni);
-  const int sb = c->tune.frodo_stream_block;
-  hipLaunchKernelGGL(k_shake<false>, dim3((2 * npad + sb - 1) / sb), dim3(sb), 0, s, ni, npad, 0, loaded_key(c), mu,
-                     MU, f.sfrag, f.efrag, f.kp, (const uint8_t*)nullptr, (uint32_t*)nullptr);
+  const int ni = (int)n;
+  launch_stream<false>(c, f, ni, 0, loaded_key(c), mu, MU, nullptr, s);
   rc = launch_gemm<kEncaps>(c, n, f, ct, mu, MU, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct,
