@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKe
     // lane `half` of word w of block b is stream bytes 136 b + 8 w + 4 half
     uint32_t* out = reinterpret_cast<uint32_t*>(raw + (size_t)r * RAWB) + half;
     for (int blk = 0; blk < 231; ++blk) {
-      keccakf1600_pair(S, (uint32_t)half);
+      keccakf1600_pair<24>(S, (uint32_t)half);
 #pragma unroll
       for (int w = 0; w < kRateShake256; ++w) out[2 * w] = S[w];
       out += 2 * kRateShake256;

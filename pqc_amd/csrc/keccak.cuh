@@ -171,9 +171,13 @@ __device__ __forceinline__ void keccak_theta_rho_pi_pair(uint32_t (&B)[25], cons
   if constexpr (I + 1 < 25) keccak_theta_rho_pi_pair<I + 1>(B, A, Cm, Rp);
 }
 
-// Both lanes of every pair must be active.  `odd` is lane & 1.
+// Both lanes of every pair must be active.  `odd` is lane & 1.  UNROLL rounds
+// per loop trip: 1 keeps the code small; 24 drops the loop's round-constant load
+// and branch (for a call site that carries a long serial chain).
+template <int UNROLL = 1>
 __device__ __forceinline__ void keccakf1600_pair(uint32_t (&A)[25], uint32_t odd) {
-#pragma unroll 1
+  static_assert(24 % UNROLL == 0, "rounds per trip");
+#pragma unroll UNROLL
   for (int round = 0; round < 24; ++round) {
     uint32_t C[5], Cm[5], Rp[5], B[25];
 #pragma unroll

@@ -1,7 +1,15 @@
 // NewHope-1024 CPA-PKE as plain batched kernels (NewHope_NIST_GPU/cpapke.cu:
-// 265-476): one kernel per reference operation, one thread per instance,
-// polynomials in coefficient-major SoA arrays (coefficient c of element x at
-// c * ld + x) so that every load and store is coalesced across the wave.
+// 265-476): one kernel per reference operation.  Polynomials are element-major
+// (element x's 1024 uint16 coefficients contiguous at x * 1024), and each kernel
+// spreads an element over as many lanes as its data flow allows:
+//   - NTT / inverse NTT: one wave per polynomial, 16 coefficients per lane in
+//     VGPRs, the 10 butterfly layers in three register-resident groups (index
+//     bits 0-3, 4-7, 8-9) with the polynomial re-laid out through LDS between
+//     groups;
+//   - poly_uniform / poly_sample: one lane per 64-coefficient SHAKE stream
+//     (16 per polynomial);
+//   - pointwise, add/sub, (de)serialisation, (de)compression, message coding:
+//     flat passes, one lane per 4 or 8 coefficients, 8- or 16-byte accesses.
 //
 // The 16-bit storage and 32-bit intermediate arithmetic of the reference are
 // kept exactly (lazy even NTT levels stored as uint16, Barrett on a uint16
@@ -40,7 +48,7 @@ constexpr uint32_t brv10(uint32_t i) {
   return r;
 }
 
-struct Tables {
+struct alignas(16) Tables {  // 16-byte aligned rows: read as uint4 by the NTT kernels
   uint16_t omegas_inv[N / 2];
   uint16_t gammas[N];
   uint16_t gammas_inv[N];
@@ -78,9 +86,8 @@ __device__ __forceinline__ uint32_t barrett(uint32_t a16) {
 }
 
 struct Pv {
-  uint16_t* p;
-  int ld;
-  __device__ __forceinline__ uint16_t& at(int c, int x) const { return p[(size_t)c * ld + x]; }
+  uint16_t* p;  // element x's coefficients at p + 1024 x
+  __device__ __forceinline__ uint16_t* el(int x) const { return p + (size_t)x * N; }
 };
 
 // ---- sponge kernels --------------------------------------------------------
@@ -97,128 +104,235 @@ __global__ void k_seed_expand(int n, uint8_t* z, const uint8_t* seed) {
   for (int w = 0; w < 8; ++w) store_u64(z + (size_t)x * 64 + 8 * w, A[w]);
 }
 
-// gen_a / poly_uniform (cpapke.cu:236-247, poly.cu:332-366)
+// gen_a / poly_uniform (cpapke.cu:236-247, poly.cu:332-366): lane (x, i) runs
+// SHAKE128(seed || i) and keeps 16-bit candidates < 5q for coefficients 64 i ..
 __global__ void k_uniform(int n, Pv a, const uint8_t* seed, size_t stride) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = t >> 4, i = t & 15;
   if (x >= n) return;
-  uint64_t s[4];
+  uint64_t A[25];
 #pragma unroll
-  for (int w = 0; w < 4; ++w) s[w] = load_u64(seed + (size_t)x * stride + 8 * w);
-  for (int i = 0; i < N / 64; ++i) {
-    uint64_t A[25];
+  for (int w = 0; w < 25; ++w) A[w] = w < 4 ? load_u64(seed + (size_t)x * stride + 8 * w) : 0;
+  A[4] = (uint64_t)i | (0x1Full << 8);
+  A[kRateShake128 - 1] ^= 0x8000000000000000ull;
+  uint16_t* dst = a.el(x) + 64 * i;
+  int ctr = 0;
+  for (int blk = 0; blk < 16 && ctr < 64; ++blk) {
+    keccakf1600(A);
 #pragma unroll
-    for (int t = 0; t < 25; ++t) A[t] = t < 4 ? s[t] : 0;
-    A[4] = (uint64_t)i | (0x1Full << 8);
-    A[kRateShake128 - 1] ^= 0x8000000000000000ull;
-    int ctr = 0;
-    for (int blk = 0; blk < 16 && ctr < 64; ++blk) {
-      keccakf1600(A);
+    for (int w = 0; w < kRateShake128; ++w) {
 #pragma unroll
-      for (int w = 0; w < kRateShake128; ++w) {
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
-          if (val < 5 * Q && ctr < 64) {
-            a.at(i * 64 + ctr, x) = (uint16_t)val;
-            ++ctr;
-          }
-        }
+      for (int h = 0; h < 4; ++h) {
+        const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
+        if (val < 5 * Q && ctr < 64) dst[ctr++] = (uint16_t)val;
       }
     }
   }
 }
 
-// poly_sample (poly.cu:396-444): hw(a) + q - hw(b) over SHAKE256(seed || nonce || i, 128)
+// poly_sample (poly.cu:396-444): lane (x, i) = coefficients 64 i .. 64 i + 63,
+// hw(a) + q - hw(b) over SHAKE256(seed || nonce || i, 128)
 __global__ void k_sample(int n, Pv r, const uint8_t* seed, size_t stride, int nonce) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = t >> 4, i = t & 15;
   if (x >= n) return;
-  uint64_t s[4];
+  uint64_t s[4], A[25];
 #pragma unroll
   for (int w = 0; w < 4; ++w) s[w] = load_u64(seed + (size_t)x * stride + 8 * w);
-  for (int i = 0; i < N / 64; ++i) {
-    uint64_t A[25];
-    absorb_fixed<kRateShake256, 4>(A, s, (uint64_t)nonce | ((uint64_t)i << 8) | (0x1Full << 16));
+  absorb_fixed<kRateShake256, 4>(A, s, (uint64_t)nonce | ((uint64_t)i << 8) | (0x1Full << 16));
+  uint4* dst = reinterpret_cast<uint4*>(r.el(x) + 64 * i);
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
+  for (int w = 0; w < 16; w += 2) {
+    uint32_t o[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t ab = (uint32_t)(A[w] >> (16 * k)) & 0xFFFFu;
-        r.at(64 * i + 4 * w + k, x) = (uint16_t)(__popc(ab & 0xFF) + Q - __popc(ab >> 8));
-      }
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t word = A[w + (k >> 1)];
+      const uint32_t ab0 = (uint32_t)(word >> (32 * (k & 1))) & 0xFFFFu;
+      const uint32_t ab1 = (uint32_t)(word >> (32 * (k & 1) + 16)) & 0xFFFFu;
+      const uint32_t c0 = (__popc(ab0 & 0xFF) + Q - __popc(ab0 >> 8)) & 0xFFFFu;
+      const uint32_t c1 = (__popc(ab1 & 0xFF) + Q - __popc(ab1 >> 8)) & 0xFFFFu;
+      o[k] = c0 | (c1 << 16);
     }
+    dst[w >> 1] = make_uint4(o[0], o[1], o[2], o[3]);
   }
 }
 
-// ---- NTT kernels ----------------------------------------------------------------
+// ---- NTT kernels: one wave per polynomial ------------------------------------------
 
-// ntt (ntt.cu:195-234) on one SoA polynomial
-__device__ __forceinline__ void ntt_core(const Pv& a, int x, const uint16_t* omega) {
-  for (int i = 0; i < 10; i += 2) {
-    int distance = 1 << i;
-    for (int start = 0; start < distance; start++) {
-      int jt = 0;
-      for (int j = start; j < N - 1; j += 2 * distance) {
-        const uint32_t W = omega[jt++];
-        const uint32_t temp = a.at(j, x), hi = a.at(j + distance, x);
-        a.at(j, x) = (uint16_t)(temp + hi);
-        a.at(j + distance, x) = (uint16_t)mont(W * (temp + 3 * Q - hi));
-      }
-    }
-    distance <<= 1;
-    for (int start = 0; start < distance; start++) {
-      int jt = 0;
-      for (int j = start; j < N - 1; j += 2 * distance) {
-        const uint32_t W = omega[jt++];
-        const uint32_t temp = a.at(j, x), hi = a.at(j + distance, x);
-        a.at(j, x) = (uint16_t)barrett(temp + hi);
-        a.at(j + distance, x) = (uint16_t)mont(W * (temp + 3 * Q - hi));
-      }
-    }
+// One butterfly of ntt (ntt.cu:195-234) on 16-bit values: even layers keep the
+// sum unreduced (stored as uint16), odd layers Barrett it; the difference is
+// multiplied by W in Montgomery form.
+template <bool REDUCE>
+__device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t W) {
+  const uint32_t t = a, h = b;
+  a = REDUCE ? barrett(t + h) : ((t + h) & 0xFFFFu);
+  b = mont(W * (t + 3 * Q - h));
+}
+
+// Layer L (distance 2^L) on register pairs (r, r + 2^RB), RB the register bit
+// that holds index bit L; the twiddle of index idx is omega[idx >> (L + 1)].
+template <int L, int RB, typename IDX>
+__device__ __forceinline__ void ntt_layer(uint32_t (&v)[16], const uint16_t* omega, IDX idx) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    if (r & (1 << RB)) continue;
+    const uint32_t W = omega[idx(r) >> (L + 1)];
+    bfly<(L & 1) != 0>(v[r], v[r + (1 << RB)], W);
   }
 }
 
-// poly_ntt (poly.cu:519-527)
-__global__ void k_ntt(int n, Pv r) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+// ntt_core on a polynomial that sits in LDS (sm) in ntt input order; returns the
+// output in v with v[r] = coefficient 4 l + (r & 3) + 256 (r >> 2).
+__device__ __forceinline__ void ntt_wave(uint32_t (&v)[16], uint16_t* sm, int l, const uint16_t* omega) {
+  // layers 0-3: v[r] = index 16 l + r
+  {
+    const uint4 p0 = *reinterpret_cast<const uint4*>(sm + 16 * l), p1 = *reinterpret_cast<const uint4*>(sm + 16 * l + 8);
+    const uint32_t w[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[2 * k] = w[k] & 0xFFFFu;
+      v[2 * k + 1] = w[k] >> 16;
+    }
+  }
+  auto ia = [l](int r) { return 16 * l + r; };
+  ntt_layer<0, 0>(v, omega, ia);
+  ntt_layer<1, 1>(v, omega, ia);
+  ntt_layer<2, 2>(v, omega, ia);
+  ntt_layer<3, 3>(v, omega, ia);
+  {
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = v[2 * k] | (v[2 * k + 1] << 16);
+    __syncthreads();
+    *reinterpret_cast<uint4*>(sm + 16 * l) = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(sm + 16 * l + 8) = make_uint4(w[4], w[5], w[6], w[7]);
+    __syncthreads();
+  }
+  // layers 4-7: v[r] = index (l & 15) + 16 r + 256 (l >> 4)
+  auto ib = [l](int r) { return (l & 15) + 16 * r + 256 * (l >> 4); };
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = sm[ib(r)];
+  ntt_layer<4, 0>(v, omega, ib);
+  ntt_layer<5, 1>(v, omega, ib);
+  ntt_layer<6, 2>(v, omega, ib);
+  ntt_layer<7, 3>(v, omega, ib);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) sm[ib(r)] = (uint16_t)v[r];
+  __syncthreads();
+  // layers 8-9: v[r] = index 4 l + (r & 3) + 256 (r >> 2)
+  auto ic = [l](int r) { return 4 * l + (r & 3) + 256 * (r >> 2); };
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint2 p = *reinterpret_cast<const uint2*>(sm + 4 * l + 256 * q);
+    v[4 * q] = p.x & 0xFFFFu;
+    v[4 * q + 1] = p.x >> 16;
+    v[4 * q + 2] = p.y & 0xFFFFu;
+    v[4 * q + 3] = p.y >> 16;
+  }
+  ntt_layer<8, 2>(v, omega, ic);
+  ntt_layer<9, 3>(v, omega, ic);
+}
+
+__device__ __forceinline__ void store_c(const uint32_t (&v)[16], uint16_t* el, int l) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    *reinterpret_cast<uint2*>(el + 4 * l + 256 * q) =
+        make_uint2(v[4 * q] | (v[4 * q + 1] << 16), v[4 * q + 2] | (v[4 * q + 3] << 16));
+}
+
+// poly_ntt (poly.cu:519-527): mul_coefficients by gammas, then ntt with the same
+// table.  Workgroup = one wave = one polynomial.
+__global__ __launch_bounds__(64) void k_ntt(int n, Pv r) {
+  __shared__ __align__(16) uint16_t sm[N];
+  const int x = blockIdx.x, l = threadIdx.x;
   if (x >= n) return;
-  for (int c = 0; c < N; ++c) r.at(c, x) = (uint16_t)mont((uint32_t)r.at(c, x) * c_tab.gammas[c]);
-  ntt_core(r, x, c_tab.gammas);
+  uint16_t* el = r.el(x);
+  const uint4* src = reinterpret_cast<const uint4*>(el + 16 * l);
+  const uint4* gam = reinterpret_cast<const uint4*>(c_tab.gammas + 16 * l);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint4 a = src[h], g = gam[h];
+    const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, gw[4] = {g.x, g.y, g.z, g.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = mont((aw[k] & 0xFFFFu) * (gw[k] & 0xFFFFu)) | (mont((aw[k] >> 16) * (gw[k] >> 16)) << 16);
+    *reinterpret_cast<uint4*>(sm + 16 * l + 8 * h) = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+  __syncthreads();
+  uint32_t v[16];
+  ntt_wave(v, sm, l, c_tab.gammas);
+  store_c(v, el, l);
 }
 
-// poly_invntt (poly.cu:538-547)
-__global__ void k_invntt(int n, Pv r) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+// poly_invntt (poly.cu:538-547): bitrev_vector, ntt with omegas_inv, then
+// mul_coefficients by gammas_inv
+__global__ __launch_bounds__(64) void k_invntt(int n, Pv r) {
+  __shared__ __align__(16) uint16_t sm[N];
+  const int x = blockIdx.x, l = threadIdx.x;
   if (x >= n) return;
-  for (int i = 0; i < N; ++i) {
-    const int j = c_tab.brv[i];
-    if (i < j) {
-      const uint16_t t = r.at(i, x);
-      r.at(i, x) = r.at(j, x);
-      r.at(j, x) = t;
+  uint16_t* el = r.el(x);
+  const uint4* src = reinterpret_cast<const uint4*>(el + 16 * l);
+  uint16_t raw[16];
+  {
+    const uint4 a = src[0], b = src[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      raw[2 * k] = (uint16_t)w[k];
+      raw[2 * k + 1] = (uint16_t)(w[k] >> 16);
     }
   }
-  ntt_core(r, x, c_tab.omegas_inv);
-  for (int c = 0; c < N; ++c) r.at(c, x) = (uint16_t)mont((uint32_t)r.at(c, x) * c_tab.gammas_inv[c]);
+  // the coefficient at ntt input position i is the stored one at brv(i): write
+  // stored coefficient c to position brv(c)
+#pragma unroll
+  for (int k = 0; k < 16; ++k) sm[c_tab.brv[16 * l + k]] = raw[k];
+  __syncthreads();
+  uint32_t v[16];
+  ntt_wave(v, sm, l, c_tab.omegas_inv);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[4 * q + k] = mont(v[4 * q + k] * c_tab.gammas_inv[4 * l + k + 256 * q]);
+  store_c(v, el, l);
 }
 
-// poly_mul_pointwise (poly.cu:455-468)
+// ---- flat passes -------------------------------------------------------------------
+
+// poly_mul_pointwise (poly.cu:455-468), 8 coefficients per lane
 __global__ void k_pointwise(int n, Pv r, Pv a, Pv b) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  for (int c = 0; c < N; ++c) {
-    const uint32_t t = mont(3186u * b.at(c, x));
-    r.at(c, x) = (uint16_t)mont((uint32_t)a.at(c, x) * t);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * (N / 8)) return;
+  const uint4 av = reinterpret_cast<const uint4*>(a.p)[t], bv = reinterpret_cast<const uint4*>(b.p)[t];
+  const uint32_t aw[4] = {av.x, av.y, av.z, av.w}, bw[4] = {bv.x, bv.y, bv.z, bv.w};
+  uint32_t o[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t t0 = mont(3186u * (bw[k] & 0xFFFFu)), t1 = mont(3186u * (bw[k] >> 16));
+    o[k] = mont((aw[k] & 0xFFFFu) * t0) | (mont((aw[k] >> 16) * t1) << 16);
   }
+  reinterpret_cast<uint4*>(r.p)[t] = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// poly_add / poly_sub (poly.cu:479-508)
+// poly_add / poly_sub (poly.cu:479-508), 8 coefficients per lane
 __global__ void k_addsub(int n, Pv r, Pv a, Pv b, int subtract) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  for (int c = 0; c < N; ++c) {
-    const uint32_t av = a.at(c, x), bv = b.at(c, x);
-    r.at(c, x) = (uint16_t)((subtract ? av + 3 * Q - bv : av + bv) % Q);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * (N / 8)) return;
+  const uint4 av = reinterpret_cast<const uint4*>(a.p)[t], bv = reinterpret_cast<const uint4*>(b.p)[t];
+  const uint32_t aw[4] = {av.x, av.y, av.z, av.w}, bw[4] = {bv.x, bv.y, bv.z, bv.w};
+  uint32_t o[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t h[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const uint32_t x = (aw[k] >> (16 * e)) & 0xFFFFu, y = (bw[k] >> (16 * e)) & 0xFFFFu;
+      h[e] = (subtract ? x + 3 * Q - y : x + y) % Q;
+    }
+    o[k] = h[0] | (h[1] << 16);
   }
+  reinterpret_cast<uint4*>(r.p)[t] = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // ---- (de)serialisation --------------------------------------------------------
@@ -239,182 +353,203 @@ __device__ __forceinline__ uint32_t flipabs(uint32_t x) {
   return (uint32_t)(uint16_t)((r + m) ^ m);
 }
 
-// poly_tobytes (poly.cu:105-127) into dst, optional seed after the polynomial
+// poly_tobytes (poly.cu:105-127): lane (x, i) packs coefficients 4 i .. 4 i + 3
+// into bytes 7 i .. 7 i + 6; lanes i < 4 also copy seed word i after the
+// polynomial (encode_pk)
 __global__ void k_tobytes(int n, uint8_t* dst, size_t stride, Pv p, const uint8_t* seed, size_t seed_stride) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  uint8_t* r = dst + (size_t)x * stride;
-  for (int i = 0; i < N / 4; i++) {
-    const uint32_t t0 = freeze(p.at(4 * i, x)), t1 = freeze(p.at(4 * i + 1, x)), t2 = freeze(p.at(4 * i + 2, x)),
-                   t3 = freeze(p.at(4 * i + 3, x));
-    r[7 * i + 0] = (uint8_t)(t0 & 0xff);
-    r[7 * i + 1] = (uint8_t)((t0 >> 8) | (t1 << 6));
-    r[7 * i + 2] = (uint8_t)(t1 >> 2);
-    r[7 * i + 3] = (uint8_t)((t1 >> 10) | (t2 << 4));
-    r[7 * i + 4] = (uint8_t)(t2 >> 4);
-    r[7 * i + 5] = (uint8_t)((t2 >> 12) | (t3 << 2));
-    r[7 * i + 6] = (uint8_t)(t3 >> 6);
-  }
-  if (seed) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w) store_u64(r + POLYBYTES + 8 * w, load_u64(seed + (size_t)x * seed_stride + 8 * w));
-  }
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * (N / 4)) return;
+  const int x = (int)(t / (N / 4)), i = (int)(t % (N / 4));
+  const uint2 c = reinterpret_cast<const uint2*>(p.el(x))[i];
+  const uint32_t t0 = freeze(c.x & 0xFFFFu), t1 = freeze(c.x >> 16), t2 = freeze(c.y & 0xFFFFu), t3 = freeze(c.y >> 16);
+  uint8_t* r = dst + (size_t)x * stride + 7 * i;
+  r[0] = (uint8_t)(t0 & 0xff);
+  r[1] = (uint8_t)((t0 >> 8) | (t1 << 6));
+  r[2] = (uint8_t)(t1 >> 2);
+  r[3] = (uint8_t)((t1 >> 10) | (t2 << 4));
+  r[4] = (uint8_t)(t2 >> 4);
+  r[5] = (uint8_t)((t2 >> 12) | (t3 << 2));
+  r[6] = (uint8_t)(t3 >> 6);
+  if (seed && i < 4)
+    store_u64(dst + (size_t)x * stride + POLYBYTES + 8 * i, load_u64(seed + (size_t)x * seed_stride + 8 * i));
 }
 
-// poly_frombytes (poly.cu:62-74), optional seed copy-out (decode_pk)
+// poly_frombytes (poly.cu:62-74): lane (x, i) unpacks bytes 7 i .. 7 i + 6;
+// lanes i < 4 copy seed word i out (decode_pk)
 __global__ void k_frombytes(int n, Pv r, const uint8_t* src, size_t stride, uint8_t* seed_out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  const uint8_t* a = src + (size_t)x * stride;
-  for (int i = 0; i < N / 4; i++) {
-    r.at(4 * i + 0, x) = (uint16_t)(a[7 * i + 0] | (((uint32_t)a[7 * i + 1] & 0x3f) << 8));
-    r.at(4 * i + 1, x) = (uint16_t)((a[7 * i + 1] >> 6) | ((uint32_t)a[7 * i + 2] << 2) | (((uint32_t)a[7 * i + 3] & 0x0f) << 10));
-    r.at(4 * i + 2, x) = (uint16_t)((a[7 * i + 3] >> 4) | ((uint32_t)a[7 * i + 4] << 4) | (((uint32_t)a[7 * i + 5] & 0x03) << 12));
-    r.at(4 * i + 3, x) = (uint16_t)((a[7 * i + 5] >> 2) | ((uint32_t)a[7 * i + 6] << 6));
-  }
-  if (seed_out) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w) store_u64(seed_out + (size_t)x * 32 + 8 * w, load_u64(a + POLYBYTES + 8 * w));
-  }
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * (N / 4)) return;
+  const int x = (int)(t / (N / 4)), i = (int)(t % (N / 4));
+  const uint8_t* a = src + (size_t)x * stride + 7 * i;
+  const uint32_t c0 = a[0] | (((uint32_t)a[1] & 0x3f) << 8);
+  const uint32_t c1 = (a[1] >> 6) | ((uint32_t)a[2] << 2) | (((uint32_t)a[3] & 0x0f) << 10);
+  const uint32_t c2 = (a[3] >> 4) | ((uint32_t)a[4] << 4) | (((uint32_t)a[5] & 0x03) << 12);
+  const uint32_t c3 = (a[5] >> 2) | ((uint32_t)a[6] << 6);
+  reinterpret_cast<uint2*>(r.el(x))[i] = make_uint2((c0 & 0xFFFFu) | (c1 << 16), (c2 & 0xFFFFu) | ((c3 & 0xFFFFu) << 16));
+  if (seed_out && i < 4)
+    store_u64(seed_out + (size_t)x * 32 + 8 * i, load_u64(src + (size_t)x * stride + POLYBYTES + 8 * i));
 }
 
-// poly_compress (poly.cu:165-186) into ct + POLYBYTES
+// poly_compress (poly.cu:165-186): lane (x, g) packs coefficients 8 g .. 8 g + 7
+// into ct + POLYBYTES + 3 g
 __global__ void k_compress(int n, uint8_t* ct, Pv p) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  uint8_t* r = ct + (size_t)x * CT + POLYBYTES;
-  for (int i = 0, k = 0; i < N; i += 8, k += 3) {
-    uint32_t t[8];
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * (N / 8)) return;
+  const int x = (int)(t / (N / 8)), g = (int)(t % (N / 8));
+  const uint4 c = reinterpret_cast<const uint4*>(p.el(x))[g];
+  const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
+  uint32_t v[8];
 #pragma unroll
-    for (int j = 0; j < 8; j++) t[j] = (((freeze(p.at(i + j, x)) << 3) + Q / 2) / Q) & 7;
-    r[k] = (uint8_t)(t[0] | (t[1] << 3) | (t[2] << 6));
-    r[k + 1] = (uint8_t)((t[2] >> 2) | (t[3] << 1) | (t[4] << 4) | (t[5] << 7));
-    r[k + 2] = (uint8_t)((t[5] >> 1) | (t[6] << 2) | (t[7] << 5));
-  }
+  for (int j = 0; j < 8; j++) v[j] = (((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) / Q) & 7;
+  uint8_t* r = ct + (size_t)x * CT + POLYBYTES + 3 * g;
+  r[0] = (uint8_t)(v[0] | (v[1] << 3) | (v[2] << 6));
+  r[1] = (uint8_t)((v[2] >> 2) | (v[3] << 1) | (v[4] << 4) | (v[5] << 7));
+  r[2] = (uint8_t)((v[5] >> 1) | (v[6] << 2) | (v[7] << 5));
 }
 
-// poly_decompress (poly.cu:197-216) from ct + POLYBYTES
+// poly_decompress (poly.cu:197-216): lane (x, g) = coefficients 8 g .. 8 g + 7
 __global__ void k_decompress(int n, Pv r, const uint8_t* ct) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  const uint8_t* a = ct + (size_t)x * CT + POLYBYTES;
-  for (int i = 0; i < N; i += 8, a += 3) {
-    uint32_t t[8];
-    t[0] = a[0] & 7;
-    t[1] = (a[0] >> 3) & 7;
-    t[2] = (a[0] >> 6) | ((a[1] << 2) & 4);
-    t[3] = (a[1] >> 1) & 7;
-    t[4] = (a[1] >> 4) & 7;
-    t[5] = (a[1] >> 7) | ((a[2] << 1) & 6);
-    t[6] = (a[2] >> 2) & 7;
-    t[7] = (a[2] >> 5);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * (N / 8)) return;
+  const int x = (int)(t / (N / 8)), g = (int)(t % (N / 8));
+  const uint8_t* a = ct + (size_t)x * CT + POLYBYTES + 3 * g;
+  const uint32_t a0 = a[0], a1 = a[1], a2 = a[2];
+  uint32_t v[8];
+  v[0] = a0 & 7;
+  v[1] = (a0 >> 3) & 7;
+  v[2] = (a0 >> 6) | ((a1 << 2) & 4);
+  v[3] = (a1 >> 1) & 7;
+  v[4] = (a1 >> 4) & 7;
+  v[5] = (a1 >> 7) | ((a2 << 1) & 6);
+  v[6] = (a2 >> 2) & 7;
+  v[7] = (a2 >> 5);
+  uint32_t o[4];
 #pragma unroll
-    for (int j = 0; j < 8; j++) r.at(i + j, x) = (uint16_t)((t[j] * Q + 4) >> 3);
-  }
+  for (int k = 0; k < 4; ++k)
+    o[k] = (((v[2 * k] * Q + 4) >> 3) & 0xFFFFu) | ((((v[2 * k + 1] * Q + 4) >> 3) & 0xFFFFu) << 16);
+  reinterpret_cast<uint4*>(r.el(x))[g] = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// poly_frommsg_n (poly.cu:226-251)
+// poly_frommsg_n (poly.cu:226-251): lane (x, i) spreads message byte i over
+// coefficients 8 i + j + 256 q
 __global__ void k_frommsg(int n, Pv r, const uint8_t* msg) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  for (int i = 0; i < 32; i++) {
-    const uint32_t b = msg[(size_t)x * 32 + i];
-    for (int j = 0; j < 8; j++) {
-      const uint16_t v = (uint16_t)((0u - ((b >> j) & 1)) & (Q / 2));
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 32) return;
+  const int x = t >> 5, i = t & 31;
+  const uint32_t b = msg[(size_t)x * 32 + i];
+  uint32_t o[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) r.at(8 * i + j + 256 * q, x) = v;
-    }
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t v0 = (0u - ((b >> (2 * k)) & 1)) & (Q / 2), v1 = (0u - ((b >> (2 * k + 1)) & 1)) & (Q / 2);
+    o[k] = v0 | (v1 << 16);
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) reinterpret_cast<uint4*>(r.el(x) + 8 * i + 256 * q)[0] = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// poly_tomsg_n (poly.cu:261-291)
+// poly_tomsg_n (poly.cu:261-291): lane (x, i) decodes message byte i
 __global__ void k_tomsg(int n, uint8_t* msg, Pv p) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  for (int i = 0; i < 32; i++) {
-    uint32_t byte = 0;
-    for (int j = 0; j < 8; j++) {
-      const int c = 8 * i + j;
-      uint32_t t = flipabs(p.at(c, x)) + flipabs(p.at(c + 256, x)) + flipabs(p.at(c + 512, x)) +
-                   flipabs(p.at(c + 768, x));
-      t = (t - Q) & 0xFFFFu;
-      byte |= (t >> 15) << j;
-    }
-    msg[(size_t)x * 32 + i] = (uint8_t)byte;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 32) return;
+  const int x = t >> 5, i = t & 31;
+  uint32_t c[4][8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint4 w = reinterpret_cast<const uint4*>(p.el(x) + 8 * i + 256 * q)[0];
+    const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[q][j] = (ww[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
   }
+  uint32_t byte = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    uint32_t v = flipabs(c[0][j]) + flipabs(c[1][j]) + flipabs(c[2][j]) + flipabs(c[3][j]);
+    v = (v - Q) & 0xFFFFu;
+    byte |= (v >> 15) << j;
+  }
+  msg[(size_t)x * 32 + i] = (uint8_t)byte;
 }
 
 // ---- orchestration (cpapke.cu:265-476) -------------------------------------------
 
 struct Scratch {
   pqc_ctx* c;
-  Pv poly(int i) const { return Pv{c->nhpolys + (size_t)i * N * c->ld, (int)c->ld}; }
+  Pv poly(int i) const { return Pv{c->nhpolys + (size_t)i * N * c->ld}; }
 };
 
-#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__)
+// grid for `threads` lanes at the tuned block size
+struct Launch {
+  int block;
+  dim3 grid(size_t threads) const { return dim3((unsigned)((threads + block - 1) / block)); }
+};
+#define LAUNCH(kern, threads, ...) hipLaunchKernelGGL(kern, L.grid(threads), dim3(L.block), 0, s, __VA_ARGS__)
+#define LAUNCH_NTT(kern, ...) hipLaunchKernelGGL(kern, dim3(n), dim3(64), 0, s, __VA_ARGS__)
 
 }  // namespace newhope
 
 int newhope_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
   using namespace newhope;
-  const int block = c->tune.newhope_block;
-  const int n = (int)n_, grid = (n + block - 1) / block;
+  const newhope::Launch L{c->tune.newhope_block};
+  const int n = (int)n_;
+  const size_t n1 = n_, n16 = 16 * n_, n128 = 128 * n_, n256 = 256 * n_;
   Scratch S{c};
   uint8_t* z = c->nhbytes;  // 64 bytes per element: publicseed || noiseseed
   const int AHAT = 0, EHAT = 1, AS = 2, BHAT = 3, SHAT = 4;
-  LAUNCH(k_seed_expand, n, z, seed);
-  LAUNCH(k_uniform, n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
-  LAUNCH(k_sample, n, S.poly(SHAT), (const uint8_t*)z + 32, (size_t)64, 0);
-  LAUNCH(k_ntt, n, S.poly(SHAT));
-  LAUNCH(k_sample, n, S.poly(EHAT), (const uint8_t*)z + 32, (size_t)64, 1);
-  LAUNCH(k_ntt, n, S.poly(EHAT));
-  LAUNCH(k_pointwise, n, S.poly(AS), S.poly(SHAT), S.poly(AHAT));
-  LAUNCH(k_addsub, n, S.poly(BHAT), S.poly(EHAT), S.poly(AS), 0);
-  LAUNCH(k_tobytes, n, sk, (size_t)SK, S.poly(SHAT), (const uint8_t*)nullptr, (size_t)0);
-  LAUNCH(k_tobytes, n, pk, (size_t)PK, S.poly(BHAT), (const uint8_t*)z, (size_t)64);
+  LAUNCH(k_seed_expand, n1, n, z, seed);
+  LAUNCH(k_uniform, n16, n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
+  LAUNCH(k_sample, n16, n, S.poly(SHAT), (const uint8_t*)z + 32, (size_t)64, 0);
+  LAUNCH_NTT(k_ntt, n, S.poly(SHAT));
+  LAUNCH(k_sample, n16, n, S.poly(EHAT), (const uint8_t*)z + 32, (size_t)64, 1);
+  LAUNCH_NTT(k_ntt, n, S.poly(EHAT));
+  LAUNCH(k_pointwise, n128, n, S.poly(AS), S.poly(SHAT), S.poly(AHAT));
+  LAUNCH(k_addsub, n128, n, S.poly(BHAT), S.poly(EHAT), S.poly(AS), 0);
+  LAUNCH(k_tobytes, n256, n, sk, (size_t)SK, S.poly(SHAT), (const uint8_t*)nullptr, (size_t)0);
+  LAUNCH(k_tobytes, n256, n, pk, (size_t)PK, S.poly(BHAT), (const uint8_t*)z, (size_t)64);
   return hip_err(hipGetLastError());
 }
 
 int newhope_enc(pqc_ctx* c, size_t n_, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
                 hipStream_t s) {
   using namespace newhope;
-  const int block = c->tune.newhope_block;
-  const int n = (int)n_, grid = (n + block - 1) / block;
+  const newhope::Launch L{c->tune.newhope_block};
+  const int n = (int)n_;
+  const size_t n16 = 16 * n_, n32 = 32 * n_, n128 = 128 * n_, n256 = 256 * n_;
   Scratch S{c};
-  uint8_t* seed = c->nhbytes;  // publicseed copy, 32 bytes per element (stride 64)
+  uint8_t* seed = c->nhbytes;  // publicseed copy, 32 bytes per element
   const int SP = 0, EP = 1, VP = 2, AHAT = 3, BHAT = 4, EPP = 5, UHAT = 6, V = 7;
-  LAUNCH(k_frommsg, n, S.poly(V), m);
-  LAUNCH(k_frombytes, n, S.poly(BHAT), pk, (size_t)PK, seed);
-  LAUNCH(k_uniform, n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
-  LAUNCH(k_sample, n, S.poly(SP), coin, (size_t)32, 0);
-  LAUNCH(k_sample, n, S.poly(EP), coin, (size_t)32, 1);
-  LAUNCH(k_sample, n, S.poly(EPP), coin, (size_t)32, 2);
-  LAUNCH(k_ntt, n, S.poly(SP));
-  LAUNCH(k_ntt, n, S.poly(EP));
-  LAUNCH(k_pointwise, n, S.poly(UHAT), S.poly(AHAT), S.poly(SP));
-  LAUNCH(k_addsub, n, S.poly(UHAT), S.poly(UHAT), S.poly(EP), 0);
-  LAUNCH(k_pointwise, n, S.poly(VP), S.poly(BHAT), S.poly(SP));
-  LAUNCH(k_invntt, n, S.poly(VP));
-  LAUNCH(k_addsub, n, S.poly(VP), S.poly(VP), S.poly(EPP), 0);
-  LAUNCH(k_addsub, n, S.poly(VP), S.poly(VP), S.poly(V), 0);
-  LAUNCH(k_tobytes, n, ct, (size_t)CT, S.poly(UHAT), (const uint8_t*)nullptr, (size_t)0);
-  LAUNCH(k_compress, n, ct, S.poly(VP));
+  LAUNCH(k_frommsg, n32, n, S.poly(V), m);
+  LAUNCH(k_frombytes, n256, n, S.poly(BHAT), pk, (size_t)PK, seed);
+  LAUNCH(k_uniform, n16, n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
+  LAUNCH(k_sample, n16, n, S.poly(SP), coin, (size_t)32, 0);
+  LAUNCH(k_sample, n16, n, S.poly(EP), coin, (size_t)32, 1);
+  LAUNCH(k_sample, n16, n, S.poly(EPP), coin, (size_t)32, 2);
+  LAUNCH_NTT(k_ntt, n, S.poly(SP));
+  LAUNCH_NTT(k_ntt, n, S.poly(EP));
+  LAUNCH(k_pointwise, n128, n, S.poly(UHAT), S.poly(AHAT), S.poly(SP));
+  LAUNCH(k_addsub, n128, n, S.poly(UHAT), S.poly(UHAT), S.poly(EP), 0);
+  LAUNCH(k_pointwise, n128, n, S.poly(VP), S.poly(BHAT), S.poly(SP));
+  LAUNCH_NTT(k_invntt, n, S.poly(VP));
+  LAUNCH(k_addsub, n128, n, S.poly(VP), S.poly(VP), S.poly(EPP), 0);
+  LAUNCH(k_addsub, n128, n, S.poly(VP), S.poly(VP), S.poly(V), 0);
+  LAUNCH(k_tobytes, n256, n, ct, (size_t)CT, S.poly(UHAT), (const uint8_t*)nullptr, (size_t)0);
+  LAUNCH(k_compress, n128, n, ct, S.poly(VP));
   return hip_err(hipGetLastError());
 }
 
 int newhope_dec(pqc_ctx* c, size_t n_, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   using namespace newhope;
-  const int block = c->tune.newhope_block;
-  const int n = (int)n_, grid = (n + block - 1) / block;
+  const newhope::Launch L{c->tune.newhope_block};
+  const int n = (int)n_;
+  const size_t n32 = 32 * n_, n128 = 128 * n_, n256 = 256 * n_;
   Scratch S{c};
   const int VP = 0, UHAT = 1, TMP = 2, SHAT = 3;
-  LAUNCH(k_frombytes, n, S.poly(SHAT), sk, (size_t)SK, (uint8_t*)nullptr);
-  LAUNCH(k_frombytes, n, S.poly(UHAT), ct, (size_t)CT, (uint8_t*)nullptr);
-  LAUNCH(k_decompress, n, S.poly(VP), ct);
-  LAUNCH(k_pointwise, n, S.poly(TMP), S.poly(SHAT), S.poly(UHAT));
-  LAUNCH(k_invntt, n, S.poly(TMP));
-  LAUNCH(k_addsub, n, S.poly(TMP), S.poly(TMP), S.poly(VP), 1);
-  LAUNCH(k_tomsg, n, m, S.poly(TMP));
+  LAUNCH(k_frombytes, n256, n, S.poly(SHAT), sk, (size_t)SK, (uint8_t*)nullptr);
+  LAUNCH(k_frombytes, n256, n, S.poly(UHAT), ct, (size_t)CT, (uint8_t*)nullptr);
+  LAUNCH(k_decompress, n128, n, S.poly(VP), ct);
+  LAUNCH(k_pointwise, n128, n, S.poly(TMP), S.poly(SHAT), S.poly(UHAT));
+  LAUNCH_NTT(k_invntt, n, S.poly(TMP));
+  LAUNCH(k_addsub, n128, n, S.poly(TMP), S.poly(TMP), S.poly(VP), 1);
+  LAUNCH(k_tomsg, n32, n, m, S.poly(TMP));
   return hip_err(hipGetLastError());
 }
 
