@@ -349,6 +349,17 @@ def bench_newhope(args):
     ms = wall * 1e3 / args.steps
     log("newhope triples: %.3f ms/step" % ms)
     ok = bool((m2.download((n, 32)) == msg.download((n, 32))).all())
+
+    # The step's largest kernel is poly_sample (5 launches per triple batch); time
+    # it alone on the same stream, and the NTT (HBM figure) beside it.
+    polys = pqc.DeviceBuffer(n * 2048)
+    L = pqc.lib()
+    sample_ms = timed_run(lambda: L.pqc_newhope1024_poly_sample(n, polys.ptr, coin.ptr, 32, 0, stream.handle),
+                          stream, 20, 2, pqc)
+    ntt_ms = timed_run(lambda: L.pqc_newhope1024_poly_transform(0, n, polys.ptr, stream.handle), stream, 20, 2, pqc)
+    sample_ops = n * 16 * OPS_PER_KECCAK  # 16 SHAKE256 permutations per polynomial
+    ntt_bytes = n * 2048 * 2              # read + write one uint16 polynomial
+    log("poly_sample %.1f us, poly_ntt %.1f us" % (sample_ms * 1e3, ntt_ms * 1e3))
     out = {
         "metric": "NewHope-1024 CPA keygen+encaps+decaps triples/s at batch 16384 (plain batched kernels)",
         "value": round(n / (ms / 1e3), 1),
@@ -365,8 +376,18 @@ def bench_newhope(args):
         "config": {"workload": "newhope1024_cpa_triples", "batch": n,
                    "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
                    "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
-                   "messages_recovered": ok},
-        "roofline": None,
+                   "messages_recovered": ok,
+                   "poly_ntt": {"kernel_ms": round(ntt_ms, 4), "bytes": ntt_bytes,
+                                "hbm_gbps": round(ntt_bytes / ntt_ms / 1e6, 1),
+                                "hbm_frac": round(ntt_bytes / ntt_ms / 1e6 / HBM_PEAK_GBPS, 4)}},
+        "roofline": {"bound": "valu", "kernel": "newhope::k_sample (poly_sample, 16 SHAKE256 streams per polynomial)",
+                     "achieved": round(sample_ops / sample_ms / 1e9, 2), "peak": VALU_PEAK_TOPS,
+                     "unit": "T int32 ops/s", "frac": round(sample_ops / sample_ms / 1e9 / VALU_PEAK_TOPS, 4),
+                     "work_per_launch": "%d polynomials x 16 Keccak-f x 7440 two-input 32-bit ops" % n,
+                     "kernel_ms": round(sample_ms, 4), "traffic": None,
+                     "note": "7440 counts two-input ops; v_bitop3_b32 retires two of them in one lane-cycle "
+                             "(4320 VALU instructions per permutation), so the rate can pass the 1-op peak",
+                     "valu_instr_frac": round(n * 16 * 4320 / sample_ms / 1e9 / VALU_PEAK_TOPS, 4)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

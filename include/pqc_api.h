@@ -347,6 +347,15 @@ int pqc_event_elapsed_ms(float *ms, void *start, void *stop);
  * of 8.  Replaces sha3_256_n / sha3_512_n (fips202.cu:634-717).
  * pqc_kyber_poly_transform: n polynomials of 256 int16 in place; op 0 =
  * poly_ntt (ntt + Barrett), 1 = poly_invntt.
+ * pqc_newhope1024_poly_transform: n polynomials of 1024 uint16 in place
+ * (element-contiguous, device memory); op 0 = poly_ntt (mul_coefficients by
+ * gammas + ntt), 1 = poly_invntt (bitrev_vector + ntt + mul_coefficients),
+ * replacing NewHope_NIST_GPU/poly.cu:519-547 (the reference runs it per
+ * polynomial inside its CPA kernels; here one wave per polynomial).
+ * pqc_newhope1024_poly_sample: n noise polynomials (1024 uint16 each, device)
+ * from the 32-byte seeds at seeds + i*seed_stride (seed_stride a multiple of 8)
+ * and one nonce byte: poly_sample (NewHope_NIST_GPU/poly.cu:396-444), one lane
+ * per 64-coefficient SHAKE256 stream.
  * pqc_frodo976_gemm: re-runs only the shared-A product
  * [B'' | V] = S'.[A | B] + [E' | E''] of the last Frodo-976 batch of n >= this n
  * elements (frodo_mul_add_sa_plus_e_gpu + frodo_mul_add_sb_plus_e_kernel,
@@ -357,6 +366,9 @@ int pqc_event_elapsed_ms(float *ms, void *start, void *stop);
 int pqc_keccak_batch(int op, size_t n, uint8_t *out, size_t outlen, const uint8_t *in, size_t inlen,
                      void *stream);
 int pqc_kyber_poly_transform(int op, size_t n, int16_t *polys, void *stream);
+int pqc_newhope1024_poly_transform(int op, size_t n, uint16_t *polys, void *stream);
+int pqc_newhope1024_poly_sample(size_t n, uint16_t *polys, const uint8_t *seeds, size_t seed_stride, int nonce,
+                                void *stream);
 int pqc_frodo976_gemm(pqc_ctx *ctx, size_t n, const uint8_t *ct, void *stream);
 
 const char *pqc_version(void);

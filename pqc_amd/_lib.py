@@ -67,6 +67,8 @@ SIGNATURES = {
     "pqc_event_elapsed_ms": (_I, [C.POINTER(C.c_float), _P, _P]),
     "pqc_keccak_batch": (_I, [_I, _SZ, _P, _SZ, _P, _SZ, _P]),
     "pqc_kyber_poly_transform": (_I, [_I, _SZ, _P, _P]),
+    "pqc_newhope1024_poly_transform": (_I, [_I, _SZ, _P, _P]),
+    "pqc_newhope1024_poly_sample": (_I, [_SZ, _P, _P, _SZ, _I, _P]),
     "pqc_frodo976_gemm": (_I, [_P, _SZ, _P, _P]),
     "pqc_newhope1024_cpa_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _P, _U, _P]),

@@ -86,6 +86,9 @@ int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_
 int newhope_enc(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
                 hipStream_t s);
 int newhope_dec(pqc_ctx* c, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
+int newhope_poly_transform(int op, size_t n, uint16_t* polys, hipStream_t s);
+int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,
+                        hipStream_t s);
 
 // FrodoKEM-976 (frodo.hip)
 size_t frodo_bytes_per_elem();

@@ -536,6 +536,30 @@ int newhope_enc(pqc_ctx* c, size_t n_, uint8_t* ct, const uint8_t* m, const uint
   return hip_err(hipGetLastError());
 }
 
+int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,
+                        hipStream_t s) {
+  using namespace newhope;
+  if (nonce < 0 || nonce > 255 || seed_stride < 32 || seed_stride % 8) return PQC_ERR_INVALID_ARG;
+  if (n == 0) return PQC_OK;
+  if (n > (size_t)INT32_MAX / 16) return PQC_ERR_BATCH_TOO_LARGE;
+  const size_t threads = 16 * n;
+  hipLaunchKernelGGL(k_sample, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, (int)n, Pv{polys}, seeds,
+                     seed_stride, nonce);
+  return hip_err(hipGetLastError());
+}
+
+int newhope_poly_transform(int op, size_t n, uint16_t* polys, hipStream_t s) {
+  using namespace newhope;
+  if (op != 0 && op != 1) return PQC_ERR_INVALID_ARG;
+  if (n == 0) return PQC_OK;
+  if (n > (size_t)INT32_MAX) return PQC_ERR_BATCH_TOO_LARGE;
+  if (op == 0)
+    hipLaunchKernelGGL(k_ntt, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv{polys});
+  else
+    hipLaunchKernelGGL(k_invntt, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv{polys});
+  return hip_err(hipGetLastError());
+}
+
 int newhope_dec(pqc_ctx* c, size_t n_, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   using namespace newhope;
   const newhope::Launch L{c->tune.newhope_block};

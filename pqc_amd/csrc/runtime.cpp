@@ -789,6 +789,17 @@ int pqc_kyber_poly_transform(int op, size_t n, int16_t* polys, void* stream) {
   return pqc::kyber_poly_transform(op, n, polys, (hipStream_t)stream);
 }
 
+int pqc_newhope1024_poly_transform(int op, size_t n, uint16_t* polys, void* stream) {
+  if (!polys) return PQC_ERR_INVALID_ARG;
+  return pqc::newhope_poly_transform(op, n, polys, (hipStream_t)stream);
+}
+
+int pqc_newhope1024_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,
+                                void* stream) {
+  if (!polys || !seeds) return PQC_ERR_INVALID_ARG;
+  return pqc::newhope_poly_sample(n, polys, seeds, seed_stride, nonce, (hipStream_t)stream);
+}
+
 const char* pqc_version(void) { return "pqc-amd 0.1 (gfx950)"; }
 
 const char* pqc_strerror(int code) {

@@ -399,6 +399,24 @@ def keccak_batch(op, msgs, outlen):
     return dout.download((n, outlen))
 
 
+def newhope_poly_transform(op, polys):
+    """op 0 = poly_ntt, 1 = poly_invntt on an (n, 1024) uint16 array (device)."""
+    polys = np.ascontiguousarray(polys, dtype=np.uint16)
+    d = DeviceBuffer.from_array(polys)
+    check(L.lib().pqc_newhope1024_poly_transform(op, polys.shape[0], d.ptr, None), "pqc_newhope1024_poly_transform")
+    return d.download(polys.shape, np.uint16)
+
+
+def newhope_poly_sample(seeds, nonce):
+    """poly_sample of each 32-byte seed row with one nonce -> (n, 1024) uint16."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint8)
+    n = seeds.shape[0]
+    d_seed, d_out = DeviceBuffer.from_array(seeds), DeviceBuffer(n * 2048)
+    check(L.lib().pqc_newhope1024_poly_sample(n, d_out.ptr, d_seed.ptr, seeds.shape[1], nonce, None),
+          "pqc_newhope1024_poly_sample")
+    return d_out.download((n, 1024), np.uint16)
+
+
 def kyber_poly_transform(op, polys):
     """op 0 = poly_ntt, 1 = poly_invntt on an (n, 256) int16 array (device)."""
     polys = np.ascontiguousarray(polys, dtype=np.int16)

@@ -70,3 +70,28 @@ def test_full_size_16384_roundtrip_and_sample(ctx):
     opk, osk, oct_, _ = O.newhope_triples(np.ascontiguousarray(seed[idx]), np.ascontiguousarray(msg[idx]),
                                          np.ascontiguousarray(coin[idx]))
     assert (pk[idx] == opk).all() and (ct[idx] == oct_).all()
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_poly_transform_vs_oracle(op):
+    """The wave-per-polynomial NTT / inverse NTT on their own, element by element,
+    on in-range coefficients and on arbitrary uint16 values (the reference's
+    16-bit wraparound of the lazy layers must be reproduced exactly)."""
+    import pqc_amd
+    say("newhope poly transform op", op)
+    rng = np.random.default_rng(40 + op)
+    polys = np.concatenate([rng.integers(0, 12289, (150, 1024)), rng.integers(0, 65536, (50, 1024))]).astype(np.uint16)
+    got = pqc_amd.newhope_poly_transform(op, polys)
+    ref = O.newhope_ntt if op == 0 else O.newhope_invntt
+    for i in range(len(polys)):
+        assert (got[i] == ref(polys[i])).all(), i
+
+
+def test_poly_sample_vs_oracle():
+    import pqc_amd
+    say("newhope poly_sample")
+    seeds = O.drbg_array(70, 32, 0, 810)
+    for nonce in (0, 2, 255):
+        got = pqc_amd.newhope_poly_sample(seeds, nonce)
+        for i in range(len(seeds)):
+            assert (got[i] == O.newhope_sample(seeds[i].tobytes(), nonce)).all(), (nonce, i)
