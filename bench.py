@@ -137,7 +137,8 @@ def main():
         # each fused server slot drives 3 streams; with HIP's default of 4 hardware
         # queues per process, streams share queues and batches serialise behind each
         # other.  Must be set before the runtime initialises.
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+            os.environ["GPU_MAX_HW_QUEUES"] = "16"
         return bench_serving(args)
 
     import pqc_amd as pqc
@@ -408,6 +409,8 @@ def bench_newhope(args):
 
 
 # ---- mixed Kyber-512/768/1024 (1:2:1), 65536 encaps + decaps, fused in a graph ---------
+# Keccak-f per decapsulation: G 1 + A^T 4 K^2 + PRF 2K+1 + H(ct) ceil((ct+1)/136) + KDF 1
+MIXED_KECCAK_PER_DECAPS = {1: 1 + 16 + 5 + 6 + 1, 2: 1 + 36 + 7 + 9 + 1, 3: 1 + 64 + 9 + 12 + 1}
 def bench_mixed(args):
     import pqc_amd as pqc
     from pqc_amd import tuner as T
@@ -456,6 +459,13 @@ def bench_mixed(args):
         off, cnt = lay.region["ss"][a]
         return _Slice(ss_dec, off, cnt * 32)
 
+    # the dominant kernel, k_fused_mixed<decaps>, timed alone on the stream (with the
+    # descriptor classification it needs: two small kernels of ~40 us together)
+    dec_ms = timed_run(lambda: ctx.kem_dec_mixed(n, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ss_dec, stream=stream),
+                       stream, 10, 2, pqc)
+    dec_ops = sum(lay.count[a] * MIXED_KECCAK_PER_DECAPS[a] for a in (1, 2, 3)) * OPS_PER_KECCAK
+    log("mixed decaps alone: %.3f ms" % dec_ms)
+
     psteps = max(3, args.steps // 4)
     wall_p, _ = timed(plain_step, stream, psteps, 1, pqc)
     ms_p = wall_p * 1e3 / psteps
@@ -483,9 +493,37 @@ def bench_mixed(args):
                    "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
                    "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
                    "bitexact_fused_plain": bitexact},
-        "roofline": None,
+        "roofline": {"bound": "valu", "kernel": "k_fused_mixed<decaps> (persistent, all three parameter sets)",
+                     "achieved": round(dec_ops / dec_ms / 1e9, 3), "peak": VALU_PEAK_TOPS, "unit": "T int32 ops/s",
+                     "frac": round(dec_ops / dec_ms / 1e9 / VALU_PEAK_TOPS, 4),
+                     "work_per_launch": "Keccak-f per decaps 29 / 54 / 87 (Kyber-512 / 768 / 1024) x 7440 ops, "
+                                        "%d / %d / %d requests" % (lay.count[1], lay.count[2], lay.count[3]),
+                     "kernel_ms": round(dec_ms, 4), "traffic": None},
         "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        # the oracle on the host cores: encaps + decaps of a sample of each parameter set
+        from tests import oracle_lib as O
+        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        pk_h, sk_h, co_h = ar.download("pk"), ar.download("sk"), ar.download("coins")
+        ct_h, ssd_h = ar.download("ct"), ss_dec.download((lay.nbytes["ss"],))
+        done, dt = 0, 0.0
+        for a in (1, 2, 3):
+            m = min(lay.count[a], 192 * thr // 4)
+            pk_a, co_a = (np.ascontiguousarray(lay.view(x, kd, a)[:m]) for x, kd in ((pk_h, "pk"), (co_h, "coins")))
+            sk_a, ct_a = np.ascontiguousarray(lay.view(sk_h, "sk", a)[:m]), np.ascontiguousarray(lay.view(ct_h, "ct", a)[:m])
+            t0 = time.perf_counter()
+            oct_, oss = O.kyber_batch("enc", a + 1, [pk_a, co_a], nthreads=thr)
+            oss2 = O.kyber_batch("dec", a + 1, [ct_a, sk_a], nthreads=thr)
+            dt += time.perf_counter() - t0
+            assert (oct_ == ct_a).all() and (oss2 == lay.view(ssd_h, "ss", a)[:m]).all(), "CPU oracle and GPU disagree"
+            done += m
+        out["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "requests/s (encaps + decaps each)",
+                               "cores": thr, "kind": "port",
+                               "sample": "%d requests (first of each parameter set's slice, 1:2:1 weights not applied) "
+                                         "encapsulated and decapsulated on %d threads; outputs checked equal to the "
+                                         "GPU's" % (done, thr)}
+        out["config"]["gpu_over_cpu"] = round(out["value"] / (done / dt), 1)
     print(json.dumps(out), flush=True)
     ctx.close()
 
