@@ -565,6 +565,8 @@ def bench_serving(args):
         results[plain] = (st, m)
         log("plain=%s: %d requests, %.0f ops/s, p50 %.0f us, p99 %.0f us, mean batch %.0f, errors %d" %
             (plain, m, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch, st.errors))
+        if not plain and not args.no_cpu:
+            cpu = serving_cpu_baseline(srv, rs, algs, ops, kid, args.cpu_threads)
         srv.close()
     st, m = results[False]
     sp, mp = results[True]
@@ -589,9 +591,42 @@ def bench_serving(args):
                    "plain": {"requests": mp, "ops_per_s": round(sp.ops_per_s, 1), "p50_us": round(sp.p50_us, 1),
                              "p99_us": round(sp.p99_us, 1), "mean_batch": round(sp.mean_batch, 1)}},
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def serving_cpu_baseline(srv, rs, algs, ops, kid, cpu_threads, sample=3000):
+    """The oracle on the host cores over the first `sample` requests of the same
+    trace (same mix, same keys and inputs), one request per task on a thread pool;
+    every output is checked equal to what the server returned."""
+    from concurrent.futures import ThreadPoolExecutor
+    from tests import oracle_lib as O
+    thr = max(1, min(cpu_threads, len(os.sched_getaffinity(0))))
+    m = min(sample, rs.n)
+    keys = {}
+    for i in range(m):
+        keys.setdefault((int(algs[i]), int(kid[i])), None)
+    for a, k in keys:
+        keys[(a, k)] = srv.key(a, k)
+    ins = [rs.inbuf[rs.in_off[i]:rs.in_off[i + 1]].tobytes() for i in range(m)]
+
+    def one(i):
+        a, op = int(algs[i]), int(ops[i])
+        pk, sk = keys[(a, int(kid[i]))]
+        if a == 5:
+            return b"".join(O.frodo_enc(pk, ins[i])) if op == 1 else O.frodo_dec(ins[i], sk)
+        return b"".join(O.kyber_enc(a + 1, pk, ins[i])) if op == 1 else O.kyber_dec(a + 1, ins[i], sk)
+
+    with ThreadPoolExecutor(thr) as ex:
+        list(ex.map(one, range(min(m, 2 * thr))))  # warm
+        t0 = time.perf_counter()
+        got = list(ex.map(one, range(m)))
+        dt = time.perf_counter() - t0
+    assert all(got[i] == rs.out(i) for i in range(m)), "CPU oracle and server disagree"
+    return {"value": round(m / dt, 1), "unit": "requests/s", "cores": thr, "kind": "port",
+            "sample": "the first %d requests of the trace (same mix, keys and inputs), one request per task "
+                      "on %d threads, no arrival pacing; outputs checked equal to the server's" % (m, thr)}
 
 
 if __name__ == "__main__":
