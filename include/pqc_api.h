@@ -12,6 +12,9 @@
  *   cpapke_keypair/enc/dec ....... NewHope_NIST_GPU/cpapke.h:14-24
  *   sha3_256_n / sha3_512_n ...... Kyber_GPU_Batched/fips202.h:19-38
  *   poly_ntt_n / poly_invntt_n ... Kyber_GPU_Batched/poly.h (poly.cu:159-190)
+ *   poly_ntt / poly_invntt,
+ *   poly_sample (NewHope) ........ NewHope_NIST_GPU/poly.cu:396-444,519-547
+ *   crypto_kem_*_gpu (Frodo) ..... FrodoKEM_GPU/api.h:20-24, frodo976.cu:87-690
  *   poly_set4 scratch ............ Kyber_GPU_Batched/main.h:14-39 (now owned
  *                                  by a pqc_ctx)
  */
