@@ -122,11 +122,18 @@ def main():
                     default="kyber768")
     ap.add_argument("--rate", type=float, default=2e6, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=0.5, help="serving: trace length, seconds")
+    ap.add_argument("--fixed-rate", type=float, default=4e6,
+                    help="kyber768: Poisson arrival rate of the fixed-rate latency phase, decaps/s (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards by request with no collective, "
               "so multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    # The serving loop drives up to 3 streams per slot; with HIP's default of 4
+    # hardware queues per process, streams share queues and batches serialise
+    # behind each other.  Must be set before the runtime initialises.
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     if args.workload == "frodo976":
         return bench_frodo(args)
     if args.workload == "newhope1024":
@@ -134,11 +141,6 @@ def main():
     if args.workload == "kyber_mixed":
         return bench_mixed(args)
     if args.workload == "serving":
-        # each fused server slot drives 3 streams; with HIP's default of 4 hardware
-        # queues per process, streams share queues and batches serialise behind each
-        # other.  Must be set before the runtime initialises.
-        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
-            os.environ["GPU_MAX_HW_QUEUES"] = "16"
         return bench_serving(args)
 
     import pqc_amd as pqc
@@ -235,8 +237,40 @@ def main():
             "single_core_value": round(r_one, 1),
         }
         out["config"]["gpu_over_cpu"] = round(rate_f / r_all, 1)
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if args.fixed_rate > 0:
+        out["config"]["fixed_rate"] = fixed_rate_latency(args.fixed_rate, 0.25)
+    print(json.dumps(out), flush=True)
+
+
+def fixed_rate_latency(rate, duration, keys=1024):
+    """p50 / p99 latency of Kyber-768 decapsulations arriving as a Poisson stream at
+    `rate` per second, through the serving loop (continuous batching, 1024
+    device-resident key pairs), fused kernels vs plain per-op kernels on the same
+    trace; latency = arrival to completion, host ciphertext in to host secret out."""
+    from pqc_amd.server import RequestSet, Server
+    n = int(rate * duration)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1e6 / rate, size=n))
+    algs = np.full(n, 2, np.int32)
+    ops = np.full(n, 2, np.int32)
+    kid = rng.integers(0, keys, n).astype(np.uint32)
+    res = {"rate": rate, "requests": n}
+    for plain in (False, True):
+        srv = Server(["kyber768"], keys_per_alg=keys, key_seed=0, max_batch=65536, slots=4,
+                     arena_bytes=128 << 20, max_wait_us=100, plain=plain)
+        wm = min(n, 100000)
+        ws = RequestSet.random(algs[:wm], ops[:wm], kid[:wm], rng)
+        srv.run_trace(arrivals[:wm], ws.reqs)  # warm-up
+        rs = RequestSet.random(algs, ops, kid, rng)
+        st, _ = srv.run_trace(arrivals, rs.reqs)
+        srv.close()
+        log("fixed rate %.1fM/s plain=%s: %.0f decaps/s, p50 %.0f us, p99 %.0f us, mean batch %.0f" %
+            (rate / 1e6, plain, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch))
+        res["plain" if plain else "fused"] = {"completed_per_s": round(st.ops_per_s, 1), "p50_us": round(st.p50_us, 1),
+                                              "p99_us": round(st.p99_us, 1), "mean_batch": round(st.mean_batch, 1),
+                                              "errors": int(st.errors)}
+    return res
 
 
 # ---- FrodoKEM-976: 4096 decapsulations under one device-resident key -------------
