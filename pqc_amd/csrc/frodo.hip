@@ -422,9 +422,8 @@ constexpr int DMA_PER_STEP = 6;  // global_load_lds issued per wave per k-step
 
 template <int MODE>
 struct EpiRegs {
-  uint4 e[4];                              // E' | E'' bytes of the 4 m-tiles
   uint2 c[MODE == kDecaps ? 4 : 1][4];     // ciphertext words (decapsulation)
-  static constexpr int kLoads = MODE == kDecaps ? 20 : 4;
+  static constexpr int kLoads = MODE == kDecaps ? 16 : 0;
 };
 
 template <int MODE>
@@ -437,11 +436,7 @@ __device__ __forceinline__ uint8_t* ct_word(uint8_t* ct, int row, int n0) {
 }
 
 template <int MODE>
-__device__ __forceinline__ void epi_issue(EpiRegs<MODE>& ep, const GemmArgs& g, const int8_t* efrag, uint8_t* ct,
-                                          int nrows) {
-#pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
-    ep.e[mi] = *reinterpret_cast<const uint4*>(efrag + (((size_t)(g.mt0 + mi) * 16 + g.nb) * 64 + g.lane) * 16);
+__device__ __forceinline__ void epi_issue(EpiRegs<MODE>& ep, const GemmArgs& g, uint8_t* ct, int nrows) {
   if constexpr (MODE == kDecaps) {
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
@@ -456,8 +451,8 @@ __device__ __forceinline__ void epi_issue(EpiRegs<MODE>& ep, const GemmArgs& g, 
 }
 
 template <int MODE, int KS>
-__device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g, EpiRegs<MODE>& ep,
-                                          const int8_t* efrag, uint8_t* ct, int nrows) {
+__device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g, EpiRegs<MODE>& ep, uint8_t* ct,
+                                          int nrows) {
   // this wave's DMAs of step KS done; what may stay in flight: the DMAs of step
   // KS + 1 (issued before) and, from step 14 on, the epilogue loads (issued at 13)
   constexpr int E = EpiRegs<MODE>::kLoads;
@@ -470,7 +465,7 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
     // the explicit vmcnt values above count the epilogue loads as the newest:
     // keep the scheduler from moving them ahead of this step's DMAs
     __builtin_amdgcn_sched_barrier(0);
-    epi_issue<MODE>(ep, g, efrag, ct, nrows);
+    epi_issue<MODE>(ep, g, ct, nrows);
   }
   constexpr int st = KS % 3;
   v4i a[4];
@@ -487,12 +482,23 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
         acc[mi][ni][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b, a[mi], acc[mi][ni][d], 0, 0, 0);
     }
   }
-  if constexpr (KS + 1 < KSTEPS) gemm_step<MODE, KS + 1>(acc, g, ep, efrag, ct, nrows);
+  if constexpr (KS + 1 < KSTEPS) gemm_step<MODE, KS + 1>(acc, g, ep, ct, nrows);
 }
 
 __device__ __forceinline__ uint32_t bswap16x2(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x02030001u); }
 __device__ __forceinline__ uint32_t sx8(uint32_t w, int i) { return (uint32_t)(((int32_t)(w << (24 - 8 * i))) >> 24); }
 __device__ __forceinline__ uint32_t word_of(const uint4& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+
+// one flag byte per (request, 64-column block): lanes with (l & 15) < 8 hold
+// request 2 mt, the others 2 mt + 1
+__device__ __forceinline__ void gemm_flags(uint32_t mism, const GemmArgs& g, int mi, int n, uint8_t* flags) {
+  const uint64_t bal = __ballot(mism != 0);
+  const int req0 = 2 * (g.mt0 + mi);
+  if (g.lane == 0) {
+    if (req0 < n) flags[(size_t)req0 * GEMM_NB + g.nb] = (uint8_t)((bal & 0x00FF00FF00FF00FFull) != 0);
+    if (req0 + 1 < n) flags[(size_t)(req0 + 1) * GEMM_NB + g.nb] = (uint8_t)((bal & 0xFF00FF00FF00FF00ull) != 0);
+  }
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* __restrict__ sfrag,
@@ -500,27 +506,65 @@ __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* 
                                                  uint16_t* __restrict__ out, uint8_t* __restrict__ ct,
                                                  const uint8_t* __restrict__ mu, int mu_stride,
                                                  uint8_t* __restrict__ flags) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave-uniform in an SGPR, so every index derived from it stays scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   // XCD-aware order: dispatch slot s on XCD x -> row block 8 (s / 16) + x, column block s % 16
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int mb = 8 * (slot / GEMM_NB) + xcd, nb = slot % GEMM_NB;
   if (mb >= mblocks) return;
   const GemmArgs g{sfrag, bfrag, mb * 16 + wave * 4, nb, wave, lane};
   const int nrows = 8 * n;
-  v4i acc[4][4][2];
+  // [E' | E''] of this wave's tile (16 bytes per m-tile, in the output's lane
+  // order) is the lo digit's starting value: the epilogue then only combines digits
+  uint4 e[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int d = 0; d < 2; ++d) acc[a][b][d] = v4i{0, 0, 0, 0};
-  EpiRegs<MODE> ep;
+  for (int mi = 0; mi < 4; ++mi)
+    e[mi] = *reinterpret_cast<const uint4*>(efrag + (((size_t)(g.mt0 + mi) * 16 + nb) * 64 + lane) * 16);
   gemm_issue<0>(g);
   gemm_issue<1>(g);
-  gemm_step<MODE, 0>(acc, g, ep, efrag, ct, nrows);
+  v4i acc[4][4][2];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        acc[mi][ni][0][r] = (int)sx8(word_of(e[mi], ni), r);
+        acc[mi][ni][1][r] = 0;
+      }
+  EpiRegs<MODE> ep;
+  gemm_step<MODE, 0>(acc, g, ep, ct, nrows);
 
-  // epilogue: value = lo + 256 hi + E (mod 2^16); lane holds row 16 mt + (l & 15),
-  // columns n0 .. n0 + 3 with n0 = 16 nt + 4 (l >> 4)
+  // epilogue: value = acc_lo + 256 acc_hi (mod 2^16) (acc_lo started at E); lane
+  // holds row 16 mt + (l & 15), columns n0 .. n0 + 3 with n0 = 16 nt + 4 (l >> 4)
+  if (16 * (g.mt0 + 4) <= nrows && nb < GEMM_NB - 1) {
+    // every row is a request's and every column a B'' column (uniform)
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const int row = 16 * (g.mt0 + mi) + (lane & 15);
+      uint32_t mism = 0;
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        const int n0 = 16 * (4 * nb + ni) + 4 * (lane >> 4);
+        uint32_t v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = (uint32_t)acc[mi][ni][0][r] + ((uint32_t)acc[mi][ni][1][r] << 8);
+        if constexpr (MODE == kProducts) {
+          const uint32_t p0 = __builtin_amdgcn_perm(v[1], v[0], 0x05040100u), p1 = __builtin_amdgcn_perm(v[3], v[2], 0x05040100u);
+          *reinterpret_cast<uint2*>(out + (size_t)row * OUTC + n0) = make_uint2(p0, p1);
+        } else {
+          // two uint16 big-endian per word, as the ciphertext holds them
+          const uint32_t p0 = __builtin_amdgcn_perm(v[1], v[0], 0x04050001u), p1 = __builtin_amdgcn_perm(v[3], v[2], 0x04050001u);
+          if constexpr (MODE == kEncaps)
+            *reinterpret_cast<uint2*>(ct_word<MODE>(ct, row, n0)) = make_uint2(p0, p1);
+          else
+            mism |= (p0 ^ ep.c[mi][ni].x) | (p1 ^ ep.c[mi][ni].y);
+        }
+      }
+      if constexpr (MODE == kDecaps) gemm_flags(mism, g, mi, n, flags);
+    }
+    return;
+  }
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
     const int row = 16 * (g.mt0 + mi) + (lane & 15), req = row >> 3, k = row & 7;
@@ -529,11 +573,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* 
     for (int ni = 0; ni < 4; ++ni) {
       const int n0 = 16 * (4 * nb + ni) + 4 * (lane >> 4);
       if (row >= nrows || n0 >= N + NB) continue;
-      const uint32_t ew = word_of(ep.e[mi], ni);
       uint32_t v[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        v[r] = (uint32_t)acc[mi][ni][0][r] + ((uint32_t)acc[mi][ni][1][r] << 8) + sx8(ew, r);
+        v[r] = (uint32_t)acc[mi][ni][0][r] + ((uint32_t)acc[mi][ni][1][r] << 8);
         if constexpr (MODE != kProducts)
           if (n0 >= N) v[r] += encode_mu(mu + (size_t)req * mu_stride, k, n0 - N + r);
       }
@@ -546,15 +589,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* 
         mism |= (p0 ^ bswap16x2(ep.c[mi][ni].x)) | (p1 ^ bswap16x2(ep.c[mi][ni].y));
       }
     }
-    if constexpr (MODE == kDecaps) {
-      // lanes with (l & 15) < 8 hold request 2 mt, the others 2 mt + 1
-      const uint64_t bal = __ballot(mism != 0);
-      const int req0 = 2 * (g.mt0 + mi);
-      if (lane == 0) {
-        if (req0 < n) flags[(size_t)req0 * GEMM_NB + nb] = (uint8_t)((bal & 0x00FF00FF00FF00FFull) != 0);
-        if (req0 + 1 < n) flags[(size_t)(req0 + 1) * GEMM_NB + nb] = (uint8_t)((bal & 0xFF00FF00FF00FF00ull) != 0);
-      }
-    }
+    if constexpr (MODE == kDecaps) gemm_flags(mism, g, mi, n, flags);
   }
 }
 
