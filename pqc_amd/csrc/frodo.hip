@@ -145,9 +145,19 @@ __global__ void k_gen_a_many(int n, uint16_t* A, const uint8_t* seeds, size_t se
   }
 }
 
+// k of byte b of lane l's 16-byte fragment in k-step ks, or -1 for a byte no MFMA
+// reads.  Steps 0..14 are 64 deep (v_mfma_i32_16x16x64_i8: lane quarter l >> 4
+// holds k 16 q .. 16 q + 15); the last step holds the 16-column tail k = 960 .. 975
+// (976 = 15 x 64 + 16) for v_mfma_i32_16x16x32_i8 (quarter q holds k 960 + 8 q ..
+// +7 in bytes 0..7; quarters 2 and 3 are k >= 976, zero).
+__device__ __forceinline__ int frag_k(int ks, int lane, int b) {
+  if (ks < KSTEPS - 1) return 64 * ks + 16 * (lane >> 4) + b;
+  return b < 8 ? 64 * ks + 8 * (lane >> 4) + b : -1;
+}
+
 // Build the MFMA B-operand fragments of [A | B] as signed int8 digits.
 // Fragment (d, nt, ks), lane l holds 16 bytes: digit d of [A|B][k][n] for
-// n = 16 nt + (l & 15), k = 64 ks + 16 (l >> 4) + b, b = 0..15.
+// n = 16 nt + (l & 15), k = frag_k(ks, l, b), b = 0..15.
 __global__ void k_prep_frags(int8_t* frag, const uint16_t* A, const uint8_t* pk) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 2 * NT * KSTEPS * 64) return;
@@ -156,9 +166,9 @@ __global__ void k_prep_frags(int8_t* frag, const uint16_t* A, const uint8_t* pk)
   uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int b = 0; b < 16; ++b) {
-    const int k = 64 * ks + 16 * (lane >> 4) + b;
+    const int k = frag_k(ks, lane, b);
     uint32_t v = 0;
-    if (k < N) {
+    if (k >= 0 && k < N) {
       if (n < N)
         v = A[(size_t)k * N + n];
       else if (n < N + NB)
@@ -315,9 +325,11 @@ __global__ __launch_bounds__(256) void k_sample(int n, const uint8_t* raw, int8_
                                    : efrag + (size_t)n * N * NB + (size_t)r * 64 + (g - 2 * N * NB);
     *reinterpret_cast<uint2*>(dst) = make_uint2(lo, hi);
   } else if (g < N * NB) {
+    // column j of S' row m: k-step j >> 6, placed as frag_k lays the step out
     const int k = g / N, j = g % N, m = 8 * r + k;
-    const int lane = (m & 15) + 16 * ((j >> 4) & 3);
-    const size_t off = ((((size_t)(m >> 4) * KSTEPS + (j >> 6)) * 64 + lane) * 16) + (j & 15);
+    const int tail = j >= 64 * (KSTEPS - 1);
+    const int lane = (m & 15) + 16 * (tail ? ((j >> 3) & 1) : ((j >> 4) & 3));
+    const size_t off = ((((size_t)(m >> 4) * KSTEPS + (j >> 6)) * 64 + lane) * 16) + (tail ? 0 : (j & 15));
     *reinterpret_cast<uint2*>(sfrag + off) = make_uint2(lo, hi);
   } else {
     int m, j;  // the group's first entry; entries j .. j+3 and j+4 .. j+7 each sit in one lane's word
@@ -468,18 +480,36 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
     epi_issue<MODE>(ep, g, ct, nrows);
   }
   constexpr int st = KS % 3;
-  v4i a[4];
+  if constexpr (KS + 1 < KSTEPS) {
+    v4i a[4];
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi) a[mi] = ring_a<st>()[g.wave * 4 + mi][g.lane];
+    for (int mi = 0; mi < 4; ++mi) a[mi] = ring_a<st>()[g.wave * 4 + mi][g.lane];
 #pragma unroll
-  for (int ni = 0; ni < 4; ++ni) {
-    if (4 * g.nb + ni >= GEMM_NTILES) continue;  // uniform: padding n-tiles
+    for (int ni = 0; ni < 4; ++ni) {
+      if (4 * g.nb + ni >= GEMM_NTILES) continue;  // uniform: padding n-tiles
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      const v4i b = ring_b<st>()[2 * ni + d][g.lane];
+      for (int d = 0; d < 2; ++d) {
+        const v4i b = ring_b<st>()[2 * ni + d][g.lane];
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-        acc[mi][ni][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b, a[mi], acc[mi][ni][d], 0, 0, 0);
+        for (int mi = 0; mi < 4; ++mi)
+          acc[mi][ni][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b, a[mi], acc[mi][ni][d], 0, 0, 0);
+      }
+    }
+  } else {
+    // the 16-deep k tail (frag_k): 32-deep MFMAs on the first 8 bytes of each lane
+    long a[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) a[mi] = *reinterpret_cast<const long*>(&ring_a<st>()[g.wave * 4 + mi][g.lane]);
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      if (4 * g.nb + ni >= GEMM_NTILES) continue;
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const long b = *reinterpret_cast<const long*>(&ring_b<st>()[2 * ni + d][g.lane]);
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+          acc[mi][ni][d] = __builtin_amdgcn_mfma_i32_16x16x32_i8(b, a[mi], acc[mi][ni][d], 0, 0, 0);
+      }
     }
   }
   if constexpr (KS + 1 < KSTEPS) gemm_step<MODE, KS + 1>(acc, g, ep, ct, nrows);
