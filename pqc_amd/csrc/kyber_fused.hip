@@ -56,29 +56,47 @@ struct Lds {
 
 // ---- wave-cooperative pieces -------------------------------------------------
 
+// indcpa_dec inputs of one request, as this lane's words: the ciphertext's u and v
+// groups and the secret key's 12-bit groups.
+template <int K>
+struct DecIn {
+  uint32_t u[K][3], s[K][3], v[3];
+};
+
+template <int K>
+__device__ __forceinline__ void dec_load(DecIn<K>& in, const uint8_t* ct, const uint8_t* sk) {
+  using P = Params<K>;
+  const int l = lane_id();
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    load_group<P::du>(in.u[i], ct + 32 * P::du * i, nat_group(l));
+    load_h12(in.s[i], sk + 384 * i, ntt_group(l));
+  }
+  load_group<P::dv>(in.v, ct + P::polyvec_cbytes, nat_group(l));
+}
+
 // indcpa_dec for one request; the message is written to msg (LDS, 32 bytes).
 template <int K>
-__device__ __forceinline__ void coop_indcpa_dec(const uint8_t* ct, const uint8_t* sk, const LaneZ& z,
-                                                uint8_t* msg) {
+__device__ __forceinline__ void coop_indcpa_dec(const DecIn<K>& in, const LaneZ& z, uint8_t* msg) {
   using P = Params<K>;
   const int l = lane_id();
   P4 mp = {{0, 0, 0, 0}};
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     uint32_t t[4];
-    unpack_group<P::du>(t, ct + 32 * P::du * i, nat_group(l));
+    unpack_loaded<P::du>(t, in.u[i], nat_group(l));
     P4 u;
 #pragma unroll
     for (int r = 0; r < 4; ++r) u.v[r] = decompress_d<P::du>(t[r]);
     ntt(u, z);
     P4 s;
-    load_poly12(s, sk + 384 * i, ntt_group(l));
+    poly12_loaded(s, in.s[i]);
     basemul_acc(mp, s, u, z.bm);
   }
   reduce(mp);
   invntt(mp, z);
   uint32_t tv[4];
-  unpack_group<P::dv>(tv, ct + P::polyvec_cbytes, nat_group(l));
+  unpack_loaded<P::dv>(tv, in.v, nat_group(l));
   uint32_t nib = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -107,21 +125,45 @@ __device__ __forceinline__ void pack_pair(uint8_t* dst, const uint32_t (&t)[4]) 
   }
 }
 
+// Re-encryption inputs of one request, as this lane's words: its 16-bit PRF
+// groups (2K+1 noise polynomials), the public key's 12-bit groups and, for
+// decapsulation, the ciphertext's u and v groups to compare with.
+template <int K, bool DEC>
+struct EncIn {
+  uint32_t prf[2 * K + 1];
+  uint32_t t[K][3];
+  uint32_t cu[DEC ? K : 1][3], cv[3];
+};
+
+template <int K, bool DEC>
+__device__ __forceinline__ void enc_load(EncIn<K, DEC>& in, const uint8_t* prf, const uint8_t* pk, const uint8_t* ct) {
+  using P = Params<K>;
+  const int l = lane_id();
+  const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
+#pragma unroll
+  for (int j = 0; j < 2 * K + 1; ++j) in.prf[j] = prf16[64 * j + l];
+#pragma unroll
+  for (int j = 0; j < K; ++j) load_h12(in.t[j], pk + 384 * j, ntt_group(l));
+  if constexpr (DEC) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) load_group<P::du>(in.cu[i], ct + 32 * P::du * i, nat_group(l));
+    load_group<P::dv>(in.cv, ct + P::polyvec_cbytes, nat_group(l));
+  }
+}
+
 // Re-encryption of one request (indcpa_enc from sp-sampling on).  DEC: compare
 // with ct and return the lane's mismatch bits.  ENC: write ct.
 template <int K, bool DEC>
-__device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, const uint8_t* prf,
-                                                   const uint8_t* pk, const uint8_t* msg, uint8_t* ct,
-                                                   const LaneZ& z) {
+__device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, const EncIn<K, DEC>& in,
+                                                   const uint8_t* msg, uint8_t* ct, const LaneZ& z) {
   using P = Params<K>;
   constexpr int RING = Tile<K>::RING;
   const int l = lane_id();
   const int gn = ntt_group(l);
-  const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
   P4 sp[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    cbd_group(sp[j], prf16[64 * j + l]);
+    cbd_group(sp[j], in.prf[j]);
     ntt(sp[j], z);
   }
   uint32_t mismatch = 0;
@@ -139,18 +181,17 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
     reduce(acc);
     invntt(acc, z);
     P4 e;
-    cbd_group(e, prf16[64 * (K + i) + l]);
+    cbd_group(e, in.prf[K + i]);
     uint32_t c[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) c[r] = compress_d<P::du>(csubq(barrett(acc.v[r] + e.v[r])));
-    uint8_t* cti = ct + 32 * P::du * i;
     if constexpr (DEC) {
       uint32_t t[4];
-      unpack_group<P::du>(t, cti, nat_group(l));
+      unpack_loaded<P::du>(t, in.cu[i], nat_group(l));
 #pragma unroll
       for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
     } else {
-      pack_pair<P::du>(cti, c);
+      pack_pair<P::du>(ct + 32 * P::du * i, c);
     }
   }
   {
@@ -158,13 +199,13 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       P4 t;
-      load_poly12(t, pk + 384 * j, gn);
+      poly12_loaded(t, in.t[j]);
       basemul_acc(acc, t, sp[j], z.bm);
     }
     reduce(acc);
     invntt(acc, z);
     P4 e;
-    cbd_group(e, prf16[64 * (2 * K) + l]);
+    cbd_group(e, in.prf[2 * K]);
     const uint32_t nib = (msg[l >> 1] >> (4 * (l & 1))) & 0xF;
     uint32_t c[4];
 #pragma unroll
@@ -172,14 +213,13 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
       const int32_t kq = (int32_t)((0u - ((nib >> r) & 1)) & ((Q + 1) / 2));
       c[r] = compress_d<P::dv>(csubq(barrett(acc.v[r] + e.v[r] + kq)));
     }
-    uint8_t* ctv = ct + P::polyvec_cbytes;
     if constexpr (DEC) {
       uint32_t t[4];
-      unpack_group<P::dv>(t, ctv, nat_group(l));
+      unpack_loaded<P::dv>(t, in.cv, nat_group(l));
 #pragma unroll
       for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
     } else {
-      pack_pair<P::dv>(ctv, c);
+      pack_pair<P::dv>(ct + P::polyvec_cbytes, c);
     }
   }
   return mismatch;
@@ -264,8 +304,21 @@ struct MixedSrc {
   }
 };
 
+// Request t of the tile, from the lane that holds it (t wave-uniform): no
+// descriptor reload for the cooperative phases.
+__device__ __forceinline__ ReqP lane_req(const ReqP& me, int t) {
+  auto rd = [t](const void* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, t), hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), t);
+    return reinterpret_cast<uint8_t*>(((uint64_t)hi << 32) | lo);
+  };
+  return ReqP{rd(me.ss), rd(me.ct), rd(me.key), rd(me.coins), rd(me.prf)};
+}
+
 // One tile of up to 64 requests (indices base .. base + 63 of the source, n of
 // them valid) through the whole FO encapsulation (DEC false) or decapsulation.
+// The cooperative phases load request t + 1's inputs while request t computes
+// (one wave per SIMD: nothing else hides the loads' latency).
 template <int K, bool DEC, class Src>
 __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<Tile<K>::RING>& sh, const LaneZ& z,
                                            uint32_t* __restrict__ status) {
@@ -288,10 +341,19 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
       for (int w = 0; w < 4; ++w) hct[w] = A[w];
     }
     // ---- indcpa_dec, wave-cooperative, one request at a time
-    for (int t = 0; t < TL::T; ++t) {
-      if (base + t >= n) break;
-      const ReqP q = src.template req<K, DEC>(base + t);
-      coop_indcpa_dec<K>(q.ct, q.key, z, sh.msg[t]);
+    const int nt = min(TL::T, n - base);
+    DecIn<K> cur, nxt;
+    {
+      const ReqP q = lane_req(me, 0);
+      dec_load<K>(cur, q.ct, q.key);
+    }
+    for (int t = 0; t < nt; ++t) {
+      if (t + 1 < nt) {
+        const ReqP q = lane_req(me, t + 1);
+        dec_load<K>(nxt, q.ct, q.key);
+      }
+      coop_indcpa_dec<K>(cur, z, sh.msg[t]);
+      cur = nxt;
     }
     __syncthreads();
   } else {
@@ -338,8 +400,16 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
   __syncthreads();
 
   // ---- A^T rounds + re-encryption
+  const int nt = min(TL::T, n - base);
   int t_done = 0;  // requests whose re-encryption is finished
+  EncIn<K, DEC> cur, nxt;
   for (int g = 0; g < TL::NR; ++g) {
+    // requests all of whose entries are sampled by the end of this round
+    const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
+    if (t_done < t_end) {  // the round's first request loads during the sampling
+      const ReqP q = lane_req(me, t_done);
+      enc_load<K, DEC>(cur, q.prf, q.key + PK_OFF, q.ct);
+    }
     {
       const int E = 64 * g + lane;
       const int t = E / TL::KK;
@@ -353,18 +423,20 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
       if (valid && got < N) atomicOr(status, 1u);
     }
     __syncthreads();
-    // requests all of whose entries are now sampled
-    const int t_end = min(TL::T, (64 * (g + 1)) / TL::KK);
     for (int t = t_done; t < t_end; ++t) {
-      if (base + t >= n) break;
-      const ReqP q = src.template req<K, DEC>(base + t);
-      const uint32_t mm = coop_reencrypt<K, DEC>(sh.a, t * TL::KK, q.prf, q.key + PK_OFF, sh.msg[t], q.ct, z);
+      if (t + 1 < t_end) {
+        const ReqP q = lane_req(me, t + 1);
+        enc_load<K, DEC>(nxt, q.prf, q.key + PK_OFF, q.ct);
+      }
+      const ReqP q = lane_req(me, t);
+      const uint32_t mm = coop_reencrypt<K, DEC>(sh.a, t * TL::KK, cur, sh.msg[t], q.ct, z);
+      cur = nxt;
       if constexpr (DEC) {
         const uint32_t f = __any(mm != 0) ? 1u : 0u;
         if (lane == 0) sh.fail[t] = f;
       }
     }
-    t_done = t_end;
+    t_done = max(t_done, t_end);
     __syncthreads();
   }
 

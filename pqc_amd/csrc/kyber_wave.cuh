@@ -182,6 +182,41 @@ __device__ __forceinline__ void load_poly12(P4& p, const uint8_t* base, int g) {
   p.v[3] = (int32_t)(w2 >> 4);
 }
 
+// The same unpackings split into a load (issued early, e.g. for the next
+// request while the current one computes) and the arithmetic on the loaded words.
+template <int D>
+__device__ __forceinline__ void load_group(uint32_t (&w)[3], const uint8_t* base, int g) {
+  const uint32_t bit = (uint32_t)(4 * D) * (uint32_t)g;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(base) + (bit >> 5);
+  const int s = (int)(bit & 31);
+  w[0] = p[0];
+  w[1] = (s + 4 * D > 32) ? p[1] : 0u;  // as unpack_group: never a dword no bit is taken from
+  w[2] = (s + 4 * D > 64) ? p[2] : 0u;
+}
+
+template <int D>
+__device__ __forceinline__ void unpack_loaded(uint32_t (&t)[4], const uint32_t (&w)[3], int g) {
+  const int s = (int)(((uint32_t)(4 * D) * (uint32_t)g) & 31);
+  uint64_t v = ((uint64_t)w[0] | ((uint64_t)w[1] << 32)) >> s;
+  if (s + 4 * D > 64) v |= (uint64_t)w[2] << (64 - s);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) t[i] = (uint32_t)(v >> (D * i)) & ((1u << D) - 1);
+}
+
+__device__ __forceinline__ void load_h12(uint32_t (&h)[3], const uint8_t* base, int g) {
+  const uint16_t* p = reinterpret_cast<const uint16_t*>(base + 6 * g);
+  h[0] = p[0];
+  h[1] = p[1];
+  h[2] = p[2];
+}
+
+__device__ __forceinline__ void poly12_loaded(P4& p, const uint32_t (&h)[3]) {
+  p.v[0] = (int32_t)(h[0] & 0xFFF);
+  p.v[1] = (int32_t)((h[0] >> 12) | ((h[1] & 0xFF) << 4));
+  p.v[2] = (int32_t)((h[1] >> 8) | ((h[2] & 0xF) << 8));
+  p.v[3] = (int32_t)(h[2] >> 4);
+}
+
 // CBD (eta = 2) of the group's 16 noise bits (cbd.cu:42-63): coefficient c uses
 // bits 4c..4c+3 of the PRF output; a = b0 + b1, b = b2 + b3.
 __device__ __forceinline__ void cbd_group(P4& p, uint32_t bits16) {
