@@ -71,6 +71,7 @@ int kyber_plain_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct,
                     hipStream_t s);
 
 // fused persistent Kyber path (kyber_fused.hip)
+int kyber_fused_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s);
 int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk,
                     const uint8_t* coins, hipStream_t s);
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,

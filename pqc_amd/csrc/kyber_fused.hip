@@ -468,6 +468,173 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
   __syncthreads();
 }
 
+// ---- key generation -----------------------------------------------------------------
+// indcpa_keypair (Kyber_GPU_Batched/indcpa.cu:474-564) + the KEM tail: (rho, sigma)
+// = G(d); s_j = CBD(PRF(sigma, j)), e_i = CBD(PRF(sigma, K + i)), both through
+// poly_ntt (ntt + Barrett); t_i = frommont(sum_j A[i][j] o s_j) + e_i, Barrett;
+// pk = t || rho, sk = s || pk || H(pk) || z, polynomials packed 12 bits per
+// coefficient after csubq (polyvec_tobytes).  A is not transposed:
+// A[i][j] = SHAKE128(rho || j || i) (gen_matrix_n with transposed = 0).
+
+// 12-bit pack of a group in its NTT-order slot (the inverse of load_poly12)
+__device__ __forceinline__ void store_poly12(uint8_t* base, int g, const P4& p) {
+  uint16_t* h = reinterpret_cast<uint16_t*>(base + 6 * g);
+  const uint32_t c0 = (uint32_t)p.v[0], c1 = (uint32_t)p.v[1], c2 = (uint32_t)p.v[2], c3 = (uint32_t)p.v[3];
+  h[0] = (uint16_t)(c0 | (c1 << 12));
+  h[1] = (uint16_t)((c1 >> 4) | (c2 << 8));
+  h[2] = (uint16_t)((c2 >> 8) | (c3 << 4));
+}
+
+template <int K>
+struct KeyIn {
+  uint32_t prf[2 * K];
+};
+
+template <int K>
+__device__ __forceinline__ void key_load(KeyIn<K>& in, const uint8_t* prf) {
+  const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) in.prf[j] = prf16[64 * j + lane_id()];
+}
+
+// one request's s and t, written into sk_cpa, pk and sk's copy of pk
+template <int K>
+__device__ __forceinline__ void coop_keypair(const int16_t* ring, int e0, const KeyIn<K>& in, uint8_t* pk, uint8_t* sk,
+                                             const LaneZ& z) {
+  using P = Params<K>;
+  constexpr int RING = Tile<K>::RING;
+  const int gn = ntt_group(lane_id());
+  P4 sv[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    cbd_group(sv[j], in.prf[j]);
+    ntt(sv[j], z);
+    reduce(sv[j]);
+    P4 c;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c.v[r] = csubq(sv[j].v[r]);
+    store_poly12(sk + 384 * j, gn, c);
+  }
+  const int32_t f = (int32_t)((1ull << 32) % Q);  // frommont: x 2^32 (poly.cu:235-247)
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    P4 acc = {{0, 0, 0, 0}};
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int slot = (e0 + i * K + j) % RING;
+      const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
+      const uint32_t w0 = e32[0], w1 = e32[1];
+      P4 a = {{(int32_t)(w0 & 0xFFFF), (int32_t)(w0 >> 16), (int32_t)(w1 & 0xFFFF), (int32_t)(w1 >> 16)}};
+      basemul_acc(acc, a, sv[j], z.bm);
+    }
+    reduce(acc);
+    P4 e;
+    cbd_group(e, in.prf[K + i]);
+    ntt(e, z);
+    reduce(e);
+    P4 c;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c.v[r] = csubq(barrett(fqmul(acc.v[r], f) + e.v[r]));
+    store_poly12(pk + 384 * i, gn, c);
+    store_poly12(sk + P::sk_cpa_bytes + 384 * i, gn, c);
+  }
+}
+
+struct KeySrc {
+  uint8_t* pk;
+  uint8_t* sk;
+  const uint8_t* coins;  // 64 bytes per request: d || z
+  uint8_t* prf;
+};
+
+template <int K>
+__device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, int n, Lds<Tile<K>::RING>& sh,
+                                                   const LaneZ& z, uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  using TL = Tile<K>;
+  const int lane = threadIdx.x;
+  const int r = base + lane;
+  const bool active = r < n;
+  const int rc = active ? r : base;  // clamped request for address math
+  uint8_t* pk = src.pk + (size_t)rc * P::pk_bytes;
+  uint8_t* sk = src.sk + (size_t)rc * P::sk_bytes;
+  uint8_t* prf = src.prf + (size_t)rc * TL::PRFB;
+  const uint8_t* coins = src.coins + (size_t)rc * 64;
+  // ---- (rho, sigma) = G(d); PRF(sigma, 0 .. 2K-1) -> scratch; rho -> LDS and both pk copies
+  {
+    uint64_t d[4], A[25], sg[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) d[w] = load_u64(coins + 8 * w);
+    absorb_fixed<kRateSha3_512, 4>(A, d, 0x06);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      sg[w] = A[4 + w];
+      store_u64(sh.msg[lane] + 8 * w, A[w]);
+      if (active) {
+        store_u64(pk + P::polyvec_bytes + 8 * w, A[w]);
+        store_u64(sk + P::sk_cpa_bytes + P::polyvec_bytes + 8 * w, A[w]);
+      }
+    }
+    for (int nonce = 0; nonce < 2 * K; ++nonce) {
+      absorb_fixed<kRateShake256, 4>(A, sg, (uint64_t)nonce | (0x1Full << 8));
+      if (active) {
+#pragma unroll
+        for (int w = 0; w < 16; ++w) store_u64(prf + 128 * nonce + 8 * w, A[w]);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- A rounds + t, s
+  const int nt = min(TL::T, n - base);
+  int t_done = 0;
+  KeyIn<K> cur, nxt;
+  for (int g = 0; g < TL::NR; ++g) {
+    const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
+    if (t_done < t_end) key_load<K>(cur, src.prf + (size_t)(base + t_done) * TL::PRFB);
+    {
+      const int E = 64 * g + lane;
+      const int t = E / TL::KK;
+      const int ij = E % TL::KK;
+      const bool valid = t < nt;
+      uint64_t rho[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) rho[w] = load_u64(sh.msg[valid ? t : 0] + 8 * w);
+      const int got = sample_entry(sh.a + (E % TL::RING) * kAStride, rho, ij % K, ij / K, valid);
+      if (valid && got < N) atomicOr(status, 1u);
+    }
+    __syncthreads();
+    for (int t = t_done; t < t_end; ++t) {
+      if (t + 1 < t_end) key_load<K>(nxt, src.prf + (size_t)(base + t + 1) * TL::PRFB);
+      coop_keypair<K>(sh.a, t * TL::KK, cur, src.pk + (size_t)(base + t) * P::pk_bytes,
+                      src.sk + (size_t)(base + t) * P::sk_bytes, z);
+      cur = nxt;
+    }
+    t_done = max(t_done, t_end);
+    __syncthreads();
+  }
+  // ---- H(pk) over the packed key (written by the whole wave), z
+  {
+    uint64_t A[25];
+    sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(pk), P::pk_bytes / 8, 0x06);
+    if (active) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        store_u64(sk + P::hpk_off + 8 * w, A[w]);
+        store_u64(sk + P::z_off + 8 * w, load_u64(coins + 32 + 8 * w));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_fused_keypair(int n, int ntiles, KeySrc src, uint32_t* __restrict__ status) {
+  __shared__ Lds<Tile<K>::RING> sh;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) fused_keypair_tile<K>(src, tile * 64, n, sh, z, status);
+}
+
 template <int K, bool DEC>
 __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, DenseSrc src, uint32_t* __restrict__ status) {
   __shared__ Lds<Tile<K>::RING> sh;
@@ -670,8 +837,29 @@ static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
   return hip_err(hipGetLastError());
 }
 
+template <int K>
+static int launch_fused_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+  if (n == 0) return PQC_OK;
+  const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;
+  if (grid > ntiles) grid = ntiles;
+  if (grid < 1) grid = 1;
+  const KeySrc src{pk, sk, coins, c->kprf};
+  hipLaunchKernelGGL((k_fused_keypair<K>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, src, c->status);
+  return hip_err(hipGetLastError());
+}
+
 }  // namespace fused
 }  // namespace kyber
+
+int kyber_fused_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+  switch (k) {
+    case 2: return kyber::fused::launch_fused_keypair<2>(c, n, pk, sk, coins, s);
+    case 3: return kyber::fused::launch_fused_keypair<3>(c, n, pk, sk, coins, s);
+    case 4: return kyber::fused::launch_fused_keypair<4>(c, n, pk, sk, coins, s);
+    default: return PQC_ERR_UNSUPPORTED;
+  }
+}
 
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation

@@ -254,8 +254,10 @@ int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, con
   switch (op) {
     case OP_KEYPAIR: {
       const Buf b[3] = {{o0, a.pk, true}, {o1, a.sk, true}, {i0, 64, false}};
-      // key generation runs on the plain kernels
-      return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::kyber_plain_keypair(c, k, n, d[0], d[1], d[2], s); });
+      return staged(c, n, flags, s, b, [&](uint8_t** d) {
+        return plain ? pqc::kyber_plain_keypair(c, k, n, d[0], d[1], d[2], s)
+                     : pqc::kyber_fused_keypair(c, k, n, d[0], d[1], d[2], s);
+      });
     }
     case OP_ENC: {
       const Buf b[4] = {{o0, a.ct, true}, {o1, a.ss, true}, {i0, a.pk, false}, {i1, 32, false}};

@@ -116,6 +116,16 @@ def test_ragged_batches_kyber768(ctx, n):
     assert (gct == ct).all() and (gss2 == ss).all()
 
 
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("n", [1, 63, 65])
+def test_ragged_fused_keypair_vs_oracle(ctx, k, n):
+    say("fused keypair k", k, "n", n)
+    coins = O.drbg_array(n, 64, seed=0, stream=700 + 10 * k + n)
+    pk, sk = O.kyber_batch("keypair", k, [coins])
+    gpk, gsk = ctx.keypair_host(ALG[k], coins)
+    assert (gpk == pk).all() and (gsk == sk).all()
+
+
 def test_batch_of_one_equals_batch_of_n(ctx):
     say("batch-of-1 vs batch-of-n")
     coins, ecoins, pk, sk, ct, ss = oracle_triples(3, 40, 900)
@@ -174,6 +184,11 @@ def test_full_size_fused_equals_plain_kyber768(pqc):
     ct, ss = pqc.DeviceBuffer(n * 1088), pqc.DeviceBuffer(n * 32)
     ssf, ssp = pqc.DeviceBuffer(n * 32), pqc.DeviceBuffer(n * 32)
     ctx.keypair_derand("kyber768", n, pk, sk, dco)
+    # fused key generation == plain key generation on every element
+    pk2, sk2 = pqc.DeviceBuffer(n * 1184), pqc.DeviceBuffer(n * 2400)
+    ctx.keypair_derand("kyber768", n, pk2, sk2, dco, plain=True)
+    assert (pk.download((n, 1184)) == pk2.download((n, 1184))).all()
+    assert (sk.download((n, 2400)) == sk2.download((n, 2400))).all()
     ctx.enc_derand("kyber768", n, ct, ss, pk, dec_)
     # tamper 1/8 of the ciphertexts in place
     h_ct = ct.download((n, 1088))
