@@ -229,4 +229,32 @@ __device__ __forceinline__ void sponge_absorb_words(uint64_t (&A)[25], const uin
   }
 }
 
+// The same, with block b + 1's words loaded while block b is permuted: for a
+// kernel that runs one wave per SIMD, where nothing else hides the load latency
+// of a long absorb (2 RATE extra registers).
+template <int RATE>
+__device__ __forceinline__ void sponge_absorb_words_prefetch(uint64_t (&A)[25], const uint64_t* __restrict__ p,
+                                                             int nwords, uint8_t domain) {
+  keccak_zero(A);
+  const int nblocks = nwords / RATE + 1;
+  uint64_t nxt[RATE];
+#pragma unroll
+  for (int w = 0; w < RATE; ++w) nxt[w] = w < nwords ? p[w] : 0;
+  for (int b = 0; b < nblocks; ++b) {
+    const int rem = nwords - b * RATE;  // words left, >= 0
+    uint64_t cur[RATE];
+#pragma unroll
+    for (int w = 0; w < RATE; ++w) cur[w] = nxt[w];
+#pragma unroll
+    for (int w = 0; w < RATE; ++w) nxt[w] = w < rem - RATE ? p[(b + 1) * RATE + w] : 0;
+#pragma unroll
+    for (int w = 0; w < RATE; ++w) {
+      if (w < rem) A[w] ^= cur[w];
+      if (w == rem) A[w] ^= (uint64_t)domain;
+    }
+    if (b == nblocks - 1) A[RATE - 1] ^= 0x8000000000000000ull;
+    keccakf1600(A);
+  }
+}
+
 }  // namespace pqc

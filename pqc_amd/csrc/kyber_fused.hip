@@ -336,7 +336,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
     // ---- H(ct), one request per lane
     {
       uint64_t A[25];
-      sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
+      sponge_absorb_words_prefetch<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
 #pragma unroll
       for (int w = 0; w < 4; ++w) hct[w] = A[w];
     }
@@ -367,7 +367,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
 #pragma unroll
       for (int w = 0; w < 4; ++w) store_u64(sh.msg[lane] + 8 * w, A[w]);
     }
-    sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.key), P::pk_bytes / 8, 0x06);
+    sponge_absorb_words_prefetch<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.key), P::pk_bytes / 8, 0x06);
 #pragma unroll
     for (int w = 0; w < 4; ++w) hct[w] = A[w];  // holds H(pk) until G
     __syncthreads();
@@ -403,6 +403,16 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
   const int nt = min(TL::T, n - base);
   int t_done = 0;  // requests whose re-encryption is finished
   EncIn<K, DEC> cur, nxt;
+  // rho of this lane's entry in round g: the public seed of request E / KK, its
+  // key pointer taken from the lane that holds it (invalid entries use lane 0's)
+  auto load_rho = [&](uint64_t (&rho)[4], int g) {
+    const int t = (64 * g + lane) / TL::KK;
+    const uint8_t* key = reinterpret_cast<const uint8_t*>(__shfl((long long)me.key, t < nt ? t : 0));
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rho[w] = load_u64(key + PK_OFF + P::polyvec_bytes + 8 * w);
+  };
+  uint64_t rho_next[4];
+  load_rho(rho_next, 0);
   for (int g = 0; g < TL::NR; ++g) {
     // requests all of whose entries are sampled by the end of this round
     const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
@@ -414,15 +424,16 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
       const int E = 64 * g + lane;
       const int t = E / TL::KK;
       const int ij = E % TL::KK;
-      const bool valid = base + t < n;
-      const uint8_t* seed = src.template req<K, DEC>(valid ? base + t : rc).key + PK_OFF + P::polyvec_bytes;
+      const bool valid = t < nt;
       uint64_t rho[4];
 #pragma unroll
-      for (int w = 0; w < 4; ++w) rho[w] = load_u64(seed + 8 * w);
+      for (int w = 0; w < 4; ++w) rho[w] = rho_next[w];
       const int got = sample_entry(sh.a + (E % TL::RING) * kAStride, rho, ij / K, ij % K, valid);
       if (valid && got < N) atomicOr(status, 1u);
     }
     __syncthreads();
+    // the next round's rho loads during this round's re-encryptions
+    if (g + 1 < TL::NR) load_rho(rho_next, g + 1);
     for (int t = t_done; t < t_end; ++t) {
       if (t + 1 < t_end) {
         const ReqP q = lane_req(me, t + 1);
@@ -452,7 +463,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
         m[4 + w] = hct[w];
       }
     } else {
-      sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
+      sponge_absorb_words_prefetch<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
         m[w] = kbar[w];
@@ -615,7 +626,7 @@ __device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, 
   // ---- H(pk) over the packed key (written by the whole wave), z
   {
     uint64_t A[25];
-    sponge_absorb_words<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(pk), P::pk_bytes / 8, 0x06);
+    sponge_absorb_words_prefetch<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(pk), P::pk_bytes / 8, 0x06);
     if (active) {
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
