@@ -241,10 +241,10 @@ __device__ __forceinline__ size_t eoff(int m, int j) {
   return ((((size_t)(m >> 4) * 16 + (j >> 6)) * 64 + lane) * 4 + ((j >> 4) & 3)) * 4 + (j & 3);
 }
 
-// The stream is written raw (RAWB bytes per request, the SHAKE256 output in
-// order) and sampled by k_sample: the serial sponge chain of a lane pair is then
-// one permutation plus 17 dword stores per block, with no sampling or index
-// arithmetic on it.
+// The stream is written raw (RAWB bytes per request: each 8-byte SHAKE256
+// output word as its even-bit then its odd-bit 32-bit word) and sampled by
+// k_sample: the serial sponge chain of a lane pair is then one permutation plus
+// 17 dword stores per block, with no sampling or index arithmetic on it.
 constexpr int RAWB = 31424;  // 231 blocks x 136 bytes, rounded up to 16
 static_assert(231 * 136 <= RAWB && 231 * 68 >= NSAMP && RAWB % 16 == 0, "raw stream size");
 
@@ -281,13 +281,16 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKe
     }
     const uint64_t init[4] = {0x96 | (sd[0] << 8), (sd[0] >> 56) | (sd[1] << 8), (sd[1] >> 56) | (sd[2] << 8),
                               (sd[2] >> 56) | (0x1Full << 8)};
+    // the stream's state bit-interleaved (keccakf1600_pair_il): this lane holds
+    // the even (half 0) or odd (half 1) bits of every 64-bit lane
 #pragma unroll
-    for (int w = 0; w < 25; ++w) S[w] = w < 4 ? (uint32_t)(init[w < 4 ? w : 0] >> (32 * half)) : 0u;
-    S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;
-    // lane `half` of word w of block b is stream bytes 136 b + 8 w + 4 half
+    for (int w = 0; w < 25; ++w) S[w] = w < 4 ? il_word(init[w < 4 ? w : 0], (uint32_t)half) : 0u;
+    S[kRateShake256 - 1] ^= half ? 0x80000000u : 0u;  // pad bit 63 = odd bit 31
+    // lane `half` of stream word w of block b (bytes 136 b + 8 w .. + 7) goes to
+    // 136 b + 8 w + 4 half: the even-bit word, then the odd-bit word
     uint32_t* out = reinterpret_cast<uint32_t*>(raw + (size_t)r * RAWB) + half;
     for (int blk = 0; blk < 231; ++blk) {
-      keccakf1600_pair<24>(S, (uint32_t)half);
+      keccakf1600_pair_il<24>(S, (uint32_t)half);
 #pragma unroll
       for (int w = 0; w < kRateShake256; ++w) out[2 * w] = S[w];
       out += 2 * kRateShake256;
@@ -309,6 +312,19 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKe
 // ROWS = true: S' and E' as plain int8 rows [8n][976] in sfrag / efrag, E'' after
 // E' as [n][64] (the many-key products).
 constexpr int kSampleThreads = NSAMP / 8;  // per request
+
+// bits 0..15 of the 16-bit half `h` of v spread to the even bit positions
+__device__ __forceinline__ uint32_t il_spread(uint32_t v, int h) {
+  v = __builtin_amdgcn_perm(0u, v, h ? 0x0C030C02u : 0x0C010C00u);  // byte b of the half -> byte 2 b
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  return (v | (v << 1)) & 0x55555555u;
+}
+
+// 32-bit half h of the 64-bit word whose even bits are e and odd bits are o
+__device__ __forceinline__ uint32_t il_join(uint32_t e, uint32_t o, int h) {
+  return il_spread(e, h) | (il_spread(o, h) << 1);
+}
 static_assert(NSAMP % 8 == 0 && (N * NB) % 8 == 0, "sample groups");
 
 template <bool ROWS>
@@ -316,9 +332,12 @@ __global__ __launch_bounds__(256) void k_sample(int n, const uint8_t* raw, int8_
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * kSampleThreads) return;
   const int r = t / kSampleThreads, g = 8 * (t - r * kSampleThreads);
+  // two stream words, each as (even-bit word, odd-bit word)
   const uint4 v = *reinterpret_cast<const uint4*>(raw + (size_t)r * RAWB + 2 * g);
-  const uint32_t lo = bytes2(sample2(v.x)) | (bytes2(sample2(v.y)) << 16);
-  const uint32_t hi = bytes2(sample2(v.z)) | (bytes2(sample2(v.w)) << 16);
+  const uint32_t w0 = il_join(v.x, v.y, 0), w1 = il_join(v.x, v.y, 1);
+  const uint32_t w2 = il_join(v.z, v.w, 0), w3 = il_join(v.z, v.w, 1);
+  const uint32_t lo = bytes2(sample2(w0)) | (bytes2(sample2(w1)) << 16);
+  const uint32_t hi = bytes2(sample2(w2)) | (bytes2(sample2(w3)) << 16);
   if constexpr (ROWS) {
     int8_t* dst = g < N * NB       ? sfrag + (size_t)r * N * NB + g
                   : g < 2 * N * NB ? efrag + (size_t)r * N * NB + (g - N * NB)

@@ -198,6 +198,88 @@ __device__ __forceinline__ void keccakf1600_pair(uint32_t (&A)[25], uint32_t odd
   }
 }
 
+// ---- bit-interleaved lane pairs ------------------------------------------------
+// A 64-bit lane x as two 32-bit words, E (even lane of the pair) = bits 0, 2, ..,
+// 62 of x and O (odd lane) = bits 1, 3, .., 63.  rot64(x, 2k) is then rot32 of
+// each word by k (lane-local: no exchange), and rot64(x, 2k + 1) is
+// E' = rot32(O, k + 1), O' = rot32(E, k): the partner's word rotated by
+// k + 1 - half.  Of rho's 24 rotations 12 are even, so a round needs 12 fewer
+// cross-lane moves than the lo/hi split (keccakf1600_pair).
+constexpr uint32_t il_bits(uint64_t x) {  // bits 0, 2, .., 62 of x
+  uint32_t r = 0;
+  for (int i = 0; i < 32; ++i) r |= (uint32_t)((x >> (2 * i)) & 1u) << i;
+  return r;
+}
+
+struct IlRc {
+  uint32_t e[24], o[24];
+};
+constexpr IlRc il_round_constants() {
+  IlRc t{};
+  for (unsigned r = 0; r < 24; ++r) {
+    t.e[r] = il_bits(keccak_rc(r));
+    t.o[r] = il_bits(keccak_rc(r) >> 1);
+  }
+  return t;
+}
+static constexpr IlRc kIlRc = il_round_constants();
+
+// this lane's interleaved word of the 64-bit value x (half = lane & 1)
+__device__ __forceinline__ uint32_t il_word(uint64_t x, uint32_t half) {
+  x = (x >> half) & 0x5555555555555555ull;
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  return (uint32_t)(x | (x >> 16));
+}
+
+template <int N>
+__device__ __forceinline__ uint32_t rotl_il(uint32_t own, uint32_t half) {
+  if constexpr (N == 0) {
+    return own;
+  } else if constexpr (N % 2 == 0) {
+    return __builtin_amdgcn_alignbit(own, own, 32 - N / 2);
+  } else {
+    const uint32_t partner = pair_swap(own);
+    // rotate left by N / 2 + 1 - half = right by 31 - N / 2 + half (mod 32)
+    return __builtin_amdgcn_alignbit(partner, partner, 31u - (uint32_t)(N / 2) + half);
+  }
+}
+
+template <int I>
+__device__ __forceinline__ void keccak_theta_rho_pi_il(uint32_t (&B)[25], const uint32_t (&A)[25],
+                                                       const uint32_t (&Cm)[5], const uint32_t (&Rp)[5],
+                                                       uint32_t half) {
+  constexpr int x = I % 5;
+  B[kKeccakMaps.pi_dst[I]] = rotl_il<kKeccakMaps.rho[I]>(xor3_32(A[I], Cm[x], Rp[x]), half);
+  if constexpr (I + 1 < 25) keccak_theta_rho_pi_il<I + 1>(B, A, Cm, Rp, half);
+}
+
+// Keccak-f[1600] on a bit-interleaved lane pair; half = lane & 1, both lanes active.
+template <int UNROLL = 1>
+__device__ __forceinline__ void keccakf1600_pair_il(uint32_t (&A)[25], uint32_t half) {
+  static_assert(24 % UNROLL == 0, "rounds per trip");
+#pragma unroll UNROLL
+  for (int round = 0; round < 24; ++round) {
+    uint32_t C[5], Cm[5], Rp[5], B[25];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) C[x] = xor3_32(xor3_32(A[x], A[x + 5], A[x + 10]), A[x + 15], A[x + 20]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) {
+      Cm[x] = C[(x + 4) % 5];
+      Rp[x] = rotl_il<1>(C[(x + 1) % 5], half);
+    }
+    keccak_theta_rho_pi_il<0>(B, A, Cm, Rp, half);
+#pragma unroll
+    for (int y = 0; y < 5; ++y)
+#pragma unroll
+      for (int x = 0; x < 5; ++x)
+        A[x + 5 * y] = chi_32(B[x + 5 * y], B[(x + 1) % 5 + 5 * y], B[(x + 2) % 5 + 5 * y]);
+    A[0] ^= half ? kIlRc.o[round] : kIlRc.e[round];
+  }
+}
+
 __device__ __forceinline__ void keccak_zero(uint64_t (&A)[25]) {
 #pragma unroll
   for (int i = 0; i < 25; ++i) A[i] = 0;
