@@ -37,11 +37,11 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz int32 o
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per fused launch at batch 65536 from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs, each scaled by the same counter's reading of a 256 MiB
-# device copy (profiles/kyber768_decaps_b65536.md): 454.5 MB read + 60.8 MB written.
-MEASURED_TRAFFIC_BYTES_B65536 = 515.3e6
+# device copy (profiles/kyber768_decaps_b65536.md): 456.9 MB read + 60.8 MB written.
+MEASURED_TRAFFIC_BYTES_B65536 = 517.7e6
 # HBM bytes per Frodo-976 decapsulation GEMM launch at batch 4096, same method
-# (profiles/frodo976_decaps_b4096.md): 197.7 MB read + 0.07 MB written.
-MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 197.8e6
+# (profiles/frodo976_decaps_b4096.md): 192.3 MB read + 0.07 MB written.
+MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 192.4e6
 
 
 def log(*a):
