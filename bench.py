@@ -89,6 +89,35 @@ def timed_run(fn, stream, launches, warmup, pqc):
     return a.elapsed_ms(b) / launches
 
 
+def host_buffer_rates(pqc, ctx, n, ct, sk, ss_want, reps=3):
+    """PCIe-inclusive decapsulation rate: ciphertexts and secret keys start in
+    host memory and the shared secrets end there (PQC_F_HOST_PTRS), from
+    pageable numpy arrays (staged through the context's pinned double buffers)
+    and from pqc_host_alloc memory (copied directly).  Not the headline value,
+    which has its inputs resident in HBM."""
+    ct_h, sk_h = ct.download((n, 1088)), sk.download((n, 2400))
+    res = {}
+    pin = [pqc.PinnedBuffer(n * w) for w in (1088, 2400, 32)]
+    p_ct, p_sk, p_ss = pin[0].view((n, 1088)), pin[1].view((n, 2400)), pin[2].view((n, 32))
+    p_ct[:] = ct_h
+    p_sk[:] = sk_h
+    for name, args_ in (("pageable", (ct_h, sk_h, None)), ("pinned", (p_ct, p_sk, p_ss))):
+        got = ctx.dec_host(ALG, args_[0], args_[1], out=args_[2])  # warm-up + check
+        assert (got == ss_want).all(), "host-buffer decaps differ from the device-buffer run"
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.dec_host(ALG, args_[0], args_[1], out=args_[2])
+        dt = (time.perf_counter() - t0) / reps
+        res[name + "_per_s"] = round(n / dt, 1)
+        res[name + "_ms"] = round(dt * 1e3, 3)
+        res[name + "_GBps_h2d"] = round(n * (1088 + 2400) / dt / 1e9, 2)
+        log("host buffers (%s): %.3f ms per %d decaps" % (name, dt * 1e3, n))
+    for b in pin:
+        b.free()
+    res["note"] = "ct + sk host->device, decaps, ss device->host per call; bytes match the device run"
+    return res
+
+
 def cpu_baseline(ct_h, sk_h, ss_gpu, threads):
     from tests import oracle_lib as O
     n1 = min(4096, len(ct_h))
@@ -193,6 +222,8 @@ def main():
                      "fused_over_plain": round(wp / wf, 3), "bitexact": bool((out_f == out_p).all())}
         log("%s: fused %.3f ms, plain %.3f ms" % (name, wf * 1e3 / tsteps, wp * 1e3 / tsteps))
 
+    host_io = host_buffer_rates(pqc, ctx, n, ct, sk, a)
+
     ms_f = wall_f * 1e3 / args.steps
     ms_p = wall_p * 1e3 / max(3, args.steps // 4)
     kern_ms = float(np.mean(per_f))
@@ -226,6 +257,7 @@ def main():
             "bitexact_fused_plain_encaps": bitexact,
             "keypair": tri["keypair"],
             "encaps": tri["encaps"],
+            "host_buffers": host_io,
         },
         "roofline": {
             "bound": "valu",

@@ -47,7 +47,11 @@ extern "C" {
 
 /* ---- call flags ------------------------------------------------------------- */
 #define PQC_F_HOST_PTRS 0x1u /* buffers are host memory: staged through the
-                                context, and the call returns when done */
+                                context, and the call returns when done.
+                                Pageable memory moves in 16 MiB chunks through
+                                two pinned buffers (host copy of one chunk
+                                overlaps the DMA of the other); memory from
+                                pqc_host_alloc is copied directly. */
 #define PQC_F_PLAIN 0x2u     /* run the plain per-op kernels instead of the
                                 fused persistent kernels (same outputs) */
 
@@ -74,6 +78,14 @@ typedef struct pqc_tuning {
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
 int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */
+
+/* Pinned (page-locked) host memory for PQC_F_HOST_PTRS buffers: the DMA
+ * engines read and write it directly, with no staging copy.  Replaces the
+ * reference's cudaMemcpy from pageable buffers inside its timed loop
+ * (Kyber_GPU_Batched/main.cu:376-423).  Free only after the calls that use
+ * the block have returned.                                                    */
+int pqc_host_alloc(size_t bytes, void **out);
+int pqc_host_free(void *p); /* PQC_ERR_INVALID_ARG for a pointer pqc_host_alloc did not return */
 
 /* Byte sizes of one element: public key, secret key, ciphertext, shared secret. */
 int pqc_alg_sizes(int alg, size_t *pk, size_t *sk, size_t *ct, size_t *ss);

@@ -45,9 +45,11 @@ struct pqc_ctx {
   uint8_t* fr_elem = nullptr;
   int fr_loaded = 0, fr_has_sk = 0;
 
-  // staging for host-pointer calls
+  // staging for host-pointer calls: device buffers, and (hostio.cpp) the
+  // pinned double buffers with their copy threads
   uint8_t* stage = nullptr;
   size_t stage_bytes = 0;
+  void* hostio = nullptr;
 
   uint32_t* status = nullptr;  // device word: bit 0 = a sampler hit its block cap
 
@@ -61,6 +63,14 @@ struct pqc_ctx {
 namespace pqc {
 
 inline int hip_err(hipError_t e) { return e == hipSuccess ? PQC_OK : PQC_ERR_DEVICE; }
+
+// host-buffer transfers (hostio.cpp): pageable memory is staged through pinned
+// double buffers in chunks, pqc_host_alloc memory is copied directly; both
+// are ordered on stream s, and the caller synchronises before reading dst
+int host_to_device(pqc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
+int device_to_host(pqc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
+bool host_is_pinned(const void* p, size_t bytes);
+void hostio_free(pqc_ctx* c);
 
 // plain per-op Kyber path (kyber_plain.hip); all pointers are device pointers
 int kyber_plain_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins,

@@ -194,9 +194,7 @@ int staged(pqc_ctx* c, size_t n, unsigned flags, hipStream_t s, const Buf (&bufs
     for (size_t i = 0; i < NB; ++i) {
       dev[i] = p;
       p += (n * bufs[i].per + 255) & ~(size_t)255;
-      if (!bufs[i].out &&
-          hipMemcpyAsync(dev[i], bufs[i].p, n * bufs[i].per, hipMemcpyHostToDevice, s) != hipSuccess)
-        return PQC_ERR_DEVICE;
+      if (!bufs[i].out && (rc = pqc::host_to_device(c, dev[i], bufs[i].p, n * bufs[i].per, s))) return rc;
     }
   } else {
     for (size_t i = 0; i < NB; ++i)
@@ -206,9 +204,7 @@ int staged(pqc_ctx* c, size_t n, unsigned flags, hipStream_t s, const Buf (&bufs
   if (rc) return rc;
   if (host) {
     for (size_t i = 0; i < NB; ++i)
-      if (bufs[i].out &&
-          hipMemcpyAsync((void*)bufs[i].p, dev[i], n * bufs[i].per, hipMemcpyDeviceToHost, s) != hipSuccess)
-        return PQC_ERR_DEVICE;
+      if (bufs[i].out && (rc = pqc::device_to_host(c, (void*)bufs[i].p, dev[i], n * bufs[i].per, s))) return rc;
     if (hipStreamSynchronize(s) != hipSuccess) return PQC_ERR_DEVICE;
     uint32_t st = 0;
     if (hipMemcpy(&st, c->status, 4, hipMemcpyDeviceToHost) != hipSuccess) return PQC_ERR_DEVICE;
@@ -406,6 +402,7 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->nhpolys) (void)hipFree(c->nhpolys);
   if (c->nhbytes) (void)hipFree(c->nhbytes);
   if (c->stage) (void)hipFree(c->stage);
+  pqc::hostio_free(c);
   if (c->status) (void)hipFree(c->status);
   delete c;
 }

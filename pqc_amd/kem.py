@@ -85,6 +85,36 @@ class DeviceBuffer:
             pass
 
 
+class PinnedBuffer:
+    """Page-locked host memory from pqc_host_alloc; `array` is a uint8 numpy view.
+
+    Host-pointer calls (PQC_F_HOST_PTRS) on views of it skip the staging copy."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(L.lib().pqc_host_alloc(nbytes, C.byref(p)), "pqc_host_alloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+        self.array = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(self.ptr))
+
+    def view(self, shape, dtype=np.uint8):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        assert n <= self.nbytes
+        return self.array[:n].view(dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            check(L.lib().pqc_host_free(self.ptr), "pqc_host_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Stream:
     def __init__(self):
         p = C.c_void_p()
@@ -210,12 +240,13 @@ class Context:
                                            stream.handle if stream else None), "dec")
 
     # ---- host numpy arrays (staged, synchronous) -----------------------------
-    def keypair_host(self, alg, coins, plain=False):
+    def keypair_host(self, alg, coins, plain=False, out=None):
+        """out: optional (pk, sk) arrays to write into (e.g. PinnedBuffer views)."""
         pkb, skb, _, _ = sizes(alg)
         coins = np.ascontiguousarray(coins, dtype=np.uint8)
         n = coins.shape[0]
-        pk = np.empty((n, pkb), np.uint8)
-        sk = np.empty((n, skb), np.uint8)
+        pk, sk = out if out is not None else (np.empty((n, pkb), np.uint8), np.empty((n, skb), np.uint8))
+        assert pk.shape == (n, pkb) and sk.shape == (n, skb)
         flags = L.F_HOST_PTRS | (L.F_PLAIN if plain else 0)
         check(L.lib().crypto_kem_keypair_derand_batch(self.handle, alg_id(alg), n, _ptr(pk), _ptr(sk), _ptr(coins),
                                                       flags, None), "keypair")
@@ -233,12 +264,14 @@ class Context:
                                                   _ptr(coins), flags, None), "enc")
         return ct, ss
 
-    def dec_host(self, alg, ct, sk, plain=False):
+    def dec_host(self, alg, ct, sk, plain=False, out=None):
+        """out: optional ss array to write into (e.g. a PinnedBuffer view)."""
         _, _, _, ssb = sizes(alg)
         ct = np.ascontiguousarray(ct, dtype=np.uint8)
         sk = np.ascontiguousarray(sk, dtype=np.uint8)
         n = ct.shape[0]
-        ss = np.empty((n, ssb), np.uint8)
+        ss = out if out is not None else np.empty((n, ssb), np.uint8)
+        assert ss.shape == (n, ssb)
         flags = L.F_HOST_PTRS | (L.F_PLAIN if plain else 0)
         check(L.lib().crypto_kem_dec_batch(self.handle, alg_id(alg), n, _ptr(ss), _ptr(ct), _ptr(sk), flags, None),
               "dec")

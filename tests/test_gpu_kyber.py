@@ -207,3 +207,47 @@ def test_full_size_fused_equals_plain_kyber768(pqc):
     want = O.kyber_batch("dec", 3, [np.ascontiguousarray(h_ct[idx]), np.ascontiguousarray(h_sk[idx])])
     assert (a[idx] == want).all()
     ctx.close()
+
+
+def test_host_staging_pageable_and_pinned(pqc):
+    """Host-buffer calls larger than one 16 MiB staging chunk, from pageable numpy
+    arrays and from pqc_host_alloc memory, give the same bytes as device calls."""
+    n = 12288  # sk 29.5 MB and pk 14.5 MB: two chunks each way
+    say("host staging", n)
+    ctx = pqc.Context(["kyber768"], max_batch=n)
+    rng = np.random.default_rng(7)
+    coins = rng.integers(0, 256, (n, 64), dtype=np.uint8)
+    ecoins = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    dco, dec_ = pqc.DeviceBuffer.from_array(coins), pqc.DeviceBuffer.from_array(ecoins)
+    pk, sk = pqc.DeviceBuffer(n * 1184), pqc.DeviceBuffer(n * 2400)
+    ct, ss = pqc.DeviceBuffer(n * 1088), pqc.DeviceBuffer(n * 32)
+    ctx.keypair_derand("kyber768", n, pk, sk, dco)
+    ctx.enc_derand("kyber768", n, ct, ss, pk, dec_)
+    d_pk, d_sk = pk.download((n, 1184)), sk.download((n, 2400))
+    d_ct, d_ss = ct.download((n, 1088)), ss.download((n, 32))
+    d_ct[::5, 100] ^= 1  # some implicit rejections
+    ss2 = pqc.DeviceBuffer(n * 32)
+    ct.upload(d_ct)
+    ctx.dec("kyber768", n, ss2, ct, sk)
+    want_dec = ss2.download((n, 32))
+
+    h_pk, h_sk = ctx.keypair_host("kyber768", coins)
+    assert (h_pk == d_pk).all() and (h_sk == d_sk).all()
+    h_ct, h_ss = ctx.enc_host("kyber768", d_pk, ecoins)
+    h_ct[::5, 100] ^= 1
+    assert (h_ct == d_ct).all()
+    assert (ctx.dec_host("kyber768", h_ct, h_sk) == want_dec).all()
+
+    say("host staging pinned", n)
+    bufs = [pqc.PinnedBuffer(n * w) for w in (1184, 2400, 1088, 2400, 32)]
+    p_pk, p_sk = bufs[0].view((n, 1184)), bufs[1].view((n, 2400))
+    ctx.keypair_host("kyber768", coins, out=(p_pk, p_sk))
+    assert (p_pk == d_pk).all() and (p_sk == d_sk).all()
+    p_ct, p_sk2, p_ss = bufs[2].view((n, 1088)), bufs[3].view((n, 2400)), bufs[4].view((n, 32))
+    p_ct[:] = d_ct
+    p_sk2[:] = d_sk
+    ctx.dec_host("kyber768", p_ct, p_sk2, out=p_ss)
+    assert (p_ss == want_dec).all()
+    for b in bufs:
+        b.free()
+    ctx.close()
