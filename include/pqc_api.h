@@ -336,6 +336,15 @@ int pqc_frodo976_products(pqc_ctx *ctx, size_t n, uint16_t *bpp, uint16_t *v, un
 void pqc_randombytes_seed(uint64_t seed); /* deterministic SHAKE256 DRBG */
 void pqc_randombytes_system(void);        /* OS randomness (default) */
 int pqc_randombytes(uint8_t *out, size_t n);
+/* The NIST PQC known-answer-test source under the names of the reference's
+ * rng.h (Kyber_GPU_Batched/rng.h:22-26, whose rng.cu:17-39 body is a counter):
+ * AES-256 CTR_DRBG without derivation function (SP 800-90A 10.2.1), seeded
+ * with 48 bytes.  After randombytes_init every non-derand entry point draws
+ * from it, call by call as the NIST submissions do, so PQCgenKAT_kem.c links
+ * against this library unchanged.  pqc_randombytes_seed/_system switch away. */
+void randombytes_init(unsigned char *entropy_input, unsigned char *personalization_string,
+                      int security_strength);
+int randombytes(unsigned char *x, unsigned long long xlen);
 
 /* ---- device plumbing, so that callers need no other GPU binding -------------- */
 int pqc_device_count(int *count);

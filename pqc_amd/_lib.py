@@ -39,6 +39,8 @@ SIGNATURES = {
     "pqc_ctx_set_tuning": (_I, [_P, _I, _I]),
     "pqc_alg_sizes": (_I, [_I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ)]),
     "pqc_host_alloc": (_I, [_SZ, C.POINTER(_P)]),
+    "randombytes_init": (None, [_P, _P, _I]),
+    "randombytes": (_I, [_P, C.c_ulonglong]),
     "pqc_host_free": (_I, [_P]),
     "crypto_kem_keypair_derand_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
     "crypto_kem_enc_derand_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _P, _U, _P]),

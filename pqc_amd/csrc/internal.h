@@ -121,6 +121,11 @@ int gather_records(size_t n, uint8_t* dst, size_t bytes, const uint8_t* src, siz
                    hipStream_t s);
 // the library's deterministic generator: SHAKE256("pqc-mi355x" || seed || stream || index)
 void host_drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, uint64_t index);
+
+// NIST PQC known-answer-test DRBG (nist_drbg.cpp): active after randombytes_init
+bool nist_drbg_active();
+void nist_drbg_disable();
+void nist_drbg_generate(uint8_t* x, size_t len);
 int frodo_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* mu,
                     hipStream_t s);
 int frodo_enc_batch_strided(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, size_t pk_stride,

@@ -275,13 +275,19 @@ int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, con
 int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1) {
   pqc_ctx* c = default_ctx();
   if (!c) return PQC_ERR_NO_DEVICE;
+  // Coins are drawn call by call as the NIST submissions do, so that the
+  // known-answer-test DRBG reproduces their outputs: Kyber keypair d then z
+  // (indcpa_keypair, then crypto_kem_keypair's z), encaps 32 bytes; Frodo
+  // keypair s || seedSE || z in one draw, encaps mu (FrodoKEM_GPU/kem.c:33,90).
   uint8_t coins[64];
+  const bool frodo = alg == PQC_ALG_FRODO976;
   if (op == OP_KEYPAIR) {
-    if (pqc_randombytes(coins, 64)) return PQC_ERR_DEVICE;
+    if (frodo ? pqc_randombytes(coins, 64) : (pqc_randombytes(coins, 32) || pqc_randombytes(coins + 32, 32)))
+      return PQC_ERR_DEVICE;
     i0 = coins;
     i1 = coins;
   } else if (op == OP_ENC) {
-    if (pqc_randombytes(coins, 32)) return PQC_ERR_DEVICE;
+    if (pqc_randombytes(coins, frodo ? 24 : 32)) return PQC_ERR_DEVICE;
     i1 = coins;
   }
   return kem_call(c, alg, op, 1, o0, o1, i0, i1, PQC_F_HOST_PTRS, nullptr);
@@ -676,6 +682,7 @@ PQC_NIST_SET(kyber1024, PQC_ALG_KYBER1024)
 PQC_NIST_SET(frodo976, PQC_ALG_FRODO976)
 
 void pqc_randombytes_seed(uint64_t seed) {
+  pqc::nist_drbg_disable();
   std::lock_guard<std::mutex> lk(g_rng_mu);
   g_rng_det = true;
   g_rng_seed = seed;
@@ -683,12 +690,17 @@ void pqc_randombytes_seed(uint64_t seed) {
 }
 
 void pqc_randombytes_system(void) {
+  pqc::nist_drbg_disable();
   std::lock_guard<std::mutex> lk(g_rng_mu);
   g_rng_det = false;
 }
 
 int pqc_randombytes(uint8_t* out, size_t n) {
   if (!out && n) return PQC_ERR_INVALID_ARG;
+  if (pqc::nist_drbg_active()) {
+    pqc::nist_drbg_generate(out, n);
+    return PQC_OK;
+  }
   std::lock_guard<std::mutex> lk(g_rng_mu);
   if (g_rng_det) {
     drbg(out, n, g_rng_seed, 0xFFFFFFFFull, g_rng_index++);

@@ -158,10 +158,13 @@ def test_nist_dropins_deterministic(pqc):
         pk, sk = pqc.crypto_kem_keypair(alg)
         ct, ss = pqc.crypto_kem_enc(pk, alg)
         assert pqc.crypto_kem_dec(ct, sk, alg) == ss
-        # the same DRBG draws through the oracle
+        # the same DRBG draws through the oracle: keypair d then z, encaps m
         import struct
-        d0 = hashlib.shake_256(b"pqc-mi355x" + struct.pack("<QQQ", 42, 0xFFFFFFFF, 0)).digest(64)
-        d1 = hashlib.shake_256(b"pqc-mi355x" + struct.pack("<QQQ", 42, 0xFFFFFFFF, 1)).digest(32)
+
+        def draw(i):
+            return hashlib.shake_256(b"pqc-mi355x" + struct.pack("<QQQ", 42, 0xFFFFFFFF, i)).digest(32)
+        d0 = draw(0) + draw(1)
+        d1 = draw(2)
         opk, osk = O.kyber_keypair(k, d0)
         assert (pk, sk) == (opk, osk)
         oct_, oss = O.kyber_enc(k, opk, d1)
