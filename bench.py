@@ -10,7 +10,7 @@ generation and encapsulation kernels.  Prints one JSON line.
 --workload frodo976 measures FrodoKEM-976 decapsulation of 4096 ciphertexts
 under one device-resident key, with the shared-A int8-MFMA product timed on
 its own; --workload newhope1024 measures 16384 NewHope-1024 CPA
-keygen+encaps+decaps triples; --workload kyber_mixed measures 65536 mixed
+keygen+encaps+decaps triples (and the CCA-KEM's rates beside them); --workload kyber_mixed measures 65536 mixed
 Kyber-512/768/1024 requests (1:2:1) through the graph-captured fused kernels;
 --workload serving replays a Poisson trace (2M requests/s, 90% Kyber 1:2:1,
 10% FrodoKEM-976, 1024 device-resident key pairs per set) through the serving
@@ -421,7 +421,7 @@ NEWHOPE_BATCH = 16384
 def bench_newhope(args):
     import pqc_amd as pqc
     n = NEWHOPE_BATCH if args.batch == BATCH else args.batch
-    ctx = pqc.Context(["newhope1024cpa"], max_batch=n)
+    ctx = pqc.Context(["newhope1024cpa", "newhope1024"], max_batch=n)
     stream = pqc.Stream()
     rng = np.random.default_rng(0)
     seed, msg, coin = (pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8)) for _ in range(3))
@@ -437,6 +437,7 @@ def bench_newhope(args):
     ms = wall * 1e3 / args.steps
     log("newhope triples: %.3f ms/step" % ms)
     ok = bool((m2.download((n, 32)) == msg.download((n, 32))).all())
+    cca = newhope_cca_rates(pqc, ctx, stream, n, rng, max(3, args.steps // 4))
 
     # The step's largest kernel is poly_sample (5 launches per triple batch); time
     # it alone on the same stream, and the NTT (HBM figure) beside it.
@@ -465,6 +466,7 @@ def bench_newhope(args):
                    "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
                    "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
                    "messages_recovered": ok,
+                   "cca_kem": cca,
                    "poly_ntt": {"kernel_ms": round(ntt_ms, 4), "bytes": ntt_bytes,
                                 "hbm_gbps": round(ntt_bytes / ntt_ms / 1e6, 1),
                                 "hbm_frac": round(ntt_bytes / ntt_ms / 1e6 / HBM_PEAK_GBPS, 4)}},
@@ -493,6 +495,25 @@ def bench_newhope(args):
         out["config"]["gpu_over_cpu"] = round(out["value"] / (m / dt), 1)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def newhope_cca_rates(pqc, ctx, stream, n, rng, steps):
+    """NewHope-1024 CCA-KEM keygen, encaps and decaps over the same batch size
+    (the specification's transform over the CPA kernels above)."""
+    kc = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 64), dtype=np.uint8))
+    ec = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8))
+    pk, sk, ct = pqc.DeviceBuffer(n * 1824), pqc.DeviceBuffer(n * 3680), pqc.DeviceBuffer(n * 2208)
+    ss, ss2 = pqc.DeviceBuffer(n * 32), pqc.DeviceBuffer(n * 32)
+    res = {}
+    for name, fn in (("keypair", lambda: ctx.keypair_derand("newhope1024", n, pk, sk, kc, stream=stream)),
+                     ("encaps", lambda: ctx.enc_derand("newhope1024", n, ct, ss, pk, ec, stream=stream)),
+                     ("decaps", lambda: ctx.dec("newhope1024", n, ss2, ct, sk, stream=stream))):
+        wall, _ = timed(fn, stream, steps, 1, pqc)
+        res[name + "_per_s"] = round(n / (wall / steps), 1)
+        log("newhope cca %s: %.3f ms" % (name, wall * 1e3 / steps))
+    res["shared_secrets_equal"] = bool((ss.download((n, 32)) == ss2.download((n, 32))).all())
+    res["sizes"] = "pk 1824, sk 3680, ct 2208, ss 32"
+    return res
 
 
 # ---- mixed Kyber-512/768/1024 (1:2:1), 65536 encaps + decaps, fused in a graph ---------

@@ -43,6 +43,8 @@ extern "C" {
 #define PQC_ALG_KYBER1024 3       /* round 2, k=4, du=11, dv=5 */
 #define PQC_ALG_NEWHOPE1024_CPA 4 /* NewHope-1024 CPA-PKE */
 #define PQC_ALG_FRODO976 5        /* FrodoKEM-976-SHAKE */
+#define PQC_ALG_NEWHOPE1024 6     /* NewHope-1024 CCA-KEM: pk 1824, sk 3680, ct 2208, ss 32
+                                     (NewHope_NIST_GPU/params.h:37-39) */
 #define PQC_ALG_MASK(a) (1u << (a))
 
 /* ---- call flags ------------------------------------------------------------- */
@@ -302,6 +304,11 @@ int pqc_kyber1024_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, con
 int pqc_frodo976_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
 int pqc_frodo976_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
 int pqc_frodo976_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+/* NewHope-1024 CCA-KEM: the sizes of NewHope_NIST_GPU/params.h:37-39 and api.h:11-14;
+ * the reference has no KEM code, the transform is the NewHope specification's */
+int pqc_newhope1024_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int pqc_newhope1024_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int pqc_newhope1024_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
 
 /* ---- NewHope-1024 CPA-PKE, plain batched kernels ------------------------------
  * Replaces cpapke_keypair/enc/dec (NewHope_NIST_GPU/cpapke.h:14-24).

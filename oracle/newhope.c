@@ -18,6 +18,8 @@
  * uint16_t/uint32_t arithmetic is kept exactly as the reference stores it.
  * The keypair seed expansion is SHAKE256(seed, 64) -> publicseed ||
  * noiseseed (cpapke.cu:283-298) without the reference's stride mismatch (Q6).
+ * The CCA-KEM at the end is the NewHope specification's transform over this
+ * CPA-PKE (the reference has its sizes, params.h:37-39, but no code).
  */
 #include <string.h>
 #include <stdlib.h>
@@ -362,6 +364,50 @@ int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk) {
     oracle_newhope_invntt(tmp);
     poly_sub(tmp, tmp, vprime);
     poly_tomsg(m, tmp);
+    return 0;
+}
+
+/* NewHope-1024 CCA-KEM: the Fujisaki-Okamoto transform of the NewHope
+ * specification (CCA-KEM Gen / Encaps / Decaps, SHAKE256 for every hash) over the
+ * CPA-PKE above.  The reference fixes its sizes (NewHope_NIST_GPU/params.h:37-39:
+ * sk = sk_cpa || pk || H(pk) || s, ct = ct_cpa || d) and its verify/cmov
+ * (verify.cu), but has no KEM code.  Keypair coins: CPA seed || s. */
+#define NH_PK (NH_POLYBYTES + NH_SYM)
+#define NH_CT (NH_POLYBYTES + NH_CBYTES)
+#define NH_CCA_SK (NH_POLYBYTES + NH_PK + 2 * NH_SYM)
+#define NH_CCA_CT (NH_CT + NH_SYM)
+
+int oracle_newhope_cca_keypair(uint8_t *pk, uint8_t *sk, const uint8_t coins[64]) {
+    oracle_newhope_cpa_keypair(pk, sk, coins);
+    memcpy(sk + NH_POLYBYTES, pk, NH_PK);
+    oracle_shake256(sk + NH_POLYBYTES + NH_PK, NH_SYM, pk, NH_PK);
+    memcpy(sk + NH_CCA_SK - NH_SYM, coins + NH_SYM, NH_SYM);
+    return 0;
+}
+
+int oracle_newhope_cca_enc(uint8_t *ct, uint8_t ss[32], const uint8_t *pk, const uint8_t coins[32]) {
+    uint8_t buf[2 * NH_SYM], kcd[3 * NH_SYM];
+    oracle_shake256(buf, NH_SYM, coins, NH_SYM);        /* mu = H(coins): raw coins are not released */
+    oracle_shake256(buf + NH_SYM, NH_SYM, pk, NH_PK);   /* H(pk) */
+    oracle_shake256(kcd, 3 * NH_SYM, buf, 2 * NH_SYM);  /* K || coin' || d */
+    oracle_newhope_cpa_enc(ct, buf, pk, kcd + NH_SYM);
+    memcpy(ct + NH_CT, kcd + 2 * NH_SYM, NH_SYM);
+    oracle_shake256(kcd + NH_SYM, NH_SYM, ct, NH_CCA_CT); /* H(c || d) */
+    oracle_shake256(ss, NH_SYM, kcd, 2 * NH_SYM);
+    return 0;
+}
+
+int oracle_newhope_cca_dec(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk) {
+    uint8_t buf[2 * NH_SYM], kcd[3 * NH_SYM], ct2[NH_CCA_CT];
+    oracle_newhope_cpa_dec(buf, ct, sk);
+    memcpy(buf + NH_SYM, sk + NH_POLYBYTES + NH_PK, NH_SYM);
+    oracle_shake256(kcd, 3 * NH_SYM, buf, 2 * NH_SYM);
+    oracle_newhope_cpa_enc(ct2, buf, sk + NH_POLYBYTES, kcd + NH_SYM);
+    memcpy(ct2 + NH_CT, kcd + 2 * NH_SYM, NH_SYM);
+    const int fail = memcmp(ct, ct2, NH_CCA_CT) != 0;
+    oracle_shake256(kcd + NH_SYM, NH_SYM, ct, NH_CCA_CT);
+    if (fail) memcpy(kcd, sk + NH_CCA_SK - NH_SYM, NH_SYM); /* implicit rejection: s */
+    oracle_shake256(ss, NH_SYM, kcd, 2 * NH_SYM);
     return 0;
 }
 

@@ -87,6 +87,9 @@ void oracle_newhope_poly_sample(uint16_t *r, const uint8_t seed[32], uint8_t non
 int oracle_newhope_cpa_keypair(uint8_t *pk, uint8_t *sk, const uint8_t seed[32]);
 int oracle_newhope_cpa_enc(uint8_t *c, const uint8_t m[32], const uint8_t *pk, const uint8_t coin[32]);
 int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk);
+int oracle_newhope_cca_keypair(uint8_t *pk, uint8_t *sk, const uint8_t coins[64]);
+int oracle_newhope_cca_enc(uint8_t *ct, uint8_t ss[32], const uint8_t *pk, const uint8_t coins[32]);
+int oracle_newhope_cca_dec(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk);
 int oracle_newhope_triple_batch(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
                                 const uint8_t *seed, const uint8_t *msg, const uint8_t *coin, int nthreads);
 

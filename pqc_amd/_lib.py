@@ -22,6 +22,7 @@ ALG_KYBER768 = 2
 ALG_KYBER1024 = 3
 ALG_NEWHOPE1024_CPA = 4
 ALG_FRODO976 = 5
+ALG_NEWHOPE1024 = 6
 
 F_HOST_PTRS = 0x1
 F_PLAIN = 0x2
@@ -116,7 +117,7 @@ SIGNATURES = {
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }
-for _alg in ("kyber512", "kyber768", "kyber1024", "frodo976"):
+for _alg in ("kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024"):
     SIGNATURES["pqc_%s_crypto_kem_keypair" % _alg] = (_I, [_P, _P])
     SIGNATURES["pqc_%s_crypto_kem_enc" % _alg] = (_I, [_P, _P, _P])
     SIGNATURES["pqc_%s_crypto_kem_dec" % _alg] = (_I, [_P, _P, _P])

@@ -31,6 +31,8 @@ struct pqc_ctx {
   // and 64 bytes per element (seed expansion)
   uint16_t* nhpolys = nullptr;
   uint8_t* nhbytes = nullptr;
+  // NewHope-1024 CCA-KEM: per-element row of hashes, flags and the re-encryption
+  uint8_t* nhkem = nullptr;
 
   // FrodoKEM-976: device-resident key, expanded A (uint16 and MFMA digit
   // fragments), per-element scratch (S' fragments, E', E'', GEMM output, mu, k,
@@ -97,6 +99,12 @@ int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_
 int newhope_enc(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
                 hipStream_t s);
 int newhope_dec(pqc_ctx* c, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
+// NewHope-1024 CCA-KEM over the CPA kernels (newhope.hip); keypair coins 64 bytes
+constexpr int kNewhopeKemBytes = 2432;
+int newhope_cca_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s);
+int newhope_cca_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+                    hipStream_t s);
+int newhope_cca_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
 int newhope_poly_transform(int op, size_t n, uint16_t* polys, hipStream_t s);
 int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,
                         hipStream_t s);

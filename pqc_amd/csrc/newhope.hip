@@ -93,12 +93,12 @@ struct Pv {
 // ---- sponge kernels --------------------------------------------------------
 
 // keypair seed expansion: z = SHAKE256(seed, 64) (cpapke.cu:283-298)
-__global__ void k_seed_expand(int n, uint8_t* z, const uint8_t* seed) {
+__global__ void k_seed_expand(int n, uint8_t* z, const uint8_t* seed, size_t seed_stride) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   uint64_t m[4], A[25];
 #pragma unroll
-  for (int w = 0; w < 4; ++w) m[w] = load_u64(seed + (size_t)x * 32 + 8 * w);
+  for (int w = 0; w < 4; ++w) m[w] = load_u64(seed + (size_t)x * seed_stride + 8 * w);
   absorb_fixed<kRateShake256, 4>(A, m, 0x1F);
 #pragma unroll
   for (int w = 0; w < 8; ++w) store_u64(z + (size_t)x * 64 + 8 * w, A[w]);
@@ -392,7 +392,7 @@ __global__ void k_frombytes(int n, Pv r, const uint8_t* src, size_t stride, uint
 
 // poly_compress (poly.cu:165-186): lane (x, g) packs coefficients 8 g .. 8 g + 7
 // into ct + POLYBYTES + 3 g
-__global__ void k_compress(int n, uint8_t* ct, Pv p) {
+__global__ void k_compress(int n, uint8_t* ct, size_t ct_stride, Pv p) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 8)) return;
   const int x = (int)(t / (N / 8)), g = (int)(t % (N / 8));
@@ -401,18 +401,18 @@ __global__ void k_compress(int n, uint8_t* ct, Pv p) {
   uint32_t v[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) v[j] = (((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) / Q) & 7;
-  uint8_t* r = ct + (size_t)x * CT + POLYBYTES + 3 * g;
+  uint8_t* r = ct + (size_t)x * ct_stride + POLYBYTES + 3 * g;
   r[0] = (uint8_t)(v[0] | (v[1] << 3) | (v[2] << 6));
   r[1] = (uint8_t)((v[2] >> 2) | (v[3] << 1) | (v[4] << 4) | (v[5] << 7));
   r[2] = (uint8_t)((v[5] >> 1) | (v[6] << 2) | (v[7] << 5));
 }
 
 // poly_decompress (poly.cu:197-216): lane (x, g) = coefficients 8 g .. 8 g + 7
-__global__ void k_decompress(int n, Pv r, const uint8_t* ct) {
+__global__ void k_decompress(int n, Pv r, const uint8_t* ct, size_t ct_stride) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 8)) return;
   const int x = (int)(t / (N / 8)), g = (int)(t % (N / 8));
-  const uint8_t* a = ct + (size_t)x * CT + POLYBYTES + 3 * g;
+  const uint8_t* a = ct + (size_t)x * ct_stride + POLYBYTES + 3 * g;
   const uint32_t a0 = a[0], a1 = a[1], a2 = a[2];
   uint32_t v[8];
   v[0] = a0 & 7;
@@ -432,11 +432,11 @@ __global__ void k_decompress(int n, Pv r, const uint8_t* ct) {
 
 // poly_frommsg_n (poly.cu:226-251): lane (x, i) spreads message byte i over
 // coefficients 8 i + j + 256 q
-__global__ void k_frommsg(int n, Pv r, const uint8_t* msg) {
+__global__ void k_frommsg(int n, Pv r, const uint8_t* msg, size_t msg_stride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 32) return;
   const int x = t >> 5, i = t & 31;
-  const uint32_t b = msg[(size_t)x * 32 + i];
+  const uint32_t b = msg[(size_t)x * msg_stride + i];
   uint32_t o[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -448,7 +448,7 @@ __global__ void k_frommsg(int n, Pv r, const uint8_t* msg) {
 }
 
 // poly_tomsg_n (poly.cu:261-291): lane (x, i) decodes message byte i
-__global__ void k_tomsg(int n, uint8_t* msg, Pv p) {
+__global__ void k_tomsg(int n, uint8_t* msg, size_t msg_stride, Pv p) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 32) return;
   const int x = t >> 5, i = t & 31;
@@ -467,7 +467,73 @@ __global__ void k_tomsg(int n, uint8_t* msg, Pv p) {
     v = (v - Q) & 0xFFFFu;
     byte |= (v >> 15) << j;
   }
-  msg[(size_t)x * 32 + i] = (uint8_t)byte;
+  msg[(size_t)x * msg_stride + i] = (uint8_t)byte;
+}
+
+// ---- CCA-KEM glue: one lane per element for the sponges, flat copies -------------
+
+// out[x] = SHAKE256(in[x], 8 words bytes), first 8 OW bytes
+template <int OW>
+__global__ void k_shake256_words(int n, uint8_t* out, size_t out_stride, const uint8_t* in, size_t in_stride,
+                                 int words) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  uint64_t A[25];
+  sponge_absorb_words<kRateShake256>(A, reinterpret_cast<const uint64_t*>(in + (size_t)x * in_stride), words, 0x1F);
+#pragma unroll
+  for (int w = 0; w < OW; ++w) store_u64(out + (size_t)x * out_stride + 8 * w, A[w]);
+}
+
+// out[x] = SHAKE256(a[x] (32 bytes) || b[x] (32 bytes)), first 8 OW bytes
+template <int OW>
+__global__ void k_shake256_2x32(int n, uint8_t* out, size_t out_stride, const uint8_t* a, size_t a_stride,
+                                const uint8_t* b, size_t b_stride) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  uint64_t m[8], A[25];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    m[w] = load_u64(a + (size_t)x * a_stride + 8 * w);
+    m[4 + w] = load_u64(b + (size_t)x * b_stride + 8 * w);
+  }
+  absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+#pragma unroll
+  for (int w = 0; w < OW; ++w) store_u64(out + (size_t)x * out_stride + 8 * w, A[w]);
+}
+
+// dst[x][0 .. 8 words) = src[x][0 .. 8 words), one lane per word
+__global__ void k_copy_words(int n, uint8_t* dst, size_t dst_stride, const uint8_t* src, size_t src_stride,
+                             int words) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * words) return;
+  const size_t x = t / words, w = t % words;
+  store_u64(dst + x * dst_stride + 8 * w, load_u64(src + x * src_stride + 8 * w));
+}
+
+// fail[x] = 1 if a[x] and b[x] differ anywhere in `words` words, else 0: one wave
+// per element, no early exit (verify.cu:15-28 without its data-dependent branch)
+__global__ __launch_bounds__(64) void k_verify_words(int n, uint32_t* fail, size_t fail_stride, const uint8_t* a,
+                                                    size_t a_stride, const uint8_t* b, size_t b_stride, int words) {
+  const int x = blockIdx.x;
+  if (x >= n) return;
+  const uint64_t* pa = reinterpret_cast<const uint64_t*>(a + (size_t)x * a_stride);
+  const uint64_t* pb = reinterpret_cast<const uint64_t*>(b + (size_t)x * b_stride);
+  uint64_t d = 0;
+  for (int w = threadIdx.x; w < words; w += 64) d |= pa[w] ^ pb[w];
+  const bool any = __any(d != 0);
+  if (threadIdx.x == 0) *reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(fail) + (size_t)x * fail_stride) = any;
+}
+
+// k[x] = fail ? z[x] : k[x], 32 bytes, by mask (verify.cu:40-50 cmov)
+__global__ void k_cmov32(int n, uint8_t* k, size_t k_stride, const uint8_t* z, size_t z_stride, const uint32_t* fail,
+                         size_t fail_stride) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * 4) return;
+  const size_t x = t >> 2, w = t & 3;
+  const uint64_t f = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(fail) + x * fail_stride);
+  const uint64_t mask = 0ull - (f & 1);
+  uint8_t* pk = k + x * k_stride + 8 * w;
+  store_u64(pk, (load_u64(pk) & ~mask) | (load_u64(z + x * z_stride + 8 * w) & mask));
 }
 
 // ---- orchestration (cpapke.cu:265-476) -------------------------------------------
@@ -487,15 +553,18 @@ struct Launch {
 
 }  // namespace newhope
 
-int newhope_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
-  using namespace newhope;
-  const newhope::Launch L{c->tune.newhope_block};
-  const int n = (int)n_;
-  const size_t n1 = n_, n16 = 16 * n_, n128 = 128 * n_, n256 = 256 * n_;
+namespace newhope {
+
+// cpapke_keypair (cpapke.cu:265-338) with pk at stride PK and sk at sk_stride
+// (the CCA-KEM's sk holds sk_cpa first), seeds at seed_stride
+int keypair_impl(pqc_ctx* c, int n, uint8_t* pk, uint8_t* sk, size_t sk_stride, const uint8_t* seed,
+                 size_t seed_stride, hipStream_t s) {
+  const Launch L{c->tune.newhope_block};
+  const size_t n1 = n, n16 = 16 * (size_t)n, n128 = 128 * (size_t)n, n256 = 256 * (size_t)n;
   Scratch S{c};
   uint8_t* z = c->nhbytes;  // 64 bytes per element: publicseed || noiseseed
   const int AHAT = 0, EHAT = 1, AS = 2, BHAT = 3, SHAT = 4;
-  LAUNCH(k_seed_expand, n1, n, z, seed);
+  LAUNCH(k_seed_expand, n1, n, z, seed, seed_stride);
   LAUNCH(k_uniform, n16, n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
   LAUNCH(k_sample, n16, n, S.poly(SHAT), (const uint8_t*)z + 32, (size_t)64, 0);
   LAUNCH_NTT(k_ntt, n, S.poly(SHAT));
@@ -503,26 +572,25 @@ int newhope_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8
   LAUNCH_NTT(k_ntt, n, S.poly(EHAT));
   LAUNCH(k_pointwise, n128, n, S.poly(AS), S.poly(SHAT), S.poly(AHAT));
   LAUNCH(k_addsub, n128, n, S.poly(BHAT), S.poly(EHAT), S.poly(AS), 0);
-  LAUNCH(k_tobytes, n256, n, sk, (size_t)SK, S.poly(SHAT), (const uint8_t*)nullptr, (size_t)0);
+  LAUNCH(k_tobytes, n256, n, sk, sk_stride, S.poly(SHAT), (const uint8_t*)nullptr, (size_t)0);
   LAUNCH(k_tobytes, n256, n, pk, (size_t)PK, S.poly(BHAT), (const uint8_t*)z, (size_t)64);
   return hip_err(hipGetLastError());
 }
 
-int newhope_enc(pqc_ctx* c, size_t n_, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
-                hipStream_t s) {
-  using namespace newhope;
-  const newhope::Launch L{c->tune.newhope_block};
-  const int n = (int)n_;
-  const size_t n16 = 16 * n_, n32 = 32 * n_, n128 = 128 * n_, n256 = 256 * n_;
+// cpapke_enc (cpapke.cu:353-429), every buffer at its own stride
+int enc_impl(pqc_ctx* c, int n, uint8_t* ct, size_t ct_stride, const uint8_t* m, size_t m_stride, const uint8_t* pk,
+             size_t pk_stride, const uint8_t* coin, size_t coin_stride, hipStream_t s) {
+  const Launch L{c->tune.newhope_block};
+  const size_t n16 = 16 * (size_t)n, n32 = 32 * (size_t)n, n128 = 128 * (size_t)n, n256 = 256 * (size_t)n;
   Scratch S{c};
   uint8_t* seed = c->nhbytes;  // publicseed copy, 32 bytes per element
   const int SP = 0, EP = 1, VP = 2, AHAT = 3, BHAT = 4, EPP = 5, UHAT = 6, V = 7;
-  LAUNCH(k_frommsg, n32, n, S.poly(V), m);
-  LAUNCH(k_frombytes, n256, n, S.poly(BHAT), pk, (size_t)PK, seed);
+  LAUNCH(k_frommsg, n32, n, S.poly(V), m, m_stride);
+  LAUNCH(k_frombytes, n256, n, S.poly(BHAT), pk, pk_stride, seed);
   LAUNCH(k_uniform, n16, n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
-  LAUNCH(k_sample, n16, n, S.poly(SP), coin, (size_t)32, 0);
-  LAUNCH(k_sample, n16, n, S.poly(EP), coin, (size_t)32, 1);
-  LAUNCH(k_sample, n16, n, S.poly(EPP), coin, (size_t)32, 2);
+  LAUNCH(k_sample, n16, n, S.poly(SP), coin, coin_stride, 0);
+  LAUNCH(k_sample, n16, n, S.poly(EP), coin, coin_stride, 1);
+  LAUNCH(k_sample, n16, n, S.poly(EPP), coin, coin_stride, 2);
   LAUNCH_NTT(k_ntt, n, S.poly(SP));
   LAUNCH_NTT(k_ntt, n, S.poly(EP));
   LAUNCH(k_pointwise, n128, n, S.poly(UHAT), S.poly(AHAT), S.poly(SP));
@@ -531,8 +599,104 @@ int newhope_enc(pqc_ctx* c, size_t n_, uint8_t* ct, const uint8_t* m, const uint
   LAUNCH_NTT(k_invntt, n, S.poly(VP));
   LAUNCH(k_addsub, n128, n, S.poly(VP), S.poly(VP), S.poly(EPP), 0);
   LAUNCH(k_addsub, n128, n, S.poly(VP), S.poly(VP), S.poly(V), 0);
-  LAUNCH(k_tobytes, n256, n, ct, (size_t)CT, S.poly(UHAT), (const uint8_t*)nullptr, (size_t)0);
-  LAUNCH(k_compress, n128, n, ct, S.poly(VP));
+  LAUNCH(k_tobytes, n256, n, ct, ct_stride, S.poly(UHAT), (const uint8_t*)nullptr, (size_t)0);
+  LAUNCH(k_compress, n128, n, ct, ct_stride, S.poly(VP));
+  return hip_err(hipGetLastError());
+}
+
+// cpapke_dec (cpapke.cu:443-476), every buffer at its own stride
+int dec_impl(pqc_ctx* c, int n, uint8_t* m, size_t m_stride, const uint8_t* ct, size_t ct_stride, const uint8_t* sk,
+             size_t sk_stride, hipStream_t s) {
+  const Launch L{c->tune.newhope_block};
+  const size_t n32 = 32 * (size_t)n, n128 = 128 * (size_t)n, n256 = 256 * (size_t)n;
+  Scratch S{c};
+  const int VP = 0, UHAT = 1, TMP = 2, SHAT = 3;
+  LAUNCH(k_frombytes, n256, n, S.poly(SHAT), sk, sk_stride, (uint8_t*)nullptr);
+  LAUNCH(k_frombytes, n256, n, S.poly(UHAT), ct, ct_stride, (uint8_t*)nullptr);
+  LAUNCH(k_decompress, n128, n, S.poly(VP), ct, ct_stride);
+  LAUNCH(k_pointwise, n128, n, S.poly(TMP), S.poly(SHAT), S.poly(UHAT));
+  LAUNCH_NTT(k_invntt, n, S.poly(TMP));
+  LAUNCH(k_addsub, n128, n, S.poly(TMP), S.poly(TMP), S.poly(VP), 1);
+  LAUNCH(k_tomsg, n32, n, m, m_stride, S.poly(TMP));
+  return hip_err(hipGetLastError());
+}
+
+// CCA-KEM layouts (NewHope_NIST_GPU/params.h:37-39) and the per-element scratch
+// row: mu or m' | H(pk) | K || coin || d | H(c) | fail | re-encrypted ct
+constexpr int CCA_SK = SK + PK + 2 * SYM;  // 3680: sk_cpa || pk || H(pk) || s
+constexpr int CCA_CT = CT + SYM;           // 2208: ct_cpa || d
+constexpr int KM_MU = 0, KM_HPK = 32, KM_KCD = 64, KM_HC = 160, KM_FAIL = 192, KM_CT2 = 224;
+constexpr int KEMB = KM_CT2 + CCA_CT;      // 2432 bytes per element
+static_assert(KEMB == kNewhopeKemBytes, "scratch row size");
+
+}  // namespace newhope
+
+int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
+  return newhope::keypair_impl(c, (int)n, pk, sk, newhope::SK, seed, 32, s);
+}
+
+int newhope_enc(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
+                hipStream_t s) {
+  using namespace newhope;
+  return enc_impl(c, (int)n, ct, CT, m, 32, pk, PK, coin, 32, s);
+}
+
+int newhope_dec(pqc_ctx* c, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  using namespace newhope;
+  return dec_impl(c, (int)n, m, 32, ct, CT, sk, SK, s);
+}
+
+// NewHope-1024 CCA-KEM, the NewHope specification's transform over the CPA-PKE
+// (SHAKE256 for every hash).  Keypair coins: CPA seed || s, 64 bytes.
+int newhope_cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+  using namespace newhope;
+  const Launch L{c->tune.newhope_block};
+  const int n = (int)n_;
+  int rc = keypair_impl(c, n, pk, sk, CCA_SK, coins, 64, s);
+  if (rc) return rc;
+  LAUNCH(k_copy_words, n_ * (PK / 8), n, sk + SK, (size_t)CCA_SK, (const uint8_t*)pk, (size_t)PK, PK / 8);
+  LAUNCH(k_shake256_words<4>, n_, n, sk + SK + PK, (size_t)CCA_SK, (const uint8_t*)pk, (size_t)PK, PK / 8);
+  LAUNCH(k_copy_words, n_ * 4, n, sk + CCA_SK - SYM, (size_t)CCA_SK, coins + SYM, (size_t)64, 4);
+  return hip_err(hipGetLastError());
+}
+
+int newhope_cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+                    hipStream_t s) {
+  using namespace newhope;
+  const Launch L{c->tune.newhope_block};
+  const int n = (int)n_;
+  uint8_t* km = c->nhkem;
+  const size_t kb = KEMB;
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_MU, kb, coins, (size_t)32, 4);       // mu = H(coins)
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HPK, kb, pk, (size_t)PK, PK / 8);    // H(pk)
+  LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
+  int rc = enc_impl(c, n, ct, CCA_CT, km + KM_MU, kb, pk, PK, km + KM_KCD + SYM, kb, s);
+  if (rc) return rc;
+  LAUNCH(k_copy_words, n_ * 4, n, ct + CT, (size_t)CCA_CT, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, (const uint8_t*)ct, (size_t)CCA_CT, CCA_CT / 8);      // H(c)
+  LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);
+  return hip_err(hipGetLastError());
+}
+
+int newhope_cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  using namespace newhope;
+  const Launch L{c->tune.newhope_block};
+  const int n = (int)n_;
+  uint8_t* km = c->nhkem;
+  const size_t kb = KEMB;
+  int rc = dec_impl(c, n, km + KM_MU, kb, ct, CCA_CT, sk, CCA_SK, s);  // m'
+  if (rc) return rc;
+  LAUNCH(k_copy_words, n_ * 4, n, km + KM_HPK, kb, sk + SK + PK, (size_t)CCA_SK, 4);
+  LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
+  rc = enc_impl(c, n, km + KM_CT2, kb, km + KM_MU, kb, sk + SK, CCA_SK, km + KM_KCD + SYM, kb, s);
+  if (rc) return rc;
+  LAUNCH(k_copy_words, n_ * 4, n, km + KM_CT2 + CT, kb, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d'
+  hipLaunchKernelGGL(k_verify_words, dim3((unsigned)n), dim3(64), 0, s, n, reinterpret_cast<uint32_t*>(km + KM_FAIL),
+                     kb, ct, (size_t)CCA_CT, (const uint8_t*)km + KM_CT2, kb, CCA_CT / 8);
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, ct, (size_t)CCA_CT, CCA_CT / 8);  // H(c)
+  LAUNCH(k_cmov32, n_ * 4, n, km + KM_KCD, kb, sk + CCA_SK - SYM, (size_t)CCA_SK,
+         reinterpret_cast<const uint32_t*>(km + KM_FAIL), kb);
+  LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);
   return hip_err(hipGetLastError());
 }
 
@@ -557,23 +721,6 @@ int newhope_poly_transform(int op, size_t n, uint16_t* polys, hipStream_t s) {
     hipLaunchKernelGGL(k_ntt, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv{polys});
   else
     hipLaunchKernelGGL(k_invntt, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv{polys});
-  return hip_err(hipGetLastError());
-}
-
-int newhope_dec(pqc_ctx* c, size_t n_, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
-  using namespace newhope;
-  const newhope::Launch L{c->tune.newhope_block};
-  const int n = (int)n_;
-  const size_t n32 = 32 * n_, n128 = 128 * n_, n256 = 256 * n_;
-  Scratch S{c};
-  const int VP = 0, UHAT = 1, TMP = 2, SHAT = 3;
-  LAUNCH(k_frombytes, n256, n, S.poly(SHAT), sk, (size_t)SK, (uint8_t*)nullptr);
-  LAUNCH(k_frombytes, n256, n, S.poly(UHAT), ct, (size_t)CT, (uint8_t*)nullptr);
-  LAUNCH(k_decompress, n128, n, S.poly(VP), ct);
-  LAUNCH(k_pointwise, n128, n, S.poly(TMP), S.poly(SHAT), S.poly(UHAT));
-  LAUNCH_NTT(k_invntt, n, S.poly(TMP));
-  LAUNCH(k_addsub, n128, n, S.poly(TMP), S.poly(TMP), S.poly(VP), 1);
-  LAUNCH(k_tomsg, n32, n, m, S.poly(TMP));
   return hip_err(hipGetLastError());
 }
 

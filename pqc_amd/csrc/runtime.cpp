@@ -26,6 +26,7 @@ bool alg_info(int alg, AlgInfo* a) {
     case PQC_ALG_KYBER1024: *a = {1, 4, 1568, 3168, 1568, 32}; return true;
     case PQC_ALG_NEWHOPE1024_CPA: *a = {2, 0, 1824, 1792, 2176, 32}; return true;
     case PQC_ALG_FRODO976: *a = {3, 0, 15632, 31296, 15744, 24}; return true;
+    case PQC_ALG_NEWHOPE1024: *a = {4, 0, 1824, 3680, 2208, 32}; return true;
     default: return false;
   }
 }
@@ -142,7 +143,7 @@ pqc_ctx* default_ctx() {
   if (!g_default[dev]) {
     pqc_ctx* c = nullptr;
     unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024) |
-                    PQC_ALG_MASK(PQC_ALG_FRODO976);
+                    PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024);
     if (pqc_ctx_create(&c, mask, 64, 0) != PQC_OK) return nullptr;
     g_default[dev] = c;
   }
@@ -224,10 +225,28 @@ int check_call(pqc_ctx* c, int alg, size_t n) {
 int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, const uint8_t* i0,
              const uint8_t* i1, unsigned flags, void* stream) {
   AlgInfo a;
-  if (!alg_info(alg, &a) || (a.kind != 1 && a.kind != 3)) return PQC_ERR_UNSUPPORTED;
+  if (!alg_info(alg, &a) || (a.kind != 1 && a.kind != 3 && a.kind != 4)) return PQC_ERR_UNSUPPORTED;
   int rc = check_call(c, alg, n);
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (a.kind == 4) {  // NewHope-1024 CCA-KEM
+    switch (op) {
+      case OP_KEYPAIR: {
+        const Buf b[3] = {{o0, a.pk, true}, {o1, a.sk, true}, {i0, 64, false}};
+        return staged(c, n, flags, s, b,
+                      [&](uint8_t** d) { return pqc::newhope_cca_keypair(c, n, d[0], d[1], d[2], s); });
+      }
+      case OP_ENC: {
+        const Buf b[4] = {{o0, a.ct, true}, {o1, a.ss, true}, {i0, a.pk, false}, {i1, 32, false}};
+        return staged(c, n, flags, s, b,
+                      [&](uint8_t** d) { return pqc::newhope_cca_enc(c, n, d[0], d[1], d[2], d[3], s); });
+      }
+      default: {
+        const Buf b[3] = {{o0, a.ss, true}, {i0, a.ct, false}, {i1, a.sk, false}};
+        return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_cca_dec(c, n, d[0], d[1], d[2], s); });
+      }
+    }
+  }
   if (a.kind == 3) {  // FrodoKEM-976 with one key per element
     switch (op) {
       case OP_KEYPAIR: {
@@ -276,8 +295,9 @@ int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const ui
   pqc_ctx* c = default_ctx();
   if (!c) return PQC_ERR_NO_DEVICE;
   // Coins are drawn call by call as the NIST submissions do, so that the
-  // known-answer-test DRBG reproduces their outputs: Kyber keypair d then z
-  // (indcpa_keypair, then crypto_kem_keypair's z), encaps 32 bytes; Frodo
+  // known-answer-test DRBG reproduces their outputs: Kyber and NewHope keypair
+  // CPA seed then z / s (cpapke/indcpa keypair, then crypto_kem_keypair's own
+  // draw), encaps 32 bytes; Frodo
   // keypair s || seedSE || z in one draw, encaps mu (FrodoKEM_GPU/kem.c:33,90).
   uint8_t coins[64];
   const bool frodo = alg == PQC_ALG_FRODO976;
@@ -304,7 +324,7 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
   if (rc) return rc;
   const unsigned known = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) |
                          PQC_ALG_MASK(PQC_ALG_KYBER1024) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA) |
-                         PQC_ALG_MASK(PQC_ALG_FRODO976);
+                         PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024);
   if (alg_mask == 0 || (alg_mask & ~known)) return PQC_ERR_UNSUPPORTED;
   pqc_ctx* c = new pqc_ctx();
   if (hipGetDevice(&c->device) != hipSuccess) {
@@ -343,13 +363,18 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     c->kmix_order = (uint32_t*)p;
     c->kmix_counts = (uint32_t*)((uint8_t*)p + (size_t)3 * c->ld * sizeof(uint32_t));
   }
-  if (alg_mask & PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA)) {
+  if (alg_mask & (PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024))) {
     rc = ctx_alloc(c, &p, (size_t)8 * 1024 * c->ld * sizeof(uint16_t));
     if (rc) goto fail;
     c->nhpolys = (uint16_t*)p;
     rc = ctx_alloc(c, &p, (size_t)64 * c->ld);
     if (rc) goto fail;
     c->nhbytes = (uint8_t*)p;
+  }
+  if (alg_mask & PQC_ALG_MASK(PQC_ALG_NEWHOPE1024)) {
+    rc = ctx_alloc(c, &p, (size_t)pqc::kNewhopeKemBytes * c->ld);
+    if (rc) goto fail;
+    c->nhkem = (uint8_t*)p;
   }
   if (alg_mask & PQC_ALG_MASK(PQC_ALG_FRODO976)) {
     rc = ctx_alloc(c, &p, (size_t)976 * 976 * 2);
@@ -407,6 +432,7 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->fr_elem) (void)hipFree(c->fr_elem);
   if (c->nhpolys) (void)hipFree(c->nhpolys);
   if (c->nhbytes) (void)hipFree(c->nhbytes);
+  if (c->nhkem) (void)hipFree(c->nhkem);
   if (c->stage) (void)hipFree(c->stage);
   pqc::hostio_free(c);
   if (c->status) (void)hipFree(c->status);
@@ -680,6 +706,7 @@ PQC_NIST_SET(kyber512, PQC_ALG_KYBER512)
 PQC_NIST_SET(kyber768, PQC_ALG_KYBER768)
 PQC_NIST_SET(kyber1024, PQC_ALG_KYBER1024)
 PQC_NIST_SET(frodo976, PQC_ALG_FRODO976)
+PQC_NIST_SET(newhope1024, PQC_ALG_NEWHOPE1024)
 
 void pqc_randombytes_seed(uint64_t seed) {
   pqc::nist_drbg_disable();

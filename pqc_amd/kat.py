@@ -8,7 +8,8 @@ library's own (randombytes_init / randombytes, host code); `coins` replays the
 draws each parameter set makes, so a whole file is three batched GPU calls.
 
 Draw order per parameter set (what the single-instance entry points do too):
-Kyber keypair d (32) then z (32), encaps m (32, hashed inside); FrodoKEM-976
+Kyber and NewHope keypair CPA seed (32) then z / s (32), encaps m (32, hashed
+inside); FrodoKEM-976
 keypair s || seedSE || z (64, FrodoKEM_GPU/kem.c:33), encaps mu (24, kem.c:90).
 """
 import ctypes as C
@@ -19,7 +20,7 @@ from . import _lib as L
 from .kem import Context
 
 RSP_NAMES = {"kyber512": "Kyber512", "kyber768": "Kyber768", "kyber1024": "Kyber1024",
-             "frodo976": "FrodoKEM-976-SHAKE"}
+             "frodo976": "FrodoKEM-976-SHAKE", "newhope1024": "NewHope1024-CCAKEM"}
 FIELDS = ("seed", "pk", "sk", "ct", "ss")
 
 

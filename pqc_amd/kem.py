@@ -19,6 +19,7 @@ ALGS = {
     "kyber1024": L.ALG_KYBER1024,
     "newhope1024cpa": L.ALG_NEWHOPE1024_CPA,
     "frodo976": L.ALG_FRODO976,
+    "newhope1024": L.ALG_NEWHOPE1024,
 }
 KYBER_K = {L.ALG_KYBER512: 2, L.ALG_KYBER768: 3, L.ALG_KYBER1024: 4}
 

@@ -268,6 +268,27 @@ def newhope_dec(c, sk):
     return bytes(m)
 
 
+NH_CCA_SK, NH_CCA_CT = 1792 + 1824 + 64, 2176 + 32
+
+
+def newhope_cca_keypair(coins64):
+    pk, sk = buf(NH_PK), buf(NH_CCA_SK)
+    assert lib().oracle_newhope_cca_keypair(pk, sk, bytes(coins64)) == 0
+    return bytes(pk), bytes(sk)
+
+
+def newhope_cca_enc(pk, coins32):
+    ct, ss = buf(NH_CCA_CT), buf(32)
+    assert lib().oracle_newhope_cca_enc(ct, ss, bytes(pk), bytes(coins32)) == 0
+    return bytes(ct), bytes(ss)
+
+
+def newhope_cca_dec(ct, sk):
+    ss = buf(32)
+    assert lib().oracle_newhope_cca_dec(ss, bytes(ct), bytes(sk)) == 0
+    return bytes(ss)
+
+
 def newhope_triples(seed, msg, coin, nthreads=8):
     n = seed.shape[0]
     pk = np.empty((n, NH_PK), np.uint8)

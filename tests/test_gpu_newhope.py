@@ -95,3 +95,79 @@ def test_poly_sample_vs_oracle():
         got = pqc_amd.newhope_poly_sample(seeds, nonce)
         for i in range(len(seeds)):
             assert (got[i] == O.newhope_sample(seeds[i].tobytes(), nonce)).all(), (nonce, i)
+
+
+# ---- NewHope-1024 CCA-KEM over the CPA kernels ---------------------------------------
+@pytest.fixture(scope="module")
+def cctx():
+    import pqc_amd
+    c = pqc_amd.Context(["newhope1024"], max_batch=16384)
+    yield c
+    c.close()
+
+
+def _oracle_cca(kc, ec):
+    outs = [O.newhope_cca_keypair(kc[i].tobytes()) for i in range(len(kc))]
+    encs = [O.newhope_cca_enc(pk, ec[i].tobytes()) for i, (pk, _) in enumerate(outs)]
+    return outs, encs
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 200])
+def test_cca_vs_oracle(cctx, n):
+    say("newhope cca n", n)
+    kc = O.drbg_array(n, 64, 0, 900 + n)
+    ec = O.drbg_array(n, 32, 0, 901 + n)
+    pk, sk = cctx.keypair_host("newhope1024", kc)
+    ct, ss = cctx.enc_host("newhope1024", pk, ec)
+    ss2 = cctx.dec_host("newhope1024", ct, sk)
+    keys, encs = _oracle_cca(kc, ec)
+    for i in range(n):
+        assert (pk[i].tobytes(), sk[i].tobytes()) == keys[i]
+        assert (ct[i].tobytes(), ss[i].tobytes()) == encs[i]
+    assert (ss2 == ss).all()
+
+
+def test_cca_implicit_rejection_vs_oracle(cctx):
+    """Tampered bytes in u, v and d, and random-byte ciphertexts: the GPU's shared
+    secrets equal the oracle's (H(s || H(ct')))."""
+    say("newhope cca rejection")
+    n = 96
+    kc = O.drbg_array(n, 64, 0, 950)
+    ec = O.drbg_array(n, 32, 0, 951)
+    pk, sk = cctx.keypair_host("newhope1024", kc)
+    ct, ss = cctx.enc_host("newhope1024", pk, ec)
+    bad = ct.copy()
+    pos = np.array([0, 1791, 1792, 2175, 2176, 2207])
+    for i in range(n):
+        bad[i, pos[i % len(pos)]] ^= 0x10 if i % 3 else 0x01
+    bad[-8:] = np.random.default_rng(5).integers(0, 256, (8, 2208), dtype=np.uint8)
+    got = cctx.dec_host("newhope1024", bad, sk)
+    for i in range(n):
+        assert got[i].tobytes() == O.newhope_cca_dec(bad[i].tobytes(), sk[i].tobytes())
+    assert (got != ss).any(axis=1).all()
+
+
+def test_cca_full_size_16384_roundtrip_and_sample(cctx):
+    import pqc_amd
+    n = 16384
+    say("newhope cca full size", n)
+    rng = np.random.default_rng(12)
+    kc = rng.integers(0, 256, (n, 64), dtype=np.uint8)
+    ec = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    d_kc, d_ec = pqc_amd.DeviceBuffer.from_array(kc), pqc_amd.DeviceBuffer.from_array(ec)
+    pk, sk = pqc_amd.DeviceBuffer(n * 1824), pqc_amd.DeviceBuffer(n * 3680)
+    ct, ss, ss2 = pqc_amd.DeviceBuffer(n * 2208), pqc_amd.DeviceBuffer(n * 32), pqc_amd.DeviceBuffer(n * 32)
+    cctx.keypair_derand("newhope1024", n, pk, sk, d_kc)
+    cctx.enc_derand("newhope1024", n, ct, ss, pk, d_ec)
+    h_ct = ct.download((n, 2208))
+    h_ct[::7, 2200] ^= 0x80  # every 7th ciphertext rejected
+    ct.upload(h_ct)
+    cctx.dec("newhope1024", n, ss2, ct, sk)
+    a, e = ss2.download((n, 32)), ss.download((n, 32))
+    good = np.ones(n, bool)
+    good[::7] = False
+    assert (a[good] == e[good]).all() and (a[~good] != e[~good]).any(axis=1).all()
+    h_pk, h_sk = pk.download((n, 1824)), sk.download((n, 3680))
+    for i in rng.choice(n, 12, replace=False):
+        assert (h_pk[i].tobytes(), h_sk[i].tobytes()) == O.newhope_cca_keypair(kc[i].tobytes())
+        assert a[i].tobytes() == O.newhope_cca_dec(h_ct[i].tobytes(), h_sk[i].tobytes())

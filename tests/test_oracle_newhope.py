@@ -152,3 +152,46 @@ def test_triple_batch_driver():
     assert (m2 == msg).all()
     for i in range(n):
         assert O.newhope_keypair(seed[i].tobytes()) == (pk[i].tobytes(), sk[i].tobytes())
+
+
+# ---- CCA-KEM: layout and equations of the specification's transform, with hashlib ----
+def _shake(data, n):
+    return hashlib.shake_256(bytes(data)).digest(n)
+
+
+@pytest.mark.parametrize("idx", range(3))
+def test_cca_keypair_layout(idx):
+    coins = O.drbg(64, seed=0, stream=60, index=idx)
+    pk, sk = O.newhope_cca_keypair(coins)
+    cpk, csk = O.newhope_keypair(coins[:32])
+    assert pk == cpk and len(sk) == 3680
+    assert sk[:1792] == csk and sk[1792:3616] == pk
+    assert sk[3616:3648] == _shake(pk, 32) and sk[3648:] == coins[32:]
+
+
+@pytest.mark.parametrize("idx", range(3))
+def test_cca_encaps_equations(idx):
+    """ct_cpa decrypts to mu = H(coins); d and ss follow from hashlib alone:
+    (K, coin', d) = H(mu || H(pk), 96), ct_cpa = Enc(pk, mu, coin'),
+    ss = H(K || H(ct_cpa || d))."""
+    pk, sk = O.newhope_cca_keypair(O.drbg(64, seed=0, stream=61, index=idx))
+    coins = O.drbg(32, seed=0, stream=62, index=idx)
+    ct, ss = O.newhope_cca_enc(pk, coins)
+    mu = _shake(coins, 32)
+    assert O.newhope_dec(ct[:2176], sk[:1792]) == mu
+    kcd = _shake(mu + _shake(pk, 32), 96)
+    assert ct[:2176] == O.newhope_enc(mu, pk, kcd[32:64])
+    assert ct[2176:] == kcd[64:]
+    assert ss == _shake(kcd[:32] + _shake(ct, 32), 32)
+    assert O.newhope_cca_dec(ct, sk) == ss
+
+
+@pytest.mark.parametrize("pos", [0, 1791, 1792, 2175, 2176, 2207])
+def test_cca_implicit_rejection(pos):
+    """A changed ciphertext byte anywhere (u, v, d) gives H(s || H(ct')), s from the secret key."""
+    pk, sk = O.newhope_cca_keypair(O.drbg(64, seed=0, stream=63, index=pos))
+    ct, ss = O.newhope_cca_enc(pk, O.drbg(32, seed=0, stream=64, index=pos))
+    bad = bytearray(ct)
+    bad[pos] ^= 0x04
+    got = O.newhope_cca_dec(bytes(bad), sk)
+    assert got != ss and got == _shake(sk[3648:] + _shake(bad, 32), 32)
