@@ -1,5 +1,6 @@
 /*
- * NewHope-1024 CPA-PKE restatement for the oracle (TEST INFRASTRUCTURE ONLY).
+ * NewHope-1024 (and, through newhope512.c, NewHope-512) CPA-PKE and CCA-KEM
+ * restatement for the oracle (TEST INFRASTRUCTURE ONLY).
  *
  * Per-element C following the batched CUDA sources of NewHope_NIST_GPU:
  *   params ........... params.h:9-39 (n = 1024, q = 12289, k = 8)
@@ -26,7 +27,24 @@
 #include <pthread.h>
 #include "oracle.h"
 
+/* N = 1024 by default; newhope512.c compiles this file again with N = 512
+ * (precomp.cu:75-190 tables, g = 10968; ntt.cu:130-180 NTT with 9 levels and
+ * % q on the odd ones; poly.cu:244,279 message coding over 2 coefficients).
+ * NH_FN names the exported functions of each build. */
+#ifndef NH_N
 #define NH_N 1024
+#endif
+#if NH_N == 1024
+#define NH_FN(x) oracle_newhope_##x
+#define NH_G 7
+#define NH_LOG 10
+#elif NH_N == 512
+#define NH_FN(x) oracle_newhope512_##x
+#define NH_G 10968
+#define NH_LOG 9
+#else
+#error "NH_N must be 512 or 1024"
+#endif
 #define NH_Q 12289
 #define NH_SYM 32
 #define NH_POLYBYTES ((14 * NH_N) / 8)
@@ -49,10 +67,10 @@ static uint32_t invmod(uint32_t a) { return powmod(a, NH_Q - 2); }
 
 static void nh_init(void) {
     if (g_nh_init) return;
-    const uint32_t g = 7, mont = (1u << 18) % NH_Q;
+    const uint32_t g = NH_G, mont = (1u << 18) % NH_Q;
     for (unsigned i = 0; i < NH_N; i++) {
         unsigned r = 0;
-        for (int b = 0; b < 10; b++) r |= ((i >> b) & 1u) << (9 - b);
+        for (int b = 0; b < NH_LOG; b++) r |= ((i >> b) & 1u) << (NH_LOG - 1 - b);
         g_brv[i] = (uint16_t)r;
     }
     const uint32_t g2inv = invmod(g * g % NH_Q), ginv = invmod(g), ninv = invmod(NH_N);
@@ -65,7 +83,7 @@ static void nh_init(void) {
     g_nh_init = 1;
 }
 
-void oracle_newhope_tables(uint16_t omegas_inv[512], uint16_t gammas[1024], uint16_t gammas_inv[1024]) {
+void NH_FN(tables)(uint16_t omegas_inv[NH_N / 2], uint16_t gammas[NH_N], uint16_t gammas_inv[NH_N]) {
     nh_init();
     memcpy(omegas_inv, g_omegas_inv, sizeof g_omegas_inv);
     memcpy(gammas, g_gammas, sizeof g_gammas);
@@ -73,7 +91,7 @@ void oracle_newhope_tables(uint16_t omegas_inv[512], uint16_t gammas[1024], uint
 }
 
 /* reduce.cu:22-31 */
-uint16_t oracle_newhope_montgomery_reduce(uint32_t a) {
+uint16_t NH_FN(montgomery_reduce)(uint32_t a) {
     uint32_t u = a * 12287u;
     u &= (1u << 18) - 1;
     u *= NH_Q;
@@ -82,15 +100,15 @@ uint16_t oracle_newhope_montgomery_reduce(uint32_t a) {
 }
 
 /* reduce.cu:33-47 */
-uint16_t oracle_newhope_barrett_reduce(uint16_t a) {
+uint16_t NH_FN(barrett_reduce)(uint16_t a) {
     uint32_t u = ((uint32_t)a * 5) >> 16;
     u *= NH_Q;
     a = (uint16_t)(a - u);
     return a;
 }
 
-#define mont oracle_newhope_montgomery_reduce
-#define barr oracle_newhope_barrett_reduce
+#define mont NH_FN(montgomery_reduce)
+#define barr NH_FN(barrett_reduce)
 
 /* ntt.cu:92-109 */
 static void bitrev_vector(uint16_t *p) {
@@ -109,9 +127,10 @@ static void mul_coefficients(uint16_t *p, const uint16_t *f) {
     for (unsigned i = 0; i < NH_N; i++) p[i] = mont((uint32_t)p[i] * f[i]);
 }
 
-/* ntt.cu:195-234: levels in pairs, even lazy, odd Barrett */
+/* ntt.cu:195-234 (N = 1024): levels in pairs, even lazy, odd Barrett;
+ * ntt.cu:130-180 (N = 512): 9 levels, even lazy, odd reduced with % q */
 static void ntt_core(uint16_t *a, const uint16_t *omega) {
-    for (int i = 0; i < 10; i += 2) {
+    for (int i = 0; i < NH_LOG; i += 2) {
         int distance = 1 << i;
         for (int start = 0; start < distance; start++) {
             int jt = 0;
@@ -122,13 +141,18 @@ static void ntt_core(uint16_t *a, const uint16_t *omega) {
                 a[j + distance] = mont((uint32_t)W * ((uint32_t)temp + 3 * NH_Q - a[j + distance]));
             }
         }
+        if (i + 1 >= NH_LOG) break;
         distance <<= 1;
         for (int start = 0; start < distance; start++) {
             int jt = 0;
             for (int j = start; j < NH_N - 1; j += 2 * distance) {
                 uint16_t W = omega[jt++];
                 uint16_t temp = a[j];
+#if NH_N == 1024
                 a[j] = barr((uint16_t)(temp + a[j + distance]));
+#else
+                a[j] = (uint16_t)((temp + a[j + distance]) % NH_Q);
+#endif
                 a[j + distance] = mont((uint32_t)W * ((uint32_t)temp + 3 * NH_Q - a[j + distance]));
             }
         }
@@ -136,14 +160,14 @@ static void ntt_core(uint16_t *a, const uint16_t *omega) {
 }
 
 /* poly.cu:519-527 */
-void oracle_newhope_ntt(uint16_t *r) {
+void NH_FN(ntt)(uint16_t *r) {
     nh_init();
     mul_coefficients(r, g_gammas);
     ntt_core(r, g_gammas);
 }
 
 /* poly.cu:538-547 */
-void oracle_newhope_invntt(uint16_t *r) {
+void NH_FN(invntt)(uint16_t *r) {
     nh_init();
     bitrev_vector(r);
     ntt_core(r, g_omegas_inv);
@@ -233,7 +257,7 @@ static void poly_frommsg(uint16_t *r, const uint8_t *msg) {
     for (unsigned i = 0; i < 32; i++)
         for (unsigned j = 0; j < 8; j++) {
             unsigned mask = -(unsigned)((msg[i] >> j) & 1);
-            for (unsigned q = 0; q < 4; q++) r[8 * i + j + 256 * q] = (uint16_t)(mask & (NH_Q / 2));
+            for (unsigned q = 0; q < NH_N / 256; q++) r[8 * i + j + 256 * q] = (uint16_t)(mask & (NH_Q / 2));
         }
 }
 
@@ -243,9 +267,13 @@ static void poly_tomsg(uint8_t *msg, const uint16_t *x) {
     for (unsigned i = 0; i < 256; i++) {
         uint16_t t = flipabs(x[i]);
         t = (uint16_t)(t + flipabs(x[i + 256]));
+#if NH_N == 1024
         t = (uint16_t)(t + flipabs(x[i + 512]));
         t = (uint16_t)(t + flipabs(x[i + 768]));
         t = (uint16_t)(t - NH_Q);
+#else
+        t = (uint16_t)(t - NH_Q / 2);
+#endif
         t >>= 15;
         msg[i >> 3] |= (uint8_t)(t << (i & 7));
     }
@@ -253,7 +281,7 @@ static void poly_tomsg(uint8_t *msg, const uint16_t *x) {
 
 /* poly.cu:332-366: 16 SHAKE128(seed || i) streams, 16-bit candidates < 5q kept
  * unreduced, 64 coefficients each */
-void oracle_newhope_poly_uniform(uint16_t *a, const uint8_t seed[32]) {
+void NH_FN(poly_uniform)(uint16_t *a, const uint8_t seed[32]) {
     uint8_t ext[NH_SYM + 1], buf[168 * 8];
     memcpy(ext, seed, NH_SYM);
     for (unsigned i = 0; i < NH_N / 64; i++) {
@@ -274,7 +302,7 @@ void oracle_newhope_poly_uniform(uint16_t *a, const uint8_t seed[32]) {
 }
 
 /* poly.cu:376-444: hw(a) + q - hw(b) over SHAKE256(seed || nonce || i, 128) */
-void oracle_newhope_poly_sample(uint16_t *r, const uint8_t seed[32], uint8_t nonce) {
+void NH_FN(poly_sample)(uint16_t *r, const uint8_t seed[32], uint8_t nonce) {
     uint8_t ext[NH_SYM + 2], buf[128];
     memcpy(ext, seed, NH_SYM);
     ext[NH_SYM] = nonce;
@@ -309,17 +337,17 @@ static void poly_sub(uint16_t *r, const uint16_t *a, const uint16_t *b) {
 }
 
 /* cpapke.cu:265-338 */
-int oracle_newhope_cpa_keypair(uint8_t *pk, uint8_t *sk, const uint8_t seed[32]) {
+int NH_FN(cpa_keypair)(uint8_t *pk, uint8_t *sk, const uint8_t seed[32]) {
     uint16_t ahat[NH_N], ehat[NH_N], ahat_shat[NH_N], bhat[NH_N], shat[NH_N];
     uint8_t z[2 * NH_SYM];
     nh_init();
     oracle_shake256(z, 2 * NH_SYM, seed, NH_SYM);
     const uint8_t *publicseed = z, *noiseseed = z + NH_SYM;
-    oracle_newhope_poly_uniform(ahat, publicseed);
-    oracle_newhope_poly_sample(shat, noiseseed, 0);
-    oracle_newhope_ntt(shat);
-    oracle_newhope_poly_sample(ehat, noiseseed, 1);
-    oracle_newhope_ntt(ehat);
+    NH_FN(poly_uniform)(ahat, publicseed);
+    NH_FN(poly_sample)(shat, noiseseed, 0);
+    NH_FN(ntt)(shat);
+    NH_FN(poly_sample)(ehat, noiseseed, 1);
+    NH_FN(ntt)(ehat);
     poly_mul_pointwise(ahat_shat, shat, ahat);
     poly_add(bhat, ehat, ahat_shat);
     poly_tobytes(sk, shat);
@@ -329,23 +357,23 @@ int oracle_newhope_cpa_keypair(uint8_t *pk, uint8_t *sk, const uint8_t seed[32])
 }
 
 /* cpapke.cu:353-429 */
-int oracle_newhope_cpa_enc(uint8_t *c, const uint8_t m[32], const uint8_t *pk, const uint8_t coin[32]) {
+int NH_FN(cpa_enc)(uint8_t *c, const uint8_t m[32], const uint8_t *pk, const uint8_t coin[32]) {
     uint16_t sprime[NH_N], eprime[NH_N], vprime[NH_N], ahat[NH_N], bhat[NH_N], epp[NH_N], uhat[NH_N], v[NH_N];
     uint8_t publicseed[NH_SYM];
     nh_init();
     poly_frommsg(v, m);
     poly_frombytes(bhat, pk);
     memcpy(publicseed, pk + NH_POLYBYTES, NH_SYM);
-    oracle_newhope_poly_uniform(ahat, publicseed);
-    oracle_newhope_poly_sample(sprime, coin, 0);
-    oracle_newhope_poly_sample(eprime, coin, 1);
-    oracle_newhope_poly_sample(epp, coin, 2);
-    oracle_newhope_ntt(sprime);
-    oracle_newhope_ntt(eprime);
+    NH_FN(poly_uniform)(ahat, publicseed);
+    NH_FN(poly_sample)(sprime, coin, 0);
+    NH_FN(poly_sample)(eprime, coin, 1);
+    NH_FN(poly_sample)(epp, coin, 2);
+    NH_FN(ntt)(sprime);
+    NH_FN(ntt)(eprime);
     poly_mul_pointwise(uhat, ahat, sprime);
     poly_add(uhat, uhat, eprime);
     poly_mul_pointwise(vprime, bhat, sprime);
-    oracle_newhope_invntt(vprime);
+    NH_FN(invntt)(vprime);
     poly_add(vprime, vprime, epp);
     poly_add(vprime, vprime, v);
     poly_tobytes(c, uhat);
@@ -354,20 +382,20 @@ int oracle_newhope_cpa_enc(uint8_t *c, const uint8_t m[32], const uint8_t *pk, c
 }
 
 /* cpapke.cu:443-476 */
-int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk) {
+int NH_FN(cpa_dec)(uint8_t m[32], const uint8_t *c, const uint8_t *sk) {
     uint16_t vprime[NH_N], uhat[NH_N], tmp[NH_N], shat[NH_N];
     nh_init();
     poly_frombytes(shat, sk);
     poly_frombytes(uhat, c);
     poly_decompress(vprime, c + NH_POLYBYTES);
     poly_mul_pointwise(tmp, shat, uhat);
-    oracle_newhope_invntt(tmp);
+    NH_FN(invntt)(tmp);
     poly_sub(tmp, tmp, vprime);
     poly_tomsg(m, tmp);
     return 0;
 }
 
-/* NewHope-1024 CCA-KEM: the Fujisaki-Okamoto transform of the NewHope
+/* NewHope CCA-KEM: the Fujisaki-Okamoto transform of the NewHope
  * specification (CCA-KEM Gen / Encaps / Decaps, SHAKE256 for every hash) over the
  * CPA-PKE above.  The reference fixes its sizes (NewHope_NIST_GPU/params.h:37-39:
  * sk = sk_cpa || pk || H(pk) || s, ct = ct_cpa || d) and its verify/cmov
@@ -377,32 +405,32 @@ int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk) {
 #define NH_CCA_SK (NH_POLYBYTES + NH_PK + 2 * NH_SYM)
 #define NH_CCA_CT (NH_CT + NH_SYM)
 
-int oracle_newhope_cca_keypair(uint8_t *pk, uint8_t *sk, const uint8_t coins[64]) {
-    oracle_newhope_cpa_keypair(pk, sk, coins);
+int NH_FN(cca_keypair)(uint8_t *pk, uint8_t *sk, const uint8_t coins[64]) {
+    NH_FN(cpa_keypair)(pk, sk, coins);
     memcpy(sk + NH_POLYBYTES, pk, NH_PK);
     oracle_shake256(sk + NH_POLYBYTES + NH_PK, NH_SYM, pk, NH_PK);
     memcpy(sk + NH_CCA_SK - NH_SYM, coins + NH_SYM, NH_SYM);
     return 0;
 }
 
-int oracle_newhope_cca_enc(uint8_t *ct, uint8_t ss[32], const uint8_t *pk, const uint8_t coins[32]) {
+int NH_FN(cca_enc)(uint8_t *ct, uint8_t ss[32], const uint8_t *pk, const uint8_t coins[32]) {
     uint8_t buf[2 * NH_SYM], kcd[3 * NH_SYM];
     oracle_shake256(buf, NH_SYM, coins, NH_SYM);        /* mu = H(coins): raw coins are not released */
     oracle_shake256(buf + NH_SYM, NH_SYM, pk, NH_PK);   /* H(pk) */
     oracle_shake256(kcd, 3 * NH_SYM, buf, 2 * NH_SYM);  /* K || coin' || d */
-    oracle_newhope_cpa_enc(ct, buf, pk, kcd + NH_SYM);
+    NH_FN(cpa_enc)(ct, buf, pk, kcd + NH_SYM);
     memcpy(ct + NH_CT, kcd + 2 * NH_SYM, NH_SYM);
     oracle_shake256(kcd + NH_SYM, NH_SYM, ct, NH_CCA_CT); /* H(c || d) */
     oracle_shake256(ss, NH_SYM, kcd, 2 * NH_SYM);
     return 0;
 }
 
-int oracle_newhope_cca_dec(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk) {
+int NH_FN(cca_dec)(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk) {
     uint8_t buf[2 * NH_SYM], kcd[3 * NH_SYM], ct2[NH_CCA_CT];
-    oracle_newhope_cpa_dec(buf, ct, sk);
+    NH_FN(cpa_dec)(buf, ct, sk);
     memcpy(buf + NH_SYM, sk + NH_POLYBYTES + NH_PK, NH_SYM);
     oracle_shake256(kcd, 3 * NH_SYM, buf, 2 * NH_SYM);
-    oracle_newhope_cpa_enc(ct2, buf, sk + NH_POLYBYTES, kcd + NH_SYM);
+    NH_FN(cpa_enc)(ct2, buf, sk + NH_POLYBYTES, kcd + NH_SYM);
     memcpy(ct2 + NH_CT, kcd + 2 * NH_SYM, NH_SYM);
     const int fail = memcmp(ct, ct2, NH_CCA_CT) != 0;
     oracle_shake256(kcd + NH_SYM, NH_SYM, ct, NH_CCA_CT);
@@ -421,14 +449,14 @@ typedef struct {
 static void *nh_run(void *arg) {
     nh_job *j = (nh_job *)arg;
     for (size_t e = j->lo; e < j->hi; e++) {
-        oracle_newhope_cpa_keypair(j->pk + e * 1824, j->sk + e * 1792, j->seed + e * 32);
-        oracle_newhope_cpa_enc(j->ct + e * 2176, j->msg + e * 32, j->pk + e * 1824, j->coin + e * 32);
-        oracle_newhope_cpa_dec(j->m2 + e * 32, j->ct + e * 2176, j->sk + e * 1792);
+        NH_FN(cpa_keypair)(j->pk + e * NH_PK, j->sk + e * NH_POLYBYTES, j->seed + e * 32);
+        NH_FN(cpa_enc)(j->ct + e * NH_CT, j->msg + e * 32, j->pk + e * NH_PK, j->coin + e * 32);
+        NH_FN(cpa_dec)(j->m2 + e * 32, j->ct + e * NH_CT, j->sk + e * NH_POLYBYTES);
     }
     return NULL;
 }
 
-int oracle_newhope_triple_batch(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
+int NH_FN(triple_batch)(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
                                 const uint8_t *seed, const uint8_t *msg, const uint8_t *coin, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;

@@ -90,6 +90,22 @@ int oracle_newhope_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk);
 int oracle_newhope_cca_keypair(uint8_t *pk, uint8_t *sk, const uint8_t coins[64]);
 int oracle_newhope_cca_enc(uint8_t *ct, uint8_t ss[32], const uint8_t *pk, const uint8_t coins[32]);
 int oracle_newhope_cca_dec(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk);
+/* NewHope-512: the same functions at N = 512 (newhope512.c) */
+void oracle_newhope512_tables(uint16_t omegas_inv[256], uint16_t gammas[512], uint16_t gammas_inv[512]);
+uint16_t oracle_newhope512_montgomery_reduce(uint32_t a);
+uint16_t oracle_newhope512_barrett_reduce(uint16_t a);
+void oracle_newhope512_ntt(uint16_t *r);
+void oracle_newhope512_invntt(uint16_t *r);
+void oracle_newhope512_poly_uniform(uint16_t *a, const uint8_t seed[32]);
+void oracle_newhope512_poly_sample(uint16_t *r, const uint8_t seed[32], uint8_t nonce);
+int oracle_newhope512_cpa_keypair(uint8_t *pk, uint8_t *sk, const uint8_t seed[32]);
+int oracle_newhope512_cpa_enc(uint8_t *c, const uint8_t m[32], const uint8_t *pk, const uint8_t coin[32]);
+int oracle_newhope512_cpa_dec(uint8_t m[32], const uint8_t *c, const uint8_t *sk);
+int oracle_newhope512_cca_keypair(uint8_t *pk, uint8_t *sk, const uint8_t coins[64]);
+int oracle_newhope512_cca_enc(uint8_t *ct, uint8_t ss[32], const uint8_t *pk, const uint8_t coins[32]);
+int oracle_newhope512_cca_dec(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk);
+int oracle_newhope512_triple_batch(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
+                                   const uint8_t *seed, const uint8_t *msg, const uint8_t *coin, int nthreads);
 int oracle_newhope_triple_batch(size_t n, uint8_t *pk, uint8_t *sk, uint8_t *ct, uint8_t *m2,
                                 const uint8_t *seed, const uint8_t *msg, const uint8_t *coin, int nthreads);
 

@@ -1,8 +1,9 @@
-"""Writes tests/golden/newhope1024_tables.json: the numeric NTT tables the
-reference tabulates for n = 1024 in NewHope_NIST_GPU/precomp.cu
+"""Writes tests/golden/newhope1024_tables.json and newhope512_tables.json: the
+numeric NTT tables the reference tabulates in NewHope_NIST_GPU/precomp.cu
 (omegas_inv_bitrev_montgomery, gammas_bitrev_montgomery,
-gammas_inv_montgomery; the __device__ arrays at precomp.cu:191-371) and the
-10-bit bit-reversal table of NewHope_NIST_GPU/ntt.cu:46.
+gammas_inv_montgomery; for n = 1024 the __device__ arrays at
+precomp.cu:191-371, for n = 512 the host arrays at precomp.cu:75-190) and the
+bit-reversal tables of NewHope_NIST_GPU/ntt.cu:46 (10-bit) and :19 (9-bit).
 
 Data only: it pins the oracle's closed-form table generator against the
 reference's own tables.  Run from the repository root where the reference is
@@ -37,6 +38,18 @@ def main():
     }
     dst = Path(__file__).with_name("newhope1024_tables.json")
     dst.write_text(json.dumps(out) + "\n")
+    print("wrote", dst)
+    # n = 512: the host arrays of the NEWHOPE_N == 512 branches (precomp.cu:75-190,
+    # ntt.cu:19), which come first in each file
+    out512 = {
+        "source": "NewHope_NIST_GPU/precomp.cu:75-190, ntt.cu:19",
+        "omegas_inv_bitrev_montgomery": extract(pre, "omegas_inv_bitrev_montgomery", 256, device=False),
+        "gammas_bitrev_montgomery": extract(pre, "gammas_bitrev_montgomery", 512, device=False),
+        "gammas_inv_montgomery": extract(pre, "gammas_inv_montgomery", 512, device=False),
+        "bitrev_table": extract(ntt, "bitrev_table", 512, device=False),
+    }
+    dst = Path(__file__).with_name("newhope512_tables.json")
+    dst.write_text(json.dumps(out512) + "\n")
     print("wrote", dst)
 
 

@@ -212,90 +212,104 @@ def kyber_batch(op, k, arrays, nthreads=8):
 
 # ---- NewHope-1024 CPA ------------------------------------------------------------
 NH_PK, NH_SK, NH_CT = 1824, 1792, 2176
+NH_CCA_SK, NH_CCA_CT = 1792 + 1824 + 64, 2176 + 32
+
+
+def nh_sizes(n=1024):
+    """(pk, sk, ct, cca_sk, cca_ct) of NewHope-n: 14-bit polynomials, 3-bit v."""
+    poly, cb = 14 * n // 8, 3 * n // 8
+    pk, ct = poly + 32, poly + cb
+    return pk, poly, ct, poly + pk + 64, ct + 32
+
+
+def _nh(name, n):
+    return getattr(lib(), ("oracle_newhope_" if n == 1024 else "oracle_newhope512_") + name)
 
 
 def _u16(a):
     return np.ascontiguousarray(a, dtype=np.uint16)
 
 
-def newhope_tables():
-    o = np.zeros(512, np.uint16)
-    g = np.zeros(1024, np.uint16)
-    gi = np.zeros(1024, np.uint16)
-    lib().oracle_newhope_tables(ptr(o), ptr(g), ptr(gi))
+def newhope_tables(n=1024):
+    o = np.zeros(n // 2, np.uint16)
+    g = np.zeros(n, np.uint16)
+    gi = np.zeros(n, np.uint16)
+    _nh("tables", n)(ptr(o), ptr(g), ptr(gi))
     return o, g, gi
 
 
-def newhope_ntt(a):
+def newhope_ntt(a, n=1024):
     a = _u16(a).copy()
-    lib().oracle_newhope_ntt(ptr(a))
+    assert a.size == n
+    _nh("ntt", n)(ptr(a))
     return a
 
 
-def newhope_invntt(a):
+def newhope_invntt(a, n=1024):
     a = _u16(a).copy()
-    lib().oracle_newhope_invntt(ptr(a))
+    assert a.size == n
+    _nh("invntt", n)(ptr(a))
     return a
 
 
-def newhope_uniform(seed):
-    a = np.zeros(1024, np.uint16)
-    lib().oracle_newhope_poly_uniform(ptr(a), bytes(seed))
+def newhope_uniform(seed, n=1024):
+    a = np.zeros(n, np.uint16)
+    _nh("poly_uniform", n)(ptr(a), bytes(seed))
     return a
 
 
-def newhope_sample(seed, nonce):
-    a = np.zeros(1024, np.uint16)
-    lib().oracle_newhope_poly_sample(ptr(a), bytes(seed), C.c_uint8(nonce))
+def newhope_sample(seed, nonce, n=1024):
+    a = np.zeros(n, np.uint16)
+    _nh("poly_sample", n)(ptr(a), bytes(seed), C.c_uint8(nonce))
     return a
 
 
-def newhope_keypair(seed):
-    pk, sk = buf(NH_PK), buf(NH_SK)
-    assert lib().oracle_newhope_cpa_keypair(pk, sk, bytes(seed)) == 0
+def newhope_keypair(seed, n=1024):
+    pkb, skb, _, _, _ = nh_sizes(n)
+    pk, sk = buf(pkb), buf(skb)
+    assert _nh("cpa_keypair", n)(pk, sk, bytes(seed)) == 0
     return bytes(pk), bytes(sk)
 
 
-def newhope_enc(m, pk, coin):
-    c = buf(NH_CT)
-    assert lib().oracle_newhope_cpa_enc(c, bytes(m), bytes(pk), bytes(coin)) == 0
+def newhope_enc(m, pk, coin, n=1024):
+    c = buf(nh_sizes(n)[2])
+    assert _nh("cpa_enc", n)(c, bytes(m), bytes(pk), bytes(coin)) == 0
     return bytes(c)
 
 
-def newhope_dec(c, sk):
+def newhope_dec(c, sk, n=1024):
     m = buf(32)
-    assert lib().oracle_newhope_cpa_dec(m, bytes(c), bytes(sk)) == 0
+    assert _nh("cpa_dec", n)(m, bytes(c), bytes(sk)) == 0
     return bytes(m)
 
 
-NH_CCA_SK, NH_CCA_CT = 1792 + 1824 + 64, 2176 + 32
-
-
-def newhope_cca_keypair(coins64):
-    pk, sk = buf(NH_PK), buf(NH_CCA_SK)
-    assert lib().oracle_newhope_cca_keypair(pk, sk, bytes(coins64)) == 0
+def newhope_cca_keypair(coins64, n=1024):
+    pkb, _, _, skb, _ = nh_sizes(n)
+    pk, sk = buf(pkb), buf(skb)
+    assert _nh("cca_keypair", n)(pk, sk, bytes(coins64)) == 0
     return bytes(pk), bytes(sk)
 
 
-def newhope_cca_enc(pk, coins32):
-    ct, ss = buf(NH_CCA_CT), buf(32)
-    assert lib().oracle_newhope_cca_enc(ct, ss, bytes(pk), bytes(coins32)) == 0
+def newhope_cca_enc(pk, coins32, n=1024):
+    ct, ss = buf(nh_sizes(n)[4]), buf(32)
+    assert _nh("cca_enc", n)(ct, ss, bytes(pk), bytes(coins32)) == 0
     return bytes(ct), bytes(ss)
 
 
-def newhope_cca_dec(ct, sk):
+def newhope_cca_dec(ct, sk, n=1024):
     ss = buf(32)
-    assert lib().oracle_newhope_cca_dec(ss, bytes(ct), bytes(sk)) == 0
+    assert _nh("cca_dec", n)(ss, bytes(ct), bytes(sk)) == 0
     return bytes(ss)
 
 
-def newhope_triples(seed, msg, coin, nthreads=8):
+def newhope_triples(seed, msg, coin, nthreads=8, N=1024):
     n = seed.shape[0]
-    pk = np.empty((n, NH_PK), np.uint8)
-    sk = np.empty((n, NH_SK), np.uint8)
-    ct = np.empty((n, NH_CT), np.uint8)
+    pkb, skb, ctb, _, _ = nh_sizes(N)
+    pk = np.empty((n, pkb), np.uint8)
+    sk = np.empty((n, skb), np.uint8)
+    ct = np.empty((n, ctb), np.uint8)
     m2 = np.empty((n, 32), np.uint8)
-    lib().oracle_newhope_triple_batch(C.c_size_t(n), ptr(pk), ptr(sk), ptr(ct), ptr(m2), ptr(seed), ptr(msg),
+    _nh("triple_batch", N)(C.c_size_t(n), ptr(pk), ptr(sk), ptr(ct), ptr(m2), ptr(seed), ptr(msg),
                                       ptr(coin), nthreads)
     return pk, sk, ct, m2
 
