@@ -421,7 +421,7 @@ NEWHOPE_BATCH = 16384
 def bench_newhope(args):
     import pqc_amd as pqc
     n = NEWHOPE_BATCH if args.batch == BATCH else args.batch
-    ctx = pqc.Context(["newhope1024cpa", "newhope1024"], max_batch=n)
+    ctx = pqc.Context(["newhope1024cpa", "newhope1024", "newhope512cpa", "newhope512"], max_batch=n)
     stream = pqc.Stream()
     rng = np.random.default_rng(0)
     seed, msg, coin = (pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8)) for _ in range(3))
@@ -438,6 +438,7 @@ def bench_newhope(args):
     log("newhope triples: %.3f ms/step" % ms)
     ok = bool((m2.download((n, 32)) == msg.download((n, 32))).all())
     cca = newhope_cca_rates(pqc, ctx, stream, n, rng, max(3, args.steps // 4))
+    nh512 = newhope512_rates(pqc, ctx, stream, n, rng, max(3, args.steps // 4))
 
     # The step's largest kernel is poly_sample (5 launches per triple batch); time
     # it alone on the same stream, and the NTT (HBM figure) beside it.
@@ -467,6 +468,7 @@ def bench_newhope(args):
                    "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
                    "messages_recovered": ok,
                    "cca_kem": cca,
+                   "newhope512": nh512,
                    "poly_ntt": {"kernel_ms": round(ntt_ms, 4), "bytes": ntt_bytes,
                                 "hbm_gbps": round(ntt_bytes / ntt_ms / 1e6, 1),
                                 "hbm_frac": round(ntt_bytes / ntt_ms / 1e6 / HBM_PEAK_GBPS, 4)}},
@@ -497,23 +499,41 @@ def bench_newhope(args):
     ctx.close()
 
 
-def newhope_cca_rates(pqc, ctx, stream, n, rng, steps):
-    """NewHope-1024 CCA-KEM keygen, encaps and decaps over the same batch size
+def newhope_cca_rates(pqc, ctx, stream, n, rng, steps, alg="newhope1024"):
+    """NewHope CCA-KEM keygen, encaps and decaps over the same batch size
     (the specification's transform over the CPA kernels above)."""
+    pkb, skb, ctb, _ = pqc.sizes(alg)
     kc = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 64), dtype=np.uint8))
     ec = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8))
-    pk, sk, ct = pqc.DeviceBuffer(n * 1824), pqc.DeviceBuffer(n * 3680), pqc.DeviceBuffer(n * 2208)
+    pk, sk, ct = pqc.DeviceBuffer(n * pkb), pqc.DeviceBuffer(n * skb), pqc.DeviceBuffer(n * ctb)
     ss, ss2 = pqc.DeviceBuffer(n * 32), pqc.DeviceBuffer(n * 32)
     res = {}
-    for name, fn in (("keypair", lambda: ctx.keypair_derand("newhope1024", n, pk, sk, kc, stream=stream)),
-                     ("encaps", lambda: ctx.enc_derand("newhope1024", n, ct, ss, pk, ec, stream=stream)),
-                     ("decaps", lambda: ctx.dec("newhope1024", n, ss2, ct, sk, stream=stream))):
+    for name, fn in (("keypair", lambda: ctx.keypair_derand(alg, n, pk, sk, kc, stream=stream)),
+                     ("encaps", lambda: ctx.enc_derand(alg, n, ct, ss, pk, ec, stream=stream)),
+                     ("decaps", lambda: ctx.dec(alg, n, ss2, ct, sk, stream=stream))):
         wall, _ = timed(fn, stream, steps, 1, pqc)
         res[name + "_per_s"] = round(n / (wall / steps), 1)
-        log("newhope cca %s: %.3f ms" % (name, wall * 1e3 / steps))
+        log("%s cca %s: %.3f ms" % (alg, name, wall * 1e3 / steps))
     res["shared_secrets_equal"] = bool((ss.download((n, 32)) == ss2.download((n, 32))).all())
-    res["sizes"] = "pk 1824, sk 3680, ct 2208, ss 32"
+    res["sizes"] = "pk %d, sk %d, ct %d, ss 32" % (pkb, skb, ctb)
     return res
+
+
+def newhope512_rates(pqc, ctx, stream, n, rng, steps):
+    """NewHope-512: CPA triples through the N = 512 kernels and the CCA-KEM."""
+    seed, msg, coin = (pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8)) for _ in range(3))
+    pk, sk, ct, m2 = (pqc.DeviceBuffer(n * b) for b in (928, 896, 1088, 32))
+
+    def triple():
+        ctx.newhope_keypair(n, pk, sk, seed, stream=stream, N=512)
+        ctx.newhope_enc(n, ct, msg, pk, coin, stream=stream, N=512)
+        ctx.newhope_dec(n, m2, ct, sk, stream=stream, N=512)
+
+    wall, _ = timed(triple, stream, steps, 1, pqc)
+    log("newhope512 triples: %.3f ms" % (wall * 1e3 / steps))
+    return {"cpa_triples_per_s": round(n / (wall / steps), 1),
+            "messages_recovered": bool((m2.download((n, 32)) == msg.download((n, 32))).all()),
+            "cca_kem": newhope_cca_rates(pqc, ctx, stream, n, rng, steps, "newhope512")}
 
 
 # ---- mixed Kyber-512/768/1024 (1:2:1), 65536 encaps + decaps, fused in a graph ---------

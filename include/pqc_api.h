@@ -45,6 +45,8 @@ extern "C" {
 #define PQC_ALG_FRODO976 5        /* FrodoKEM-976-SHAKE */
 #define PQC_ALG_NEWHOPE1024 6     /* NewHope-1024 CCA-KEM: pk 1824, sk 3680, ct 2208, ss 32
                                      (NewHope_NIST_GPU/params.h:37-39) */
+#define PQC_ALG_NEWHOPE512 7      /* NewHope-512 CCA-KEM: pk 928, sk 1888, ct 1120, ss 32 */
+#define PQC_ALG_NEWHOPE512_CPA 8  /* NewHope-512 CPA-PKE (the NEWHOPE_N == 512 branches) */
 #define PQC_ALG_MASK(a) (1u << (a))
 
 /* ---- call flags ------------------------------------------------------------- */
@@ -309,6 +311,10 @@ int pqc_frodo976_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, cons
 int pqc_newhope1024_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
 int pqc_newhope1024_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
 int pqc_newhope1024_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
+/* NewHope-512 CCA-KEM: the same transform at N = 512 */
+int pqc_newhope512_crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
+int pqc_newhope512_crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
+int pqc_newhope512_crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);
 
 /* ---- NewHope-1024 CPA-PKE, plain batched kernels ------------------------------
  * Replaces cpapke_keypair/enc/dec (NewHope_NIST_GPU/cpapke.h:14-24).
@@ -320,6 +326,16 @@ int pqc_newhope1024_cpa_enc_batch(pqc_ctx *ctx, size_t n, uint8_t *ct, const uin
                                   const uint8_t *pk, const uint8_t *coins, unsigned flags, void *stream);
 int pqc_newhope1024_cpa_dec_batch(pqc_ctx *ctx, size_t n, uint8_t *msgs, const uint8_t *ct,
                                   const uint8_t *sk, unsigned flags, void *stream);
+/* NewHope-512 CPA-PKE: the reference's NEWHOPE_N == 512 configuration
+ * (params.h:9-39 at N 512; precomp.cu:75-190; ntt.cu:13-37,130-180;
+ * poly.cu:244,279).  Element sizes: seed 32, pk 928, sk 896, ct 1088.
+ * Requires PQC_ALG_NEWHOPE512_CPA in the context's mask.                        */
+int pqc_newhope512_cpa_keypair_batch(pqc_ctx *ctx, size_t n, uint8_t *pk, uint8_t *sk,
+                                     const uint8_t *seeds, unsigned flags, void *stream);
+int pqc_newhope512_cpa_enc_batch(pqc_ctx *ctx, size_t n, uint8_t *ct, const uint8_t *msgs,
+                                 const uint8_t *pk, const uint8_t *coins, unsigned flags, void *stream);
+int pqc_newhope512_cpa_dec_batch(pqc_ctx *ctx, size_t n, uint8_t *msgs, const uint8_t *ct,
+                                 const uint8_t *sk, unsigned flags, void *stream);
 
 /* ---- FrodoKEM-976-SHAKE: batches under one device-resident key ------------------
  * Replaces crypto_kem_keypair_gpu/enc/dec (FrodoKEM_GPU/api.h:20-24,

@@ -23,6 +23,8 @@ ALG_KYBER1024 = 3
 ALG_NEWHOPE1024_CPA = 4
 ALG_FRODO976 = 5
 ALG_NEWHOPE1024 = 6
+ALG_NEWHOPE512 = 7
+ALG_NEWHOPE512_CPA = 8
 
 F_HOST_PTRS = 0x1
 F_PLAIN = 0x2
@@ -78,6 +80,9 @@ SIGNATURES = {
     "pqc_newhope1024_cpa_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_dec_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
+    "pqc_newhope512_cpa_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
+    "pqc_newhope512_cpa_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _P, _U, _P]),
+    "pqc_newhope512_cpa_dec_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_frodo976_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_frodo976_load_key": (_I, [_P, _P, _I, _U, _P]),
     "pqc_frodo976_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
@@ -117,7 +122,7 @@ SIGNATURES = {
     "pqc_version": (C.c_char_p, []),
     "pqc_strerror": (C.c_char_p, [_I]),
 }
-for _alg in ("kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024"):
+for _alg in ("kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024", "newhope512"):
     SIGNATURES["pqc_%s_crypto_kem_keypair" % _alg] = (_I, [_P, _P])
     SIGNATURES["pqc_%s_crypto_kem_enc" % _alg] = (_I, [_P, _P, _P])
     SIGNATURES["pqc_%s_crypto_kem_dec" % _alg] = (_I, [_P, _P, _P])

@@ -1,37 +1,51 @@
-// NewHope-1024 CPA-PKE as plain batched kernels (NewHope_NIST_GPU/cpapke.cu:
-// 265-476): one kernel per reference operation.  Polynomials are element-major
-// (element x's 1024 uint16 coefficients contiguous at x * 1024), and each kernel
-// spreads an element over as many lanes as its data flow allows:
-//   - NTT / inverse NTT: one wave per polynomial, 16 coefficients per lane in
-//     VGPRs, the 10 butterfly layers in three register-resident groups (index
-//     bits 0-3, 4-7, 8-9) with the polynomial re-laid out through LDS between
-//     groups;
+// NewHope CPA-PKE (N = 1024 and N = 512) as plain batched kernels
+// (NewHope_NIST_GPU/cpapke.cu:265-476): one kernel per reference operation,
+// and the NewHope CCA-KEM over them.  Polynomials are element-major (element
+// x's N uint16 coefficients contiguous at x * N), and each kernel spreads an
+// element over as many lanes as its data flow allows:
+//   - NTT / inverse NTT: one wave per polynomial, N / 64 coefficients per lane
+//     in VGPRs, the butterfly layers in register-resident groups (N = 1024:
+//     index bits 0-3, 4-7, 8-9; N = 512: bits 0-2, 3-5, 6-8) with the
+//     polynomial re-laid out through LDS between groups;
 //   - poly_uniform / poly_sample: one lane per 64-coefficient SHAKE stream
-//     (16 per polynomial);
+//     (N / 64 per polynomial);
 //   - pointwise, add/sub, (de)serialisation, (de)compression, message coding:
 //     flat passes, one lane per 4 or 8 coefficients, 8- or 16-byte accesses.
 //
 // The 16-bit storage and 32-bit intermediate arithmetic of the reference are
 // kept exactly (lazy even NTT levels stored as uint16, Barrett on a uint16
-// argument, Montgomery with R = 2^18 in uint32), so results are identical for
-// every input, including out-of-range ones.
+// argument at N = 1024 and % q at N = 512 on the odd levels, Montgomery with
+// R = 2^18 in uint32), so results are identical for every input, including
+// out-of-range ones.  N = 512 follows the reference's NEWHOPE_N == 512 branches
+// (precomp.cu:75-190 tables, ntt.cu:130-180, poly.cu:244,279), which its
+// batched GPU path never compiled.
 #include "internal.h"
 #include "sponge.cuh"
 
 namespace pqc {
 namespace newhope {
 
-constexpr int N = 1024;
 constexpr uint32_t Q = 12289;
 constexpr int SYM = 32;
-constexpr int POLYBYTES = 14 * N / 8;  // 1792
-constexpr int CBYTES = 3 * N / 8;      // 384
-constexpr int PK = POLYBYTES + SYM;    // 1824
-constexpr int SK = POLYBYTES;          // 1792
-constexpr int CT = POLYBYTES + CBYTES; // 2176
 
-// ---- tables: the Pari/GP definitions of precomp.cu:5-72 (n = 1024, g = 7,
-// mont = 2^18), evaluated at compile time -------------------------------------
+// sizes of NewHope-N (params.h:26-39)
+template <int N>
+struct Par {
+  static constexpr int POLYBYTES = 14 * N / 8;          // 1792 / 896
+  static constexpr int CBYTES = 3 * N / 8;              // 384 / 192
+  static constexpr int PK = POLYBYTES + SYM;            // 1824 / 928
+  static constexpr int SK = POLYBYTES;                  // 1792 / 896
+  static constexpr int CT = POLYBYTES + CBYTES;         // 2176 / 1088
+  static constexpr int CCA_SK = SK + PK + 2 * SYM;      // 3680 / 1888: sk_cpa || pk || H(pk) || s
+  static constexpr int CCA_CT = CT + SYM;               // 2208 / 1120: ct_cpa || d
+  static constexpr int LOG = N == 1024 ? 10 : 9;
+  static constexpr uint32_t G = N == 1024 ? 7 : 10968;  // precomp.cu: g = Mod(7 | 10968, q)
+  static constexpr int STREAMS = N / 64;                // SHAKE streams per polynomial
+  static constexpr int COPIES = N / 256;                // coefficients per message bit
+};
+
+// ---- tables: the Pari/GP definitions of precomp.cu:5-72 (mont = 2^18),
+// evaluated at compile time ---------------------------------------------------
 constexpr uint32_t powmod(uint32_t b, uint32_t e) {
   uint64_t r = 1, x = b % Q;
   while (e) {
@@ -42,12 +56,13 @@ constexpr uint32_t powmod(uint32_t b, uint32_t e) {
   return (uint32_t)r;
 }
 
-constexpr uint32_t brv10(uint32_t i) {
+constexpr uint32_t bitrev(uint32_t i, int bits) {
   uint32_t r = 0;
-  for (int b = 0; b < 10; b++) r |= ((i >> b) & 1u) << (9 - b);
+  for (int b = 0; b < bits; b++) r |= ((i >> b) & 1u) << (bits - 1 - b);
   return r;
 }
 
+template <int N>
 struct alignas(16) Tables {  // 16-byte aligned rows: read as uint4 by the NTT kernels
   uint16_t omegas_inv[N / 2];
   uint16_t gammas[N];
@@ -55,20 +70,30 @@ struct alignas(16) Tables {  // 16-byte aligned rows: read as uint4 by the NTT k
   uint16_t brv[N];
 };
 
-constexpr Tables make_tables() {
-  Tables t{};
-  const uint32_t g = 7, mont = (1u << 18) % Q;
+template <int N>
+constexpr Tables<N> make_tables() {
+  Tables<N> t{};
+  constexpr int B = Par<N>::LOG;
+  const uint32_t g = Par<N>::G, mont = (1u << 18) % Q;
   const uint32_t g2inv = powmod(g * g % Q, Q - 2), ginv = powmod(g, Q - 2), ninv = powmod(N, Q - 2);
-  for (uint32_t k = 0; k < N / 2; k++) t.omegas_inv[k] = (uint16_t)((uint64_t)powmod(g2inv, brv10(2 * k)) * mont % Q);
+  for (uint32_t k = 0; k < N / 2; k++) t.omegas_inv[k] = (uint16_t)((uint64_t)powmod(g2inv, bitrev(2 * k, B)) * mont % Q);
   for (uint32_t k = 0; k < N; k++) {
-    t.gammas[k] = (uint16_t)((uint64_t)powmod(g, brv10(k)) * mont % Q);
+    t.gammas[k] = (uint16_t)((uint64_t)powmod(g, bitrev(k, B)) * mont % Q);
     t.gammas_inv[k] = (uint16_t)((uint64_t)powmod(ginv, k) * ninv % Q * mont % Q);
-    t.brv[k] = (uint16_t)brv10(k);
+    t.brv[k] = (uint16_t)bitrev(k, B);
   }
   return t;
 }
 
-__constant__ Tables c_tab = make_tables();
+__constant__ Tables<1024> c_tab1024 = make_tables<1024>();
+__constant__ Tables<512> c_tab512 = make_tables<512>();
+
+template <int N>
+__device__ __forceinline__ const Tables<N>& tab();
+template <>
+__device__ __forceinline__ const Tables<1024>& tab<1024>() { return c_tab1024; }
+template <>
+__device__ __forceinline__ const Tables<512>& tab<512>() { return c_tab512; }
 
 // reduce.cu:22-31
 __device__ __forceinline__ uint32_t mont(uint32_t a) {
@@ -85,8 +110,9 @@ __device__ __forceinline__ uint32_t barrett(uint32_t a16) {
   return (a - u) & 0xFFFFu;
 }
 
+template <int N>
 struct Pv {
-  uint16_t* p;  // element x's coefficients at p + 1024 x
+  uint16_t* p;  // element x's coefficients at p + N x
   __device__ __forceinline__ uint16_t* el(int x) const { return p + (size_t)x * N; }
 };
 
@@ -106,9 +132,11 @@ __global__ void k_seed_expand(int n, uint8_t* z, const uint8_t* seed, size_t see
 
 // gen_a / poly_uniform (cpapke.cu:236-247, poly.cu:332-366): lane (x, i) runs
 // SHAKE128(seed || i) and keeps 16-bit candidates < 5q for coefficients 64 i ..
-__global__ void k_uniform(int n, Pv a, const uint8_t* seed, size_t stride) {
+template <int N>
+__global__ void k_uniform(int n, Pv<N> a, const uint8_t* seed, size_t stride) {
+  constexpr int S = Par<N>::STREAMS;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int x = t >> 4, i = t & 15;
+  const int x = t / S, i = t % S;
   if (x >= n) return;
   uint64_t A[25];
 #pragma unroll
@@ -132,9 +160,11 @@ __global__ void k_uniform(int n, Pv a, const uint8_t* seed, size_t stride) {
 
 // poly_sample (poly.cu:396-444): lane (x, i) = coefficients 64 i .. 64 i + 63,
 // hw(a) + q - hw(b) over SHAKE256(seed || nonce || i, 128)
-__global__ void k_sample(int n, Pv r, const uint8_t* seed, size_t stride, int nonce) {
+template <int N>
+__global__ void k_sample(int n, Pv<N> r, const uint8_t* seed, size_t stride, int nonce) {
+  constexpr int S = Par<N>::STREAMS;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int x = t >> 4, i = t & 15;
+  const int x = t / S, i = t % S;
   if (x >= n) return;
   uint64_t s[4], A[25];
 #pragma unroll
@@ -159,30 +189,31 @@ __global__ void k_sample(int n, Pv r, const uint8_t* seed, size_t stride, int no
 
 // ---- NTT kernels: one wave per polynomial ------------------------------------------
 
-// One butterfly of ntt (ntt.cu:195-234) on 16-bit values: even layers keep the
-// sum unreduced (stored as uint16), odd layers Barrett it; the difference is
+// One butterfly of ntt on 16-bit values: even layers keep the sum unreduced
+// (stored as uint16), odd layers reduce it (MODE 1: Barrett, ntt.cu:195-234 at
+// N = 1024; MODE 2: % q, ntt.cu:130-180 at N = 512); the difference is
 // multiplied by W in Montgomery form.
-template <bool REDUCE>
+template <int MODE>
 __device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t W) {
   const uint32_t t = a, h = b;
-  a = REDUCE ? barrett(t + h) : ((t + h) & 0xFFFFu);
+  a = MODE == 0 ? ((t + h) & 0xFFFFu) : MODE == 1 ? barrett(t + h) : (t + h) % Q;
   b = mont(W * (t + 3 * Q - h));
 }
 
 // Layer L (distance 2^L) on register pairs (r, r + 2^RB), RB the register bit
 // that holds index bit L; the twiddle of index idx is omega[idx >> (L + 1)].
-template <int L, int RB, typename IDX>
-__device__ __forceinline__ void ntt_layer(uint32_t (&v)[16], const uint16_t* omega, IDX idx) {
+template <int R, int L, int RB, int ODD, typename IDX>
+__device__ __forceinline__ void ntt_layer(uint32_t (&v)[R], const uint16_t* omega, IDX idx) {
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
+  for (int r = 0; r < R; ++r) {
     if (r & (1 << RB)) continue;
     const uint32_t W = omega[idx(r) >> (L + 1)];
-    bfly<(L & 1) != 0>(v[r], v[r + (1 << RB)], W);
+    bfly<(L & 1) ? ODD : 0>(v[r], v[r + (1 << RB)], W);
   }
 }
 
-// ntt_core on a polynomial that sits in LDS (sm) in ntt input order; returns the
-// output in v with v[r] = coefficient 4 l + (r & 3) + 256 (r >> 2).
+// ntt_core at N = 1024 on a polynomial that sits in LDS (sm) in ntt input
+// order; returns the output in v with v[r] = coefficient 4 l + (r & 3) + 256 (r >> 2).
 __device__ __forceinline__ void ntt_wave(uint32_t (&v)[16], uint16_t* sm, int l, const uint16_t* omega) {
   // layers 0-3: v[r] = index 16 l + r
   {
@@ -195,10 +226,10 @@ __device__ __forceinline__ void ntt_wave(uint32_t (&v)[16], uint16_t* sm, int l,
     }
   }
   auto ia = [l](int r) { return 16 * l + r; };
-  ntt_layer<0, 0>(v, omega, ia);
-  ntt_layer<1, 1>(v, omega, ia);
-  ntt_layer<2, 2>(v, omega, ia);
-  ntt_layer<3, 3>(v, omega, ia);
+  ntt_layer<16, 0, 0, 1>(v, omega, ia);
+  ntt_layer<16, 1, 1, 1>(v, omega, ia);
+  ntt_layer<16, 2, 2, 1>(v, omega, ia);
+  ntt_layer<16, 3, 3, 1>(v, omega, ia);
   {
     uint32_t w[8];
 #pragma unroll
@@ -212,10 +243,10 @@ __device__ __forceinline__ void ntt_wave(uint32_t (&v)[16], uint16_t* sm, int l,
   auto ib = [l](int r) { return (l & 15) + 16 * r + 256 * (l >> 4); };
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = sm[ib(r)];
-  ntt_layer<4, 0>(v, omega, ib);
-  ntt_layer<5, 1>(v, omega, ib);
-  ntt_layer<6, 2>(v, omega, ib);
-  ntt_layer<7, 3>(v, omega, ib);
+  ntt_layer<16, 4, 0, 1>(v, omega, ib);
+  ntt_layer<16, 5, 1, 1>(v, omega, ib);
+  ntt_layer<16, 6, 2, 1>(v, omega, ib);
+  ntt_layer<16, 7, 3, 1>(v, omega, ib);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 16; ++r) sm[ib(r)] = (uint16_t)v[r];
@@ -230,78 +261,139 @@ __device__ __forceinline__ void ntt_wave(uint32_t (&v)[16], uint16_t* sm, int l,
     v[4 * q + 2] = p.y & 0xFFFFu;
     v[4 * q + 3] = p.y >> 16;
   }
-  ntt_layer<8, 2>(v, omega, ic);
-  ntt_layer<9, 3>(v, omega, ic);
+  ntt_layer<16, 8, 2, 1>(v, omega, ic);
+  ntt_layer<16, 9, 3, 1>(v, omega, ic);
 }
 
-__device__ __forceinline__ void store_c(const uint32_t (&v)[16], uint16_t* el, int l) {
+// ntt_core at N = 512 (nine layers): returns v[r] = coefficient l + 64 r.
+__device__ __forceinline__ void ntt_wave(uint32_t (&v)[8], uint16_t* sm, int l, const uint16_t* omega) {
+  // layers 0-2: v[r] = index 8 l + r
+  {
+    const uint4 p = *reinterpret_cast<const uint4*>(sm + 8 * l);
+    const uint32_t w[4] = {p.x, p.y, p.z, p.w};
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-    *reinterpret_cast<uint2*>(el + 4 * l + 256 * q) =
-        make_uint2(v[4 * q] | (v[4 * q + 1] << 16), v[4 * q + 2] | (v[4 * q + 3] << 16));
+    for (int k = 0; k < 4; ++k) {
+      v[2 * k] = w[k] & 0xFFFFu;
+      v[2 * k + 1] = w[k] >> 16;
+    }
+  }
+  auto ia = [l](int r) { return 8 * l + r; };
+  ntt_layer<8, 0, 0, 2>(v, omega, ia);
+  ntt_layer<8, 1, 1, 2>(v, omega, ia);
+  ntt_layer<8, 2, 2, 2>(v, omega, ia);
+  {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = v[2 * k] | (v[2 * k + 1] << 16);
+    __syncthreads();
+    *reinterpret_cast<uint4*>(sm + 8 * l) = make_uint4(w[0], w[1], w[2], w[3]);
+    __syncthreads();
+  }
+  // layers 3-5: v[r] = index (l & 7) + 8 r + 64 (l >> 3)
+  auto ib = [l](int r) { return (l & 7) + 8 * r + 64 * (l >> 3); };
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = sm[ib(r)];
+  ntt_layer<8, 3, 0, 2>(v, omega, ib);
+  ntt_layer<8, 4, 1, 2>(v, omega, ib);
+  ntt_layer<8, 5, 2, 2>(v, omega, ib);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) sm[ib(r)] = (uint16_t)v[r];
+  __syncthreads();
+  // layers 6-8: v[r] = index l + 64 r
+  auto ic = [l](int r) { return l + 64 * r; };
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = sm[ic(r)];
+  ntt_layer<8, 6, 0, 2>(v, omega, ic);
+  ntt_layer<8, 7, 1, 2>(v, omega, ic);
+  ntt_layer<8, 8, 2, 2>(v, omega, ic);
+}
+
+// output position of register r of lane l after ntt_wave
+template <int N>
+__device__ __forceinline__ int out_index(int l, int r) {
+  return N == 1024 ? 4 * l + (r & 3) + 256 * (r >> 2) : l + 64 * r;
+}
+
+template <int N>
+__device__ __forceinline__ void store_out(const uint32_t (&v)[N / 64], uint16_t* el, int l) {
+  if constexpr (N == 1024) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      *reinterpret_cast<uint2*>(el + 4 * l + 256 * q) =
+          make_uint2(v[4 * q] | (v[4 * q + 1] << 16), v[4 * q + 2] | (v[4 * q + 3] << 16));
+  } else {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) el[l + 64 * r] = (uint16_t)v[r];
+  }
 }
 
 // poly_ntt (poly.cu:519-527): mul_coefficients by gammas, then ntt with the same
-// table.  Workgroup = one wave = one polynomial.
-__global__ __launch_bounds__(64) void k_ntt(int n, Pv r) {
+// table.  Workgroup = one wave = one polynomial; lane l starts with
+// coefficients (N / 64) l .. (N / 64) l + N / 64 - 1.
+template <int N>
+__global__ __launch_bounds__(64) void k_ntt(int n, Pv<N> r) {
+  constexpr int C = N / 64;
   __shared__ __align__(16) uint16_t sm[N];
   const int x = blockIdx.x, l = threadIdx.x;
   if (x >= n) return;
   uint16_t* el = r.el(x);
-  const uint4* src = reinterpret_cast<const uint4*>(el + 16 * l);
-  const uint4* gam = reinterpret_cast<const uint4*>(c_tab.gammas + 16 * l);
+  const uint4* src = reinterpret_cast<const uint4*>(el + C * l);
+  const uint4* gam = reinterpret_cast<const uint4*>(tab<N>().gammas + C * l);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < C / 8; ++h) {
     const uint4 a = src[h], g = gam[h];
     const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, gw[4] = {g.x, g.y, g.z, g.w};
     uint32_t o[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       o[k] = mont((aw[k] & 0xFFFFu) * (gw[k] & 0xFFFFu)) | (mont((aw[k] >> 16) * (gw[k] >> 16)) << 16);
-    *reinterpret_cast<uint4*>(sm + 16 * l + 8 * h) = make_uint4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<uint4*>(sm + C * l + 8 * h) = make_uint4(o[0], o[1], o[2], o[3]);
   }
   __syncthreads();
-  uint32_t v[16];
-  ntt_wave(v, sm, l, c_tab.gammas);
-  store_c(v, el, l);
+  uint32_t v[C];
+  ntt_wave(v, sm, l, tab<N>().gammas);
+  store_out<N>(v, el, l);
 }
 
 // poly_invntt (poly.cu:538-547): bitrev_vector, ntt with omegas_inv, then
 // mul_coefficients by gammas_inv
-__global__ __launch_bounds__(64) void k_invntt(int n, Pv r) {
+template <int N>
+__global__ __launch_bounds__(64) void k_invntt(int n, Pv<N> r) {
+  constexpr int C = N / 64;
   __shared__ __align__(16) uint16_t sm[N];
   const int x = blockIdx.x, l = threadIdx.x;
   if (x >= n) return;
   uint16_t* el = r.el(x);
-  const uint4* src = reinterpret_cast<const uint4*>(el + 16 * l);
-  uint16_t raw[16];
-  {
-    const uint4 a = src[0], b = src[1];
-    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  const uint4* src = reinterpret_cast<const uint4*>(el + C * l);
+  uint16_t raw[C];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      raw[2 * k] = (uint16_t)w[k];
-      raw[2 * k + 1] = (uint16_t)(w[k] >> 16);
+  for (int h = 0; h < C / 8; ++h) {
+    const uint4 a = src[h];
+    const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      raw[8 * h + 2 * k] = (uint16_t)w[k];
+      raw[8 * h + 2 * k + 1] = (uint16_t)(w[k] >> 16);
     }
   }
   // the coefficient at ntt input position i is the stored one at brv(i): write
   // stored coefficient c to position brv(c)
 #pragma unroll
-  for (int k = 0; k < 16; ++k) sm[c_tab.brv[16 * l + k]] = raw[k];
+  for (int k = 0; k < C; ++k) sm[tab<N>().brv[C * l + k]] = raw[k];
   __syncthreads();
-  uint32_t v[16];
-  ntt_wave(v, sm, l, c_tab.omegas_inv);
+  uint32_t v[C];
+  ntt_wave(v, sm, l, tab<N>().omegas_inv);
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[4 * q + k] = mont(v[4 * q + k] * c_tab.gammas_inv[4 * l + k + 256 * q]);
-  store_c(v, el, l);
+  for (int k = 0; k < C; ++k) v[k] = mont(v[k] * tab<N>().gammas_inv[out_index<N>(l, k)]);
+  store_out<N>(v, el, l);
 }
 
 // ---- flat passes -------------------------------------------------------------------
 
 // poly_mul_pointwise (poly.cu:455-468), 8 coefficients per lane
-__global__ void k_pointwise(int n, Pv r, Pv a, Pv b) {
+template <int N>
+__global__ void k_pointwise(int n, Pv<N> r, Pv<N> a, Pv<N> b) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 8)) return;
   const uint4 av = reinterpret_cast<const uint4*>(a.p)[t], bv = reinterpret_cast<const uint4*>(b.p)[t];
@@ -316,7 +408,8 @@ __global__ void k_pointwise(int n, Pv r, Pv a, Pv b) {
 }
 
 // poly_add / poly_sub (poly.cu:479-508), 8 coefficients per lane
-__global__ void k_addsub(int n, Pv r, Pv a, Pv b, int subtract) {
+template <int N>
+__global__ void k_addsub(int n, Pv<N> r, Pv<N> a, Pv<N> b, int subtract) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 8)) return;
   const uint4 av = reinterpret_cast<const uint4*>(a.p)[t], bv = reinterpret_cast<const uint4*>(b.p)[t];
@@ -356,7 +449,8 @@ __device__ __forceinline__ uint32_t flipabs(uint32_t x) {
 // poly_tobytes (poly.cu:105-127): lane (x, i) packs coefficients 4 i .. 4 i + 3
 // into bytes 7 i .. 7 i + 6; lanes i < 4 also copy seed word i after the
 // polynomial (encode_pk)
-__global__ void k_tobytes(int n, uint8_t* dst, size_t stride, Pv p, const uint8_t* seed, size_t seed_stride) {
+template <int N>
+__global__ void k_tobytes(int n, uint8_t* dst, size_t stride, Pv<N> p, const uint8_t* seed, size_t seed_stride) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 4)) return;
   const int x = (int)(t / (N / 4)), i = (int)(t % (N / 4));
@@ -371,12 +465,13 @@ __global__ void k_tobytes(int n, uint8_t* dst, size_t stride, Pv p, const uint8_
   r[5] = (uint8_t)((t2 >> 12) | (t3 << 2));
   r[6] = (uint8_t)(t3 >> 6);
   if (seed && i < 4)
-    store_u64(dst + (size_t)x * stride + POLYBYTES + 8 * i, load_u64(seed + (size_t)x * seed_stride + 8 * i));
+    store_u64(dst + (size_t)x * stride + Par<N>::POLYBYTES + 8 * i, load_u64(seed + (size_t)x * seed_stride + 8 * i));
 }
 
 // poly_frombytes (poly.cu:62-74): lane (x, i) unpacks bytes 7 i .. 7 i + 6;
 // lanes i < 4 copy seed word i out (decode_pk)
-__global__ void k_frombytes(int n, Pv r, const uint8_t* src, size_t stride, uint8_t* seed_out) {
+template <int N>
+__global__ void k_frombytes(int n, Pv<N> r, const uint8_t* src, size_t stride, uint8_t* seed_out) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 4)) return;
   const int x = (int)(t / (N / 4)), i = (int)(t % (N / 4));
@@ -387,12 +482,13 @@ __global__ void k_frombytes(int n, Pv r, const uint8_t* src, size_t stride, uint
   const uint32_t c3 = (a[5] >> 2) | ((uint32_t)a[6] << 6);
   reinterpret_cast<uint2*>(r.el(x))[i] = make_uint2((c0 & 0xFFFFu) | (c1 << 16), (c2 & 0xFFFFu) | ((c3 & 0xFFFFu) << 16));
   if (seed_out && i < 4)
-    store_u64(seed_out + (size_t)x * 32 + 8 * i, load_u64(src + (size_t)x * stride + POLYBYTES + 8 * i));
+    store_u64(seed_out + (size_t)x * 32 + 8 * i, load_u64(src + (size_t)x * stride + Par<N>::POLYBYTES + 8 * i));
 }
 
 // poly_compress (poly.cu:165-186): lane (x, g) packs coefficients 8 g .. 8 g + 7
 // into ct + POLYBYTES + 3 g
-__global__ void k_compress(int n, uint8_t* ct, size_t ct_stride, Pv p) {
+template <int N>
+__global__ void k_compress(int n, uint8_t* ct, size_t ct_stride, Pv<N> p) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 8)) return;
   const int x = (int)(t / (N / 8)), g = (int)(t % (N / 8));
@@ -401,18 +497,19 @@ __global__ void k_compress(int n, uint8_t* ct, size_t ct_stride, Pv p) {
   uint32_t v[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) v[j] = (((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) / Q) & 7;
-  uint8_t* r = ct + (size_t)x * ct_stride + POLYBYTES + 3 * g;
+  uint8_t* r = ct + (size_t)x * ct_stride + Par<N>::POLYBYTES + 3 * g;
   r[0] = (uint8_t)(v[0] | (v[1] << 3) | (v[2] << 6));
   r[1] = (uint8_t)((v[2] >> 2) | (v[3] << 1) | (v[4] << 4) | (v[5] << 7));
   r[2] = (uint8_t)((v[5] >> 1) | (v[6] << 2) | (v[7] << 5));
 }
 
 // poly_decompress (poly.cu:197-216): lane (x, g) = coefficients 8 g .. 8 g + 7
-__global__ void k_decompress(int n, Pv r, const uint8_t* ct, size_t ct_stride) {
+template <int N>
+__global__ void k_decompress(int n, Pv<N> r, const uint8_t* ct, size_t ct_stride) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * (N / 8)) return;
   const int x = (int)(t / (N / 8)), g = (int)(t % (N / 8));
-  const uint8_t* a = ct + (size_t)x * ct_stride + POLYBYTES + 3 * g;
+  const uint8_t* a = ct + (size_t)x * ct_stride + Par<N>::POLYBYTES + 3 * g;
   const uint32_t a0 = a[0], a1 = a[1], a2 = a[2];
   uint32_t v[8];
   v[0] = a0 & 7;
@@ -431,8 +528,9 @@ __global__ void k_decompress(int n, Pv r, const uint8_t* ct, size_t ct_stride) {
 }
 
 // poly_frommsg_n (poly.cu:226-251): lane (x, i) spreads message byte i over
-// coefficients 8 i + j + 256 q
-__global__ void k_frommsg(int n, Pv r, const uint8_t* msg, size_t msg_stride) {
+// coefficients 8 i + j + 256 q, q < N / 256
+template <int N>
+__global__ void k_frommsg(int n, Pv<N> r, const uint8_t* msg, size_t msg_stride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 32) return;
   const int x = t >> 5, i = t & 31;
@@ -444,17 +542,21 @@ __global__ void k_frommsg(int n, Pv r, const uint8_t* msg, size_t msg_stride) {
     o[k] = v0 | (v1 << 16);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) reinterpret_cast<uint4*>(r.el(x) + 8 * i + 256 * q)[0] = make_uint4(o[0], o[1], o[2], o[3]);
+  for (int q = 0; q < Par<N>::COPIES; ++q)
+    reinterpret_cast<uint4*>(r.el(x) + 8 * i + 256 * q)[0] = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// poly_tomsg_n (poly.cu:261-291): lane (x, i) decodes message byte i
-__global__ void k_tomsg(int n, uint8_t* msg, size_t msg_stride, Pv p) {
+// poly_tomsg_n (poly.cu:261-291): lane (x, i) decodes message byte i from the
+// sum of flipabs over the N / 256 copies, compared with q (N = 1024) or q / 2
+template <int N>
+__global__ void k_tomsg(int n, uint8_t* msg, size_t msg_stride, Pv<N> p) {
+  constexpr int CP = Par<N>::COPIES;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 32) return;
   const int x = t >> 5, i = t & 31;
-  uint32_t c[4][8];
+  uint32_t c[CP][8];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < CP; ++q) {
     const uint4 w = reinterpret_cast<const uint4*>(p.el(x) + 8 * i + 256 * q)[0];
     const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
@@ -463,8 +565,10 @@ __global__ void k_tomsg(int n, uint8_t* msg, size_t msg_stride, Pv p) {
   uint32_t byte = 0;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    uint32_t v = flipabs(c[0][j]) + flipabs(c[1][j]) + flipabs(c[2][j]) + flipabs(c[3][j]);
-    v = (v - Q) & 0xFFFFu;
+    uint32_t v = 0;
+#pragma unroll
+    for (int q = 0; q < CP; ++q) v += flipabs(c[q][j]);
+    v = (v - (N == 1024 ? Q : Q / 2)) & 0xFFFFu;
     byte |= (v >> 15) << j;
   }
   msg[(size_t)x * msg_stride + i] = (uint8_t)byte;
@@ -538,9 +642,10 @@ __global__ void k_cmov32(int n, uint8_t* k, size_t k_stride, const uint8_t* z, s
 
 // ---- orchestration (cpapke.cu:265-476) -------------------------------------------
 
+template <int N>
 struct Scratch {
   pqc_ctx* c;
-  Pv poly(int i) const { return Pv{c->nhpolys + (size_t)i * N * c->ld}; }
+  Pv<N> poly(int i) const { return Pv<N>{c->nhpolys + (size_t)i * N * c->ld}; }
 };
 
 // grid for `threads` lanes at the tuned block size
@@ -551,153 +656,177 @@ struct Launch {
 #define LAUNCH(kern, threads, ...) hipLaunchKernelGGL(kern, L.grid(threads), dim3(L.block), 0, s, __VA_ARGS__)
 #define LAUNCH_NTT(kern, ...) hipLaunchKernelGGL(kern, dim3(n), dim3(64), 0, s, __VA_ARGS__)
 
-}  // namespace newhope
-
-namespace newhope {
-
 // cpapke_keypair (cpapke.cu:265-338) with pk at stride PK and sk at sk_stride
 // (the CCA-KEM's sk holds sk_cpa first), seeds at seed_stride
+template <int N>
 int keypair_impl(pqc_ctx* c, int n, uint8_t* pk, uint8_t* sk, size_t sk_stride, const uint8_t* seed,
                  size_t seed_stride, hipStream_t s) {
   const Launch L{c->tune.newhope_block};
-  const size_t n1 = n, n16 = 16 * (size_t)n, n128 = 128 * (size_t)n, n256 = 256 * (size_t)n;
-  Scratch S{c};
+  const size_t n1 = n, nS = (size_t)Par<N>::STREAMS * n, n8 = (size_t)n * (N / 8), n4 = (size_t)n * (N / 4);
+  Scratch<N> S{c};
   uint8_t* z = c->nhbytes;  // 64 bytes per element: publicseed || noiseseed
   const int AHAT = 0, EHAT = 1, AS = 2, BHAT = 3, SHAT = 4;
   LAUNCH(k_seed_expand, n1, n, z, seed, seed_stride);
-  LAUNCH(k_uniform, n16, n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
-  LAUNCH(k_sample, n16, n, S.poly(SHAT), (const uint8_t*)z + 32, (size_t)64, 0);
-  LAUNCH_NTT(k_ntt, n, S.poly(SHAT));
-  LAUNCH(k_sample, n16, n, S.poly(EHAT), (const uint8_t*)z + 32, (size_t)64, 1);
-  LAUNCH_NTT(k_ntt, n, S.poly(EHAT));
-  LAUNCH(k_pointwise, n128, n, S.poly(AS), S.poly(SHAT), S.poly(AHAT));
-  LAUNCH(k_addsub, n128, n, S.poly(BHAT), S.poly(EHAT), S.poly(AS), 0);
-  LAUNCH(k_tobytes, n256, n, sk, sk_stride, S.poly(SHAT), (const uint8_t*)nullptr, (size_t)0);
-  LAUNCH(k_tobytes, n256, n, pk, (size_t)PK, S.poly(BHAT), (const uint8_t*)z, (size_t)64);
+  LAUNCH(k_uniform<N>, nS, n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
+  LAUNCH(k_sample<N>, nS, n, S.poly(SHAT), (const uint8_t*)z + 32, (size_t)64, 0);
+  LAUNCH_NTT(k_ntt<N>, n, S.poly(SHAT));
+  LAUNCH(k_sample<N>, nS, n, S.poly(EHAT), (const uint8_t*)z + 32, (size_t)64, 1);
+  LAUNCH_NTT(k_ntt<N>, n, S.poly(EHAT));
+  LAUNCH(k_pointwise<N>, n8, n, S.poly(AS), S.poly(SHAT), S.poly(AHAT));
+  LAUNCH(k_addsub<N>, n8, n, S.poly(BHAT), S.poly(EHAT), S.poly(AS), 0);
+  LAUNCH(k_tobytes<N>, n4, n, sk, sk_stride, S.poly(SHAT), (const uint8_t*)nullptr, (size_t)0);
+  LAUNCH(k_tobytes<N>, n4, n, pk, (size_t)Par<N>::PK, S.poly(BHAT), (const uint8_t*)z, (size_t)64);
   return hip_err(hipGetLastError());
 }
 
 // cpapke_enc (cpapke.cu:353-429), every buffer at its own stride
+template <int N>
 int enc_impl(pqc_ctx* c, int n, uint8_t* ct, size_t ct_stride, const uint8_t* m, size_t m_stride, const uint8_t* pk,
              size_t pk_stride, const uint8_t* coin, size_t coin_stride, hipStream_t s) {
   const Launch L{c->tune.newhope_block};
-  const size_t n16 = 16 * (size_t)n, n32 = 32 * (size_t)n, n128 = 128 * (size_t)n, n256 = 256 * (size_t)n;
-  Scratch S{c};
+  const size_t nS = (size_t)Par<N>::STREAMS * n, n32 = 32 * (size_t)n, n8 = (size_t)n * (N / 8), n4 = (size_t)n * (N / 4);
+  Scratch<N> S{c};
   uint8_t* seed = c->nhbytes;  // publicseed copy, 32 bytes per element
   const int SP = 0, EP = 1, VP = 2, AHAT = 3, BHAT = 4, EPP = 5, UHAT = 6, V = 7;
-  LAUNCH(k_frommsg, n32, n, S.poly(V), m, m_stride);
-  LAUNCH(k_frombytes, n256, n, S.poly(BHAT), pk, pk_stride, seed);
-  LAUNCH(k_uniform, n16, n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
-  LAUNCH(k_sample, n16, n, S.poly(SP), coin, coin_stride, 0);
-  LAUNCH(k_sample, n16, n, S.poly(EP), coin, coin_stride, 1);
-  LAUNCH(k_sample, n16, n, S.poly(EPP), coin, coin_stride, 2);
-  LAUNCH_NTT(k_ntt, n, S.poly(SP));
-  LAUNCH_NTT(k_ntt, n, S.poly(EP));
-  LAUNCH(k_pointwise, n128, n, S.poly(UHAT), S.poly(AHAT), S.poly(SP));
-  LAUNCH(k_addsub, n128, n, S.poly(UHAT), S.poly(UHAT), S.poly(EP), 0);
-  LAUNCH(k_pointwise, n128, n, S.poly(VP), S.poly(BHAT), S.poly(SP));
-  LAUNCH_NTT(k_invntt, n, S.poly(VP));
-  LAUNCH(k_addsub, n128, n, S.poly(VP), S.poly(VP), S.poly(EPP), 0);
-  LAUNCH(k_addsub, n128, n, S.poly(VP), S.poly(VP), S.poly(V), 0);
-  LAUNCH(k_tobytes, n256, n, ct, ct_stride, S.poly(UHAT), (const uint8_t*)nullptr, (size_t)0);
-  LAUNCH(k_compress, n128, n, ct, ct_stride, S.poly(VP));
+  LAUNCH(k_frommsg<N>, n32, n, S.poly(V), m, m_stride);
+  LAUNCH(k_frombytes<N>, n4, n, S.poly(BHAT), pk, pk_stride, seed);
+  LAUNCH(k_uniform<N>, nS, n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
+  LAUNCH(k_sample<N>, nS, n, S.poly(SP), coin, coin_stride, 0);
+  LAUNCH(k_sample<N>, nS, n, S.poly(EP), coin, coin_stride, 1);
+  LAUNCH(k_sample<N>, nS, n, S.poly(EPP), coin, coin_stride, 2);
+  LAUNCH_NTT(k_ntt<N>, n, S.poly(SP));
+  LAUNCH_NTT(k_ntt<N>, n, S.poly(EP));
+  LAUNCH(k_pointwise<N>, n8, n, S.poly(UHAT), S.poly(AHAT), S.poly(SP));
+  LAUNCH(k_addsub<N>, n8, n, S.poly(UHAT), S.poly(UHAT), S.poly(EP), 0);
+  LAUNCH(k_pointwise<N>, n8, n, S.poly(VP), S.poly(BHAT), S.poly(SP));
+  LAUNCH_NTT(k_invntt<N>, n, S.poly(VP));
+  LAUNCH(k_addsub<N>, n8, n, S.poly(VP), S.poly(VP), S.poly(EPP), 0);
+  LAUNCH(k_addsub<N>, n8, n, S.poly(VP), S.poly(VP), S.poly(V), 0);
+  LAUNCH(k_tobytes<N>, n4, n, ct, ct_stride, S.poly(UHAT), (const uint8_t*)nullptr, (size_t)0);
+  LAUNCH(k_compress<N>, n8, n, ct, ct_stride, S.poly(VP));
   return hip_err(hipGetLastError());
 }
 
 // cpapke_dec (cpapke.cu:443-476), every buffer at its own stride
+template <int N>
 int dec_impl(pqc_ctx* c, int n, uint8_t* m, size_t m_stride, const uint8_t* ct, size_t ct_stride, const uint8_t* sk,
              size_t sk_stride, hipStream_t s) {
   const Launch L{c->tune.newhope_block};
-  const size_t n32 = 32 * (size_t)n, n128 = 128 * (size_t)n, n256 = 256 * (size_t)n;
-  Scratch S{c};
+  const size_t n32 = 32 * (size_t)n, n8 = (size_t)n * (N / 8), n4 = (size_t)n * (N / 4);
+  Scratch<N> S{c};
   const int VP = 0, UHAT = 1, TMP = 2, SHAT = 3;
-  LAUNCH(k_frombytes, n256, n, S.poly(SHAT), sk, sk_stride, (uint8_t*)nullptr);
-  LAUNCH(k_frombytes, n256, n, S.poly(UHAT), ct, ct_stride, (uint8_t*)nullptr);
-  LAUNCH(k_decompress, n128, n, S.poly(VP), ct, ct_stride);
-  LAUNCH(k_pointwise, n128, n, S.poly(TMP), S.poly(SHAT), S.poly(UHAT));
-  LAUNCH_NTT(k_invntt, n, S.poly(TMP));
-  LAUNCH(k_addsub, n128, n, S.poly(TMP), S.poly(TMP), S.poly(VP), 1);
-  LAUNCH(k_tomsg, n32, n, m, m_stride, S.poly(TMP));
+  LAUNCH(k_frombytes<N>, n4, n, S.poly(SHAT), sk, sk_stride, (uint8_t*)nullptr);
+  LAUNCH(k_frombytes<N>, n4, n, S.poly(UHAT), ct, ct_stride, (uint8_t*)nullptr);
+  LAUNCH(k_decompress<N>, n8, n, S.poly(VP), ct, ct_stride);
+  LAUNCH(k_pointwise<N>, n8, n, S.poly(TMP), S.poly(SHAT), S.poly(UHAT));
+  LAUNCH_NTT(k_invntt<N>, n, S.poly(TMP));
+  LAUNCH(k_addsub<N>, n8, n, S.poly(TMP), S.poly(TMP), S.poly(VP), 1);
+  LAUNCH(k_tomsg<N>, n32, n, m, m_stride, S.poly(TMP));
   return hip_err(hipGetLastError());
 }
 
-// CCA-KEM layouts (NewHope_NIST_GPU/params.h:37-39) and the per-element scratch
-// row: mu or m' | H(pk) | K || coin || d | H(c) | fail | re-encrypted ct
-constexpr int CCA_SK = SK + PK + 2 * SYM;  // 3680: sk_cpa || pk || H(pk) || s
-constexpr int CCA_CT = CT + SYM;           // 2208: ct_cpa || d
+// Per-element CCA scratch row (one size for both N): mu or m' | H(pk) |
+// K || coin || d | H(c) | fail | re-encrypted ct
 constexpr int KM_MU = 0, KM_HPK = 32, KM_KCD = 64, KM_HC = 160, KM_FAIL = 192, KM_CT2 = 224;
-constexpr int KEMB = KM_CT2 + CCA_CT;      // 2432 bytes per element
-static_assert(KEMB == kNewhopeKemBytes, "scratch row size");
+static_assert(KM_CT2 + Par<1024>::CCA_CT == kNewhopeKemBytes, "scratch row size");
 
-}  // namespace newhope
-
-int newhope_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
-  return newhope::keypair_impl(c, (int)n, pk, sk, newhope::SK, seed, 32, s);
-}
-
-int newhope_enc(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
-                hipStream_t s) {
-  using namespace newhope;
-  return enc_impl(c, (int)n, ct, CT, m, 32, pk, PK, coin, 32, s);
-}
-
-int newhope_dec(pqc_ctx* c, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
-  using namespace newhope;
-  return dec_impl(c, (int)n, m, 32, ct, CT, sk, SK, s);
-}
-
-// NewHope-1024 CCA-KEM, the NewHope specification's transform over the CPA-PKE
+// NewHope CCA-KEM, the NewHope specification's transform over the CPA-PKE
 // (SHAKE256 for every hash).  Keypair coins: CPA seed || s, 64 bytes.
-int newhope_cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
-  using namespace newhope;
+template <int N>
+int cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+  using P = Par<N>;
   const Launch L{c->tune.newhope_block};
   const int n = (int)n_;
-  int rc = keypair_impl(c, n, pk, sk, CCA_SK, coins, 64, s);
+  int rc = keypair_impl<N>(c, n, pk, sk, P::CCA_SK, coins, 64, s);
   if (rc) return rc;
-  LAUNCH(k_copy_words, n_ * (PK / 8), n, sk + SK, (size_t)CCA_SK, (const uint8_t*)pk, (size_t)PK, PK / 8);
-  LAUNCH(k_shake256_words<4>, n_, n, sk + SK + PK, (size_t)CCA_SK, (const uint8_t*)pk, (size_t)PK, PK / 8);
-  LAUNCH(k_copy_words, n_ * 4, n, sk + CCA_SK - SYM, (size_t)CCA_SK, coins + SYM, (size_t)64, 4);
+  LAUNCH(k_copy_words, n_ * (P::PK / 8), n, sk + P::SK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK, P::PK / 8);
+  LAUNCH(k_shake256_words<4>, n_, n, sk + P::SK + P::PK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK,
+         P::PK / 8);
+  LAUNCH(k_copy_words, n_ * 4, n, sk + P::CCA_SK - SYM, (size_t)P::CCA_SK, coins + SYM, (size_t)64, 4);
   return hip_err(hipGetLastError());
 }
 
-int newhope_cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
-                    hipStream_t s) {
-  using namespace newhope;
+template <int N>
+int cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+            hipStream_t s) {
+  using P = Par<N>;
   const Launch L{c->tune.newhope_block};
   const int n = (int)n_;
   uint8_t* km = c->nhkem;
-  const size_t kb = KEMB;
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_MU, kb, coins, (size_t)32, 4);       // mu = H(coins)
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HPK, kb, pk, (size_t)PK, PK / 8);    // H(pk)
+  const size_t kb = kNewhopeKemBytes;
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_MU, kb, coins, (size_t)32, 4);              // mu = H(coins)
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HPK, kb, pk, (size_t)P::PK, P::PK / 8);    // H(pk)
   LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
-  int rc = enc_impl(c, n, ct, CCA_CT, km + KM_MU, kb, pk, PK, km + KM_KCD + SYM, kb, s);
+  int rc = enc_impl<N>(c, n, ct, P::CCA_CT, km + KM_MU, kb, pk, P::PK, km + KM_KCD + SYM, kb, s);
   if (rc) return rc;
-  LAUNCH(k_copy_words, n_ * 4, n, ct + CT, (size_t)CCA_CT, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, (const uint8_t*)ct, (size_t)CCA_CT, CCA_CT / 8);      // H(c)
+  LAUNCH(k_copy_words, n_ * 4, n, ct + P::CT, (size_t)P::CCA_CT, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, (const uint8_t*)ct, (size_t)P::CCA_CT, P::CCA_CT / 8);      // H(c)
   LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);
   return hip_err(hipGetLastError());
 }
 
-int newhope_cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
-  using namespace newhope;
+template <int N>
+int cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  using P = Par<N>;
   const Launch L{c->tune.newhope_block};
   const int n = (int)n_;
   uint8_t* km = c->nhkem;
-  const size_t kb = KEMB;
-  int rc = dec_impl(c, n, km + KM_MU, kb, ct, CCA_CT, sk, CCA_SK, s);  // m'
+  const size_t kb = kNewhopeKemBytes;
+  int rc = dec_impl<N>(c, n, km + KM_MU, kb, ct, P::CCA_CT, sk, P::CCA_SK, s);  // m'
   if (rc) return rc;
-  LAUNCH(k_copy_words, n_ * 4, n, km + KM_HPK, kb, sk + SK + PK, (size_t)CCA_SK, 4);
+  LAUNCH(k_copy_words, n_ * 4, n, km + KM_HPK, kb, sk + P::SK + P::PK, (size_t)P::CCA_SK, 4);
   LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
-  rc = enc_impl(c, n, km + KM_CT2, kb, km + KM_MU, kb, sk + SK, CCA_SK, km + KM_KCD + SYM, kb, s);
+  rc = enc_impl<N>(c, n, km + KM_CT2, kb, km + KM_MU, kb, sk + P::SK, P::CCA_SK, km + KM_KCD + SYM, kb, s);
   if (rc) return rc;
-  LAUNCH(k_copy_words, n_ * 4, n, km + KM_CT2 + CT, kb, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d'
+  LAUNCH(k_copy_words, n_ * 4, n, km + KM_CT2 + P::CT, kb, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d'
   hipLaunchKernelGGL(k_verify_words, dim3((unsigned)n), dim3(64), 0, s, n, reinterpret_cast<uint32_t*>(km + KM_FAIL),
-                     kb, ct, (size_t)CCA_CT, (const uint8_t*)km + KM_CT2, kb, CCA_CT / 8);
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, ct, (size_t)CCA_CT, CCA_CT / 8);  // H(c)
-  LAUNCH(k_cmov32, n_ * 4, n, km + KM_KCD, kb, sk + CCA_SK - SYM, (size_t)CCA_SK,
+                     kb, ct, (size_t)P::CCA_CT, (const uint8_t*)km + KM_CT2, kb, P::CCA_CT / 8);
+  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, ct, (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
+  LAUNCH(k_cmov32, n_ * 4, n, km + KM_KCD, kb, sk + P::CCA_SK - SYM, (size_t)P::CCA_SK,
          reinterpret_cast<const uint32_t*>(km + KM_FAIL), kb);
   LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);
   return hip_err(hipGetLastError());
+}
+
+}  // namespace newhope
+
+#define NH_DISPATCH(N, call) ((N) == 1024 ? newhope::call<1024> : newhope::call<512>)
+
+int newhope_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
+  if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
+  const size_t sks = N == 1024 ? newhope::Par<1024>::SK : newhope::Par<512>::SK;
+  return NH_DISPATCH(N, keypair_impl)(c, (int)n, pk, sk, sks, seed, 32, s);
+}
+
+int newhope_enc(pqc_ctx* c, int N, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
+                hipStream_t s) {
+  if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
+  const size_t cts = N == 1024 ? newhope::Par<1024>::CT : newhope::Par<512>::CT;
+  const size_t pks = N == 1024 ? newhope::Par<1024>::PK : newhope::Par<512>::PK;
+  return NH_DISPATCH(N, enc_impl)(c, (int)n, ct, cts, m, 32, pk, pks, coin, 32, s);
+}
+
+int newhope_dec(pqc_ctx* c, int N, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
+  const size_t cts = N == 1024 ? newhope::Par<1024>::CT : newhope::Par<512>::CT;
+  const size_t sks = N == 1024 ? newhope::Par<1024>::SK : newhope::Par<512>::SK;
+  return NH_DISPATCH(N, dec_impl)(c, (int)n, m, 32, ct, cts, sk, sks, s);
+}
+
+int newhope_cca_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+  if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
+  return NH_DISPATCH(N, cca_keypair)(c, n, pk, sk, coins, s);
+}
+
+int newhope_cca_enc(pqc_ctx* c, int N, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+                    hipStream_t s) {
+  if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
+  return NH_DISPATCH(N, cca_enc)(c, n, ct, ss, pk, coins, s);
+}
+
+int newhope_cca_dec(pqc_ctx* c, int N, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
+  return NH_DISPATCH(N, cca_dec)(c, n, ss, ct, sk, s);
 }
 
 int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,
@@ -707,8 +836,8 @@ int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t 
   if (n == 0) return PQC_OK;
   if (n > (size_t)INT32_MAX / 16) return PQC_ERR_BATCH_TOO_LARGE;
   const size_t threads = 16 * n;
-  hipLaunchKernelGGL(k_sample, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, (int)n, Pv{polys}, seeds,
-                     seed_stride, nonce);
+  hipLaunchKernelGGL(k_sample<1024>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, (int)n,
+                     Pv<1024>{polys}, seeds, seed_stride, nonce);
   return hip_err(hipGetLastError());
 }
 
@@ -718,9 +847,9 @@ int newhope_poly_transform(int op, size_t n, uint16_t* polys, hipStream_t s) {
   if (n == 0) return PQC_OK;
   if (n > (size_t)INT32_MAX) return PQC_ERR_BATCH_TOO_LARGE;
   if (op == 0)
-    hipLaunchKernelGGL(k_ntt, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv{polys});
+    hipLaunchKernelGGL(k_ntt<1024>, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv<1024>{polys});
   else
-    hipLaunchKernelGGL(k_invntt, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv{polys});
+    hipLaunchKernelGGL(k_invntt<1024>, dim3((unsigned)n), dim3(64), 0, s, (int)n, Pv<1024>{polys});
   return hip_err(hipGetLastError());
 }
 

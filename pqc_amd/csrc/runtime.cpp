@@ -14,7 +14,7 @@
 namespace {
 
 struct AlgInfo {
-  int kind;  // 1 = Kyber
+  int kind;  // 1 = Kyber (k = K), 2 = NewHope CPA, 3 = Frodo, 4 = NewHope CCA (k = N)
   int k;
   size_t pk, sk, ct, ss;
 };
@@ -26,10 +26,16 @@ bool alg_info(int alg, AlgInfo* a) {
     case PQC_ALG_KYBER1024: *a = {1, 4, 1568, 3168, 1568, 32}; return true;
     case PQC_ALG_NEWHOPE1024_CPA: *a = {2, 0, 1824, 1792, 2176, 32}; return true;
     case PQC_ALG_FRODO976: *a = {3, 0, 15632, 31296, 15744, 24}; return true;
-    case PQC_ALG_NEWHOPE1024: *a = {4, 0, 1824, 3680, 2208, 32}; return true;
+    case PQC_ALG_NEWHOPE1024: *a = {4, 1024, 1824, 3680, 2208, 32}; return true;
+    case PQC_ALG_NEWHOPE512: *a = {4, 512, 928, 1888, 1120, 32}; return true;
+    case PQC_ALG_NEWHOPE512_CPA: *a = {2, 512, 928, 896, 1088, 32}; return true;
     default: return false;
   }
 }
+
+constexpr unsigned kNewhopeCca = PQC_ALG_MASK(PQC_ALG_NEWHOPE1024) | PQC_ALG_MASK(PQC_ALG_NEWHOPE512);
+constexpr unsigned kNewhopeAny =
+    kNewhopeCca | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA) | PQC_ALG_MASK(PQC_ALG_NEWHOPE512_CPA);
 
 int check_device() {
   int n = 0;
@@ -143,7 +149,8 @@ pqc_ctx* default_ctx() {
   if (!g_default[dev]) {
     pqc_ctx* c = nullptr;
     unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024) |
-                    PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024);
+                    PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024) |
+                    PQC_ALG_MASK(PQC_ALG_NEWHOPE512);
     if (pqc_ctx_create(&c, mask, 64, 0) != PQC_OK) return nullptr;
     g_default[dev] = c;
   }
@@ -229,21 +236,21 @@ int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, con
   int rc = check_call(c, alg, n);
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if (a.kind == 4) {  // NewHope-1024 CCA-KEM
+  if (a.kind == 4) {  // NewHope CCA-KEM, N = a.k
     switch (op) {
       case OP_KEYPAIR: {
         const Buf b[3] = {{o0, a.pk, true}, {o1, a.sk, true}, {i0, 64, false}};
         return staged(c, n, flags, s, b,
-                      [&](uint8_t** d) { return pqc::newhope_cca_keypair(c, n, d[0], d[1], d[2], s); });
+                      [&](uint8_t** d) { return pqc::newhope_cca_keypair(c, a.k, n, d[0], d[1], d[2], s); });
       }
       case OP_ENC: {
         const Buf b[4] = {{o0, a.ct, true}, {o1, a.ss, true}, {i0, a.pk, false}, {i1, 32, false}};
         return staged(c, n, flags, s, b,
-                      [&](uint8_t** d) { return pqc::newhope_cca_enc(c, n, d[0], d[1], d[2], d[3], s); });
+                      [&](uint8_t** d) { return pqc::newhope_cca_enc(c, a.k, n, d[0], d[1], d[2], d[3], s); });
       }
       default: {
         const Buf b[3] = {{o0, a.ss, true}, {i0, a.ct, false}, {i1, a.sk, false}};
-        return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_cca_dec(c, n, d[0], d[1], d[2], s); });
+        return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_cca_dec(c, a.k, n, d[0], d[1], d[2], s); });
       }
     }
   }
@@ -324,7 +331,8 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
   if (rc) return rc;
   const unsigned known = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) |
                          PQC_ALG_MASK(PQC_ALG_KYBER1024) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA) |
-                         PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024);
+                         PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024) |
+                         PQC_ALG_MASK(PQC_ALG_NEWHOPE512) | PQC_ALG_MASK(PQC_ALG_NEWHOPE512_CPA);
   if (alg_mask == 0 || (alg_mask & ~known)) return PQC_ERR_UNSUPPORTED;
   pqc_ctx* c = new pqc_ctx();
   if (hipGetDevice(&c->device) != hipSuccess) {
@@ -363,7 +371,7 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     c->kmix_order = (uint32_t*)p;
     c->kmix_counts = (uint32_t*)((uint8_t*)p + (size_t)3 * c->ld * sizeof(uint32_t));
   }
-  if (alg_mask & (PQC_ALG_MASK(PQC_ALG_NEWHOPE1024_CPA) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024))) {
+  if (alg_mask & kNewhopeAny) {
     rc = ctx_alloc(c, &p, (size_t)8 * 1024 * c->ld * sizeof(uint16_t));
     if (rc) goto fail;
     c->nhpolys = (uint16_t*)p;
@@ -371,7 +379,7 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     if (rc) goto fail;
     c->nhbytes = (uint8_t*)p;
   }
-  if (alg_mask & PQC_ALG_MASK(PQC_ALG_NEWHOPE1024)) {
+  if (alg_mask & kNewhopeCca) {
     rc = ctx_alloc(c, &p, (size_t)pqc::kNewhopeKemBytes * c->ld);
     if (rc) goto fail;
     c->nhkem = (uint8_t*)p;
@@ -539,7 +547,7 @@ int pqc_newhope1024_cpa_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{pk, 1824, true}, {sk, 1792, true}, {seeds, 32, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_keypair(c, n, d[0], d[1], d[2], s); });
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_keypair(c, 1024, n, d[0], d[1], d[2], s); });
 }
 
 int pqc_newhope1024_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* msgs, const uint8_t* pk,
@@ -548,7 +556,8 @@ int pqc_newhope1024_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[4] = {{ct, 2176, true}, {msgs, 32, false}, {pk, 1824, false}, {coins, 32, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_enc(c, n, d[0], d[1], d[2], d[3], s); });
+  return staged(c, n, flags, s, b,
+                [&](uint8_t** d) { return pqc::newhope_enc(c, 1024, n, d[0], d[1], d[2], d[3], s); });
 }
 
 int pqc_newhope1024_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uint8_t* ct, const uint8_t* sk,
@@ -557,7 +566,35 @@ int pqc_newhope1024_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uin
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{msgs, 32, true}, {ct, 2176, false}, {sk, 1792, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, n, d[0], d[1], d[2], s); });
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, 1024, n, d[0], d[1], d[2], s); });
+}
+
+int pqc_newhope512_cpa_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seeds,
+                                     unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_NEWHOPE512_CPA, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[3] = {{pk, 928, true}, {sk, 896, true}, {seeds, 32, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_keypair(c, 512, n, d[0], d[1], d[2], s); });
+}
+
+int pqc_newhope512_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* msgs, const uint8_t* pk,
+                                 const uint8_t* coins, unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_NEWHOPE512_CPA, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[4] = {{ct, 1088, true}, {msgs, 32, false}, {pk, 928, false}, {coins, 32, false}};
+  return staged(c, n, flags, s, b,
+                [&](uint8_t** d) { return pqc::newhope_enc(c, 512, n, d[0], d[1], d[2], d[3], s); });
+}
+
+int pqc_newhope512_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uint8_t* ct, const uint8_t* sk,
+                                 unsigned flags, void* stream) {
+  int rc = check_call(c, PQC_ALG_NEWHOPE512_CPA, n);
+  if (rc || n == 0) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Buf b[3] = {{msgs, 32, true}, {ct, 1088, false}, {sk, 896, false}};
+  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, 512, n, d[0], d[1], d[2], s); });
 }
 
 int pqc_frodo976_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* randomness,
@@ -707,6 +744,7 @@ PQC_NIST_SET(kyber768, PQC_ALG_KYBER768)
 PQC_NIST_SET(kyber1024, PQC_ALG_KYBER1024)
 PQC_NIST_SET(frodo976, PQC_ALG_FRODO976)
 PQC_NIST_SET(newhope1024, PQC_ALG_NEWHOPE1024)
+PQC_NIST_SET(newhope512, PQC_ALG_NEWHOPE512)
 
 void pqc_randombytes_seed(uint64_t seed) {
   pqc::nist_drbg_disable();

@@ -20,7 +20,8 @@ from . import _lib as L
 from .kem import Context
 
 RSP_NAMES = {"kyber512": "Kyber512", "kyber768": "Kyber768", "kyber1024": "Kyber1024",
-             "frodo976": "FrodoKEM-976-SHAKE", "newhope1024": "NewHope1024-CCAKEM"}
+             "frodo976": "FrodoKEM-976-SHAKE", "newhope1024": "NewHope1024-CCAKEM",
+             "newhope512": "NewHope512-CCAKEM"}
 FIELDS = ("seed", "pk", "sk", "ct", "ss")
 
 

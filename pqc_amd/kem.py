@@ -20,6 +20,8 @@ ALGS = {
     "newhope1024cpa": L.ALG_NEWHOPE1024_CPA,
     "frodo976": L.ALG_FRODO976,
     "newhope1024": L.ALG_NEWHOPE1024,
+    "newhope512": L.ALG_NEWHOPE512,
+    "newhope512cpa": L.ALG_NEWHOPE512_CPA,
 }
 KYBER_K = {L.ALG_KYBER512: 2, L.ALG_KYBER768: 3, L.ALG_KYBER1024: 4}
 
@@ -280,32 +282,37 @@ class Context:
 
 
     # ---- NewHope-1024 CPA-PKE (cpapke_keypair/enc/dec with a batch count) -------
-    def newhope_keypair(self, n, pk, sk, seeds, stream=None, host=False):
+    # ---- NewHope CPA-PKE, N = 1024 (default) or 512 -------------------------------
+    def newhope_keypair(self, n, pk, sk, seeds, stream=None, host=False, N=1024):
         flags = L.F_HOST_PTRS if host else 0
-        check(L.lib().pqc_newhope1024_cpa_keypair_batch(self.handle, n, _ptr(pk), _ptr(sk), _ptr(seeds), flags,
-                                                        stream.handle if stream else None), "newhope keypair")
+        check(getattr(L.lib(), "pqc_newhope%d_cpa_keypair_batch" % N)(
+            self.handle, n, _ptr(pk), _ptr(sk), _ptr(seeds), flags, stream.handle if stream else None),
+            "newhope keypair")
 
-    def newhope_enc(self, n, ct, msgs, pk, coins, stream=None, host=False):
+    def newhope_enc(self, n, ct, msgs, pk, coins, stream=None, host=False, N=1024):
         flags = L.F_HOST_PTRS if host else 0
-        check(L.lib().pqc_newhope1024_cpa_enc_batch(self.handle, n, _ptr(ct), _ptr(msgs), _ptr(pk), _ptr(coins),
-                                                    flags, stream.handle if stream else None), "newhope enc")
+        check(getattr(L.lib(), "pqc_newhope%d_cpa_enc_batch" % N)(
+            self.handle, n, _ptr(ct), _ptr(msgs), _ptr(pk), _ptr(coins), flags, stream.handle if stream else None),
+            "newhope enc")
 
-    def newhope_dec(self, n, msgs, ct, sk, stream=None, host=False):
+    def newhope_dec(self, n, msgs, ct, sk, stream=None, host=False, N=1024):
         flags = L.F_HOST_PTRS if host else 0
-        check(L.lib().pqc_newhope1024_cpa_dec_batch(self.handle, n, _ptr(msgs), _ptr(ct), _ptr(sk), flags,
-                                                    stream.handle if stream else None), "newhope dec")
+        check(getattr(L.lib(), "pqc_newhope%d_cpa_dec_batch" % N)(
+            self.handle, n, _ptr(msgs), _ptr(ct), _ptr(sk), flags, stream.handle if stream else None),
+            "newhope dec")
 
-    def newhope_triples_host(self, seeds, msgs, coins):
+    def newhope_triples_host(self, seeds, msgs, coins, N=1024):
         """keypair + enc + dec over host arrays; returns (pk, sk, ct, m')."""
         n = seeds.shape[0]
-        pk = np.empty((n, 1824), np.uint8)
-        sk = np.empty((n, 1792), np.uint8)
-        ct = np.empty((n, 2176), np.uint8)
+        poly = 14 * N // 8
+        pk = np.empty((n, poly + 32), np.uint8)
+        sk = np.empty((n, poly), np.uint8)
+        ct = np.empty((n, poly + 3 * N // 8), np.uint8)
         m2 = np.empty((n, 32), np.uint8)
         seeds, msgs, coins = (np.ascontiguousarray(x, dtype=np.uint8) for x in (seeds, msgs, coins))
-        self.newhope_keypair(n, pk, sk, seeds, host=True)
-        self.newhope_enc(n, ct, msgs, pk, coins, host=True)
-        self.newhope_dec(n, m2, ct, sk, host=True)
+        self.newhope_keypair(n, pk, sk, seeds, host=True, N=N)
+        self.newhope_enc(n, ct, msgs, pk, coins, host=True, N=N)
+        self.newhope_dec(n, m2, ct, sk, host=True, N=N)
         return pk, sk, ct, m2
 
     # ---- mixed Kyber-512/768/1024 batches (device arenas + descriptors) ----------

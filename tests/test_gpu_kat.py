@@ -21,14 +21,14 @@ def rsp(alg):
     return os.path.join(HERE, "golden", "kat_%s.rsp" % alg)
 
 
-@pytest.mark.parametrize("alg", ["kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024"])
+@pytest.mark.parametrize("alg", ["kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024", "newhope512"])
 def test_gpu_reproduces_kat_file(alg):
     from pqc_amd import kat
     say("kat batch", alg)
     assert kat.check_rsp(rsp(alg), alg) == []
 
 
-@pytest.mark.parametrize("alg", ["kyber768", "kyber1024", "frodo976", "newhope1024"])
+@pytest.mark.parametrize("alg", ["kyber768", "kyber1024", "frodo976", "newhope1024", "newhope512"])
 def test_nist_harness_sequence(alg):
     """randombytes_init(seed) then the single-instance drop-ins give the file's entry."""
     import pqc_amd

@@ -1,4 +1,5 @@
-"""GPU parity of the NewHope-1024 CPA batched kernels against the CPU oracle."""
+"""GPU parity of the NewHope CPA batched kernels (N = 1024 and 512) and the
+NewHope CCA-KEM against the CPU oracle, bit-exact."""
 import sys
 
 import numpy as np
@@ -171,3 +172,75 @@ def test_cca_full_size_16384_roundtrip_and_sample(cctx):
     for i in rng.choice(n, 12, replace=False):
         assert (h_pk[i].tobytes(), h_sk[i].tobytes()) == O.newhope_cca_keypair(kc[i].tobytes())
         assert a[i].tobytes() == O.newhope_cca_dec(h_ct[i].tobytes(), h_sk[i].tobytes())
+
+
+# ---- NewHope-512: CPA kernels and CCA-KEM -------------------------------------------
+@pytest.fixture(scope="module")
+def ctx512():
+    import pqc_amd
+    c = pqc_amd.Context(["newhope512cpa", "newhope512"], max_batch=16384)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n", [1, 63, 130])
+def test_newhope512_triples_vs_oracle(ctx512, n):
+    say("newhope512 triples n", n)
+    seed = O.drbg_array(n, 32, 0, 1700 + n)
+    msg = O.drbg_array(n, 32, 0, 1701 + n)
+    coin = O.drbg_array(n, 32, 0, 1702 + n)
+    pk, sk, ct, m2 = ctx512.newhope_triples_host(seed, msg, coin, N=512)
+    opk, osk, oct_, om2 = O.newhope_triples(seed, msg, coin, N=512)
+    assert (pk == opk).all()
+    assert (sk == osk).all()
+    assert (ct == oct_).all()
+    assert (m2 == om2).all() and (m2 == msg).all()
+
+
+def test_newhope512_decrypt_garbage_vs_oracle(ctx512):
+    say("newhope512 garbage ct")
+    n = 64
+    seed = O.drbg_array(n, 32, 0, 1800)
+    pk, sk, _, _ = O.newhope_triples(seed, O.drbg_array(n, 32, 0, 1801), O.drbg_array(n, 32, 0, 1802), N=512)
+    ct = np.random.default_rng(4).integers(0, 256, (n, 1088), dtype=np.uint8)
+    m = np.empty((n, 32), np.uint8)
+    ctx512.newhope_dec(n, m, ct, sk, host=True, N=512)
+    for i in range(n):
+        assert m[i].tobytes() == O.newhope_dec(ct[i].tobytes(), sk[i].tobytes(), 512)
+
+
+@pytest.mark.parametrize("n", [1, 65, 200])
+def test_newhope512_cca_vs_oracle(ctx512, n):
+    say("newhope512 cca n", n)
+    kc = O.drbg_array(n, 64, 0, 1900 + n)
+    ec = O.drbg_array(n, 32, 0, 1901 + n)
+    pk, sk = ctx512.keypair_host("newhope512", kc)
+    ct, ss = ctx512.enc_host("newhope512", pk, ec)
+    bad = ct.copy()
+    bad[::3, 7] ^= 0x20
+    got = ctx512.dec_host("newhope512", bad, sk)
+    for i in range(n):
+        okp, osk = O.newhope_cca_keypair(kc[i].tobytes(), 512)
+        assert (pk[i].tobytes(), sk[i].tobytes()) == (okp, osk)
+        assert (ct[i].tobytes(), ss[i].tobytes()) == O.newhope_cca_enc(okp, ec[i].tobytes(), 512)
+        assert got[i].tobytes() == O.newhope_cca_dec(bad[i].tobytes(), osk, 512)
+    assert (got[1::3] == ss[1::3]).all() and (got[::3] != ss[::3]).any(axis=1).all()
+
+
+def test_newhope512_full_size_16384(ctx512):
+    import pqc_amd
+    n = 16384
+    say("newhope512 full size", n)
+    rng = np.random.default_rng(13)
+    seed, msg, coin = (rng.integers(0, 256, (n, 32), dtype=np.uint8) for _ in range(3))
+    d_seed, d_msg, d_coin = (pqc_amd.DeviceBuffer.from_array(x) for x in (seed, msg, coin))
+    pk, sk, ct, m2 = (pqc_amd.DeviceBuffer(n * b) for b in (928, 896, 1088, 32))
+    ctx512.newhope_keypair(n, pk, sk, d_seed, N=512)
+    ctx512.newhope_enc(n, ct, d_msg, pk, d_coin, N=512)
+    ctx512.newhope_dec(n, m2, ct, sk, N=512)
+    assert (m2.download((n, 32)) == msg).all()
+    h_pk, h_ct = pk.download((n, 928)), ct.download((n, 1088))
+    for i in rng.choice(n, 12, replace=False):
+        opk, _ = O.newhope_keypair(seed[i].tobytes(), 512)
+        assert h_pk[i].tobytes() == opk
+        assert h_ct[i].tobytes() == O.newhope_enc(msg[i].tobytes(), opk, coin[i].tobytes(), 512)

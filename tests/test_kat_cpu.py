@@ -175,7 +175,7 @@ def test_rsp_write_read_roundtrip(tmp_path):
     assert p.read_text().startswith("# Kyber512\n\ncount = 0\nseed = ")
 
 
-@pytest.mark.parametrize("alg", ["kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024"])
+@pytest.mark.parametrize("alg", ["kyber512", "kyber768", "kyber1024", "frodo976", "newhope1024", "newhope512"])
 def test_kat_fixture_matches_oracle(alg):
     """The committed .rsp: NIST seeds, sizes, and count 0 recomputed by the oracle
     from the DRBG's coins."""
@@ -193,10 +193,11 @@ def test_kat_fixture_matches_oracle(alg):
         pk, sk = O.frodo_keypair(kc)
         ct, ss = O.frodo_enc(pk, ec)
         assert O.frodo_dec(ct, sk) == ss
-    elif alg == "newhope1024":
-        pk, sk = O.newhope_cca_keypair(kc)
-        ct, ss = O.newhope_cca_enc(pk, ec)
-        assert O.newhope_cca_dec(ct, sk) == ss
+    elif alg.startswith("newhope"):
+        N = int(alg[7:])
+        pk, sk = O.newhope_cca_keypair(kc, N)
+        ct, ss = O.newhope_cca_enc(pk, ec, N)
+        assert O.newhope_cca_dec(ct, sk, N) == ss
     else:
         k = {"kyber512": 2, "kyber768": 3, "kyber1024": 4}[alg]
         pk, sk = O.kyber_keypair(k, kc)
