@@ -118,6 +118,28 @@ def host_buffer_rates(pqc, ctx, n, ct, sk, ss_want, reps=3):
     return res
 
 
+def small_batch_latency(pqc, ctx, stream, ct, sk, ss, ss_want, reps=50):
+    """Kernel time of decapsulation batches of 1, 64 and 256 requests (HIP events,
+    p50 / p99 over `reps` launches): the one-workgroup-per-request kernel that
+    such batches take by default, and the 64-request tile kernel forced with
+    lowlat_max = -1.  The outputs are checked equal to the full batch's."""
+    from pqc_amd import tuner as T
+    out = {}
+    for m in (1, 64, 256):
+        row = {}
+        for name, lim in (("lowlat", 0), ("tiles", -1)):
+            T.put_tuning(ctx, lowlat_max=lim)
+            _, per = timed(lambda: ctx.dec(ALG, m, ss, ct, sk, stream=stream), stream, reps, 3, pqc)
+            got = ss.download((m, 32))
+            assert (got == ss_want[:m]).all(), "small-batch decaps differ from the full batch"
+            row[name] = {"p50": round(float(np.percentile(per, 50)) * 1e3, 1),
+                         "p99": round(float(np.percentile(per, 99)) * 1e3, 1)}
+        out["n%d" % m] = row
+        log("decaps n=%d: lowlat p50 %.1f us, tiles p50 %.1f us" % (m, row["lowlat"]["p50"], row["tiles"]["p50"]))
+    T.put_tuning(ctx, lowlat_max=0)
+    return out
+
+
 def cpu_baseline(ct_h, sk_h, ss_gpu, threads):
     from tests import oracle_lib as O
     n1 = min(4096, len(ct_h))
@@ -223,6 +245,7 @@ def main():
         log("%s: fused %.3f ms, plain %.3f ms" % (name, wf * 1e3 / tsteps, wp * 1e3 / tsteps))
 
     host_io = host_buffer_rates(pqc, ctx, n, ct, sk, a)
+    small = small_batch_latency(pqc, ctx, stream, ct, sk, ss_f, a)
 
     ms_f = wall_f * 1e3 / args.steps
     ms_p = wall_p * 1e3 / max(3, args.steps // 4)
@@ -258,6 +281,7 @@ def main():
             "keypair": tri["keypair"],
             "encaps": tri["encaps"],
             "host_buffers": host_io,
+            "small_batch_latency_us": small,
         },
         "roofline": {
             "bound": "valu",

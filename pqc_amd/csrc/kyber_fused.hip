@@ -413,7 +413,10 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
   };
   uint64_t rho_next[4];
   load_rho(rho_next, 0);
-  for (int g = 0; g < TL::NR; ++g) {
+  // a partial tile needs only the rounds that hold its nt * K^2 entries (a lone
+  // request: one round instead of K^2, each at least 4 Keccak-f)
+  const int ng = min(TL::NR, (nt * TL::KK + 63) / 64);
+  for (int g = 0; g < ng; ++g) {
     // requests all of whose entries are sampled by the end of this round
     const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
     if (t_done < t_end) {  // the round's first request loads during the sampling
@@ -433,7 +436,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
     }
     __syncthreads();
     // the next round's rho loads during this round's re-encryptions
-    if (g + 1 < TL::NR) load_rho(rho_next, g + 1);
+    if (g + 1 < ng) load_rho(rho_next, g + 1);
     for (int t = t_done; t < t_end; ++t) {
       if (t + 1 < t_end) {
         const ReqP q = lane_req(me, t + 1);
@@ -599,7 +602,8 @@ __device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, 
   const int nt = min(TL::T, n - base);
   int t_done = 0;
   KeyIn<K> cur, nxt;
-  for (int g = 0; g < TL::NR; ++g) {
+  const int ng = min(TL::NR, (nt * TL::KK + 63) / 64);  // rounds that hold the tile's entries
+  for (int g = 0; g < ng; ++g) {
     const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
     if (t_done < t_end) key_load<K>(cur, src.prf + (size_t)(base + t_done) * TL::PRFB);
     {
@@ -652,6 +656,146 @@ __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, DenseSrc src, u
   LaneZ z;
   load_lane_z(z, &c_lane_tw);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) fused_tile<K, DEC>(src, tile * 64, n, sh, z, status);
+}
+
+// ---- low-latency decapsulation: one workgroup of four waves per request ---------------
+// The tile kernel's latency is one lane's serial sponge chain (H(ct), G, the
+// 2K + 1 PRF outputs, the A^T entries, KDF) plus the wave's cooperative work,
+// about 22 Keccak-f for a lone Kyber-768 request.  For small batches (the
+// single-instance drop-in, a serving loop at low load) this kernel spreads one
+// request over four waves and many lanes:
+//   phase 1  wave 0: indcpa_dec (wave-cooperative), then G and the 2K + 1 PRF
+//                    outputs on 2K + 1 lanes at once;
+//            wave 1: the K^2 A^T entries on K^2 lanes at once;
+//            wave 2: the first blocks of H(ct) on a lane pair (keccakf1600_pair,
+//                    about 2/3 of the single-lane latency per block);
+//   phase 2  wave 0: re-encryption and compare; wave 2: the rest of H(ct), its
+//                    sponge state kept in registers across the barrier;
+//   phase 3  one lane: KDF.
+// Results are byte-identical to the tile kernel's.
+
+template <int K>
+struct alignas(16) LowLds {
+  uint8_t prf[(2 * K + 1) * 128];              // PRF(coins, nonce) at 128 nonce
+  alignas(16) uint8_t msg[32];
+  uint32_t kbar[8], hct[8];                    // 32-byte values as words
+  uint32_t fail;
+  alignas(16) int16_t a[K * K * kAStride];     // A^T entry i K + j in slot i K + j
+};
+
+// Blocks [b0, b1) of a rate-17-word absorb (SHA3-256 / SHAKE256) of nwords
+// 64-bit words on a lane pair: the even lane holds the low 32-bit halves of
+// the state words, the odd lane the high halves.  b0 = 0 starts from zero.
+__device__ __forceinline__ void absorb_range_pair(uint32_t (&S)[25], const uint8_t* __restrict__ msg, int nwords,
+                                                  int b0, int b1, uint32_t domain, uint32_t half) {
+  constexpr int R = kRateSha3_256;
+  if (b0 == 0) {
+#pragma unroll
+    for (int w = 0; w < 25; ++w) S[w] = 0;
+  }
+  const int nblocks = nwords / R + 1;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(msg) + half;
+  for (int b = b0; b < b1 && b < nblocks; ++b) {
+    const int rem = nwords - b * R;  // words left, >= 0
+#pragma unroll
+    for (int w = 0; w < R; ++w) {
+      if (w < rem) S[w] ^= p[2 * (b * R + w)];
+      if (w == rem && !half) S[w] ^= domain;
+    }
+    if (b == nblocks - 1 && half) S[R - 1] ^= 0x80000000u;
+    keccakf1600_pair(S, half);
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_lowlat_dec(int n, uint8_t* __restrict__ ss, const uint8_t* __restrict__ ct,
+                                                    const uint8_t* __restrict__ sk, uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  constexpr int KK = K * K;
+  constexpr int HW = P::ct_bytes / 8;                // H(ct) message words
+  constexpr int HB = HW / kRateSha3_256 + 1;         // H(ct) blocks, padding included
+  constexpr int HB1 = (2 * HB + 2) / 3;              // of them absorbed during phase 1
+  __shared__ LowLds<K> sh;
+  const int x = blockIdx.x;
+  if (x >= n) return;  // whole workgroup
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63;
+  const uint8_t* c = ct + (size_t)x * P::ct_bytes;
+  const uint8_t* key = sk + (size_t)x * P::sk_bytes;
+  const uint8_t* pk = key + P::sk_cpa_bytes;
+  uint32_t S[25];  // wave 2's H(ct) state, held across the first barrier
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  // ---- phase 1
+  if (wave == 0) {
+    {
+      DecIn<K> in;
+      dec_load<K>(in, c, key);
+      coop_indcpa_dec<K>(in, z, sh.msg);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (l < 2 * K + 1) {
+      // every PRF lane recomputes G (same inputs) rather than waiting for a broadcast
+      uint64_t m[8], A[25], rc4[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        m[w] = load_u64(sh.msg + 8 * w);
+        m[4 + w] = load_u64(key + P::hpk_off + 8 * w);
+      }
+      absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) rc4[w] = A[4 + w];
+      if (l == 0) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          sh.kbar[2 * w] = (uint32_t)A[w];
+          sh.kbar[2 * w + 1] = (uint32_t)(A[w] >> 32);
+        }
+      }
+      absorb_fixed<kRateShake256, 4>(A, rc4, (uint64_t)l | (0x1Full << 8));
+#pragma unroll
+      for (int w = 0; w < 16; ++w) store_u64(sh.prf + 128 * l + 8 * w, A[w]);
+    }
+  } else if (wave == 1 && l < KK) {
+    // entry ij = i K + j on lane ij: A^T[i][j] = SHAKE128(rho || i || j)
+    uint64_t rho[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rho[w] = load_u64(pk + P::polyvec_bytes + 8 * w);
+    const int got = sample_entry(sh.a + l * kAStride, rho, l / K, l % K, true);
+    if (got < N) atomicOr(status, 1u);
+  } else if (wave == 2 && l < 2) {
+    absorb_range_pair(S, c, HW, 0, HB1, 0x06, (uint32_t)l);
+  }
+  __syncthreads();
+  // ---- phase 2
+  if (wave == 0) {
+    EncIn<K, true> in;
+    enc_load<K, true>(in, sh.prf, pk, c);
+    const uint32_t mm = coop_reencrypt<K, true>(sh.a, 0, in, sh.msg, nullptr, z);
+    const uint32_t f = __any(mm != 0) ? 1u : 0u;
+    if (l == 0) sh.fail = f;
+  } else if (wave == 2 && l < 2) {
+    absorb_range_pair(S, c, HW, HB1, HB, 0x06, (uint32_t)l);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sh.hct[2 * w + l] = S[w];
+  }
+  __syncthreads();
+  // ---- phase 3: KDF(Kbar or z, H(ct))
+  if (wave == 0 && l == 0) {
+    uint64_t m[8], A[25];
+    const uint64_t mask = 0 - (uint64_t)(sh.fail & 1u);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint64_t kb = (uint64_t)sh.kbar[2 * w] | ((uint64_t)sh.kbar[2 * w + 1] << 32);
+      const uint64_t zz = load_u64(key + P::z_off + 8 * w);
+      m[w] = kb ^ (mask & (zz ^ kb));
+      m[4 + w] = (uint64_t)sh.hct[2 * w] | ((uint64_t)sh.hct[2 * w + 1] << 32);
+    }
+    absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) store_u64(ss + (size_t)x * 32 + 8 * w, A[w]);
+  }
 }
 
 // ---- mixed parameter sets ------------------------------------------------------
@@ -848,6 +992,16 @@ static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
   return hip_err(hipGetLastError());
 }
 
+// decapsulation batches of at most this many requests take k_lowlat_dec
+// (tuning.lowlat_max: 0 = this default, -1 = never)
+constexpr int kLowLatDefault = 256;
+
+template <int K>
+static int launch_lowlat_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  hipLaunchKernelGGL((k_lowlat_dec<K>), dim3((unsigned)n), dim3(256), 0, s, (int)n, ss, ct, sk, c->status);
+  return hip_err(hipGetLastError());
+}
+
 template <int K>
 static int launch_fused_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
   if (n == 0) return PQC_OK;
@@ -873,6 +1027,15 @@ int kyber_fused_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, c
 }
 
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  const int lowlat = c->tune.lowlat_max == 0 ? kyber::fused::kLowLatDefault : c->tune.lowlat_max;
+  if (n > 0 && (long long)n <= (long long)lowlat) {
+    switch (k) {
+      case 2: return kyber::fused::launch_lowlat_dec<2>(c, n, ss, ct, sk, s);
+      case 3: return kyber::fused::launch_lowlat_dec<3>(c, n, ss, ct, sk, s);
+      case 4: return kyber::fused::launch_lowlat_dec<4>(c, n, ss, ct, sk, s);
+      default: return PQC_ERR_UNSUPPORTED;
+    }
+  }
   uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation
   switch (k) {
     case 2: return kyber::fused::launch_fused<2, true>(c, n, ss, ctm, sk, nullptr, s);

@@ -254,3 +254,31 @@ def test_host_staging_pageable_and_pinned(pqc):
     for b in bufs:
         b.free()
     ctx.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("n", [1, 7, 200])
+def test_lowlat_decaps_equals_tile_and_oracle(pqc, k, n):
+    """Small decapsulation batches run one four-wave workgroup per request
+    (k_lowlat_dec); the bytes equal the 64-request tile kernel's (forced with
+    lowlat_max = -1) and the oracle's, tampered ciphertexts included."""
+    from pqc_amd import tuner as T
+    say("lowlat decaps k", k, "n", n)
+    alg = ALG[k]
+    ctx = pqc.Context([alg], max_batch=max(n, 64))
+    try:
+        kc = O.drbg_array(n, 64, 0, 3000 + 10 * k + n)
+        ec = O.drbg_array(n, 32, 0, 3001 + 10 * k + n)
+        pk, sk = ctx.keypair_host(alg, kc)
+        ct, ss = ctx.enc_host(alg, pk, ec)
+        ct[::2, n % ct.shape[1]] ^= 0x40  # every other ciphertext rejected
+        T.put_tuning(ctx, lowlat_max=0)   # default: batches up to 256 take the low-latency kernel
+        low = ctx.dec_host(alg, ct, sk)
+        T.put_tuning(ctx, lowlat_max=-1)  # tiles only
+        tile = ctx.dec_host(alg, ct, sk)
+        assert (low == tile).all()
+        assert (low[1::2] == ss[1::2]).all() and (low[::2] != ss[::2]).any(axis=1).all()
+        want = O.kyber_batch("dec", k, [np.ascontiguousarray(ct), np.ascontiguousarray(sk)])
+        assert (low == want).all()
+    finally:
+        ctx.close()
