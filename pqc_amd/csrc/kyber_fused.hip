@@ -658,12 +658,12 @@ __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, DenseSrc src, u
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) fused_tile<K, DEC>(src, tile * 64, n, sh, z, status);
 }
 
-// ---- low-latency decapsulation: one workgroup of four waves per request ---------------
-// The tile kernel's latency is one lane's serial sponge chain (H(ct), G, the
-// 2K + 1 PRF outputs, the A^T entries, KDF) plus the wave's cooperative work,
-// about 22 Keccak-f for a lone Kyber-768 request.  For small batches (the
-// single-instance drop-in, a serving loop at low load) this kernel spreads one
-// request over four waves and many lanes:
+// ---- low latency: one workgroup of four waves per request -------------------------
+// A tile's latency is one lane's serial sponge chain (H(ct), G, the 2K + 1 PRF
+// outputs, the A^T entries, KDF) plus the wave's cooperative work for all its
+// requests in turn, about 1 ms for a full tile whatever the batch size.  For
+// small batches (the single-instance drop-ins, a serving loop's batches) these
+// kernels spread one request over four waves and many lanes.  Decapsulation:
 //   phase 1  wave 0: indcpa_dec (wave-cooperative), then G and the 2K + 1 PRF
 //                    outputs on 2K + 1 lanes at once;
 //            wave 1: the K^2 A^T entries on K^2 lanes at once;
@@ -672,15 +672,17 @@ __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, DenseSrc src, u
 //   phase 2  wave 0: re-encryption and compare; wave 2: the rest of H(ct), its
 //                    sponge state kept in registers across the barrier;
 //   phase 3  one lane: KDF.
-// Results are byte-identical to the tile kernel's.
+// Encapsulation: see lowlat_enc_body.  Results are byte-identical to the tile
+// kernel's.
 
 template <int K>
 struct alignas(16) LowLds {
-  uint8_t prf[(2 * K + 1) * 128];              // PRF(coins, nonce) at 128 nonce
+  uint8_t prf[(2 * K + 1) * 128];                 // PRF(coins, nonce) at 128 nonce
   alignas(16) uint8_t msg[32];
-  uint32_t kbar[8], hct[8];                    // 32-byte values as words
+  uint32_t kbar[8], hct[8], hpk[8];               // 32-byte values as words
   uint32_t fail;
-  alignas(16) int16_t a[K * K * kAStride];     // A^T entry i K + j in slot i K + j
+  alignas(16) uint8_t ct[Params<K>::ct_bytes];    // encapsulation: the ciphertext before it is stored
+  alignas(16) int16_t a[K * K * kAStride];        // A^T entry i K + j in slot i K + j
 };
 
 // Blocks [b0, b1) of a rate-17-word absorb (SHA3-256 / SHAKE256) of nwords
@@ -707,25 +709,76 @@ __device__ __forceinline__ void absorb_range_pair(uint32_t (&S)[25], const uint8
   }
 }
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// K^2 A^T entries of one request on lanes 0 .. K^2 - 1 of the calling wave
 template <int K>
-__global__ __launch_bounds__(256) void k_lowlat_dec(int n, uint8_t* __restrict__ ss, const uint8_t* __restrict__ ct,
-                                                    const uint8_t* __restrict__ sk, uint32_t* __restrict__ status) {
+__device__ __forceinline__ void lowlat_sample(LowLds<K>& sh, const uint8_t* pk, int l, uint32_t* status) {
   using P = Params<K>;
-  constexpr int KK = K * K;
-  constexpr int HW = P::ct_bytes / 8;                // H(ct) message words
-  constexpr int HB = HW / kRateSha3_256 + 1;         // H(ct) blocks, padding included
-  constexpr int HB1 = (2 * HB + 2) / 3;              // of them absorbed during phase 1
-  __shared__ LowLds<K> sh;
-  const int x = blockIdx.x;
-  if (x >= n) return;  // whole workgroup
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int l = threadIdx.x & 63;
-  const uint8_t* c = ct + (size_t)x * P::ct_bytes;
-  const uint8_t* key = sk + (size_t)x * P::sk_bytes;
+  if (l < K * K) {
+    // entry ij = i K + j on lane ij: A^T[i][j] = SHAKE128(rho || i || j)
+    uint64_t rho[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rho[w] = load_u64(pk + P::polyvec_bytes + 8 * w);
+    const int got = sample_entry(sh.a + l * kAStride, rho, l / K, l % K, true);
+    if (got < N) atomicOr(status, 1u);
+  }
+}
+
+// (Kbar, coins) = G(msg || h) on lanes 0 .. 2K; lane j then writes PRF(coins, j)
+// to sh.prf and lane 0 Kbar to sh.kbar.  Every lane recomputes G (same inputs)
+// instead of waiting for a broadcast.
+template <int K>
+__device__ __forceinline__ void lowlat_g_prf(LowLds<K>& sh, const uint64_t (&h)[4], int l) {
+  if (l < 2 * K + 1) {
+    uint64_t m[8], A[25], rc4[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      m[w] = load_u64(sh.msg + 8 * w);
+      m[4 + w] = h[w];
+    }
+    absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rc4[w] = A[4 + w];
+    if (l == 0) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        sh.kbar[2 * w] = (uint32_t)A[w];
+        sh.kbar[2 * w + 1] = (uint32_t)(A[w] >> 32);
+      }
+    }
+    absorb_fixed<kRateShake256, 4>(A, rc4, (uint64_t)l | (0x1Full << 8));
+#pragma unroll
+    for (int w = 0; w < 16; ++w) store_u64(sh.prf + 128 * l + 8 * w, A[w]);
+  }
+}
+
+// KDF(key || H(c)) -> ss on the calling lane
+__device__ __forceinline__ void lowlat_kdf(uint8_t* ss, const uint32_t (&kb)[8], const uint32_t (&hc)[8]) {
+  uint64_t m[8], A[25];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    m[w] = (uint64_t)kb[2 * w] | ((uint64_t)kb[2 * w + 1] << 32);
+    m[4 + w] = (uint64_t)hc[2 * w] | ((uint64_t)hc[2 * w + 1] << 32);
+  }
+  absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+#pragma unroll
+  for (int w = 0; w < 4; ++w) store_u64(ss + 8 * w, A[w]);
+}
+
+// One decapsulation by the four waves of the workgroup (wave = threadIdx.x / 64)
+template <int K>
+__device__ __forceinline__ void lowlat_dec_body(uint8_t* ss, const uint8_t* c, const uint8_t* key, LowLds<K>& sh,
+                                                const LaneZ& z, int wave, int l, uint32_t* status) {
+  using P = Params<K>;
+  constexpr int HW = P::ct_bytes / 8;         // H(ct) message words
+  constexpr int HB = HW / kRateSha3_256 + 1;  // H(ct) blocks, padding included
+  constexpr int HB1 = (2 * HB + 2) / 3;       // of them absorbed during phase 1
   const uint8_t* pk = key + P::sk_cpa_bytes;
   uint32_t S[25];  // wave 2's H(ct) state, held across the first barrier
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
   // ---- phase 1
   if (wave == 0) {
     {
@@ -733,37 +786,13 @@ __global__ __launch_bounds__(256) void k_lowlat_dec(int n, uint8_t* __restrict__
       dec_load<K>(in, c, key);
       coop_indcpa_dec<K>(in, z, sh.msg);
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (l < 2 * K + 1) {
-      // every PRF lane recomputes G (same inputs) rather than waiting for a broadcast
-      uint64_t m[8], A[25], rc4[4];
+    wave_sync();
+    uint64_t h[4];
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        m[w] = load_u64(sh.msg + 8 * w);
-        m[4 + w] = load_u64(key + P::hpk_off + 8 * w);
-      }
-      absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
-#pragma unroll
-      for (int w = 0; w < 4; ++w) rc4[w] = A[4 + w];
-      if (l == 0) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          sh.kbar[2 * w] = (uint32_t)A[w];
-          sh.kbar[2 * w + 1] = (uint32_t)(A[w] >> 32);
-        }
-      }
-      absorb_fixed<kRateShake256, 4>(A, rc4, (uint64_t)l | (0x1Full << 8));
-#pragma unroll
-      for (int w = 0; w < 16; ++w) store_u64(sh.prf + 128 * l + 8 * w, A[w]);
-    }
-  } else if (wave == 1 && l < KK) {
-    // entry ij = i K + j on lane ij: A^T[i][j] = SHAKE128(rho || i || j)
-    uint64_t rho[4];
-#pragma unroll
-    for (int w = 0; w < 4; ++w) rho[w] = load_u64(pk + P::polyvec_bytes + 8 * w);
-    const int got = sample_entry(sh.a + l * kAStride, rho, l / K, l % K, true);
-    if (got < N) atomicOr(status, 1u);
+    for (int w = 0; w < 4; ++w) h[w] = load_u64(key + P::hpk_off + 8 * w);
+    lowlat_g_prf<K>(sh, h, l);
+  } else if (wave == 1) {
+    lowlat_sample<K>(sh, pk, l, status);
   } else if (wave == 2 && l < 2) {
     absorb_range_pair(S, c, HW, 0, HB1, 0x06, (uint32_t)l);
   }
@@ -781,20 +810,156 @@ __global__ __launch_bounds__(256) void k_lowlat_dec(int n, uint8_t* __restrict__
     for (int w = 0; w < 4; ++w) sh.hct[2 * w + l] = S[w];
   }
   __syncthreads();
-  // ---- phase 3: KDF(Kbar or z, H(ct))
+  // ---- phase 3: KDF(Kbar or z, H(ct)), z chosen by mask
   if (wave == 0 && l == 0) {
-    uint64_t m[8], A[25];
-    const uint64_t mask = 0 - (uint64_t)(sh.fail & 1u);
+    const uint32_t mask = 0u - (sh.fail & 1u);
+    uint32_t kb[8], hc[8];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const uint64_t kb = (uint64_t)sh.kbar[2 * w] | ((uint64_t)sh.kbar[2 * w + 1] << 32);
-      const uint64_t zz = load_u64(key + P::z_off + 8 * w);
-      m[w] = kb ^ (mask & (zz ^ kb));
-      m[4 + w] = (uint64_t)sh.hct[2 * w] | ((uint64_t)sh.hct[2 * w + 1] << 32);
+    for (int w = 0; w < 8; ++w) {
+      const uint32_t zz = reinterpret_cast<const uint32_t*>(key + P::z_off)[w];
+      kb[w] = sh.kbar[w] ^ (mask & (zz ^ sh.kbar[w]));
+      hc[w] = sh.hct[w];
     }
-    absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+    lowlat_kdf(ss, kb, hc);
+  }
+}
+
+// One encapsulation by the four waves of the workgroup
+//   phase 1  wave 0: m = H(coins); wave 1: A^T; wave 2: H(pk) on a lane pair
+//   phase 2  wave 0: G and the PRF outputs, then the encryption into LDS
+//   phase 3  wave 0: ct to memory; wave 2: H(ct) on a lane pair
+//   phase 4  one lane: KDF(Kbar || H(ct))
+template <int K>
+__device__ __forceinline__ void lowlat_enc_body(uint8_t* ss, uint8_t* c, const uint8_t* pk, const uint8_t* coins,
+                                                LowLds<K>& sh, const LaneZ& z, int wave, int l, uint32_t* status) {
+  using P = Params<K>;
+  // ---- phase 1
+  if (wave == 0 && l == 0) {
+    uint64_t A[25], c4[4];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) store_u64(ss + (size_t)x * 32 + 8 * w, A[w]);
+    for (int w = 0; w < 4; ++w) c4[w] = load_u64(coins + 8 * w);
+    absorb_fixed<kRateSha3_256, 4>(A, c4, 0x06);  // "don't release system RNG output"
+#pragma unroll
+    for (int w = 0; w < 4; ++w) store_u64(sh.msg + 8 * w, A[w]);
+  } else if (wave == 1) {
+    lowlat_sample<K>(sh, pk, l, status);
+  } else if (wave == 2 && l < 2) {
+    uint32_t S[25];
+    constexpr int W = P::pk_bytes / 8;
+    absorb_range_pair(S, pk, W, 0, W / kRateSha3_256 + 1, 0x06, (uint32_t)l);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sh.hpk[2 * w + l] = S[w];
+  }
+  __syncthreads();
+  // ---- phase 2
+  if (wave == 0) {
+    uint64_t h[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) h[w] = (uint64_t)sh.hpk[2 * w] | ((uint64_t)sh.hpk[2 * w + 1] << 32);
+    lowlat_g_prf<K>(sh, h, l);
+    wave_sync();
+    EncIn<K, false> in;
+    enc_load<K, false>(in, sh.prf, pk, nullptr);
+    (void)coop_reencrypt<K, false>(sh.a, 0, in, sh.msg, sh.ct, z);
+  }
+  __syncthreads();
+  // ---- phase 3
+  if (wave == 0) {
+    for (int w = l; w < P::ct_bytes / 8; w += 64) store_u64(c + 8 * w, load_u64(sh.ct + 8 * w));
+  } else if (wave == 2 && l < 2) {
+    uint32_t S[25];
+    constexpr int W = P::ct_bytes / 8;
+    absorb_range_pair(S, sh.ct, W, 0, W / kRateSha3_256 + 1, 0x06, (uint32_t)l);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sh.hct[2 * w + l] = S[w];
+  }
+  __syncthreads();
+  // ---- phase 4
+  if (wave == 0 && l == 0) {
+    uint32_t kb[8], hc[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      kb[w] = sh.kbar[w];
+      hc[w] = sh.hct[w];
+    }
+    lowlat_kdf(ss, kb, hc);
+  }
+}
+
+// Dense batches: request x at x * size in each buffer
+template <int K, bool DEC>
+__global__ __launch_bounds__(256) void k_lowlat(int n, uint8_t* __restrict__ ss, uint8_t* __restrict__ ct,
+                                                const uint8_t* __restrict__ key, const uint8_t* __restrict__ coins,
+                                                uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  __shared__ LowLds<K> sh;
+  const int x = blockIdx.x;
+  if (x >= n) return;  // whole workgroup
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  if constexpr (DEC)
+    lowlat_dec_body<K>(ss + (size_t)x * 32, ct + (size_t)x * P::ct_bytes, key + (size_t)x * P::sk_bytes, sh, z, wave,
+                       l, status);
+  else
+    lowlat_enc_body<K>(ss + (size_t)x * 32, ct + (size_t)x * P::ct_bytes, key + (size_t)x * P::pk_bytes,
+                       coins + (size_t)x * 32, sh, z, wave, l, status);
+}
+
+// Mixed batches: one descriptor per workgroup, checked as k_mixed_classify
+// checks it (a rejected one sets status bit 1 and is skipped)
+union LowLdsMixed {
+  LowLds<2> k2;
+  LowLds<3> k3;
+  LowLds<4> k4;
+};
+
+template <bool DEC>
+__global__ __launch_bounds__(256) void k_lowlat_mixed(int n, unsigned alg_mask, const pqc_kem_req* __restrict__ reqs,
+                                                      uint8_t* ct, uint64_t ct_bytes, const uint8_t* key,
+                                                      uint64_t key_bytes, uint8_t* ss, uint64_t ss_bytes,
+                                                      const uint8_t* coins, uint64_t coins_bytes,
+                                                      uint32_t* __restrict__ status) {
+  __shared__ LowLdsMixed sh;
+  const int x = blockIdx.x;
+  if (x >= n) return;
+  const pqc_kem_req d = reqs[x];
+  const bool known = (d.alg == PQC_ALG_KYBER512 || d.alg == PQC_ALG_KYBER768 || d.alg == PQC_ALG_KYBER1024) &&
+                     (alg_mask & PQC_ALG_MASK(d.alg));
+  const int k = known ? d.alg + 1 : 0;
+  bool ok = false;
+  if (k) {
+    const uint64_t ctb = k == 2 ? 736 : k == 3 ? 1088 : 1568;
+    const uint64_t keyb = DEC ? (k == 2 ? 1632 : k == 3 ? 2400 : 3168) : (k == 2 ? 800 : k == 3 ? 1184 : 1568);
+    ok = ((d.ct_off | d.key_off | d.ss_off | d.coins_off) & 7) == 0 && d.ct_off + ctb <= ct_bytes &&
+         d.key_off + keyb <= key_bytes && d.ss_off + 32 <= ss_bytes && (DEC || d.coins_off + 32 <= coins_bytes);
+  }
+  if (!ok) {  // uniform: every lane read the same descriptor
+    if (threadIdx.x == 0) atomicOr(status, 2u);
+    return;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  uint8_t* s_ = ss + d.ss_off;
+  uint8_t* c_ = ct + d.ct_off;
+  const uint8_t* k_ = key + d.key_off;
+  const uint8_t* r_ = DEC ? nullptr : coins + d.coins_off;
+  switch (k) {
+    case 2:
+      if constexpr (DEC) lowlat_dec_body<2>(s_, c_, k_, sh.k2, z, wave, l, status);
+      else lowlat_enc_body<2>(s_, c_, k_, r_, sh.k2, z, wave, l, status);
+      break;
+    case 3:
+      if constexpr (DEC) lowlat_dec_body<3>(s_, c_, k_, sh.k3, z, wave, l, status);
+      else lowlat_enc_body<3>(s_, c_, k_, r_, sh.k3, z, wave, l, status);
+      break;
+    default:
+      if constexpr (DEC) lowlat_dec_body<4>(s_, c_, k_, sh.k4, z, wave, l, status);
+      else lowlat_enc_body<4>(s_, c_, k_, r_, sh.k4, z, wave, l, status);
+      break;
   }
 }
 
@@ -992,13 +1157,20 @@ static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
   return hip_err(hipGetLastError());
 }
 
-// decapsulation batches of at most this many requests take k_lowlat_dec
-// (tuning.lowlat_max: 0 = this default, -1 = never)
+// Kyber batches (dense or mixed, encapsulation or decapsulation) of at most this
+// many requests take the one-workgroup-per-request kernels (tuning.lowlat_max:
+// 0 = this default, -1 = never)
 constexpr int kLowLatDefault = 256;
 
-template <int K>
-static int launch_lowlat_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
-  hipLaunchKernelGGL((k_lowlat_dec<K>), dim3((unsigned)n), dim3(256), 0, s, (int)n, ss, ct, sk, c->status);
+inline bool use_lowlat(const pqc_ctx* c, size_t n) {
+  const int lim = c->tune.lowlat_max == 0 ? kLowLatDefault : c->tune.lowlat_max;
+  return n > 0 && (long long)n <= (long long)lim;
+}
+
+template <int K, bool DEC>
+static int launch_lowlat(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const uint8_t* key, const uint8_t* coins,
+                         hipStream_t s) {
+  hipLaunchKernelGGL((k_lowlat<K, DEC>), dim3((unsigned)n), dim3(256), 0, s, (int)n, ss, ct, key, coins, c->status);
   return hip_err(hipGetLastError());
 }
 
@@ -1027,16 +1199,15 @@ int kyber_fused_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, c
 }
 
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
-  const int lowlat = c->tune.lowlat_max == 0 ? kyber::fused::kLowLatDefault : c->tune.lowlat_max;
-  if (n > 0 && (long long)n <= (long long)lowlat) {
+  uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation
+  if (kyber::fused::use_lowlat(c, n)) {
     switch (k) {
-      case 2: return kyber::fused::launch_lowlat_dec<2>(c, n, ss, ct, sk, s);
-      case 3: return kyber::fused::launch_lowlat_dec<3>(c, n, ss, ct, sk, s);
-      case 4: return kyber::fused::launch_lowlat_dec<4>(c, n, ss, ct, sk, s);
+      case 2: return kyber::fused::launch_lowlat<2, true>(c, n, ss, ctm, sk, nullptr, s);
+      case 3: return kyber::fused::launch_lowlat<3, true>(c, n, ss, ctm, sk, nullptr, s);
+      case 4: return kyber::fused::launch_lowlat<4, true>(c, n, ss, ctm, sk, nullptr, s);
       default: return PQC_ERR_UNSUPPORTED;
     }
   }
-  uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation
   switch (k) {
     case 2: return kyber::fused::launch_fused<2, true>(c, n, ss, ctm, sk, nullptr, s);
     case 3: return kyber::fused::launch_fused<3, true>(c, n, ss, ctm, sk, nullptr, s);
@@ -1047,6 +1218,14 @@ int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct,
 
 int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
                     hipStream_t s) {
+  if (kyber::fused::use_lowlat(c, n)) {
+    switch (k) {
+      case 2: return kyber::fused::launch_lowlat<2, false>(c, n, ss, ct, pk, coins, s);
+      case 3: return kyber::fused::launch_lowlat<3, false>(c, n, ss, ct, pk, coins, s);
+      case 4: return kyber::fused::launch_lowlat<4, false>(c, n, ss, ct, pk, coins, s);
+      default: return PQC_ERR_UNSUPPORTED;
+    }
+  }
   switch (k) {
     case 2: return kyber::fused::launch_fused<2, false>(c, n, ss, ct, pk, coins, s);
     case 3: return kyber::fused::launch_fused<3, false>(c, n, ss, ct, pk, coins, s);
@@ -1064,6 +1243,17 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
   using namespace kyber::fused;
   if (n == 0) return PQC_OK;
   if (!c->kmix_order || !c->kprf) return PQC_ERR_UNSUPPORTED;
+  if (use_lowlat(c, n)) {  // one workgroup per descriptor, no bucketing
+    if (dec)
+      hipLaunchKernelGGL(k_lowlat_mixed<true>, dim3((unsigned)n), dim3(256), 0, s, (int)n, c->alg_mask, reqs, ct,
+                         (uint64_t)ct_bytes, key, (uint64_t)key_bytes, ss, (uint64_t)ss_bytes, coins,
+                         (uint64_t)coins_bytes, c->status);
+    else
+      hipLaunchKernelGGL(k_lowlat_mixed<false>, dim3((unsigned)n), dim3(256), 0, s, (int)n, c->alg_mask, reqs, ct,
+                         (uint64_t)ct_bytes, key, (uint64_t)key_bytes, ss, (uint64_t)ss_bytes, coins,
+                         (uint64_t)coins_bytes, c->status);
+    return hip_err(hipGetLastError());
+  }
   const int cap = (int)c->max_batch;
   hipLaunchKernelGGL(k_mixed_zero, dim3(1), dim3(64), 0, s, c->kmix_counts);
   hipLaunchKernelGGL(k_mixed_classify, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, cap, dec,
