@@ -78,9 +78,10 @@ typedef struct pqc_tuning {
   int32_t fused_waves_per_cu; /* fused persistent grid = CUs x this: 1..4 (LDS holds 4) */
   int32_t mixed_order;        /* mixed-batch tile order: 0 = K 4,3,2; 1 = K 2,3,4; 2 = round-robin over K */
   int32_t frodo_stream_block; /* threads per workgroup of the Frodo noise-stream kernel: 64, 128 or 256 */
-  int32_t lowlat_max;         /* Kyber decapsulation batches of at most this many requests run one
-                                 four-wave workgroup per request (latency) instead of 64-request
-                                 tiles (throughput): 0 = library default (256), -1 = never, else 1..65536 */
+  int32_t lowlat_max;         /* Kyber batches of at most this many requests run one four-wave
+                                 workgroup per request (latency) instead of 64-request tiles
+                                 (throughput): 0 = library defaults (8192 decapsulations, 4096
+                                 encapsulations), -1 = never, else 1..65536 for both */
   int32_t reserved[2];        /* 0 */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);

@@ -1157,13 +1157,16 @@ static int launch_fused(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
   return hip_err(hipGetLastError());
 }
 
-// Kyber batches (dense or mixed, encapsulation or decapsulation) of at most this
-// many requests take the one-workgroup-per-request kernels (tuning.lowlat_max:
-// 0 = this default, -1 = never)
-constexpr int kLowLatDefault = 256;
+// Kyber batches (dense or mixed) of at most this many requests take the
+// one-workgroup-per-request kernels.  Measured on MI355X (Kyber-768): a tile
+// launch takes ~1.0 ms for any batch up to 65536; the per-request kernels run
+// ~10.6 M decaps/s and ~5.4 M encaps/s with ~0.12 / 0.2 ms latency, so they win
+// below ~10 K decapsulations and ~5 K encapsulations.  tuning.lowlat_max
+// overrides both (-1 = never).
+constexpr int kLowLatDefaultDec = 8192, kLowLatDefaultEnc = 4096;
 
-inline bool use_lowlat(const pqc_ctx* c, size_t n) {
-  const int lim = c->tune.lowlat_max == 0 ? kLowLatDefault : c->tune.lowlat_max;
+inline bool use_lowlat(const pqc_ctx* c, size_t n, bool dec) {
+  const int lim = c->tune.lowlat_max != 0 ? c->tune.lowlat_max : dec ? kLowLatDefaultDec : kLowLatDefaultEnc;
   return n > 0 && (long long)n <= (long long)lim;
 }
 
@@ -1200,7 +1203,7 @@ int kyber_fused_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, c
 
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation
-  if (kyber::fused::use_lowlat(c, n)) {
+  if (kyber::fused::use_lowlat(c, n, true)) {
     switch (k) {
       case 2: return kyber::fused::launch_lowlat<2, true>(c, n, ss, ctm, sk, nullptr, s);
       case 3: return kyber::fused::launch_lowlat<3, true>(c, n, ss, ctm, sk, nullptr, s);
@@ -1218,7 +1221,7 @@ int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct,
 
 int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
                     hipStream_t s) {
-  if (kyber::fused::use_lowlat(c, n)) {
+  if (kyber::fused::use_lowlat(c, n, false)) {
     switch (k) {
       case 2: return kyber::fused::launch_lowlat<2, false>(c, n, ss, ct, pk, coins, s);
       case 3: return kyber::fused::launch_lowlat<3, false>(c, n, ss, ct, pk, coins, s);
@@ -1243,7 +1246,7 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
   using namespace kyber::fused;
   if (n == 0) return PQC_OK;
   if (!c->kmix_order || !c->kprf) return PQC_ERR_UNSUPPORTED;
-  if (use_lowlat(c, n)) {  // one workgroup per descriptor, no bucketing
+  if (use_lowlat(c, n, dec != 0)) {  // one workgroup per descriptor, no bucketing
     if (dec)
       hipLaunchKernelGGL(k_lowlat_mixed<true>, dim3((unsigned)n), dim3(256), 0, s, (int)n, c->alg_mask, reqs, ct,
                          (uint64_t)ct_bytes, key, (uint64_t)key_bytes, ss, (uint64_t)ss_bytes, coins,
