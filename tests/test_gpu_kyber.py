@@ -106,14 +106,23 @@ def test_decaps_implicit_rejection_vs_oracle(ctx, k, plain):
         assert got[i].tobytes() == hashlib.shake_256(z + hashlib.sha3_256(bad[i].tobytes()).digest()).digest(32)
 
 
+@pytest.mark.parametrize("path", ["lowlat", "tiles"])
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 1000])
-def test_ragged_batches_kyber768(ctx, n):
-    say("ragged n", n)
+def test_ragged_batches_kyber768(ctx, n, path):
+    """Partial and multi-tile batches through both kernel families: one
+    workgroup per request (the default below the low-latency limits) and the
+    64-request tiles (lowlat_max = -1)."""
+    from pqc_amd import tuner as T
+    say("ragged n", n, path)
     coins, ecoins, pk, sk, ct, ss = oracle_triples(3, n, 500 + n)
-    gss = ctx.dec_host("kyber768", ct, sk)
-    assert (gss == ss).all()
-    gct, gss2 = ctx.enc_host("kyber768", pk, ecoins)
-    assert (gct == ct).all() and (gss2 == ss).all()
+    T.put_tuning(ctx, lowlat_max=0 if path == "lowlat" else -1)
+    try:
+        gss = ctx.dec_host("kyber768", ct, sk)
+        assert (gss == ss).all()
+        gct, gss2 = ctx.enc_host("kyber768", pk, ecoins)
+        assert (gct == ct).all() and (gss2 == ss).all()
+    finally:
+        T.put_tuning(ctx, lowlat_max=0)
 
 
 @pytest.mark.parametrize("k", [2, 3, 4])

@@ -132,9 +132,19 @@ def test_mixed_equals_dense_fused_and_plain(pqc, ctx, n):
         assert lay.element(ss_d, "ss", i).tobytes() == want, i
 
 
-def test_mixed_rejects_bad_descriptors(pqc, ctx):
+@pytest.fixture(params=["lowlat", "tiles"])
+def path(request, ctx):
+    """Small mixed batches run one workgroup per descriptor by default; "tiles"
+    forces the classify + bucket + tile kernels (lowlat_max = -1)."""
+    from pqc_amd import tuner as T
+    T.put_tuning(ctx, lowlat_max=0 if request.param == "lowlat" else -1)
+    yield request.param
+    T.put_tuning(ctx, lowlat_max=0)
+
+
+def test_mixed_rejects_bad_descriptors(pqc, ctx, path):
     n = 100
-    say("mixed bad descriptors")
+    say("mixed bad descriptors", path)
     lay = pqc.MixedLayout(mixed_algs(n, 20), np.random.default_rng(21))
     ar = pqc.DeviceArenas(lay)
     fill_keys_on_gpu(pqc, ctx, lay, ar, 22)
@@ -161,9 +171,9 @@ def test_mixed_rejects_bad_descriptors(pqc, ctx):
     assert ctx.take_status() == 0
 
 
-def test_graph_replay_mixed_dec(pqc, ctx):
+def test_graph_replay_mixed_dec(pqc, ctx, path):
     n = 1000
-    say("graph replay mixed dec")
+    say("graph replay mixed dec", path)
     lay = pqc.MixedLayout(mixed_algs(n, 30), np.random.default_rng(31))
     ar = pqc.DeviceArenas(lay)
     fill_keys_on_gpu(pqc, ctx, lay, ar, 32)
