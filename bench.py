@@ -331,13 +331,14 @@ def fixed_rate_latency(rate, duration, keys=1024):
     algs = np.full(n, 2, np.int32)
     ops = np.full(n, 2, np.int32)
     kid = rng.integers(0, keys, n).astype(np.uint32)
-    res = {"rate": rate, "requests": n, "slots": 2, "max_wait_us": 50}
+    res = {"rate": rate, "requests": n, "slots": 2, "max_wait_us": 25}
     for plain in (False, True):
         # two slots (one batch on the GPU while the next fills) and batches closed
-        # after 50 us: the tightest p50 / p99 measured at this rate.  Batches of up to
-        # 8192 decapsulations run the one-workgroup-per-request kernel.
+        # after 25 us, which kept batches near 600 requests on every box measured
+        # (50 us let some settle near 1000, doubling p50).  Batches of up to 8192
+        # decapsulations run the one-workgroup-per-request kernel.
         srv = Server(["kyber768"], keys_per_alg=keys, key_seed=0, max_batch=65536, slots=2,
-                     arena_bytes=128 << 20, max_wait_us=50, plain=plain)
+                     arena_bytes=128 << 20, max_wait_us=25, plain=plain)
         ws = RequestSet.random(algs, ops, kid, rng)
         srv.run_trace(arrivals, ws.reqs)  # warm-up: the same trace once
         rs = RequestSet.random(algs, ops, kid, rng)

@@ -7,15 +7,12 @@
 // and is copied directly, asynchronously on the caller's stream.  Transfers
 // below kDirectBytes (single-instance calls) go straight to hipMemcpyAsync.
 #include <algorithm>
-#include <condition_variable>
 #include <cstring>
-#include <functional>
 #include <map>
 #include <mutex>
-#include <thread>
-#include <vector>
 
 #include "internal.h"
+#include "threadpool.h"
 
 namespace pqc {
 namespace {
@@ -26,60 +23,6 @@ constexpr int kPoolThreads = 8;             // host threads per context, the cal
 
 std::mutex g_pin_mu;
 std::map<uintptr_t, size_t> g_pinned;  // start -> bytes of every live pqc_host_alloc block
-
-// A fixed set of threads that run one job at a time: run(fn) calls fn(t, T) on
-// T threads (the caller is thread 0) and returns when every call has returned.
-class CopyPool {
- public:
-  explicit CopyPool(int threads) : n_(threads) {
-    for (int t = 1; t < n_; ++t) th_.emplace_back([this, t] { worker(t); });
-  }
-  ~CopyPool() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto& t : th_) t.join();
-  }
-  void run(const std::function<void(int, int)>& fn) {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      job_ = &fn;
-      pending_ = n_ - 1;
-      ++gen_;
-    }
-    cv_.notify_all();
-    fn(0, n_);
-    std::unique_lock<std::mutex> lk(mu_);
-    done_.wait(lk, [this] { return pending_ == 0; });
-    job_ = nullptr;
-  }
-
- private:
-  void worker(int t) {
-    uint64_t seen = 0;
-    for (;;) {
-      std::unique_lock<std::mutex> lk(mu_);
-      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-      if (stop_) return;
-      seen = gen_;
-      const std::function<void(int, int)>* f = job_;
-      lk.unlock();
-      (*f)(t, n_);
-      lk.lock();
-      if (--pending_ == 0) done_.notify_one();
-    }
-  }
-  int n_;
-  std::vector<std::thread> th_;
-  std::mutex mu_;
-  std::condition_variable cv_, done_;
-  const std::function<void(int, int)>* job_ = nullptr;
-  uint64_t gen_ = 0;
-  int pending_ = 0;
-  bool stop_ = false;
-};
 
 struct HostIo {
   CopyPool pool{kPoolThreads};

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "threadpool.h"
 
 namespace {
 
@@ -104,6 +105,7 @@ struct pqc_server {
   uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_batched_reqs = 0;
   double pack_us = 0, device_us = 0, retire_us = 0;  // totals over the batches
   std::atomic<bool> stop{false};
+  pqc::CopyPool copy_pool{8};  // packs batches and delivers results (the worker thread is thread 0)
   std::thread worker;
   clk::time_point t0;
   int device = 0;
@@ -113,27 +115,7 @@ struct pqc_server {
 
 namespace {
 
-// Run fn(begin, end) over [0, n) on up to `threads` threads (the caller is one of
-// them); small ranges stay on the caller.
-template <typename F>
-void parallel_for(size_t n, int threads, F fn) {
-  const size_t per = 2048;
-  int t = (int)std::min<size_t>((size_t)threads, (n + per - 1) / per);
-  if (t <= 1) {
-    fn((size_t)0, n);
-    return;
-  }
-  std::vector<std::thread> pool;
-  const size_t chunk = (n + t - 1) / t;
-  for (int i = 1; i < t; ++i) {
-    const size_t b = i * chunk, e = std::min(n, b + chunk);
-    if (b < e) pool.emplace_back(fn, b, e);
-  }
-  fn((size_t)0, std::min(n, chunk));
-  for (auto& th : pool) th.join();
-}
-
-constexpr int kCopyThreads = 8;  // host threads that pack / unpack a batch
+constexpr size_t kCopyPer = 128;  // requests per copy thread (about 140 KB of Kyber-768 ciphertexts)
 
 void free_slot(Slot& sl) {
   if (sl.ctx) pqc_ctx_destroy(sl.ctx);
@@ -232,7 +214,7 @@ int launch(pqc_server* v, Slot& sl) {
     }
   }
   // the inputs themselves, copied by several threads
-  parallel_for(sl.reqs.size(), kCopyThreads, [&](size_t b, size_t e) {
+  v->copy_pool.parallel_for(sl.reqs.size(), kCopyPer, [&](size_t b, size_t e) {
     for (size_t i = b; i < e; ++i) memcpy(sl.h_in + sl.in_at[i], sl.reqs[i].r.in, in_bytes(sl.reqs[i].set, sl.reqs[i].r.op));
   });
   hipStream_t st = sl.s;
@@ -343,7 +325,7 @@ void worker_loop(pqc_server* v) {
         if (hipEventElapsedTime(&ms, sl.start, sl.done) == hipSuccess) v->device_us += 1e3 * ms;
         const double r0 = v->now_us();
         lk.unlock();
-        parallel_for(sl.reqs.size(), kCopyThreads, [&](size_t b, size_t e) {
+        v->copy_pool.parallel_for(sl.reqs.size(), kCopyPer, [&](size_t b, size_t e) {
           for (size_t i = b; i < e; ++i) {
             const Pending& p = sl.reqs[i];
             memcpy(p.r.out, sl.h_out + sl.out_a[i], sl.len_a[i]);
