@@ -132,18 +132,26 @@ __global__ void k_seed_expand(int n, uint8_t* z, const uint8_t* seed, size_t see
 
 // gen_a / poly_uniform (cpapke.cu:236-247, poly.cu:332-366): lane (x, i) runs
 // SHAKE128(seed || i) and keeps 16-bit candidates < 5q for coefficients 64 i ..
+// 64 i + 63.  Accepted values land at data-dependent positions, so each lane
+// collects its 64 in its own LDS row (66 uint16: 33 words, rows on distinct
+// banks) and then writes them out as eight 16-byte stores.  Fixed 128-lane
+// workgroups (16.5 KB of LDS).
+constexpr int kUniformBlock = 128;
+constexpr int kUniformRow = 66;
+
 template <int N>
-__global__ void k_uniform(int n, Pv<N> a, const uint8_t* seed, size_t stride) {
+__global__ __launch_bounds__(kUniformBlock) void k_uniform(int n, Pv<N> a, const uint8_t* seed, size_t stride) {
   constexpr int S = Par<N>::STREAMS;
+  __shared__ uint16_t stage[kUniformBlock * kUniformRow];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int x = t / S, i = t % S;
-  if (x >= n) return;
+  if (x >= n) return;  // no barrier below: every lane reads back only its own row
+  uint16_t* row = stage + threadIdx.x * kUniformRow;
   uint64_t A[25];
 #pragma unroll
   for (int w = 0; w < 25; ++w) A[w] = w < 4 ? load_u64(seed + (size_t)x * stride + 8 * w) : 0;
   A[4] = (uint64_t)i | (0x1Full << 8);
   A[kRateShake128 - 1] ^= 0x8000000000000000ull;
-  uint16_t* dst = a.el(x) + 64 * i;
   int ctr = 0;
   for (int blk = 0; blk < 16 && ctr < 64; ++blk) {
     keccakf1600(A);
@@ -152,10 +160,14 @@ __global__ void k_uniform(int n, Pv<N> a, const uint8_t* seed, size_t stride) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
         const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
-        if (val < 5 * Q && ctr < 64) dst[ctr++] = (uint16_t)val;
+        if (val < 5 * Q && ctr < 64) row[ctr++] = (uint16_t)val;
       }
     }
   }
+  const uint32_t* rw = reinterpret_cast<const uint32_t*>(row);
+  uint4* dst = reinterpret_cast<uint4*>(a.el(x) + 64 * i);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) dst[q] = make_uint4(rw[4 * q], rw[4 * q + 1], rw[4 * q + 2], rw[4 * q + 3]);
 }
 
 // poly_sample (poly.cu:396-444): lane (x, i) = coefficients 64 i .. 64 i + 63,
@@ -667,7 +679,8 @@ int keypair_impl(pqc_ctx* c, int n, uint8_t* pk, uint8_t* sk, size_t sk_stride, 
   uint8_t* z = c->nhbytes;  // 64 bytes per element: publicseed || noiseseed
   const int AHAT = 0, EHAT = 1, AS = 2, BHAT = 3, SHAT = 4;
   LAUNCH(k_seed_expand, n1, n, z, seed, seed_stride);
-  LAUNCH(k_uniform<N>, nS, n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
+  hipLaunchKernelGGL(k_uniform<N>, dim3((unsigned)((nS + kUniformBlock - 1) / kUniformBlock)), dim3(kUniformBlock), 0, s,
+                     n, S.poly(AHAT), (const uint8_t*)z, (size_t)64);
   LAUNCH(k_sample<N>, nS, n, S.poly(SHAT), (const uint8_t*)z + 32, (size_t)64, 0);
   LAUNCH_NTT(k_ntt<N>, n, S.poly(SHAT));
   LAUNCH(k_sample<N>, nS, n, S.poly(EHAT), (const uint8_t*)z + 32, (size_t)64, 1);
@@ -690,7 +703,8 @@ int enc_impl(pqc_ctx* c, int n, uint8_t* ct, size_t ct_stride, const uint8_t* m,
   const int SP = 0, EP = 1, VP = 2, AHAT = 3, BHAT = 4, EPP = 5, UHAT = 6, V = 7;
   LAUNCH(k_frommsg<N>, n32, n, S.poly(V), m, m_stride);
   LAUNCH(k_frombytes<N>, n4, n, S.poly(BHAT), pk, pk_stride, seed);
-  LAUNCH(k_uniform<N>, nS, n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
+  hipLaunchKernelGGL(k_uniform<N>, dim3((unsigned)((nS + kUniformBlock - 1) / kUniformBlock)), dim3(kUniformBlock), 0, s,
+                     n, S.poly(AHAT), (const uint8_t*)seed, (size_t)32);
   LAUNCH(k_sample<N>, nS, n, S.poly(SP), coin, coin_stride, 0);
   LAUNCH(k_sample<N>, nS, n, S.poly(EP), coin, coin_stride, 1);
   LAUNCH(k_sample<N>, nS, n, S.poly(EPP), coin, coin_stride, 2);
