@@ -685,30 +685,6 @@ struct alignas(16) LowLds {
   alignas(16) int16_t a[K * K * kAStride];        // A^T entry i K + j in slot i K + j
 };
 
-// Blocks [b0, b1) of a rate-17-word absorb (SHA3-256 / SHAKE256) of nwords
-// 64-bit words on a lane pair: the even lane holds the low 32-bit halves of
-// the state words, the odd lane the high halves.  b0 = 0 starts from zero.
-__device__ __forceinline__ void absorb_range_pair(uint32_t (&S)[25], const uint8_t* __restrict__ msg, int nwords,
-                                                  int b0, int b1, uint32_t domain, uint32_t half) {
-  constexpr int R = kRateSha3_256;
-  if (b0 == 0) {
-#pragma unroll
-    for (int w = 0; w < 25; ++w) S[w] = 0;
-  }
-  const int nblocks = nwords / R + 1;
-  const uint32_t* p = reinterpret_cast<const uint32_t*>(msg) + half;
-  for (int b = b0; b < b1 && b < nblocks; ++b) {
-    const int rem = nwords - b * R;  // words left, >= 0
-#pragma unroll
-    for (int w = 0; w < R; ++w) {
-      if (w < rem) S[w] ^= p[2 * (b * R + w)];
-      if (w == rem && !half) S[w] ^= domain;
-    }
-    if (b == nblocks - 1 && half) S[R - 1] ^= 0x80000000u;
-    keccakf1600_pair(S, half);
-  }
-}
-
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();

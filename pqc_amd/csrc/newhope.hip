@@ -600,6 +600,23 @@ __global__ void k_shake256_words(int n, uint8_t* out, size_t out_stride, const u
   for (int w = 0; w < OW; ++w) store_u64(out + (size_t)x * out_stride + 8 * w, A[w]);
 }
 
+// The same on a lane pair per element (lane 2x + h holds half h of every state
+// word, keccakf1600_pair): the latency form for the long serial absorbs of a
+// public key or a ciphertext (14 to 17 Keccak-f), with twice the lanes at work
+template <int OW>
+__global__ void k_shake256_pair(int n, uint8_t* out, size_t out_stride, const uint8_t* in, size_t in_stride,
+                                int words) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = t >> 1;
+  const uint32_t half = (uint32_t)(t & 1);
+  if (x >= n) return;  // both lanes of a pair leave together (2n lanes, even)
+  uint32_t S[25];
+  absorb_range_pair(S, in + (size_t)x * in_stride, words, 0, words / kRateShake256 + 1, 0x1F, half);
+  uint32_t* o = reinterpret_cast<uint32_t*>(out + (size_t)x * out_stride);
+#pragma unroll
+  for (int w = 0; w < OW; ++w) o[2 * w + half] = S[w];
+}
+
 // out[x] = SHAKE256(a[x] (32 bytes) || b[x] (32 bytes)), first 8 OW bytes
 template <int OW>
 __global__ void k_shake256_2x32(int n, uint8_t* out, size_t out_stride, const uint8_t* a, size_t a_stride,
@@ -754,7 +771,7 @@ int cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* 
   int rc = keypair_impl<N>(c, n, pk, sk, P::CCA_SK, coins, 64, s);
   if (rc) return rc;
   LAUNCH(k_copy_words, n_ * (P::PK / 8), n, sk + P::SK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK, P::PK / 8);
-  LAUNCH(k_shake256_words<4>, n_, n, sk + P::SK + P::PK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK,
+  LAUNCH(k_shake256_pair<4>, 2 * n_, n, sk + P::SK + P::PK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK,
          P::PK / 8);
   LAUNCH(k_copy_words, n_ * 4, n, sk + P::CCA_SK - SYM, (size_t)P::CCA_SK, coins + SYM, (size_t)64, 4);
   return hip_err(hipGetLastError());
@@ -769,12 +786,12 @@ int cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, 
   uint8_t* km = c->nhkem;
   const size_t kb = kNewhopeKemBytes;
   LAUNCH(k_shake256_words<4>, n_, n, km + KM_MU, kb, coins, (size_t)32, 4);              // mu = H(coins)
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HPK, kb, pk, (size_t)P::PK, P::PK / 8);    // H(pk)
+  LAUNCH(k_shake256_pair<4>, 2 * n_, n, km + KM_HPK, kb, pk, (size_t)P::PK, P::PK / 8);  // H(pk)
   LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
   int rc = enc_impl<N>(c, n, ct, P::CCA_CT, km + KM_MU, kb, pk, P::PK, km + KM_KCD + SYM, kb, s);
   if (rc) return rc;
   LAUNCH(k_copy_words, n_ * 4, n, ct + P::CT, (size_t)P::CCA_CT, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, (const uint8_t*)ct, (size_t)P::CCA_CT, P::CCA_CT / 8);      // H(c)
+  LAUNCH(k_shake256_pair<4>, 2 * n_, n, km + KM_HC, kb, (const uint8_t*)ct, (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
   LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);
   return hip_err(hipGetLastError());
 }
@@ -795,7 +812,7 @@ int cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t
   LAUNCH(k_copy_words, n_ * 4, n, km + KM_CT2 + P::CT, kb, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d'
   hipLaunchKernelGGL(k_verify_words, dim3((unsigned)n), dim3(64), 0, s, n, reinterpret_cast<uint32_t*>(km + KM_FAIL),
                      kb, ct, (size_t)P::CCA_CT, (const uint8_t*)km + KM_CT2, kb, P::CCA_CT / 8);
-  LAUNCH(k_shake256_words<4>, n_, n, km + KM_HC, kb, ct, (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
+  LAUNCH(k_shake256_pair<4>, 2 * n_, n, km + KM_HC, kb, ct, (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
   LAUNCH(k_cmov32, n_ * 4, n, km + KM_KCD, kb, sk + P::CCA_SK - SYM, (size_t)P::CCA_SK,
          reinterpret_cast<const uint32_t*>(km + KM_FAIL), kb);
   LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);
