@@ -32,6 +32,7 @@ ALG = "kyber768"
 K = 3
 KECCAK_PER_DECAPS = 54         # G 1 + A^T 36 + PRF 7 + H(ct) 9 + KDF 1
 OPS_PER_KECCAK = 7440          # 24 rounds x 155 64-bit ops = 7440 32-bit ops
+NEWHOPE_KECCAK_PER_ENC = 64    # gen_a 16 SHAKE128 streams (one block each) + s', e', e'' 3 x 16 SHAKE256
 BYTES_PER_DECAPS = 1088 + 2400 + 32  # compulsory: ct in, sk in, ss out
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz int32 ops
 HBM_PEAK_GBPS = 8000.0
@@ -454,30 +455,43 @@ def bench_newhope(args):
     pk, sk, ct, m2 = pqc.DeviceBuffer(n * 1824), pqc.DeviceBuffer(n * 1792), pqc.DeviceBuffer(n * 2176), \
         pqc.DeviceBuffer(n * 32)
 
-    def triple():
-        ctx.newhope_keypair(n, pk, sk, seed, stream=stream)
-        ctx.newhope_enc(n, ct, msg, pk, coin, stream=stream)
-        ctx.newhope_dec(n, m2, ct, sk, stream=stream)
+    def triple(plain=False):
+        ctx.newhope_keypair(n, pk, sk, seed, stream=stream, plain=plain)
+        ctx.newhope_enc(n, ct, msg, pk, coin, stream=stream, plain=plain)
+        ctx.newhope_dec(n, m2, ct, sk, stream=stream, plain=plain)
 
     wall, per = timed(triple, stream, args.steps, args.warmup, pqc)
     ms = wall * 1e3 / args.steps
-    log("newhope triples: %.3f ms/step" % ms)
+    log("newhope triples (fused): %.3f ms/step" % ms)
     ok = bool((m2.download((n, 32)) == msg.download((n, 32))).all())
+    ct_fused = ct.download((n, 2176))
+    pwall, _ = timed(lambda: triple(True), stream, args.steps, args.warmup, pqc)
+    plain_ms = pwall * 1e3 / args.steps
+    log("newhope triples (plain): %.3f ms/step" % plain_ms)
+    same = bool((ct.download((n, 2176)) == ct_fused).all())
+    # the three fused kernels, one launch each, timed alone on the same stream
+    kms = {name: timed_run(fn, stream, 20, 2, pqc) for name, fn in (
+        ("keypair", lambda: ctx.newhope_keypair(n, pk, sk, seed, stream=stream)),
+        ("enc", lambda: ctx.newhope_enc(n, ct, msg, pk, coin, stream=stream)),
+        ("dec", lambda: ctx.newhope_dec(n, m2, ct, sk, stream=stream)))}
+    log("fused kernels: keypair %.1f us, enc %.1f us, dec %.1f us"
+        % tuple(kms[k] * 1e3 for k in ("keypair", "enc", "dec")))
     cca = newhope_cca_rates(pqc, ctx, stream, n, rng, max(3, args.steps // 4))
     nh512 = newhope512_rates(pqc, ctx, stream, n, rng, max(3, args.steps // 4))
 
-    # The step's largest kernel is poly_sample (5 launches per triple batch); time
-    # it alone on the same stream, and the NTT (HBM figure) beside it.
+    # The per-operation path's largest kernels: poly_sample and the NTT, alone.
     polys = pqc.DeviceBuffer(n * 2048)
     L = pqc.lib()
     sample_ms = timed_run(lambda: L.pqc_newhope1024_poly_sample(n, polys.ptr, coin.ptr, 32, 0, stream.handle),
                           stream, 20, 2, pqc)
     ntt_ms = timed_run(lambda: L.pqc_newhope1024_poly_transform(0, n, polys.ptr, stream.handle), stream, 20, 2, pqc)
-    sample_ops = n * 16 * OPS_PER_KECCAK  # 16 SHAKE256 permutations per polynomial
     ntt_bytes = n * 2048 * 2              # read + write one uint16 polynomial
-    log("poly_sample %.1f us, poly_ntt %.1f us" % (sample_ms * 1e3, ntt_ms * 1e3))
+    log("plain poly_sample %.1f us, poly_ntt %.1f us" % (sample_ms * 1e3, ntt_ms * 1e3))
+    enc_ops = n * NEWHOPE_KECCAK_PER_ENC * OPS_PER_KECCAK
+    enc_ms = kms["enc"]
     out = {
-        "metric": "NewHope-1024 CPA keygen+encaps+decaps triples/s at batch 16384 (plain batched kernels)",
+        "metric": "NewHope-1024 CPA keygen+encaps+decaps triples/s at batch 16384 (fused per-element kernels vs "
+                  "plain batched kernels)",
         "value": round(n / (ms / 1e3), 1),
         "unit": "triples/s",
         "n_gpus": 1,
@@ -493,19 +507,22 @@ def bench_newhope(args):
                    "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
                    "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
                    "messages_recovered": ok,
+                   "plain_triples_per_s": round(n / (plain_ms / 1e3), 1),
+                   "plain_ms_per_step": round(plain_ms, 4),
+                   "fused_over_plain": round(plain_ms / ms, 3),
+                   "fused_equals_plain": same,
+                   "fused_kernel_us": {k: round(v * 1e3, 1) for k, v in kms.items()},
                    "cca_kem": cca,
                    "newhope512": nh512,
-                   "poly_ntt": {"kernel_ms": round(ntt_ms, 4), "bytes": ntt_bytes,
-                                "hbm_gbps": round(ntt_bytes / ntt_ms / 1e6, 1),
-                                "hbm_frac": round(ntt_bytes / ntt_ms / 1e6 / HBM_PEAK_GBPS, 4)}},
-        "roofline": {"bound": "valu", "kernel": "newhope::k_sample (poly_sample, 16 SHAKE256 streams per polynomial)",
-                     "achieved": round(sample_ops / sample_ms / 1e9, 2), "peak": VALU_PEAK_TOPS,
-                     "unit": "T int32 ops/s", "frac": round(sample_ops / sample_ms / 1e9 / VALU_PEAK_TOPS, 4),
-                     "work_per_launch": "%d polynomials x 16 Keccak-f x 7440 two-input 32-bit ops" % n,
-                     "kernel_ms": round(sample_ms, 4), "traffic": None,
-                     "note": "7440 counts two-input ops; v_bitop3_b32 retires two of them in one lane-cycle "
-                             "(4320 VALU instructions per permutation), so the rate can pass the 1-op peak",
-                     "valu_instr_frac": round(n * 16 * 4320 / sample_ms / 1e9 / VALU_PEAK_TOPS, 4)},
+                   "plain_kernels": {"poly_sample_us": round(sample_ms * 1e3, 1), "poly_ntt_us": round(ntt_ms * 1e3, 1),
+                                     "poly_ntt_hbm_frac": round(ntt_bytes / ntt_ms / 1e6 / HBM_PEAK_GBPS, 4)}},
+        "roofline": {"bound": "valu", "kernel": "newhope::k_enc_fused<1024> (cpapke_enc, one wave per element)",
+                     "achieved": round(enc_ops / enc_ms / 1e9, 2), "peak": VALU_PEAK_TOPS,
+                     "unit": "T int32 ops/s", "frac": round(enc_ops / enc_ms / 1e9 / VALU_PEAK_TOPS, 4),
+                     "work_per_launch": "%d encryptions x %d Keccak-f x 7440 two-input 32-bit ops (the sponge work "
+                                        "only: NTT, pointwise and packing work is not counted, so frac is a lower "
+                                        "bound)" % (n, NEWHOPE_KECCAK_PER_ENC),
+                     "kernel_ms": round(enc_ms, 4), "traffic": None},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

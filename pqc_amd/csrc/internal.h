@@ -94,19 +94,23 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
                       const uint8_t* key, size_t key_bytes, uint8_t* ss, size_t ss_bytes, const uint8_t* coins,
                       size_t coins_bytes, hipStream_t s);
 
-// NewHope-N CPA plain batched path (newhope.hip), N = 512 or 1024
-int newhope_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s);
+// NewHope-N CPA (newhope.hip), N = 512 or 1024: fused per-element kernels, or
+// the per-operation kernels when plain
+int newhope_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, bool plain,
+                    hipStream_t s);
 int newhope_enc(pqc_ctx* c, int N, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
+                bool plain, hipStream_t s);
+int newhope_dec(pqc_ctx* c, int N, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, bool plain,
                 hipStream_t s);
-int newhope_dec(pqc_ctx* c, int N, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
 // NewHope-N CCA-KEM over the CPA kernels (newhope.hip); keypair coins 64 bytes;
 // one scratch row of kNewhopeKemBytes per element serves both N
 constexpr int kNewhopeKemBytes = 2432;
-int newhope_cca_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins,
+int newhope_cca_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, bool plain,
                         hipStream_t s);
 int newhope_cca_enc(pqc_ctx* c, int N, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+                    bool plain, hipStream_t s);
+int newhope_cca_dec(pqc_ctx* c, int N, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, bool plain,
                     hipStream_t s);
-int newhope_cca_dec(pqc_ctx* c, int N, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s);
 int newhope_poly_transform(int op, size_t n, uint16_t* polys, hipStream_t s);
 int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,
                         hipStream_t s);

@@ -586,6 +586,366 @@ __global__ void k_tomsg(int n, uint8_t* msg, size_t msg_stride, Pv<N> p) {
   msg[(size_t)x * msg_stride + i] = (uint8_t)byte;
 }
 
+// ---- fused per-element kernels: one wave per element ------------------------------
+// Keypair, encryption and decryption each as one kernel.  The element's SHAKE
+// streams run on the wave's lanes (one stream per lane), its polynomials stay in
+// LDS or in VGPRs in ntt_wave's output layout, and HBM sees only the element's
+// inputs and outputs.  Every value is produced by the same operations, in the
+// same order and with the same 16-bit storage points as the per-operation
+// kernels above, so both paths give identical outputs.
+
+// gen_a rows in LDS: 64 coefficients of one stream, padded to 66 so that the
+// lanes' scattered accept stores fall on different banks
+constexpr int kArow = 66;
+
+__device__ __forceinline__ int arow_index(int c) { return (c >> 6) * kArow + (c & 63); }
+
+template <int N>
+__device__ __forceinline__ int brv_index(int c) {
+  return (int)(__builtin_bitreverse32((uint32_t)c) >> (32 - Par<N>::LOG));
+}
+
+// One SHAKE stream on this lane: gen_a stream i (SHAKE128(seed || i), 16-bit
+// candidates < 5q kept unreduced, poly.cu:332-366) when `uniform`, else
+// poly_sample stream i (SHAKE256(seed || nonce || i, 128 bytes), hw(a) + q -
+// hw(b), poly.cu:396-444).  Writes the stream's 64 coefficients to row[0..63].
+// Lanes that take no part pass active = false; every lane of the wave reaches
+// the block loop's vote.
+__device__ __forceinline__ void lane_stream(uint16_t* row, const uint64_t (&seed)[4], bool active, bool uniform,
+                                            int nonce, int i) {
+  uint64_t A[25];
+#pragma unroll
+  for (int w = 0; w < 25; ++w) A[w] = w < 4 ? seed[w] : 0;
+  A[4] = uniform ? ((uint64_t)i | (0x1Full << 8)) : ((uint64_t)nonce | ((uint64_t)i << 8) | (0x1Full << 16));
+  A[kRateShake128 - 1] ^= uniform ? 0x8000000000000000ull : 0;
+  A[kRateShake256 - 1] ^= uniform ? 0 : 0x8000000000000000ull;
+  keccakf1600(A);
+  if (active && !uniform) {
+#pragma unroll
+    for (int w = 0; w < 16; w += 2) {
+      uint32_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint64_t word = A[w + (k >> 1)];
+        const uint32_t ab0 = (uint32_t)(word >> (32 * (k & 1))) & 0xFFFFu;
+        const uint32_t ab1 = (uint32_t)(word >> (32 * (k & 1) + 16)) & 0xFFFFu;
+        const uint32_t c0 = (__popc(ab0 & 0xFF) + Q - __popc(ab0 >> 8)) & 0xFFFFu;
+        const uint32_t c1 = (__popc(ab1 & 0xFF) + Q - __popc(ab1 >> 8)) & 0xFFFFu;
+        o[k] = c0 | (c1 << 16);
+      }
+      reinterpret_cast<uint4*>(row)[w >> 1] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
+  int ctr = (active && uniform) ? 0 : 64;
+  for (int blk = 0; blk < 16; ++blk) {
+    if (blk > 0) {
+      if (!__any(ctr < 64)) break;
+      keccakf1600(A);
+    }
+    if (ctr < 64) {
+#pragma unroll
+      for (int w = 0; w < kRateShake128; ++w) {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
+          if (val < 5 * Q && ctr < 64) row[ctr++] = (uint16_t)val;
+        }
+      }
+    }
+  }
+}
+
+// poly_ntt of the natural-order polynomial in sm (clobbered): v[r] = output
+// coefficient out_index(l, r), as k_ntt
+template <int N>
+__device__ __forceinline__ void ntt_lds(uint32_t (&v)[N / 64], uint16_t* sm, int l) {
+  constexpr int C = N / 64;
+  uint4* p = reinterpret_cast<uint4*>(sm + C * l);
+  const uint4* gam = reinterpret_cast<const uint4*>(tab<N>().gammas + C * l);
+#pragma unroll
+  for (int h = 0; h < C / 8; ++h) {
+    const uint4 a = p[h], g = gam[h];
+    const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, gw[4] = {g.x, g.y, g.z, g.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = mont((aw[k] & 0xFFFFu) * (gw[k] & 0xFFFFu)) | (mont((aw[k] >> 16) * (gw[k] >> 16)) << 16);
+    p[h] = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+  __syncthreads();
+  ntt_wave(v, sm, l, tab<N>().gammas);
+}
+
+// poly_invntt of the polynomial held as v[r] = coefficient out_index(l, r), in
+// place, through sm (clobbered), as k_invntt
+template <int N>
+__device__ __forceinline__ void invntt_regs(uint32_t (&v)[N / 64], uint16_t* sm, int l) {
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < N / 64; ++r) sm[brv_index<N>(out_index<N>(l, r))] = (uint16_t)v[r];
+  __syncthreads();
+  ntt_wave(v, sm, l, tab<N>().omegas_inv);
+#pragma unroll
+  for (int k = 0; k < N / 64; ++k) v[k] = mont(v[k] * tab<N>().gammas_inv[out_index<N>(l, k)]);
+}
+
+// natural-order LDS polynomial <-> registers in ntt_wave's output layout
+template <int N>
+__device__ __forceinline__ void put_nat(uint16_t* sm, const uint32_t (&v)[N / 64], int l) {
+#pragma unroll
+  for (int r = 0; r < N / 64; ++r) sm[out_index<N>(l, r)] = (uint16_t)v[r];
+}
+
+// poly_frombytes of the 14-bit packing at src (8-byte aligned) into sm in
+// natural order: group g (coefficients 4 g .. 4 g + 3) is bits 0..55 of the 7
+// bytes at 7 g, read as one or two aligned words (never a word past the
+// polynomial: only groups that straddle a word boundary read the next one)
+template <int N>
+__device__ __forceinline__ void unpack14(uint16_t* sm, const uint8_t* src, int l) {
+  const uint64_t* s64 = reinterpret_cast<const uint64_t*>(src);
+#pragma unroll
+  for (int g = l; g < N / 4; g += 64) {
+    const uint32_t bo = 7u * (uint32_t)g, sh = 8u * (bo & 7u);
+    const uint64_t lo = s64[bo >> 3];
+    const uint64_t hi = (bo & 7u) > 1u ? s64[(bo >> 3) + 1] : 0;
+    const uint64_t w = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    const uint32_t c0 = (uint32_t)w & 0x3FFFu, c1 = (uint32_t)(w >> 14) & 0x3FFFu;
+    const uint32_t c2 = (uint32_t)(w >> 28) & 0x3FFFu, c3 = (uint32_t)(w >> 42) & 0x3FFFu;
+    reinterpret_cast<uint2*>(sm)[g] = make_uint2(c0 | (c1 << 16), c2 | (c3 << 16));
+  }
+}
+
+// poly_tobytes of the natural-order polynomial in sm into LDS bytes bb
+template <int N>
+__device__ __forceinline__ void pack14(uint8_t* bb, const uint16_t* sm, int l) {
+#pragma unroll
+  for (int g = l; g < N / 4; g += 64) {
+    const uint2 c = reinterpret_cast<const uint2*>(sm)[g];
+    const uint32_t t0 = freeze(c.x & 0xFFFFu), t1 = freeze(c.x >> 16), t2 = freeze(c.y & 0xFFFFu),
+                   t3 = freeze(c.y >> 16);
+    uint8_t* r = bb + 7 * g;
+    r[0] = (uint8_t)(t0 & 0xff);
+    r[1] = (uint8_t)((t0 >> 8) | (t1 << 6));
+    r[2] = (uint8_t)(t1 >> 2);
+    r[3] = (uint8_t)((t1 >> 10) | (t2 << 4));
+    r[4] = (uint8_t)(t2 >> 4);
+    r[5] = (uint8_t)((t2 >> 12) | (t3 << 2));
+    r[6] = (uint8_t)(t3 >> 6);
+  }
+}
+
+// poly_compress of the natural-order polynomial in sm into LDS bytes bb
+template <int N>
+__device__ __forceinline__ void compress3(uint8_t* bb, const uint16_t* sm, int l) {
+#pragma unroll
+  for (int g = l; g < N / 8; g += 64) {
+    const uint4 c = reinterpret_cast<const uint4*>(sm)[g];
+    const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
+    uint32_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = (((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) / Q) & 7;
+    uint8_t* r = bb + 3 * g;
+    r[0] = (uint8_t)(v[0] | (v[1] << 3) | (v[2] << 6));
+    r[1] = (uint8_t)((v[2] >> 2) | (v[3] << 1) | (v[4] << 4) | (v[5] << 7));
+    r[2] = (uint8_t)((v[5] >> 1) | (v[6] << 2) | (v[7] << 5));
+  }
+}
+
+// poly_decompress of the 3-bit packing at src into sm in natural order
+template <int N>
+__device__ __forceinline__ void decompress3(uint16_t* sm, const uint8_t* src, int l) {
+#pragma unroll
+  for (int g = l; g < N / 8; g += 64) {
+    const uint8_t* a = src + 3 * g;
+    const uint32_t a0 = a[0], a1 = a[1], a2 = a[2];
+    uint32_t v[8];
+    v[0] = a0 & 7;
+    v[1] = (a0 >> 3) & 7;
+    v[2] = (a0 >> 6) | ((a1 << 2) & 4);
+    v[3] = (a1 >> 1) & 7;
+    v[4] = (a1 >> 4) & 7;
+    v[5] = (a1 >> 7) | ((a2 << 1) & 6);
+    v[6] = (a2 >> 2) & 7;
+    v[7] = (a2 >> 5);
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = (((v[2 * k] * Q + 4) >> 3) & 0xFFFFu) | ((((v[2 * k + 1] * Q + 4) >> 3) & 0xFFFFu) << 16);
+    reinterpret_cast<uint4*>(sm)[g] = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+}
+
+// `bytes` (a multiple of 8) from LDS to global memory (8-byte aligned), one
+// word per lane per step
+__device__ __forceinline__ void copy_out(uint8_t* dst, const uint8_t* bb, int bytes, int l) {
+  for (int w = l; w < bytes / 8; w += 64) store_u64(dst + 8 * w, reinterpret_cast<const uint64_t*>(bb)[w]);
+}
+
+// cpapke_keypair (cpapke.cu:265-338) for element blockIdx.x, from z =
+// publicseed || noiseseed = SHAKE256(seed, 64) at z + 64 x (k_seed_expand: the
+// one serial permutation stays on one lane per element).  Lanes 0..S-1 run the
+// gen_a streams, S..2S-1 the streams of s (nonce 0), 2S..3S-1 those of e
+// (nonce 1); S = N / 64.
+template <int N>
+__global__ __launch_bounds__(64) void k_keypair_fused(int n, uint8_t* pk, uint8_t* sk, size_t sk_stride,
+                                                      const uint8_t* z) {
+  using P = Par<N>;
+  constexpr int C = N / 64, S = P::STREAMS;
+  __shared__ __align__(16) uint16_t sa[S * kArow];
+  __shared__ __align__(16) uint16_t ss[N];
+  __shared__ __align__(16) uint16_t se[N];
+  const int x = blockIdx.x, l = threadIdx.x;
+  if (x >= n) return;
+  const uint8_t* zx = z + (size_t)x * 64;
+  {
+    const int kind = l / S, i = l % S;
+    const bool pub = kind == 0;
+    uint64_t sd[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sd[w] = load_u64(zx + (pub ? 0 : 32) + 8 * w);
+    uint16_t* row = pub ? sa + i * kArow : (kind == 1 ? ss : se) + 64 * i;
+    lane_stream(row, sd, kind < 3, pub, kind - 1, i);
+  }
+  __syncthreads();
+  uint32_t vs[C], vb[C];
+  ntt_lds<N>(vs, ss, l);
+  ntt_lds<N>(vb, se, l);
+  // b = e + s o a (k_pointwise(AS, SHAT, AHAT), k_addsub(BHAT, EHAT, AS))
+#pragma unroll
+  for (int r = 0; r < C; ++r) {
+    const uint32_t ahat = sa[arow_index(out_index<N>(l, r))];
+    vb[r] = (vb[r] + mont(vs[r] * mont(3186u * ahat))) % Q;
+  }
+  __syncthreads();
+  put_nat<N>(ss, vs, l);
+  put_nat<N>(se, vb, l);
+  __syncthreads();
+  pack14<N>(reinterpret_cast<uint8_t*>(sa), ss, l);
+  __syncthreads();
+  copy_out(sk + (size_t)x * sk_stride, reinterpret_cast<const uint8_t*>(sa), P::POLYBYTES, l);
+  __syncthreads();
+  pack14<N>(reinterpret_cast<uint8_t*>(sa), se, l);
+  __syncthreads();
+  uint8_t* pkx = pk + (size_t)x * P::PK;
+  copy_out(pkx, reinterpret_cast<const uint8_t*>(sa), P::POLYBYTES, l);
+  if (l < 4) store_u64(pkx + P::POLYBYTES + 8 * l, load_u64(zx + 8 * l));
+}
+
+// cpapke_enc (cpapke.cu:353-429) for element blockIdx.x.  Lanes 0..S-1 run the
+// gen_a streams of the public seed, S..4S-1 the streams of s', e', e'' (coin,
+// nonces 0, 1, 2).
+template <int N>
+__global__ __launch_bounds__(64) void k_enc_fused(int n, uint8_t* ct, size_t ct_stride, const uint8_t* m,
+                                                  size_t m_stride, const uint8_t* pk, size_t pk_stride,
+                                                  const uint8_t* coin, size_t coin_stride) {
+  using P = Par<N>;
+  constexpr int C = N / 64, S = P::STREAMS;
+  __shared__ __align__(16) uint16_t sa[S * kArow];
+  __shared__ __align__(16) uint16_t s1[N];
+  __shared__ __align__(16) uint16_t s2[N];
+  __shared__ __align__(16) uint16_t s3[N];
+  const int x = blockIdx.x, l = threadIdx.x;
+  if (x >= n) return;
+  const uint8_t* pkx = pk + (size_t)x * pk_stride;
+  {
+    const int kind = l / S, i = l % S;
+    const bool pub = kind == 0;
+    const uint8_t* src = pub ? pkx + P::POLYBYTES : coin + (size_t)x * coin_stride;
+    uint64_t sd[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sd[w] = load_u64(src + 8 * w);
+    uint16_t* row = pub ? sa + i * kArow : (kind == 1 ? s1 : kind == 2 ? s2 : s3) + 64 * i;
+    lane_stream(row, sd, kind < 4, pub, kind - 1, i);
+  }
+  __syncthreads();
+  uint32_t vsp[C], vv[C];
+  ntt_lds<N>(vsp, s1, l);
+  {
+    // u = a o s' + e' (k_pointwise(UHAT, AHAT, SP), k_addsub(UHAT, UHAT, EP)),
+    // parked in natural order over the gen_a rows once every lane has read them
+    uint32_t vu[C];
+    ntt_lds<N>(vu, s2, l);
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+      const uint32_t ahat = sa[arow_index(out_index<N>(l, r))];
+      vu[r] = (mont(ahat * mont(3186u * vsp[r])) + vu[r]) % Q;
+    }
+    __syncthreads();
+    put_nat<N>(sa, vu, l);
+  }
+  unpack14<N>(s1, pkx, l);  // b (decode_pk)
+  __syncthreads();
+  // v' = invntt(b o s') (k_pointwise(VP, BHAT, SP), k_invntt)
+#pragma unroll
+  for (int r = 0; r < C; ++r) vv[r] = mont((uint32_t)s1[out_index<N>(l, r)] * mont(3186u * vsp[r]));
+  invntt_regs<N>(vv, s2, l);
+  // v = v' + e'' + frommsg(m) (k_addsub twice, k_frommsg)
+  uint64_t mw[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) mw[w] = load_u64(m + (size_t)x * m_stride + 8 * w);
+#pragma unroll
+  for (int r = 0; r < C; ++r) {
+    const int c = out_index<N>(l, r), b = c & 255;
+    const uint64_t word = (b >> 6) == 0 ? mw[0] : (b >> 6) == 1 ? mw[1] : (b >> 6) == 2 ? mw[2] : mw[3];
+    const uint32_t bit = (uint32_t)(word >> (b & 63)) & 1u;
+    const uint32_t t = (vv[r] + s3[c]) % Q;
+    vv[r] = (t + ((0u - bit) & (Q / 2))) % Q;
+  }
+  __syncthreads();
+  put_nat<N>(s3, vv, l);
+  __syncthreads();
+  pack14<N>(reinterpret_cast<uint8_t*>(s2), sa, l);
+  compress3<N>(reinterpret_cast<uint8_t*>(s1), s3, l);
+  __syncthreads();
+  uint8_t* ctx_ = ct + (size_t)x * ct_stride;
+  copy_out(ctx_, reinterpret_cast<const uint8_t*>(s2), P::POLYBYTES, l);
+  copy_out(ctx_ + P::POLYBYTES, reinterpret_cast<const uint8_t*>(s1), P::CBYTES, l);
+}
+
+// cpapke_dec (cpapke.cu:443-476) for element blockIdx.x
+template <int N>
+__global__ __launch_bounds__(64) void k_dec_fused(int n, uint8_t* m, size_t m_stride, const uint8_t* ct,
+                                                  size_t ct_stride, const uint8_t* sk, size_t sk_stride) {
+  using P = Par<N>;
+  constexpr int C = N / 64;
+  __shared__ __align__(16) uint16_t s1[N];
+  __shared__ __align__(16) uint16_t s2[N];
+  __shared__ __align__(16) uint16_t s3[N];
+  const int x = blockIdx.x, l = threadIdx.x;
+  if (x >= n) return;
+  const uint8_t* ctx_ = ct + (size_t)x * ct_stride;
+  unpack14<N>(s1, sk + (size_t)x * sk_stride, l);
+  unpack14<N>(s2, ctx_, l);
+  decompress3<N>(s3, ctx_ + P::POLYBYTES, l);
+  __syncthreads();
+  // tmp = invntt(s o u) - v (k_pointwise(TMP, SHAT, UHAT), k_invntt, k_addsub(TMP, TMP, VP, 1))
+  uint32_t vt[C];
+#pragma unroll
+  for (int r = 0; r < C; ++r) {
+    const int c = out_index<N>(l, r);
+    vt[r] = mont((uint32_t)s1[c] * mont(3186u * (uint32_t)s2[c]));
+  }
+  invntt_regs<N>(vt, s1, l);
+#pragma unroll
+  for (int r = 0; r < C; ++r) vt[r] = (vt[r] + 3 * Q - (uint32_t)s3[out_index<N>(l, r)]) % Q;
+  __syncthreads();
+  put_nat<N>(s2, vt, l);
+  __syncthreads();
+  // poly_tomsg: byte i from the N / 256 copies of coefficients 8 i + j
+  if (l < 32) {
+    uint32_t byte = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int q = 0; q < Par<N>::COPIES; ++q) v += flipabs(s2[8 * l + j + 256 * q]);
+      v = (v - (N == 1024 ? Q : Q / 2)) & 0xFFFFu;
+      byte |= (v >> 15) << j;
+    }
+    m[(size_t)x * m_stride + l] = (uint8_t)byte;
+  }
+}
+
 // ---- CCA-KEM glue: one lane per element for the sponges, flat copies -------------
 
 // out[x] = SHAKE256(in[x], 8 words bytes), first 8 OW bytes
@@ -689,8 +1049,15 @@ struct Launch {
 // (the CCA-KEM's sk holds sk_cpa first), seeds at seed_stride
 template <int N>
 int keypair_impl(pqc_ctx* c, int n, uint8_t* pk, uint8_t* sk, size_t sk_stride, const uint8_t* seed,
-                 size_t seed_stride, hipStream_t s) {
+                 size_t seed_stride, bool plain, hipStream_t s) {
   const Launch L{c->tune.newhope_block};
+  if (!plain) {
+    const size_t n1 = n;
+    LAUNCH(k_seed_expand, n1, n, c->nhbytes, seed, seed_stride);
+    hipLaunchKernelGGL(k_keypair_fused<N>, dim3((unsigned)n), dim3(64), 0, s, n, pk, sk, sk_stride,
+                       (const uint8_t*)c->nhbytes);
+    return hip_err(hipGetLastError());
+  }
   const size_t n1 = n, nS = (size_t)Par<N>::STREAMS * n, n8 = (size_t)n * (N / 8), n4 = (size_t)n * (N / 4);
   Scratch<N> S{c};
   uint8_t* z = c->nhbytes;  // 64 bytes per element: publicseed || noiseseed
@@ -712,7 +1079,12 @@ int keypair_impl(pqc_ctx* c, int n, uint8_t* pk, uint8_t* sk, size_t sk_stride, 
 // cpapke_enc (cpapke.cu:353-429), every buffer at its own stride
 template <int N>
 int enc_impl(pqc_ctx* c, int n, uint8_t* ct, size_t ct_stride, const uint8_t* m, size_t m_stride, const uint8_t* pk,
-             size_t pk_stride, const uint8_t* coin, size_t coin_stride, hipStream_t s) {
+             size_t pk_stride, const uint8_t* coin, size_t coin_stride, bool plain, hipStream_t s) {
+  if (!plain) {
+    hipLaunchKernelGGL(k_enc_fused<N>, dim3((unsigned)n), dim3(64), 0, s, n, ct, ct_stride, m, m_stride, pk, pk_stride,
+                       coin, coin_stride);
+    return hip_err(hipGetLastError());
+  }
   const Launch L{c->tune.newhope_block};
   const size_t nS = (size_t)Par<N>::STREAMS * n, n32 = 32 * (size_t)n, n8 = (size_t)n * (N / 8), n4 = (size_t)n * (N / 4);
   Scratch<N> S{c};
@@ -741,7 +1113,11 @@ int enc_impl(pqc_ctx* c, int n, uint8_t* ct, size_t ct_stride, const uint8_t* m,
 // cpapke_dec (cpapke.cu:443-476), every buffer at its own stride
 template <int N>
 int dec_impl(pqc_ctx* c, int n, uint8_t* m, size_t m_stride, const uint8_t* ct, size_t ct_stride, const uint8_t* sk,
-             size_t sk_stride, hipStream_t s) {
+             size_t sk_stride, bool plain, hipStream_t s) {
+  if (!plain) {
+    hipLaunchKernelGGL(k_dec_fused<N>, dim3((unsigned)n), dim3(64), 0, s, n, m, m_stride, ct, ct_stride, sk, sk_stride);
+    return hip_err(hipGetLastError());
+  }
   const Launch L{c->tune.newhope_block};
   const size_t n32 = 32 * (size_t)n, n8 = (size_t)n * (N / 8), n4 = (size_t)n * (N / 4);
   Scratch<N> S{c};
@@ -764,11 +1140,11 @@ static_assert(KM_CT2 + Par<1024>::CCA_CT == kNewhopeKemBytes, "scratch row size"
 // NewHope CCA-KEM, the NewHope specification's transform over the CPA-PKE
 // (SHAKE256 for every hash).  Keypair coins: CPA seed || s, 64 bytes.
 template <int N>
-int cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+int cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* coins, bool plain, hipStream_t s) {
   using P = Par<N>;
   const Launch L{c->tune.newhope_block};
   const int n = (int)n_;
-  int rc = keypair_impl<N>(c, n, pk, sk, P::CCA_SK, coins, 64, s);
+  int rc = keypair_impl<N>(c, n, pk, sk, P::CCA_SK, coins, 64, plain, s);
   if (rc) return rc;
   LAUNCH(k_copy_words, n_ * (P::PK / 8), n, sk + P::SK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK, P::PK / 8);
   LAUNCH(k_shake256_pair<4>, 2 * n_, n, sk + P::SK + P::PK, (size_t)P::CCA_SK, (const uint8_t*)pk, (size_t)P::PK,
@@ -778,7 +1154,7 @@ int cca_keypair(pqc_ctx* c, size_t n_, uint8_t* pk, uint8_t* sk, const uint8_t* 
 }
 
 template <int N>
-int cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
+int cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins, bool plain,
             hipStream_t s) {
   using P = Par<N>;
   const Launch L{c->tune.newhope_block};
@@ -788,7 +1164,7 @@ int cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, 
   LAUNCH(k_shake256_words<4>, n_, n, km + KM_MU, kb, coins, (size_t)32, 4);              // mu = H(coins)
   LAUNCH(k_shake256_pair<4>, 2 * n_, n, km + KM_HPK, kb, pk, (size_t)P::PK, P::PK / 8);  // H(pk)
   LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
-  int rc = enc_impl<N>(c, n, ct, P::CCA_CT, km + KM_MU, kb, pk, P::PK, km + KM_KCD + SYM, kb, s);
+  int rc = enc_impl<N>(c, n, ct, P::CCA_CT, km + KM_MU, kb, pk, P::PK, km + KM_KCD + SYM, kb, plain, s);
   if (rc) return rc;
   LAUNCH(k_copy_words, n_ * 4, n, ct + P::CT, (size_t)P::CCA_CT, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d
   LAUNCH(k_shake256_pair<4>, 2 * n_, n, km + KM_HC, kb, (const uint8_t*)ct, (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
@@ -797,17 +1173,17 @@ int cca_enc(pqc_ctx* c, size_t n_, uint8_t* ct, uint8_t* ss, const uint8_t* pk, 
 }
 
 template <int N>
-int cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+int cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, bool plain, hipStream_t s) {
   using P = Par<N>;
   const Launch L{c->tune.newhope_block};
   const int n = (int)n_;
   uint8_t* km = c->nhkem;
   const size_t kb = kNewhopeKemBytes;
-  int rc = dec_impl<N>(c, n, km + KM_MU, kb, ct, P::CCA_CT, sk, P::CCA_SK, s);  // m'
+  int rc = dec_impl<N>(c, n, km + KM_MU, kb, ct, P::CCA_CT, sk, P::CCA_SK, plain, s);  // m'
   if (rc) return rc;
   LAUNCH(k_copy_words, n_ * 4, n, km + KM_HPK, kb, sk + P::SK + P::PK, (size_t)P::CCA_SK, 4);
   LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
-  rc = enc_impl<N>(c, n, km + KM_CT2, kb, km + KM_MU, kb, sk + P::SK, P::CCA_SK, km + KM_KCD + SYM, kb, s);
+  rc = enc_impl<N>(c, n, km + KM_CT2, kb, km + KM_MU, kb, sk + P::SK, P::CCA_SK, km + KM_KCD + SYM, kb, plain, s);
   if (rc) return rc;
   LAUNCH(k_copy_words, n_ * 4, n, km + KM_CT2 + P::CT, kb, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d'
   hipLaunchKernelGGL(k_verify_words, dim3((unsigned)n), dim3(64), 0, s, n, reinterpret_cast<uint32_t*>(km + KM_FAIL),
@@ -823,41 +1199,45 @@ int cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t
 
 #define NH_DISPATCH(N, call) ((N) == 1024 ? newhope::call<1024> : newhope::call<512>)
 
-int newhope_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, hipStream_t s) {
+int newhope_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seed, bool plain,
+                    hipStream_t s) {
   if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
   const size_t sks = N == 1024 ? newhope::Par<1024>::SK : newhope::Par<512>::SK;
-  return NH_DISPATCH(N, keypair_impl)(c, (int)n, pk, sk, sks, seed, 32, s);
+  return NH_DISPATCH(N, keypair_impl)(c, (int)n, pk, sk, sks, seed, 32, plain, s);
 }
 
 int newhope_enc(pqc_ctx* c, int N, size_t n, uint8_t* ct, const uint8_t* m, const uint8_t* pk, const uint8_t* coin,
-                hipStream_t s) {
+                bool plain, hipStream_t s) {
   if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
   const size_t cts = N == 1024 ? newhope::Par<1024>::CT : newhope::Par<512>::CT;
   const size_t pks = N == 1024 ? newhope::Par<1024>::PK : newhope::Par<512>::PK;
-  return NH_DISPATCH(N, enc_impl)(c, (int)n, ct, cts, m, 32, pk, pks, coin, 32, s);
+  return NH_DISPATCH(N, enc_impl)(c, (int)n, ct, cts, m, 32, pk, pks, coin, 32, plain, s);
 }
 
-int newhope_dec(pqc_ctx* c, int N, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+int newhope_dec(pqc_ctx* c, int N, size_t n, uint8_t* m, const uint8_t* ct, const uint8_t* sk, bool plain,
+                hipStream_t s) {
   if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
   const size_t cts = N == 1024 ? newhope::Par<1024>::CT : newhope::Par<512>::CT;
   const size_t sks = N == 1024 ? newhope::Par<1024>::SK : newhope::Par<512>::SK;
-  return NH_DISPATCH(N, dec_impl)(c, (int)n, m, 32, ct, cts, sk, sks, s);
+  return NH_DISPATCH(N, dec_impl)(c, (int)n, m, 32, ct, cts, sk, sks, plain, s);
 }
 
-int newhope_cca_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
+int newhope_cca_keypair(pqc_ctx* c, int N, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, bool plain,
+                        hipStream_t s) {
   if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
-  return NH_DISPATCH(N, cca_keypair)(c, n, pk, sk, coins, s);
+  return NH_DISPATCH(N, cca_keypair)(c, n, pk, sk, coins, plain, s);
 }
 
 int newhope_cca_enc(pqc_ctx* c, int N, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
-                    hipStream_t s) {
+                    bool plain, hipStream_t s) {
   if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
-  return NH_DISPATCH(N, cca_enc)(c, n, ct, ss, pk, coins, s);
+  return NH_DISPATCH(N, cca_enc)(c, n, ct, ss, pk, coins, plain, s);
 }
 
-int newhope_cca_dec(pqc_ctx* c, int N, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+int newhope_cca_dec(pqc_ctx* c, int N, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, bool plain,
+                    hipStream_t s) {
   if (N != 512 && N != 1024) return PQC_ERR_INVALID_ARG;
-  return NH_DISPATCH(N, cca_dec)(c, n, ss, ct, sk, s);
+  return NH_DISPATCH(N, cca_dec)(c, n, ss, ct, sk, plain, s);
 }
 
 int newhope_poly_sample(size_t n, uint16_t* polys, const uint8_t* seeds, size_t seed_stride, int nonce,

@@ -236,21 +236,23 @@ int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, con
   int rc = check_call(c, alg, n);
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
   if (a.kind == 4) {  // NewHope CCA-KEM, N = a.k
     switch (op) {
       case OP_KEYPAIR: {
         const Buf b[3] = {{o0, a.pk, true}, {o1, a.sk, true}, {i0, 64, false}};
         return staged(c, n, flags, s, b,
-                      [&](uint8_t** d) { return pqc::newhope_cca_keypair(c, a.k, n, d[0], d[1], d[2], s); });
+                      [&](uint8_t** d) { return pqc::newhope_cca_keypair(c, a.k, n, d[0], d[1], d[2], plain, s); });
       }
       case OP_ENC: {
         const Buf b[4] = {{o0, a.ct, true}, {o1, a.ss, true}, {i0, a.pk, false}, {i1, 32, false}};
         return staged(c, n, flags, s, b,
-                      [&](uint8_t** d) { return pqc::newhope_cca_enc(c, a.k, n, d[0], d[1], d[2], d[3], s); });
+                      [&](uint8_t** d) { return pqc::newhope_cca_enc(c, a.k, n, d[0], d[1], d[2], d[3], plain, s); });
       }
       default: {
         const Buf b[3] = {{o0, a.ss, true}, {i0, a.ct, false}, {i1, a.sk, false}};
-        return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_cca_dec(c, a.k, n, d[0], d[1], d[2], s); });
+        return staged(c, n, flags, s, b,
+                      [&](uint8_t** d) { return pqc::newhope_cca_dec(c, a.k, n, d[0], d[1], d[2], plain, s); });
       }
     }
   }
@@ -271,7 +273,6 @@ int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, con
       }
     }
   }
-  const bool plain = (flags & PQC_F_PLAIN) != 0;
   const int k = a.k;
   switch (op) {
     case OP_KEYPAIR: {
@@ -548,7 +549,9 @@ int pqc_newhope1024_cpa_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{pk, 1824, true}, {sk, 1792, true}, {seeds, 32, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_keypair(c, 1024, n, d[0], d[1], d[2], s); });
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
+  return staged(c, n, flags, s, b,
+                [&](uint8_t** d) { return pqc::newhope_keypair(c, 1024, n, d[0], d[1], d[2], plain, s); });
 }
 
 int pqc_newhope1024_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* msgs, const uint8_t* pk,
@@ -557,8 +560,9 @@ int pqc_newhope1024_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[4] = {{ct, 2176, true}, {msgs, 32, false}, {pk, 1824, false}, {coins, 32, false}};
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
   return staged(c, n, flags, s, b,
-                [&](uint8_t** d) { return pqc::newhope_enc(c, 1024, n, d[0], d[1], d[2], d[3], s); });
+                [&](uint8_t** d) { return pqc::newhope_enc(c, 1024, n, d[0], d[1], d[2], d[3], plain, s); });
 }
 
 int pqc_newhope1024_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uint8_t* ct, const uint8_t* sk,
@@ -567,7 +571,9 @@ int pqc_newhope1024_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uin
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{msgs, 32, true}, {ct, 2176, false}, {sk, 1792, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, 1024, n, d[0], d[1], d[2], s); });
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
+  return staged(c, n, flags, s, b,
+                [&](uint8_t** d) { return pqc::newhope_dec(c, 1024, n, d[0], d[1], d[2], plain, s); });
 }
 
 int pqc_newhope512_cpa_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* seeds,
@@ -576,7 +582,9 @@ int pqc_newhope512_cpa_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t*
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{pk, 928, true}, {sk, 896, true}, {seeds, 32, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_keypair(c, 512, n, d[0], d[1], d[2], s); });
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
+  return staged(c, n, flags, s, b,
+                [&](uint8_t** d) { return pqc::newhope_keypair(c, 512, n, d[0], d[1], d[2], plain, s); });
 }
 
 int pqc_newhope512_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_t* msgs, const uint8_t* pk,
@@ -585,8 +593,9 @@ int pqc_newhope512_cpa_enc_batch(pqc_ctx* c, size_t n, uint8_t* ct, const uint8_
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[4] = {{ct, 1088, true}, {msgs, 32, false}, {pk, 928, false}, {coins, 32, false}};
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
   return staged(c, n, flags, s, b,
-                [&](uint8_t** d) { return pqc::newhope_enc(c, 512, n, d[0], d[1], d[2], d[3], s); });
+                [&](uint8_t** d) { return pqc::newhope_enc(c, 512, n, d[0], d[1], d[2], d[3], plain, s); });
 }
 
 int pqc_newhope512_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uint8_t* ct, const uint8_t* sk,
@@ -595,7 +604,9 @@ int pqc_newhope512_cpa_dec_batch(pqc_ctx* c, size_t n, uint8_t* msgs, const uint
   if (rc || n == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const Buf b[3] = {{msgs, 32, true}, {ct, 1088, false}, {sk, 896, false}};
-  return staged(c, n, flags, s, b, [&](uint8_t** d) { return pqc::newhope_dec(c, 512, n, d[0], d[1], d[2], s); });
+  const bool plain = (flags & PQC_F_PLAIN) != 0;
+  return staged(c, n, flags, s, b,
+                [&](uint8_t** d) { return pqc::newhope_dec(c, 512, n, d[0], d[1], d[2], plain, s); });
 }
 
 int pqc_frodo976_keypair_batch(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* randomness,

@@ -281,27 +281,26 @@ class Context:
         return ss
 
 
-    # ---- NewHope-1024 CPA-PKE (cpapke_keypair/enc/dec with a batch count) -------
     # ---- NewHope CPA-PKE, N = 1024 (default) or 512 -------------------------------
-    def newhope_keypair(self, n, pk, sk, seeds, stream=None, host=False, N=1024):
-        flags = L.F_HOST_PTRS if host else 0
+    def newhope_keypair(self, n, pk, sk, seeds, stream=None, host=False, N=1024, plain=False):
+        flags = (L.F_HOST_PTRS if host else 0) | (L.F_PLAIN if plain else 0)
         check(getattr(L.lib(), "pqc_newhope%d_cpa_keypair_batch" % N)(
             self.handle, n, _ptr(pk), _ptr(sk), _ptr(seeds), flags, stream.handle if stream else None),
             "newhope keypair")
 
-    def newhope_enc(self, n, ct, msgs, pk, coins, stream=None, host=False, N=1024):
-        flags = L.F_HOST_PTRS if host else 0
+    def newhope_enc(self, n, ct, msgs, pk, coins, stream=None, host=False, N=1024, plain=False):
+        flags = (L.F_HOST_PTRS if host else 0) | (L.F_PLAIN if plain else 0)
         check(getattr(L.lib(), "pqc_newhope%d_cpa_enc_batch" % N)(
             self.handle, n, _ptr(ct), _ptr(msgs), _ptr(pk), _ptr(coins), flags, stream.handle if stream else None),
             "newhope enc")
 
-    def newhope_dec(self, n, msgs, ct, sk, stream=None, host=False, N=1024):
-        flags = L.F_HOST_PTRS if host else 0
+    def newhope_dec(self, n, msgs, ct, sk, stream=None, host=False, N=1024, plain=False):
+        flags = (L.F_HOST_PTRS if host else 0) | (L.F_PLAIN if plain else 0)
         check(getattr(L.lib(), "pqc_newhope%d_cpa_dec_batch" % N)(
             self.handle, n, _ptr(msgs), _ptr(ct), _ptr(sk), flags, stream.handle if stream else None),
             "newhope dec")
 
-    def newhope_triples_host(self, seeds, msgs, coins, N=1024):
+    def newhope_triples_host(self, seeds, msgs, coins, N=1024, plain=False):
         """keypair + enc + dec over host arrays; returns (pk, sk, ct, m')."""
         n = seeds.shape[0]
         poly = 14 * N // 8
@@ -310,9 +309,9 @@ class Context:
         ct = np.empty((n, poly + 3 * N // 8), np.uint8)
         m2 = np.empty((n, 32), np.uint8)
         seeds, msgs, coins = (np.ascontiguousarray(x, dtype=np.uint8) for x in (seeds, msgs, coins))
-        self.newhope_keypair(n, pk, sk, seeds, host=True, N=N)
-        self.newhope_enc(n, ct, msgs, pk, coins, host=True, N=N)
-        self.newhope_dec(n, m2, ct, sk, host=True, N=N)
+        self.newhope_keypair(n, pk, sk, seeds, host=True, N=N, plain=plain)
+        self.newhope_enc(n, ct, msgs, pk, coins, host=True, N=N, plain=plain)
+        self.newhope_dec(n, m2, ct, sk, host=True, N=N, plain=plain)
         return pk, sk, ct, m2
 
     # ---- mixed Kyber-512/768/1024 batches (device arenas + descriptors) ----------

@@ -1,5 +1,6 @@
 """GPU parity of the NewHope CPA batched kernels (N = 1024 and 512) and the
-NewHope CCA-KEM against the CPU oracle, bit-exact."""
+NewHope CCA-KEM against the CPU oracle, bit-exact, through both the fused
+per-element kernels (the default) and the per-operation kernels (PQC_F_PLAIN)."""
 import sys
 
 import numpy as np
@@ -15,6 +16,9 @@ def say(*a):
     sys.stdout.flush()
 
 
+PATHS = ["fused", "plain"]
+
+
 @pytest.fixture(scope="module")
 def ctx():
     import pqc_amd
@@ -23,13 +27,14 @@ def ctx():
     c.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("n", [1, 63, 130])
-def test_triples_vs_oracle(ctx, n):
-    say("newhope triples n", n)
+def test_triples_vs_oracle(ctx, n, path):
+    say("newhope triples n", n, path)
     seed = O.drbg_array(n, 32, 0, 700 + n)
     msg = O.drbg_array(n, 32, 0, 701 + n)
     coin = O.drbg_array(n, 32, 0, 702 + n)
-    pk, sk, ct, m2 = ctx.newhope_triples_host(seed, msg, coin)
+    pk, sk, ct, m2 = ctx.newhope_triples_host(seed, msg, coin, plain=path == "plain")
     opk, osk, oct_, om2 = O.newhope_triples(seed, msg, coin)
     assert (pk == opk).all()
     assert (sk == osk).all()
@@ -37,15 +42,16 @@ def test_triples_vs_oracle(ctx, n):
     assert (m2 == om2).all() and (m2 == msg).all()
 
 
-def test_decrypt_garbage_ciphertexts_vs_oracle(ctx):
-    say("newhope garbage ct")
+@pytest.mark.parametrize("path", PATHS)
+def test_decrypt_garbage_ciphertexts_vs_oracle(ctx, path):
+    say("newhope garbage ct", path)
     n = 64
     seed = O.drbg_array(n, 32, 0, 800)
     rng = np.random.default_rng(3)
     pk, sk, _, _ = O.newhope_triples(seed, O.drbg_array(n, 32, 0, 801), O.drbg_array(n, 32, 0, 802))
     ct = rng.integers(0, 256, (n, 2176), dtype=np.uint8)
     m = np.empty((n, 32), np.uint8)
-    ctx.newhope_dec(n, m, ct, sk, host=True)
+    ctx.newhope_dec(n, m, ct, sk, host=True, plain=path == "plain")
     for i in range(n):
         assert m[i].tobytes() == O.newhope_dec(ct[i].tobytes(), sk[i].tobytes())
 
@@ -113,14 +119,16 @@ def _oracle_cca(kc, ec):
     return outs, encs
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("n", [1, 63, 65, 200])
-def test_cca_vs_oracle(cctx, n):
-    say("newhope cca n", n)
+def test_cca_vs_oracle(cctx, n, path):
+    say("newhope cca n", n, path)
+    plain = path == "plain"
     kc = O.drbg_array(n, 64, 0, 900 + n)
     ec = O.drbg_array(n, 32, 0, 901 + n)
-    pk, sk = cctx.keypair_host("newhope1024", kc)
-    ct, ss = cctx.enc_host("newhope1024", pk, ec)
-    ss2 = cctx.dec_host("newhope1024", ct, sk)
+    pk, sk = cctx.keypair_host("newhope1024", kc, plain=plain)
+    ct, ss = cctx.enc_host("newhope1024", pk, ec, plain=plain)
+    ss2 = cctx.dec_host("newhope1024", ct, sk, plain=plain)
     keys, encs = _oracle_cca(kc, ec)
     for i in range(n):
         assert (pk[i].tobytes(), sk[i].tobytes()) == keys[i]
@@ -128,10 +136,11 @@ def test_cca_vs_oracle(cctx, n):
     assert (ss2 == ss).all()
 
 
-def test_cca_implicit_rejection_vs_oracle(cctx):
+@pytest.mark.parametrize("path", PATHS)
+def test_cca_implicit_rejection_vs_oracle(cctx, path):
     """Tampered bytes in u, v and d, and random-byte ciphertexts: the GPU's shared
     secrets equal the oracle's (H(s || H(ct')))."""
-    say("newhope cca rejection")
+    say("newhope cca rejection", path)
     n = 96
     kc = O.drbg_array(n, 64, 0, 950)
     ec = O.drbg_array(n, 32, 0, 951)
@@ -142,7 +151,7 @@ def test_cca_implicit_rejection_vs_oracle(cctx):
     for i in range(n):
         bad[i, pos[i % len(pos)]] ^= 0x10 if i % 3 else 0x01
     bad[-8:] = np.random.default_rng(5).integers(0, 256, (8, 2208), dtype=np.uint8)
-    got = cctx.dec_host("newhope1024", bad, sk)
+    got = cctx.dec_host("newhope1024", bad, sk, plain=path == "plain")
     for i in range(n):
         assert got[i].tobytes() == O.newhope_cca_dec(bad[i].tobytes(), sk[i].tobytes())
     assert (got != ss).any(axis=1).all()
@@ -183,13 +192,14 @@ def ctx512():
     c.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("n", [1, 63, 130])
-def test_newhope512_triples_vs_oracle(ctx512, n):
-    say("newhope512 triples n", n)
+def test_newhope512_triples_vs_oracle(ctx512, n, path):
+    say("newhope512 triples n", n, path)
     seed = O.drbg_array(n, 32, 0, 1700 + n)
     msg = O.drbg_array(n, 32, 0, 1701 + n)
     coin = O.drbg_array(n, 32, 0, 1702 + n)
-    pk, sk, ct, m2 = ctx512.newhope_triples_host(seed, msg, coin, N=512)
+    pk, sk, ct, m2 = ctx512.newhope_triples_host(seed, msg, coin, N=512, plain=path == "plain")
     opk, osk, oct_, om2 = O.newhope_triples(seed, msg, coin, N=512)
     assert (pk == opk).all()
     assert (sk == osk).all()
@@ -197,28 +207,31 @@ def test_newhope512_triples_vs_oracle(ctx512, n):
     assert (m2 == om2).all() and (m2 == msg).all()
 
 
-def test_newhope512_decrypt_garbage_vs_oracle(ctx512):
-    say("newhope512 garbage ct")
+@pytest.mark.parametrize("path", PATHS)
+def test_newhope512_decrypt_garbage_vs_oracle(ctx512, path):
+    say("newhope512 garbage ct", path)
     n = 64
     seed = O.drbg_array(n, 32, 0, 1800)
     pk, sk, _, _ = O.newhope_triples(seed, O.drbg_array(n, 32, 0, 1801), O.drbg_array(n, 32, 0, 1802), N=512)
     ct = np.random.default_rng(4).integers(0, 256, (n, 1088), dtype=np.uint8)
     m = np.empty((n, 32), np.uint8)
-    ctx512.newhope_dec(n, m, ct, sk, host=True, N=512)
+    ctx512.newhope_dec(n, m, ct, sk, host=True, N=512, plain=path == "plain")
     for i in range(n):
         assert m[i].tobytes() == O.newhope_dec(ct[i].tobytes(), sk[i].tobytes(), 512)
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("n", [1, 65, 200])
-def test_newhope512_cca_vs_oracle(ctx512, n):
-    say("newhope512 cca n", n)
+def test_newhope512_cca_vs_oracle(ctx512, n, path):
+    say("newhope512 cca n", n, path)
+    plain = path == "plain"
     kc = O.drbg_array(n, 64, 0, 1900 + n)
     ec = O.drbg_array(n, 32, 0, 1901 + n)
-    pk, sk = ctx512.keypair_host("newhope512", kc)
-    ct, ss = ctx512.enc_host("newhope512", pk, ec)
+    pk, sk = ctx512.keypair_host("newhope512", kc, plain=plain)
+    ct, ss = ctx512.enc_host("newhope512", pk, ec, plain=plain)
     bad = ct.copy()
     bad[::3, 7] ^= 0x20
-    got = ctx512.dec_host("newhope512", bad, sk)
+    got = ctx512.dec_host("newhope512", bad, sk, plain=plain)
     for i in range(n):
         okp, osk = O.newhope_cca_keypair(kc[i].tobytes(), 512)
         assert (pk[i].tobytes(), sk[i].tobytes()) == (okp, osk)
@@ -244,3 +257,33 @@ def test_newhope512_full_size_16384(ctx512):
         opk, _ = O.newhope_keypair(seed[i].tobytes(), 512)
         assert h_pk[i].tobytes() == opk
         assert h_ct[i].tobytes() == O.newhope_enc(msg[i].tobytes(), opk, coin[i].tobytes(), 512)
+
+
+@pytest.mark.parametrize("N", [1024, 512])
+def test_fused_equals_plain_full_size_16384(ctx, ctx512, N):
+    """Every element of a 16384-element batch: the fused kernels' pk, sk, ct and
+    decrypted messages equal the per-operation kernels' (garbage ciphertexts in
+    every 5th slot for the decryption)."""
+    import pqc_amd
+    n = 16384
+    say("newhope fused == plain", N, n)
+    c = ctx if N == 1024 else ctx512
+    P = 14 * N // 8
+    rng = np.random.default_rng(20 + N)
+    seed, msg, coin = (rng.integers(0, 256, (n, 32), dtype=np.uint8) for _ in range(3))
+    d_seed, d_msg, d_coin = (pqc_amd.DeviceBuffer.from_array(x) for x in (seed, msg, coin))
+    out = {}
+    for plain in (False, True):
+        pk, sk, ct, m2 = (pqc_amd.DeviceBuffer(n * b) for b in (P + 32, P, P + 3 * N // 8, 32))
+        c.newhope_keypair(n, pk, sk, d_seed, N=N, plain=plain)
+        c.newhope_enc(n, ct, d_msg, pk, d_coin, N=N, plain=plain)
+        h_ct = ct.download((n, P + 3 * N // 8))
+        h_ct[::5] = rng.integers(0, 256, h_ct[::5].shape, dtype=np.uint8) if not plain else out[False][2][::5]
+        ct.upload(h_ct)
+        c.newhope_dec(n, m2, ct, sk, N=N, plain=plain)
+        out[plain] = (pk.download((n, P + 32)), sk.download((n, P)), h_ct, m2.download((n, 32)))
+    for a, b in zip(out[False], out[True]):
+        assert (a == b).all()
+    good = np.ones(n, bool)
+    good[::5] = False
+    assert (out[False][3][good] == msg[good]).all()
