@@ -17,6 +17,7 @@ Kyber-512/768/1024 requests (1:2:1) through the graph-captured fused kernels;
 loop, fused and plain.  All print one JSON line of the same shape.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -34,7 +35,10 @@ KECCAK_PER_DECAPS = 54         # G 1 + A^T 36 + PRF 7 + H(ct) 9 + KDF 1
 OPS_PER_KECCAK = 7440          # 24 rounds x 155 64-bit ops = 7440 32-bit ops
 NEWHOPE_KECCAK_PER_ENC = 64    # gen_a 16 SHAKE128 streams (one block each) + s', e', e'' 3 x 16 SHAKE256
 BYTES_PER_DECAPS = 1088 + 2400 + 32  # compulsory: ct in, sk in, ss out
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz int32 ops
+# 32-bit vector peak: 256 CUs x 4 SIMDs x 64 lanes per 2 cycles x 2.4 GHz = 78.6 T lane-ops/s, the
+# lane rate of the published 157.3 TFLOP/s FP32 vector peak (one FMA = 2 flops).  valu_issue_check()
+# measures the sustained issue rate on the box (pqc_valu_issue_rate).
+VALU_PEAK_TOPS = 256 * 4 * 64 / 2 * 2.4e9 / 1e12
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per fused launch at batch 65536 from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs, each scaled by the same counter's reading of a 256 MiB
@@ -88,6 +92,22 @@ def timed_run(fn, stream, launches, warmup, pqc):
     b.record(stream)
     b.synchronize()
     return a.elapsed_ms(b) / launches
+
+
+def valu_issue_check(pqc):
+    """Sustained cycles per wave64 instruction per SIMD of four instruction kinds
+    (pqc_valu_issue_rate), and the simple-op rate they imply against VALU_PEAK_TOPS."""
+    out = {}
+    for op, name in enumerate(("v_xor_b32", "v_add_u32", "v_alignbit_b32", "v_mul_lo_u32")):
+        cyc = ctypes.c_double()
+        rc = pqc.lib().pqc_valu_issue_rate(op, ctypes.byref(cyc), None)
+        if rc:
+            raise pqc.PQCError(rc, "pqc_valu_issue_rate")
+        out[name] = round(cyc.value, 3)
+    best = min(out["v_xor_b32"], out["v_add_u32"])
+    out["simple_op_rate_T"] = round(256 * 4 * 64 / best * 2.4e9 / 1e12, 2)
+    log("VALU issue: %s cycles per wave64 instruction per SIMD" % out)
+    return out
 
 
 def host_buffer_rates(pqc, ctx, n, ct, sk, ss_want, reps=3):
@@ -246,6 +266,7 @@ def main():
         log("%s: fused %.3f ms, plain %.3f ms" % (name, wf * 1e3 / tsteps, wp * 1e3 / tsteps))
 
     host_io = host_buffer_rates(pqc, ctx, n, ct, sk, a)
+    valu_check = valu_issue_check(pqc)
     small = small_batch_latency(pqc, ctx, stream, ct, sk, ss_f, a)
 
     ms_f = wall_f * 1e3 / args.steps
@@ -291,6 +312,9 @@ def main():
             "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "Tops/s (int32)",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "peak_note": "256 CUs x 4 SIMDs x 64 lanes / 2 cycles x 2.4 GHz (the FP32 vector peak's lane rate); "
+                         "issue_check: sustained cycles per wave64 instruction measured on this GPU",
+            "issue_check": valu_check,
             "work_per_launch": "%d decaps x 54 Keccak-f x 7440 int32 ops" % n,
             "kernel_ms": round(kern_ms, 4),
             "hbm_GBps_compulsory": round(n * BYTES_PER_DECAPS / (kern_ms / 1e3) / 1e9, 1),
@@ -450,6 +474,7 @@ def bench_newhope(args):
     n = NEWHOPE_BATCH if args.batch == BATCH else args.batch
     ctx = pqc.Context(["newhope1024cpa", "newhope1024", "newhope512cpa", "newhope512"], max_batch=n)
     stream = pqc.Stream()
+    calibrate_copy(args, pqc, stream)
     rng = np.random.default_rng(0)
     seed, msg, coin = (pqc.DeviceBuffer.from_array(rng.integers(0, 256, (n, 32), dtype=np.uint8)) for _ in range(3))
     pk, sk, ct, m2 = pqc.DeviceBuffer(n * 1824), pqc.DeviceBuffer(n * 1792), pqc.DeviceBuffer(n * 2176), \

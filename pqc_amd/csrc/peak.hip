@@ -1,0 +1,86 @@
+// VALU issue-rate microbenchmark: confirms on the device the 32-bit vector peak
+// that the VALU-bound kernels are priced against.  Every wave of a full grid
+// (8 waves per SIMD) runs kIters x 8 groups of eight independent instruction
+// chains of one kind, written as the instructions themselves so that the count
+// is exact; the elapsed time gives the sustained cycles per wave64 instruction
+// per SIMD.  Register-only work: no memory is touched besides one word per lane
+// written at the end.
+#include "internal.h"
+
+namespace pqc {
+namespace {
+
+constexpr int kIters = 2000;
+constexpr int kBlock = 256;        // 4 waves
+constexpr int kBlocksPerCu = 8;    // 8 waves per SIMD
+constexpr int kInstrPerIter = 64;  // 8 groups x 8 chains
+
+#define PQC_CHAINS8(ins, tail)                                                                          \
+  asm volatile(ins " %0, %0, %8" tail "\n\t" ins " %1, %1, %8" tail "\n\t" ins " %2, %2, %8" tail "\n\t" \
+               ins " %3, %3, %8" tail "\n\t" ins " %4, %4, %8" tail "\n\t" ins " %5, %5, %8" tail "\n\t" \
+               ins " %6, %6, %8" tail "\n\t" ins " %7, %7, %8" tail                                      \
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)           \
+               : "v"(b))
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_valu_rate(uint32_t* out, uint32_t seed) {
+  uint32_t a0 = threadIdx.x ^ seed, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3;
+  uint32_t a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
+  const uint32_t b = seed | 1u;
+  for (int it = 0; it < kIters; ++it) {
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      if constexpr (OP == 0) PQC_CHAINS8("v_xor_b32", "");
+      if constexpr (OP == 1) PQC_CHAINS8("v_add_u32", "");
+      if constexpr (OP == 2) PQC_CHAINS8("v_alignbit_b32", ", 7");
+      if constexpr (OP == 3) PQC_CHAINS8("v_mul_lo_u32", "");
+    }
+  }
+  out[(size_t)blockIdx.x * kBlock + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
+#undef PQC_CHAINS8
+
+}  // namespace
+}  // namespace pqc
+
+extern "C" int pqc_valu_issue_rate(int op, double* cycles_per_instr, void* stream) {
+  using namespace pqc;
+  if (!cycles_per_instr || op < 0 || op > 3) return PQC_ERR_INVALID_ARG;
+  int dev = 0, cus = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, dev) != hipSuccess || cus <= 0 || khz <= 0)
+    return PQC_ERR_NO_DEVICE;
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = cus * kBlocksPerCu;
+  uint32_t* out = nullptr;
+  if (hipMalloc(&out, (size_t)blocks * kBlock * sizeof(uint32_t)) != hipSuccess) return PQC_ERR_NO_MEMORY;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PQC_OK;
+  float best = 0.f;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PQC_ERR_DEVICE;
+  for (int rep = 0; !rc && rep < 3; ++rep) {  // the first launch also warms the clocks up
+    if (hipEventRecord(e0, s) != hipSuccess) rc = PQC_ERR_DEVICE;
+    if (!rc) {
+      auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, out, 7u); };
+      if (op == 0) launch(k_valu_rate<0>);
+      if (op == 1) launch(k_valu_rate<1>);
+      if (op == 2) launch(k_valu_rate<2>);
+      if (op == 3) launch(k_valu_rate<3>);
+      if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess)
+        rc = PQC_ERR_DEVICE;
+    }
+    float ms = 0.f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PQC_ERR_DEVICE;
+    if (!rc && (rep == 0 || ms < best)) best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  if (rc) return rc;
+  // wave64 instructions per SIMD (4 SIMDs per CU) over the elapsed cycles
+  const double per_simd = (double)blocks * (kBlock / 64) * kIters * kInstrPerIter / (4.0 * cus);
+  *cycles_per_instr = (double)best * 1e-3 * (double)khz * 1e3 / per_simd;
+  return PQC_OK;
+}
