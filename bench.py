@@ -94,19 +94,27 @@ def timed_run(fn, stream, launches, warmup, pqc):
     return a.elapsed_ms(b) / launches
 
 
+VALU_OPS = ("v_xor_b32", "v_add_u32", "v_alignbit_b32", "v_mul_lo_u32", "v_bitop3_b32", "v_lshlrev_b32",
+            "v_mad_u32_u24")
+
+
 def valu_issue_check(pqc):
-    """Sustained cycles per wave64 instruction per SIMD of four instruction kinds
-    (pqc_valu_issue_rate), and the simple-op rate they imply against VALU_PEAK_TOPS."""
+    """Sustained cycles per wave64 instruction per SIMD of seven instruction kinds
+    at 8 waves per SIMD and at 1 (pqc_valu_issue_rate), and the simple-op rate the
+    8-wave figures imply against VALU_PEAK_TOPS."""
     out = {}
-    for op, name in enumerate(("v_xor_b32", "v_add_u32", "v_alignbit_b32", "v_mul_lo_u32")):
-        cyc = ctypes.c_double()
-        rc = pqc.lib().pqc_valu_issue_rate(op, ctypes.byref(cyc), None)
-        if rc:
-            raise pqc.PQCError(rc, "pqc_valu_issue_rate")
-        out[name] = round(cyc.value, 3)
-    best = min(out["v_xor_b32"], out["v_add_u32"])
+    for wps in (8, 1):
+        row = {}
+        for op, name in enumerate(VALU_OPS):
+            cyc = ctypes.c_double()
+            rc = pqc.lib().pqc_valu_issue_rate(op, wps, ctypes.byref(cyc), None)
+            if rc:
+                raise pqc.PQCError(rc, "pqc_valu_issue_rate")
+            row[name] = round(cyc.value, 2)
+        out["cycles_%d_waves_per_simd" % wps] = row
+    best = min(out["cycles_8_waves_per_simd"]["v_xor_b32"], out["cycles_8_waves_per_simd"]["v_add_u32"])
     out["simple_op_rate_T"] = round(256 * 4 * 64 / best * 2.4e9 / 1e12, 2)
-    log("VALU issue: %s cycles per wave64 instruction per SIMD" % out)
+    log("VALU issue (cycles per wave64 instruction per SIMD): %s" % out)
     return out
 
 

@@ -423,11 +423,12 @@ int pqc_newhope1024_poly_sample(size_t n, uint16_t *polys, const uint8_t *seeds,
 int pqc_frodo976_gemm(pqc_ctx *ctx, size_t n, const uint8_t *ct, void *stream);
 /* VALU issue-rate microbenchmark (no reference counterpart): sustained cycles
  * per wave64 instruction per SIMD of one instruction kind, op 0 = v_xor_b32,
- * 1 = v_add_u32, 2 = v_alignbit_b32, 3 = v_mul_lo_u32, over a full grid at 8
- * waves per SIMD on the current device (best of three launches, synchronous on
- * the stream).  It checks on the device the 32-bit vector peak the VALU-bound
- * kernels are priced against.                                                  */
-int pqc_valu_issue_rate(int op, double *cycles_per_instr, void *stream);
+ * 1 = v_add_u32, 2 = v_alignbit_b32, 3 = v_mul_lo_u32, 4 = v_bitop3_b32,
+ * 5 = v_lshlrev_b32, 6 = v_mad_u32_u24, eight independent chains per wave, over
+ * a full grid at waves_per_simd (1..8) waves per SIMD on the current device
+ * (best of three launches, synchronous on the stream).  It checks on the device
+ * the 32-bit vector peak the VALU-bound kernels are priced against.            */
+int pqc_valu_issue_rate(int op, int waves_per_simd, double *cycles_per_instr, void *stream);
 
 const char *pqc_version(void);
 const char *pqc_strerror(int code);

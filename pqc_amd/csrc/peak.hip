@@ -1,18 +1,18 @@
 // VALU issue-rate microbenchmark: confirms on the device the 32-bit vector peak
-// that the VALU-bound kernels are priced against.  Every wave of a full grid
-// (8 waves per SIMD) runs kIters x 8 groups of eight independent instruction
-// chains of one kind, written as the instructions themselves so that the count
-// is exact; the elapsed time gives the sustained cycles per wave64 instruction
-// per SIMD.  Register-only work: no memory is touched besides one word per lane
-// written at the end.
+// that the VALU-bound kernels are priced against, and how it depends on the
+// number of waves per SIMD.  Every wave of a full grid (1 to 8 waves per SIMD)
+// runs kIters x 8 groups of eight independent instruction chains of one kind,
+// written as the instructions themselves so that the count is exact; the
+// elapsed time gives the sustained cycles per wave64 instruction per SIMD.
+// Register-only work: no memory is touched besides one word per lane written
+// at the end.
 #include "internal.h"
 
 namespace pqc {
 namespace {
 
 constexpr int kIters = 2000;
-constexpr int kBlock = 256;        // 4 waves
-constexpr int kBlocksPerCu = 8;    // 8 waves per SIMD
+constexpr int kBlock = 64;         // one wave per workgroup: waves_per_simd x 4 workgroups per CU
 constexpr int kInstrPerIter = 64;  // 8 groups x 8 chains
 
 #define PQC_CHAINS8(ins, tail)                                                                          \
@@ -20,13 +20,13 @@ constexpr int kInstrPerIter = 64;  // 8 groups x 8 chains
                ins " %3, %3, %8" tail "\n\t" ins " %4, %4, %8" tail "\n\t" ins " %5, %5, %8" tail "\n\t" \
                ins " %6, %6, %8" tail "\n\t" ins " %7, %7, %8" tail                                      \
                : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)           \
-               : "v"(b))
+               : "v"(b), "v"(c))
 
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_valu_rate(uint32_t* out, uint32_t seed) {
   uint32_t a0 = threadIdx.x ^ seed, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3;
   uint32_t a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
-  const uint32_t b = seed | 1u;
+  const uint32_t b = seed | 1u, c = seed * 3u;
   for (int it = 0; it < kIters; ++it) {
 #pragma unroll
     for (int g = 0; g < 8; ++g) {
@@ -34,6 +34,9 @@ __global__ __launch_bounds__(kBlock) void k_valu_rate(uint32_t* out, uint32_t se
       if constexpr (OP == 1) PQC_CHAINS8("v_add_u32", "");
       if constexpr (OP == 2) PQC_CHAINS8("v_alignbit_b32", ", 7");
       if constexpr (OP == 3) PQC_CHAINS8("v_mul_lo_u32", "");
+      if constexpr (OP == 4) PQC_CHAINS8("v_bitop3_b32", ", %9 bitop3:0x96");
+      if constexpr (OP == 5) PQC_CHAINS8("v_lshlrev_b32", "");
+      if constexpr (OP == 6) PQC_CHAINS8("v_mad_u32_u24", ", %9");
     }
   }
   out[(size_t)blockIdx.x * kBlock + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
@@ -43,16 +46,16 @@ __global__ __launch_bounds__(kBlock) void k_valu_rate(uint32_t* out, uint32_t se
 }  // namespace
 }  // namespace pqc
 
-extern "C" int pqc_valu_issue_rate(int op, double* cycles_per_instr, void* stream) {
+extern "C" int pqc_valu_issue_rate(int op, int waves_per_simd, double* cycles_per_instr, void* stream) {
   using namespace pqc;
-  if (!cycles_per_instr || op < 0 || op > 3) return PQC_ERR_INVALID_ARG;
+  if (!cycles_per_instr || op < 0 || op > 6 || waves_per_simd < 1 || waves_per_simd > 8) return PQC_ERR_INVALID_ARG;
   int dev = 0, cus = 0, khz = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, dev) != hipSuccess || cus <= 0 || khz <= 0)
     return PQC_ERR_NO_DEVICE;
   hipStream_t s = (hipStream_t)stream;
-  const int blocks = cus * kBlocksPerCu;
+  const int blocks = cus * 4 * waves_per_simd;
   uint32_t* out = nullptr;
   if (hipMalloc(&out, (size_t)blocks * kBlock * sizeof(uint32_t)) != hipSuccess) return PQC_ERR_NO_MEMORY;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -67,6 +70,9 @@ extern "C" int pqc_valu_issue_rate(int op, double* cycles_per_instr, void* strea
       if (op == 1) launch(k_valu_rate<1>);
       if (op == 2) launch(k_valu_rate<2>);
       if (op == 3) launch(k_valu_rate<3>);
+      if (op == 4) launch(k_valu_rate<4>);
+      if (op == 5) launch(k_valu_rate<5>);
+      if (op == 6) launch(k_valu_rate<6>);
       if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
           hipEventSynchronize(e1) != hipSuccess)
         rc = PQC_ERR_DEVICE;
@@ -80,7 +86,7 @@ extern "C" int pqc_valu_issue_rate(int op, double* cycles_per_instr, void* strea
   (void)hipFree(out);
   if (rc) return rc;
   // wave64 instructions per SIMD (4 SIMDs per CU) over the elapsed cycles
-  const double per_simd = (double)blocks * (kBlock / 64) * kIters * kInstrPerIter / (4.0 * cus);
+  const double per_simd = (double)waves_per_simd * kIters * kInstrPerIter;
   *cycles_per_instr = (double)best * 1e-3 * (double)khz * 1e3 / per_simd;
   return PQC_OK;
 }
