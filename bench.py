@@ -47,6 +47,9 @@ MEASURED_TRAFFIC_BYTES_B65536 = 517.7e6
 # HBM bytes per Frodo-976 decapsulation GEMM launch at batch 4096, same method
 # (profiles/frodo976_decaps_b4096.md): 192.3 MB read + 0.07 MB written.
 MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 192.4e6
+# HBM bytes per NewHope-1024 fused encryption launch at batch 16384, same method
+# (profiles/newhope1024_cpa_triples_b16384.md): 35.86 MB read + 35.65 MB written.
+MEASURED_TRAFFIC_NEWHOPE_ENC_B16384 = 71.5e6
 
 
 def log(*a):
@@ -555,7 +558,9 @@ def bench_newhope(args):
                      "work_per_launch": "%d encryptions x %d Keccak-f x 7440 two-input 32-bit ops (the sponge work "
                                         "only: NTT, pointwise and packing work is not counted, so frac is a lower "
                                         "bound)" % (n, NEWHOPE_KECCAK_PER_ENC),
-                     "kernel_ms": round(enc_ms, 4), "traffic": None},
+                     "kernel_ms": round(enc_ms, 4),
+                     "traffic": MEASURED_TRAFFIC_NEWHOPE_ENC_B16384 if n == NEWHOPE_BATCH else None,
+                     "traffic_compulsory": n * (1824 + 32 + 32 + 2176)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
