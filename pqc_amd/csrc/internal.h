@@ -31,8 +31,12 @@ struct pqc_ctx {
   // and 64 bytes per element (seed expansion)
   uint16_t* nhpolys = nullptr;
   uint8_t* nhbytes = nullptr;
-  // NewHope-1024 CCA-KEM: per-element row of hashes, flags and the re-encryption
+  // NewHope-1024 CCA-KEM: per-element row of hashes, flags and the re-encryption;
+  // a side stream (default priority) and two events fork the ciphertext hash of
+  // decapsulation off the caller's stream and join it back
   uint8_t* nhkem = nullptr;
+  hipStream_t nh_side = nullptr;
+  hipEvent_t nh_fork = nullptr, nh_join = nullptr;
 
   // FrodoKEM-976: device-resident key, expanded A (uint16 and MFMA digit
   // fragments), per-element scratch (S' fragments, E', E'', GEMM output, mu, k,

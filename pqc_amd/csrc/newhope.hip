@@ -1179,16 +1179,30 @@ int cca_dec(pqc_ctx* c, size_t n_, uint8_t* ss, const uint8_t* ct, const uint8_t
   const int n = (int)n_;
   uint8_t* km = c->nhkem;
   const size_t kb = kNewhopeKemBytes;
+  // H(c) depends only on the input ciphertext: it runs on the context's side
+  // stream (two lanes per element, half a wave per SIMD at 16384 elements)
+  // beside the decryption and re-encryption, and joins before the key selection
+  if (hipEventRecord(c->nh_fork, s) != hipSuccess || hipStreamWaitEvent(c->nh_side, c->nh_fork, 0) != hipSuccess)
+    return PQC_ERR_DEVICE;
+  hipLaunchKernelGGL(k_shake256_pair<4>, L.grid(2 * n_), dim3(L.block), 0, c->nh_side, n, km + KM_HC, kb, ct,
+                     (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
+  if (hipGetLastError() != hipSuccess || hipEventRecord(c->nh_join, c->nh_side) != hipSuccess) return PQC_ERR_DEVICE;
   int rc = dec_impl<N>(c, n, km + KM_MU, kb, ct, P::CCA_CT, sk, P::CCA_SK, plain, s);  // m'
-  if (rc) return rc;
+  if (rc) {
+    (void)hipStreamWaitEvent(s, c->nh_join, 0);
+    return rc;
+  }
   LAUNCH(k_copy_words, n_ * 4, n, km + KM_HPK, kb, sk + P::SK + P::PK, (size_t)P::CCA_SK, 4);
   LAUNCH(k_shake256_2x32<12>, n_, n, km + KM_KCD, kb, (const uint8_t*)km + KM_MU, kb, (const uint8_t*)km + KM_HPK, kb);
   rc = enc_impl<N>(c, n, km + KM_CT2, kb, km + KM_MU, kb, sk + P::SK, P::CCA_SK, km + KM_KCD + SYM, kb, plain, s);
-  if (rc) return rc;
+  if (rc) {
+    (void)hipStreamWaitEvent(s, c->nh_join, 0);
+    return rc;
+  }
   LAUNCH(k_copy_words, n_ * 4, n, km + KM_CT2 + P::CT, kb, (const uint8_t*)km + KM_KCD + 2 * SYM, kb, 4);  // d'
   hipLaunchKernelGGL(k_verify_words, dim3((unsigned)n), dim3(64), 0, s, n, reinterpret_cast<uint32_t*>(km + KM_FAIL),
                      kb, ct, (size_t)P::CCA_CT, (const uint8_t*)km + KM_CT2, kb, P::CCA_CT / 8);
-  LAUNCH(k_shake256_pair<4>, 2 * n_, n, km + KM_HC, kb, ct, (size_t)P::CCA_CT, P::CCA_CT / 8);  // H(c)
+  if (hipStreamWaitEvent(s, c->nh_join, 0) != hipSuccess) return PQC_ERR_DEVICE;
   LAUNCH(k_cmov32, n_ * 4, n, km + KM_KCD, kb, sk + P::CCA_SK - SYM, (size_t)P::CCA_SK,
          reinterpret_cast<const uint32_t*>(km + KM_FAIL), kb);
   LAUNCH(k_shake256_2x32<4>, n_, n, ss, (size_t)32, (const uint8_t*)km + KM_KCD, kb, (const uint8_t*)km + KM_HC, kb);

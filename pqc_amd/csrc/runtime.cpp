@@ -384,6 +384,12 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     rc = ctx_alloc(c, &p, (size_t)pqc::kNewhopeKemBytes * c->ld);
     if (rc) goto fail;
     c->nhkem = (uint8_t*)p;
+    if (hipStreamCreateWithFlags(&c->nh_side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->nh_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->nh_join, hipEventDisableTiming) != hipSuccess) {
+      rc = PQC_ERR_DEVICE;
+      goto fail;
+    }
   }
   if (alg_mask & PQC_ALG_MASK(PQC_ALG_FRODO976)) {
     rc = ctx_alloc(c, &p, (size_t)976 * 976 * 2);
@@ -442,6 +448,10 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->nhpolys) (void)hipFree(c->nhpolys);
   if (c->nhbytes) (void)hipFree(c->nhbytes);
   if (c->nhkem) (void)hipFree(c->nhkem);
+  if (c->nh_side) (void)hipStreamSynchronize(c->nh_side);
+  if (c->nh_fork) (void)hipEventDestroy(c->nh_fork);
+  if (c->nh_join) (void)hipEventDestroy(c->nh_join);
+  if (c->nh_side) (void)hipStreamDestroy(c->nh_side);
   if (c->stage) (void)hipFree(c->stage);
   pqc::hostio_free(c);
   if (c->status) (void)hipFree(c->status);
