@@ -76,7 +76,8 @@ typedef struct pqc_tuning {
   int32_t plain_block;        /* threads per workgroup, plain per-op Kyber kernels: 64..1024, multiple of 64 */
   int32_t newhope_block;      /* same for the NewHope-1024 kernels */
   int32_t fused_waves_per_cu; /* fused persistent grid = CUs x this: 1..4 (LDS holds 4) */
-  int32_t mixed_order;        /* mixed-batch tile order: 0 = K 4,3,2; 1 = K 2,3,4; 2 = round-robin over K */
+  int32_t mixed_order;        /* mixed-batch tile order: 0 = K 4,3,2; 1 = K 2,3,4; 2 = round-robin over K;
+                                 3 = K 4,3,2 with 32-request tiles for K = 4 and K = 2 */
   int32_t frodo_stream_block; /* threads per workgroup of the Frodo noise-stream kernel: 64, 128 or 256 */
   int32_t lowlat_max;         /* Kyber batches of at most this many requests run one four-wave
                                  workgroup per request (latency) instead of 64-request tiles

@@ -1041,6 +1041,14 @@ union LdsMixed {
   Lds<Tile<4>::RING> k4;
 };
 
+// Requests per tile of each bucket.  Order 3 (split) halves the K = 4 and K = 2
+// tiles and walks K 4, 3, 2: with one tile per wave the launch lasts as long as
+// its heaviest tile, and a K = 4 half tile plus a K = 2 half tile on one wave is
+// about one K = 3 tile (the per-request sponges of a half tile run on half the
+// lanes, its Aᵀ rounds and cooperative work halve).  The autotuner measures it
+// against the other orders; a tile of fewer requests is a partial tile.
+__host__ __device__ __forceinline__ int mixed_tile_req(int order, int k) { return (order == 3 && k != 3) ? 32 : 64; }
+
 template <bool DEC>
 __global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t* __restrict__ counts, int order,
                                                     uint32_t* __restrict__ status) {
@@ -1048,15 +1056,17 @@ __global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t
   LaneZ z;
   load_lane_z(z, &c_lane_tw);
   const int n2 = (int)counts[0], n3 = (int)counts[1], n4 = (int)counts[2];
-  const int t4 = (n4 + 63) / 64, t3 = (n3 + 63) / 64, t2 = (n2 + 63) / 64;
+  const int s4 = mixed_tile_req(order, 4), s3 = mixed_tile_req(order, 3), s2 = mixed_tile_req(order, 2);
+  const int t4 = (n4 + s4 - 1) / s4, t3 = (n3 + s3 - 1) / s3, t2 = (n2 + s2 - 1) / s2;
+  const int walk = order == 3 ? 0 : order;
   for (int tile = blockIdx.x; tile < t4 + t3 + t2; tile += gridDim.x) {
-    const TilePick p = pick_tile(tile, order, t4, t3, t2);
+    const TilePick p = pick_tile(tile, walk, t4, t3, t2);
     if (p.k == 4)
-      fused_tile<4, DEC>(src, p.idx * 64, n4, sh.k4, z, status);
+      fused_tile<4, DEC>(src, p.idx * s4, min(n4, (p.idx + 1) * s4), sh.k4, z, status);
     else if (p.k == 3)
-      fused_tile<3, DEC>(src, p.idx * 64, n3, sh.k3, z, status);
+      fused_tile<3, DEC>(src, p.idx * s3, min(n3, (p.idx + 1) * s3), sh.k3, z, status);
     else if (p.k == 2)
-      fused_tile<2, DEC>(src, p.idx * 64, n2, sh.k2, z, status);
+      fused_tile<2, DEC>(src, p.idx * s2, min(n2, (p.idx + 1) * s2), sh.k2, z, status);
   }
 }
 
@@ -1240,7 +1250,7 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
                      (uint64_t)ct_bytes, (uint64_t)key_bytes, (uint64_t)ss_bytes, (uint64_t)coins_bytes,
                      c->kmix_order, c->kmix_counts, c->status);
   const MixedSrc src{reqs, c->kmix_order, cap, ss, ct, key, coins, c->kprf, c->kprf_per_elem};
-  const int max_tiles = (int)((n + 63) / 64) + 2;  // buckets round up separately
+  const int max_tiles = (int)((n + 31) / 32) + 2;  // half tiles at most; buckets round up separately
   int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;
   if (grid > max_tiles) grid = max_tiles;
   if (grid < 1) grid = 1;

@@ -21,7 +21,7 @@
 namespace pqc {
 namespace tuner {
 
-constexpr int kFeatures = 13;
+constexpr int kFeatures = 14;
 
 struct Samples {
   std::vector<float> x, y;  // kFeatures per sample; log(ms)
@@ -30,7 +30,7 @@ struct Samples {
 
 static float log2f_safe(double v) { return (float)std::log2(v > 1.0 ? v : 1.0); }
 
-// one-hot family (5), K / 4, op / 2, log2(n) / 17, log2(block) / 10, waves / 4, order one-hot (3)
+// one-hot family (5), K / 4, op / 2, log2(n) / 17, log2(block) / 10, waves / 4, order one-hot (4)
 static void features(const pqc_tune_request& r, const pqc_tuning& t, float* f) {
   for (int i = 0; i < kFeatures; ++i) f[i] = 0.f;
   if (r.family >= 0 && r.family < 5) f[r.family] = 1.f;
@@ -77,7 +77,7 @@ static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_t
       break;
     case PQC_TUNE_MIXED_KYBER:
       for (int w = 1; w <= 4; ++w)
-        for (int o = 0; o < 3; ++o) {
+        for (int o = 0; o < 4; ++o) {
           pqc_tuning t = base;
           t.fused_waves_per_cu = w;
           t.mixed_order = o;

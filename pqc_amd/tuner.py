@@ -9,7 +9,7 @@ from . import _lib as L
 from ._lib import check
 
 FAMILIES = {"plain_kyber": 0, "fused_kyber": 1, "mixed_kyber": 2, "frodo": 3, "newhope": 4}
-N_FEATURES = 13
+N_FEATURES = 14
 
 
 class Tuning(C.Structure):

@@ -199,3 +199,43 @@ def test_graph_replay_mixed_dec(pqc, ctx, path):
         assert (out.download((lay.nbytes["ss"],)) == want).all(), rep
     g.destroy()
     assert ctx.take_status() == 0
+
+
+@pytest.mark.parametrize("n", [3000, 65536])
+def test_mixed_tile_orders_agree(pqc, ctx, n):
+    """Every tile order (0: K 4,3,2; 1: K 2,3,4; 2: round-robin; 3: K 4,3,2 with
+    32-request tiles for K = 4 and K = 2) gives the same ciphertexts and shared
+    secrets, through the tile kernels (lowlat_max = -1), at ragged and full
+    sizes; tampered ciphertexts included."""
+    from pqc_amd import tuner as T
+    say("mixed tile orders n", n)
+    lay = pqc.MixedLayout(mixed_algs(n, 30 + n), np.random.default_rng(31))
+    ar = pqc.DeviceArenas(lay)
+    fill_keys_on_gpu(pqc, ctx, lay, ar, 32)
+    offs = lay.offsets("ct")
+    ref = None
+    try:
+        for order in range(4):
+            say(" order", order)
+            T.put_tuning(ctx, lowlat_max=-1, mixed_order=order)
+            ar.buf["ct"].upload(np.zeros(lay.nbytes["ct"], np.uint8))
+            ctx.kem_enc_mixed(n, ar.enc_reqs, ar.buf["ct"], ar.buf["pk"], ar.buf["ss"], ar.buf["coins"])
+            ct, ss = ar.download("ct"), ar.download("ss")
+            bad = ct.copy()
+            for i in range(0, n, 5):
+                bad[int(offs[i]) + 11] ^= 0x40
+            ar.buf["ct"].upload(bad)
+            ar.buf["ss"].upload(np.zeros(lay.nbytes["ss"], np.uint8))
+            ctx.kem_dec_mixed(n, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ar.buf["ss"])
+            got = (ct, ss, ar.download("ss"))
+            assert ctx.take_status() == 0
+            if ref is None:
+                ref = got
+            else:
+                for a, b in zip(ref, got):
+                    assert (a == b).all(), order
+    finally:
+        T.put_tuning(ctx, lowlat_max=0, mixed_order=0)
+    sso = lay.offsets("ss")
+    same = np.array([(ref[2][int(o):int(o) + 32] == ref[1][int(o):int(o) + 32]).all() for o in sso])
+    assert (same == (np.arange(n) % 5 != 0)).all()
