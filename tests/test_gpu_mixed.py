@@ -201,7 +201,7 @@ def test_graph_replay_mixed_dec(pqc, ctx, path):
     assert ctx.take_status() == 0
 
 
-@pytest.mark.parametrize("n", [3000, 65536])
+@pytest.mark.parametrize("n", [5, 65, 3000, 65536])
 def test_mixed_tile_orders_agree(pqc, ctx, n):
     """Every tile order (0: K 4,3,2; 1: K 2,3,4; 2: round-robin; 3: K 4,3,2 with
     32-request tiles for K = 4 and K = 2) gives the same ciphertexts and shared
