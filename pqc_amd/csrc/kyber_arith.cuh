@@ -59,10 +59,28 @@ __device__ __forceinline__ int32_t csubq(int32_t a) {
   return a + ((a >> 31) & Q);
 }
 
+// floor(n / q) for n <= kDivQMax as (n * m) >> 33, m = ceil(2^33 / q), from
+// the 24-bit high product: two instructions where the compiler's division by
+// a constant takes five.  Exact: with e = m q - 2^33, n m / 2^33 = n / q +
+// n e / (q 2^33), and n e < 2^33 keeps the floor.  Written as the instruction
+// itself because the compiler does not form v_mul_hi_u32_u24 from C.
+constexpr uint32_t kDivQM = (uint32_t)(((1ull << 33) + Q - 1) / Q);
+constexpr uint32_t kDivQMax = (uint32_t)((Q - 1) << 11) + Q / 2;  // compress_d's largest numerator (d <= 11)
+static_assert(kDivQM < (1u << 24) && kDivQMax < (1u << 24), "24-bit operands");
+static_assert((uint64_t)((uint64_t)kDivQM * Q - (1ull << 33)) * kDivQMax < (1ull << 33), "exact division");
+static_assert(kDivQM == 0x275F6Fu, "literal below");
+
+__device__ __forceinline__ uint32_t div_q(uint32_t n) {
+  uint32_t hi;
+  asm("v_mul_hi_u32_u24 %0, 0x275f6f, %1" : "=v"(hi) : "v"(n));
+  return hi >> 1;
+}
+
 // round(x * 2^d / q) mod 2^d for x in [0, q) (poly.cu:34, polyvec.cu:33).
 template <int D>
 __device__ __forceinline__ uint32_t compress_d(int32_t x) {
-  return ((((uint32_t)x << D) + Q / 2) / (uint32_t)Q) & ((1u << D) - 1);
+  static_assert(D >= 1 && D <= 11, "numerator bound");
+  return div_q(((uint32_t)x << D) + Q / 2) & ((1u << D) - 1);
 }
 
 // (t q + 2^(d-1)) >> d (poly.cu:59-66, polyvec.cu:71-78).
