@@ -95,6 +95,25 @@ __device__ __forceinline__ const Tables<1024>& tab<1024>() { return c_tab1024; }
 template <>
 __device__ __forceinline__ const Tables<512>& tab<512>() { return c_tab512; }
 
+// floor(n / q) for n < 2^21: the high word of the 24-bit product n m with
+// m = ceil(2^32 / q), one v_mul_hi_u32_u24 where the compiler's division by a
+// constant takes five instructions.  Exact: n m / 2^32 = n / q + n e / (q 2^32)
+// with e = m q - 2^32, and n e < 2^32.  Every operand here is below 2^17 (sums of
+// two uint16 values, or of a uint16 value and 3q).  Written as the instruction
+// itself because the compiler does not form it from C.
+constexpr uint32_t kDivQM = (uint32_t)(((1ull << 32) + Q - 1) / Q);
+static_assert(kDivQM < (1u << 24), "24-bit operand");
+static_assert((((uint64_t)kDivQM * Q - (1ull << 32)) << 21) < (1ull << 32), "exact below 2^21");
+static_assert(kDivQM == 0x55539u, "literal below");
+
+__device__ __forceinline__ uint32_t div_q(uint32_t n) {
+  uint32_t r;
+  asm("v_mul_hi_u32_u24 %0, 0x55539, %1" : "=v"(r) : "v"(n));
+  return r;
+}
+
+__device__ __forceinline__ uint32_t mod_q(uint32_t n) { return n - div_q(n) * Q; }
+
 // reduce.cu:22-31
 __device__ __forceinline__ uint32_t mont(uint32_t a) {
   uint32_t u = (a * 12287u) & ((1u << 18) - 1);
@@ -208,7 +227,7 @@ __global__ void k_sample(int n, Pv<N> r, const uint8_t* seed, size_t stride, int
 template <int MODE>
 __device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t W) {
   const uint32_t t = a, h = b;
-  a = MODE == 0 ? ((t + h) & 0xFFFFu) : MODE == 1 ? barrett(t + h) : (t + h) % Q;
+  a = MODE == 0 ? ((t + h) & 0xFFFFu) : MODE == 1 ? barrett(t + h) : mod_q(t + h);
   b = mont(W * (t + 3 * Q - h));
 }
 
@@ -433,7 +452,7 @@ __global__ void k_addsub(int n, Pv<N> r, Pv<N> a, Pv<N> b, int subtract) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const uint32_t x = (aw[k] >> (16 * e)) & 0xFFFFu, y = (bw[k] >> (16 * e)) & 0xFFFFu;
-      h[e] = (subtract ? x + 3 * Q - y : x + y) % Q;
+      h[e] = mod_q(subtract ? x + 3 * Q - y : x + y);
     }
     o[k] = h[0] | (h[1] << 16);
   }
@@ -444,7 +463,7 @@ __global__ void k_addsub(int n, Pv<N> r, Pv<N> a, Pv<N> b, int subtract) {
 
 // coeff_freeze (poly.cu:21-33)
 __device__ __forceinline__ uint32_t freeze(uint32_t x) {
-  const uint32_t r = x % Q;
+  const uint32_t r = mod_q(x & 0xFFFFu);
   const uint32_t m = (r - Q) & 0xFFFFu;
   const uint32_t c = (uint32_t)((int32_t)(int16_t)m >> 15) & 0xFFFFu;
   return (m ^ ((r ^ m) & c)) & 0xFFFFu;
@@ -508,7 +527,7 @@ __global__ void k_compress(int n, uint8_t* ct, size_t ct_stride, Pv<N> p) {
   const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
   uint32_t v[8];
 #pragma unroll
-  for (int j = 0; j < 8; j++) v[j] = (((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) / Q) & 7;
+  for (int j = 0; j < 8; j++) v[j] = div_q((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) & 7;
   uint8_t* r = ct + (size_t)x * ct_stride + Par<N>::POLYBYTES + 3 * g;
   r[0] = (uint8_t)(v[0] | (v[1] << 3) | (v[2] << 6));
   r[1] = (uint8_t)((v[2] >> 2) | (v[3] << 1) | (v[4] << 4) | (v[5] << 7));
@@ -743,7 +762,7 @@ __device__ __forceinline__ void compress3(uint8_t* bb, const uint16_t* sm, int l
     const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
     uint32_t v[8];
 #pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = (((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) / Q) & 7;
+    for (int j = 0; j < 8; j++) v[j] = div_q((freeze((cw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) << 3) + Q / 2) & 7;
     uint8_t* r = bb + 3 * g;
     r[0] = (uint8_t)(v[0] | (v[1] << 3) | (v[2] << 6));
     r[1] = (uint8_t)((v[2] >> 2) | (v[3] << 1) | (v[4] << 4) | (v[5] << 7));
@@ -814,7 +833,7 @@ __global__ __launch_bounds__(64) void k_keypair_fused(int n, uint8_t* pk, uint8_
 #pragma unroll
   for (int r = 0; r < C; ++r) {
     const uint32_t ahat = sa[arow_index(out_index<N>(l, r))];
-    vb[r] = (vb[r] + mont(vs[r] * mont(3186u * ahat))) % Q;
+    vb[r] = mod_q(vb[r] + mont(vs[r] * mont(3186u * ahat)));
   }
   __syncthreads();
   put_nat<N>(ss, vs, l);
@@ -868,7 +887,7 @@ __global__ __launch_bounds__(64) void k_enc_fused(int n, uint8_t* ct, size_t ct_
 #pragma unroll
     for (int r = 0; r < C; ++r) {
       const uint32_t ahat = sa[arow_index(out_index<N>(l, r))];
-      vu[r] = (mont(ahat * mont(3186u * vsp[r])) + vu[r]) % Q;
+      vu[r] = mod_q(mont(ahat * mont(3186u * vsp[r])) + vu[r]);
     }
     __syncthreads();
     put_nat<N>(sa, vu, l);
@@ -888,8 +907,8 @@ __global__ __launch_bounds__(64) void k_enc_fused(int n, uint8_t* ct, size_t ct_
     const int c = out_index<N>(l, r), b = c & 255;
     const uint64_t word = (b >> 6) == 0 ? mw[0] : (b >> 6) == 1 ? mw[1] : (b >> 6) == 2 ? mw[2] : mw[3];
     const uint32_t bit = (uint32_t)(word >> (b & 63)) & 1u;
-    const uint32_t t = (vv[r] + s3[c]) % Q;
-    vv[r] = (t + ((0u - bit) & (Q / 2))) % Q;
+    const uint32_t t = mod_q(vv[r] + s3[c]);
+    vv[r] = mod_q(t + ((0u - bit) & (Q / 2)));
   }
   __syncthreads();
   put_nat<N>(s3, vv, l);
@@ -927,7 +946,7 @@ __global__ __launch_bounds__(64) void k_dec_fused(int n, uint8_t* m, size_t m_st
   }
   invntt_regs<N>(vt, s1, l);
 #pragma unroll
-  for (int r = 0; r < C; ++r) vt[r] = (vt[r] + 3 * Q - (uint32_t)s3[out_index<N>(l, r)]) % Q;
+  for (int r = 0; r < C; ++r) vt[r] = mod_q(vt[r] + 3 * Q - (uint32_t)s3[out_index<N>(l, r)]);
   __syncthreads();
   put_nat<N>(s2, vt, l);
   __syncthreads();

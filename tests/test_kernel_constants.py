@@ -34,3 +34,13 @@ def test_compress_matches_definition_for_all_widths():
         want = (num // np.uint64(Q)) & np.uint64((1 << d) - 1)
         got = ((num * np.uint64(m)) >> np.uint64(33)) & np.uint64((1 << d) - 1)
         assert (got == want).all(), d
+
+
+def test_newhope_div_q_is_exact_below_2_21():
+    """newhope.hip's div_q: the high word of n * m (m = ceil(2^32 / 12289)) is
+    floor(n / 12289) for every n < 2^21; its operands stay below 2^17."""
+    src = open(os.path.join(HERE, "..", "pqc_amd", "csrc", "newhope.hip")).read()
+    m = int(re.search(r'v_mul_hi_u32_u24 %0, (0x[0-9a-fA-F]+), %1', src).group(1), 16)
+    assert m < (1 << 24)
+    n = np.arange(0, 1 << 21, dtype=np.uint64)
+    assert ((n * np.uint64(m)) >> np.uint64(32) == n // np.uint64(12289)).all()
