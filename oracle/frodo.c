@@ -266,7 +266,7 @@ int oracle_frodo_dec_a(uint8_t ss[FSS], const uint8_t *ct, const uint8_t *sk, co
 
 int oracle_frodo_dec(uint8_t ss[FSS], const uint8_t *ct, const uint8_t *sk) { return oracle_frodo_dec_a(ss, ct, sk, NULL); }
 
-/* threaded drivers sharing one A (one key pair): the CPU baseline for config 3 */
+/* threaded drivers sharing one A (one key pair): the CPU baseline of the Frodo-976 shared-A decaps workload */
 typedef struct {
     int op;
     size_t lo, hi;

@@ -439,7 +439,7 @@ int NH_FN(cca_dec)(uint8_t ss[32], const uint8_t *ct, const uint8_t *sk) {
     return 0;
 }
 
-/* threaded triple driver (CPU baseline for config 2) */
+/* threaded triple driver (the CPU baseline of the NewHope-1024 CPA triple workload) */
 typedef struct {
     size_t lo, hi;
     uint8_t *pk, *sk, *ct, *m2;
