@@ -31,6 +31,7 @@ namespace fused {
 __constant__ LaneTwiddles c_lane_tw = make_lane_twiddles();
 
 constexpr int kAStride = 262;   // int16 slots per A entry: 256 + guard, odd dword count
+static_assert(kAStride >= N + 4, "guard slots for a word's four candidates past the 256th accept");
 constexpr int kMaxBlocks = 32;  // SHAKE128 blocks per entry before giving up (never reached)
 
 // A tile is 64 requests, one per lane in the per-request phases.  Its 64*K*K
@@ -229,8 +230,10 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
 
 // SHAKE128(rho || i || j) rejection-sampled into an LDS entry (gen_matrix_n,
 // indcpa.cu:339-403, rej_uniform indcpa.cu:290-322).  Every candidate is
-// written: accepted ones to slot min(ctr, 256), rejected ones to the guard
-// slot 256.  Returns the accept count (>= 256 unless the block cap was hit).
+// written: accepted ones to slot ctr, rejected ones to the guard slot 256.  The
+// count is clamped to 256 after each word's four candidates, so accepts past
+// the 256th land in guard slots 256..259 (kAStride leaves 262).  Returns the
+// accept count, clamped (256 unless the block cap was hit).
 __device__ __forceinline__ int sample_entry(int16_t* dst, const uint64_t (&rho)[4], int x, int y, bool valid) {
   uint64_t A[25];
 #pragma unroll
@@ -248,10 +251,11 @@ __device__ __forceinline__ int sample_entry(int16_t* dst, const uint64_t (&rho)[
         const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
         const bool ok = val < 19u * Q;
         const int32_t red = (int32_t)val - (int32_t)(val >> 12) * Q;
-        const int slot = ok ? min(ctr, N) : N;
+        const int slot = ok ? ctr : N;
         dst[slot] = (int16_t)red;
         ctr += ok ? 1 : 0;
       }
+      ctr = min(ctr, N);
     }
   }
   return ctr;
