@@ -210,7 +210,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards by request with no collective, "
-              "so multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+              "so multi-GPU runs are not measured here (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     # The serving loop drives up to 3 streams per slot; with HIP's default of 4
     # hardware queues per process, streams share queues and batches serialise
