@@ -293,8 +293,85 @@ int pqc_server_run_trace(pqc_server *srv, size_t n, const double *arrival_us, co
                          double *latency_us, pqc_trace_stats *out);
 
 /* ---- NIST drop-ins: one instance, host pointers, synchronous -----------------
- * Unprefixed names are Kyber-768 (Kyber_GPU_Batched/api.h:18-22 with
- * KYBER_K 3).  Each call uses a library-owned context on the current device. */
+ * The sizes and name of Kyber_GPU_Batched/api.h:10-16 and FrodoKEM_GPU/api.h:11-17
+ * (CRYPTO_SECRETKEYBYTES, CRYPTO_PUBLICKEYBYTES, CRYPTO_CIPHERTEXTBYTES,
+ * CRYPTO_BYTES, CRYPTO_ALGNAME) describe the parameter set that the unprefixed
+ * crypto_kem_keypair / enc / dec mean.  Define PQC_NIST_ALG to one of the
+ * PQC_ALG_* KEM ids before including this header to choose it; the default is
+ * PQC_ALG_KYBER768, which the exported unprefixed symbols implement.  Any other
+ * choice maps the unprefixed names onto the prefixed functions below by macro,
+ * so a caller written against the reference's api.h compiles and links
+ * unchanged.  Note: the reference's own params.h:13 defaults to KYBER_K 4
+ * ("Kyber1024"); a caller that relied on that default defines
+ * PQC_NIST_ALG PQC_ALG_KYBER1024.  With PQC_ALG_FRODO976, the reference's
+ * crypto_kem_keypair_gpu and crypto_kem_keypair_random (FrodoKEM_GPU/api.h:21-22)
+ * also map onto the Frodo key generation.  NewHope: NewHope_NIST_GPU/api.h:10-23
+ * names only the CPA-KEM sizes, whose KEM functions the reference never wrote;
+ * here PQC_ALG_NEWHOPE1024 / PQC_ALG_NEWHOPE512 select the CCA-KEM drop-ins
+ * (sizes of params.h:37-39).
+ *
+ * Every call runs on a library-owned context of the current device.  Calls
+ * from several host threads are safe: each thread uses its own context (made
+ * on its first call, freed when the thread exits).                           */
+#ifndef PQC_NIST_ALG
+#define PQC_NIST_ALG PQC_ALG_KYBER768
+#endif
+#if PQC_NIST_ALG == PQC_ALG_KYBER512
+#define CRYPTO_SECRETKEYBYTES 1632
+#define CRYPTO_PUBLICKEYBYTES 800
+#define CRYPTO_CIPHERTEXTBYTES 736
+#define CRYPTO_BYTES 32
+#define CRYPTO_ALGNAME "Kyber512"
+#define crypto_kem_keypair pqc_kyber512_crypto_kem_keypair
+#define crypto_kem_enc pqc_kyber512_crypto_kem_enc
+#define crypto_kem_dec pqc_kyber512_crypto_kem_dec
+#elif PQC_NIST_ALG == PQC_ALG_KYBER768
+#define CRYPTO_SECRETKEYBYTES 2400
+#define CRYPTO_PUBLICKEYBYTES 1184
+#define CRYPTO_CIPHERTEXTBYTES 1088
+#define CRYPTO_BYTES 32
+#define CRYPTO_ALGNAME "Kyber768"
+#elif PQC_NIST_ALG == PQC_ALG_KYBER1024
+#define CRYPTO_SECRETKEYBYTES 3168
+#define CRYPTO_PUBLICKEYBYTES 1568
+#define CRYPTO_CIPHERTEXTBYTES 1568
+#define CRYPTO_BYTES 32
+#define CRYPTO_ALGNAME "Kyber1024"
+#define crypto_kem_keypair pqc_kyber1024_crypto_kem_keypair
+#define crypto_kem_enc pqc_kyber1024_crypto_kem_enc
+#define crypto_kem_dec pqc_kyber1024_crypto_kem_dec
+#elif PQC_NIST_ALG == PQC_ALG_FRODO976
+#define CRYPTO_SECRETKEYBYTES 31296
+#define CRYPTO_PUBLICKEYBYTES 15632
+#define CRYPTO_CIPHERTEXTBYTES 15744
+#define CRYPTO_BYTES 24
+#define CRYPTO_ALGNAME "FrodoKEM-976"
+#define crypto_kem_keypair pqc_frodo976_crypto_kem_keypair
+#define crypto_kem_keypair_gpu pqc_frodo976_crypto_kem_keypair
+#define crypto_kem_keypair_random pqc_frodo976_crypto_kem_keypair
+#define crypto_kem_enc pqc_frodo976_crypto_kem_enc
+#define crypto_kem_dec pqc_frodo976_crypto_kem_dec
+#elif PQC_NIST_ALG == PQC_ALG_NEWHOPE1024
+#define CRYPTO_SECRETKEYBYTES 3680
+#define CRYPTO_PUBLICKEYBYTES 1824
+#define CRYPTO_CIPHERTEXTBYTES 2208
+#define CRYPTO_BYTES 32
+#define CRYPTO_ALGNAME "NewHope1024-CCAKEM"
+#define crypto_kem_keypair pqc_newhope1024_crypto_kem_keypair
+#define crypto_kem_enc pqc_newhope1024_crypto_kem_enc
+#define crypto_kem_dec pqc_newhope1024_crypto_kem_dec
+#elif PQC_NIST_ALG == PQC_ALG_NEWHOPE512
+#define CRYPTO_SECRETKEYBYTES 1888
+#define CRYPTO_PUBLICKEYBYTES 928
+#define CRYPTO_CIPHERTEXTBYTES 1120
+#define CRYPTO_BYTES 32
+#define CRYPTO_ALGNAME "NewHope512-CCAKEM"
+#define crypto_kem_keypair pqc_newhope512_crypto_kem_keypair
+#define crypto_kem_enc pqc_newhope512_crypto_kem_enc
+#define crypto_kem_dec pqc_newhope512_crypto_kem_dec
+#else
+#error "PQC_NIST_ALG must be a KEM: PQC_ALG_KYBER512/768/1024, PQC_ALG_FRODO976, PQC_ALG_NEWHOPE1024/512"
+#endif
 int crypto_kem_keypair(unsigned char *pk, unsigned char *sk);
 int crypto_kem_enc(unsigned char *ct, unsigned char *ss, const unsigned char *pk);
 int crypto_kem_dec(unsigned char *ss, const unsigned char *ct, const unsigned char *sk);

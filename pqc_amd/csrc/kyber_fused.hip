@@ -21,6 +21,7 @@
 // comparing ciphertext bytes: every width packs 256 coefficients into whole
 // bytes with no unused bits.
 #include "internal.h"
+#include "req_check.h"
 #include "kyber_wave.cuh"
 #include "sponge.cuh"
 
@@ -905,17 +906,8 @@ __global__ __launch_bounds__(256) void k_lowlat_mixed(int n, unsigned alg_mask, 
   const int x = blockIdx.x;
   if (x >= n) return;
   const pqc_kem_req d = reqs[x];
-  const bool known = (d.alg == PQC_ALG_KYBER512 || d.alg == PQC_ALG_KYBER768 || d.alg == PQC_ALG_KYBER1024) &&
-                     (alg_mask & PQC_ALG_MASK(d.alg));
-  const int k = known ? d.alg + 1 : 0;
-  bool ok = false;
-  if (k) {
-    const uint64_t ctb = k == 2 ? 736 : k == 3 ? 1088 : 1568;
-    const uint64_t keyb = DEC ? (k == 2 ? 1632 : k == 3 ? 2400 : 3168) : (k == 2 ? 800 : k == 3 ? 1184 : 1568);
-    ok = ((d.ct_off | d.key_off | d.ss_off | d.coins_off) & 7) == 0 && d.ct_off + ctb <= ct_bytes &&
-         d.key_off + keyb <= key_bytes && d.ss_off + 32 <= ss_bytes && (DEC || d.coins_off + 32 <= coins_bytes);
-  }
-  if (!ok) {  // uniform: every lane read the same descriptor
+  const int k = pqc_req_k(d, DEC, alg_mask, ct_bytes, key_bytes, ss_bytes, coins_bytes);
+  if (!k) {  // uniform: every lane read the same descriptor
     if (threadIdx.x == 0) atomicOr(status, 2u);
     return;
   }
@@ -963,18 +955,7 @@ __global__ void k_mixed_classify(int n, int cap, int dec, unsigned alg_mask, con
   int k = 0;
   if (i < n) {
     const pqc_kem_req d = reqs[i];
-    const bool known = (d.alg == PQC_ALG_KYBER512 || d.alg == PQC_ALG_KYBER768 || d.alg == PQC_ALG_KYBER1024) &&
-                       (alg_mask & PQC_ALG_MASK(d.alg));
-    const int kk = known ? d.alg + 1 : 0;
-    if (kk) {
-      // per-K sizes (round 2): ct, sk, pk
-      const uint64_t ctb = kk == 2 ? 736 : kk == 3 ? 1088 : 1568;
-      const uint64_t keyb = dec ? (kk == 2 ? 1632 : kk == 3 ? 2400 : 3168) : (kk == 2 ? 800 : kk == 3 ? 1184 : 1568);
-      const bool ok = ((d.ct_off | d.key_off | d.ss_off | d.coins_off) & 7) == 0 && d.ct_off + ctb <= ct_bytes &&
-                      d.key_off + keyb <= key_bytes && d.ss_off + 32 <= ss_bytes &&
-                      (dec || d.coins_off + 32 <= coins_bytes);
-      k = ok ? kk : 0;
-    }
+    k = pqc_req_k(d, dec != 0, alg_mask, ct_bytes, key_bytes, ss_bytes, coins_bytes);
     if (!k) atomicOr(status, 2u);
   }
 #pragma unroll

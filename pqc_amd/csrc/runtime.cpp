@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "internal.h"
 
@@ -130,9 +131,13 @@ void drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, uint64_t ind
   }
 }
 
-// ---- library-owned context for the NIST drop-ins ----------------------------------
+// ---- library-owned contexts for the NIST drop-ins ---------------------------------
+// One pool per device.  A drop-in call checks a context out for its whole
+// duration and returns it afterwards, so two host threads never share a
+// context's staging buffer, pinned double buffers or status word.  The pool
+// grows to the largest number of concurrent callers and is kept until exit.
 std::mutex g_default_mu;
-pqc_ctx* g_default[64] = {nullptr};
+std::vector<pqc_ctx*> g_default_free[64];
 
 }  // namespace
 
@@ -142,19 +147,29 @@ void pqc::host_drbg(uint8_t* out, size_t len, uint64_t seed, uint64_t stream, ui
 
 namespace {
 
-pqc_ctx* default_ctx() {
+pqc_ctx* default_ctx_acquire(int* dev_out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  std::lock_guard<std::mutex> lk(g_default_mu);
-  if (!g_default[dev]) {
-    pqc_ctx* c = nullptr;
-    unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024) |
-                    PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024) |
-                    PQC_ALG_MASK(PQC_ALG_NEWHOPE512);
-    if (pqc_ctx_create(&c, mask, 64, 0) != PQC_OK) return nullptr;
-    g_default[dev] = c;
+  *dev_out = dev;
+  {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default_free[dev].empty()) {
+      pqc_ctx* c = g_default_free[dev].back();
+      g_default_free[dev].pop_back();
+      return c;
+    }
   }
-  return g_default[dev];
+  pqc_ctx* c = nullptr;
+  unsigned mask = PQC_ALG_MASK(PQC_ALG_KYBER512) | PQC_ALG_MASK(PQC_ALG_KYBER768) | PQC_ALG_MASK(PQC_ALG_KYBER1024) |
+                  PQC_ALG_MASK(PQC_ALG_FRODO976) | PQC_ALG_MASK(PQC_ALG_NEWHOPE1024) |
+                  PQC_ALG_MASK(PQC_ALG_NEWHOPE512);
+  if (pqc_ctx_create(&c, mask, 64, 0) != PQC_OK) return nullptr;
+  return c;
+}
+
+void default_ctx_release(int dev, pqc_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  g_default_free[dev].push_back(c);
 }
 
 int stage_ensure(pqc_ctx* c, size_t bytes) {
@@ -193,7 +208,13 @@ int staged(pqc_ctx* c, size_t n, unsigned flags, hipStream_t s, const Buf (&bufs
     dev[i] = (uint8_t*)bufs[i].p;
   }
   const bool host = (flags & PQC_F_HOST_PTRS) != 0;
+  // Host-pointer calls report only the status bits they raise themselves:
+  // bits left by earlier device-pointer calls stay for pqc_ctx_take_status.
+  uint32_t st0 = 0;
   if (host) {
+    if (hipMemcpyAsync(&st0, c->status, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return PQC_ERR_DEVICE;
     size_t total = 0;
     for (size_t i = 0; i < NB; ++i) total += ((n * bufs[i].per + 255) & ~(size_t)255);
     int rc = stage_ensure(c, total);
@@ -216,7 +237,7 @@ int staged(pqc_ctx* c, size_t n, unsigned flags, hipStream_t s, const Buf (&bufs
     if (hipStreamSynchronize(s) != hipSuccess) return PQC_ERR_DEVICE;
     uint32_t st = 0;
     if (hipMemcpy(&st, c->status, 4, hipMemcpyDeviceToHost) != hipSuccess) return PQC_ERR_DEVICE;
-    if (st) return PQC_ERR_DEVICE;
+    if (st & ~st0) return PQC_ERR_DEVICE;
   }
   return PQC_OK;
 }
@@ -299,9 +320,7 @@ int kem_call(pqc_ctx* c, int alg, Op op, size_t n, uint8_t* o0, uint8_t* o1, con
   }
 }
 
-int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1) {
-  pqc_ctx* c = default_ctx();
-  if (!c) return PQC_ERR_NO_DEVICE;
+int single_on(pqc_ctx* c, int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1) {
   // Coins are drawn call by call as the NIST submissions do, so that the
   // known-answer-test DRBG reproduces their outputs: Kyber and NewHope keypair
   // CPA seed then z / s (cpapke/indcpa keypair, then crypto_kem_keypair's own
@@ -319,6 +338,15 @@ int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const ui
     i1 = coins;
   }
   return kem_call(c, alg, op, 1, o0, o1, i0, i1, PQC_F_HOST_PTRS, nullptr);
+}
+
+int single(int alg, Op op, uint8_t* o0, uint8_t* o1, const uint8_t* i0, const uint8_t* i1) {
+  int dev = 0;
+  pqc_ctx* c = default_ctx_acquire(&dev);
+  if (!c) return PQC_ERR_NO_DEVICE;
+  const int rc = single_on(c, alg, op, o0, o1, i0, i1);
+  default_ctx_release(dev, c);
+  return rc;
 }
 
 }  // namespace

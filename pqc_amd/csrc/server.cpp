@@ -77,6 +77,7 @@ struct Slot {
   std::vector<Pending> reqs;
   std::vector<size_t> out_a, out_b, len_a, len_b;  // where each request's output lies in h_out
   std::vector<size_t> in_at;                       // where each request's input goes in h_in
+  uint32_t* h_status = nullptr;  // pinned: the device status words of ctx and ctx_fd after the batch
   size_t bytes = 0;
   bool busy = false;
 };
@@ -124,7 +125,7 @@ void free_slot(Slot& sl) {
     if (t) (void)hipStreamDestroy(t);
   for (hipEvent_t e : {sl.done, sl.in_ready, sl.frodo_done, sl.fd_done, sl.start})
     if (e) (void)hipEventDestroy(e);
-  for (void* p : {(void*)sl.h_in, (void*)sl.h_out, (void*)sl.h_desc, (void*)sl.h_kid})
+  for (void* p : {(void*)sl.h_in, (void*)sl.h_out, (void*)sl.h_desc, (void*)sl.h_kid, (void*)sl.h_status})
     if (p) (void)hipHostFree(p);
   for (void* p : {(void*)sl.d_in, (void*)sl.d_out, (void*)sl.d_desc, (void*)sl.d_kid, (void*)sl.d_keys})
     if (p) (void)hipFree(p);
@@ -221,6 +222,10 @@ int launch(pqc_server* v, Slot& sl) {
   const size_t n = sl.reqs.size();
   const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
   if (hipEventRecord(sl.start, st) != hipSuccess) return PQC_ERR_DEVICE;
+  sl.h_status[0] = sl.h_status[1] = 0;
+  if (hipMemsetAsync(sl.ctx->status, 0, 4, st) != hipSuccess ||
+      (sl.ctx_fd && hipMemsetAsync(sl.ctx_fd->status, 0, 4, st) != hipSuccess))
+    return PQC_ERR_DEVICE;
   if (hipMemcpyAsync(sl.d_in, sl.h_in, io, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
   if (hipMemcpyAsync(sl.d_kid, sl.h_kid, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
   // fused mode: Frodo encapsulation and decapsulation each run on a stream of their
@@ -291,6 +296,10 @@ int launch(pqc_server* v, Slot& sl) {
     rc = PQC_ERR_DEVICE;
   if (rc) return rc;
   if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
+  // the contexts' status words (cleared before the batch) come back with the outputs
+  if (hipMemcpyAsync(sl.h_status, sl.ctx->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      (sl.ctx_fd && hipMemcpyAsync(sl.h_status + 1, sl.ctx_fd->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess))
+    return PQC_ERR_DEVICE;
   if (hipEventRecord(sl.done, st) != hipSuccess) return PQC_ERR_DEVICE;
   return PQC_OK;
 }
@@ -319,7 +328,9 @@ void worker_loop(pqc_server* v) {
       if (!sl.busy) continue;
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
-      const int status = q == hipSuccess ? PQC_OK : PQC_ERR_DEVICE;
+      // a status bit raised on the device (a rejected descriptor, a sampler at its
+      // block cap) fails the whole batch: its outputs are not trusted
+      const int status = q != hipSuccess ? PQC_ERR_DEVICE : (sl.h_status[0] | sl.h_status[1]) ? PQC_ERR_DEVICE : PQC_OK;
       if (status == PQC_OK) {  // the outputs to the callers' buffers, without holding the lock
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, sl.start, sl.done) == hipSuccess) v->device_us += 1e3 * ms;
@@ -502,7 +513,7 @@ int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
     const size_t nb = cfg->max_batch;
     if (!rc && (hipHostMalloc(&sl.h_in, v->arena) != hipSuccess || hipHostMalloc(&sl.h_out, v->arena) != hipSuccess ||
                 hipHostMalloc(&sl.h_desc, nb * sizeof(pqc_kem_req)) != hipSuccess ||
-                hipHostMalloc(&sl.h_kid, nb * 4) != hipSuccess))
+                hipHostMalloc(&sl.h_kid, nb * 4) != hipSuccess || hipHostMalloc(&sl.h_status, 16) != hipSuccess))
       rc = PQC_ERR_NO_MEMORY;
     if (!rc && (hipMalloc(&sl.d_in, v->arena) != hipSuccess || hipMalloc(&sl.d_out, v->arena) != hipSuccess ||
                 hipMalloc(&sl.d_desc, nb * sizeof(pqc_kem_req)) != hipSuccess ||
@@ -614,6 +625,27 @@ double pqc_server_now_us(pqc_server* v) { return v ? v->now_us() : 0.0; }
 // Replay an arrival trace from the calling thread: request i is submitted at
 // arrival_us[i] (server clock, offset to "now" at the call), completions are
 // drained as they come.  latency_us[i] (optional) = completion - arrival.
+}  // extern "C"
+
+namespace {
+// An unfinished trace returns only once the requests it submitted are done
+// (their outputs land in the caller's buffers, which must not be freed first)
+// or the GPU made no progress for 30 s; their completions are discarded so
+// that later polls do not see trace indices as tags.
+int abandon_trace(pqc_server* v, int rc) {
+  (void)pqc_server_drain(v);
+  std::vector<pqc_server_completion> comp(4096);
+  size_t got = 0;
+  do {
+    got = 0;
+    pqc_server_poll(v, comp.data(), comp.size(), &got, 0);
+  } while (got);
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
 int pqc_server_run_trace(pqc_server* v, size_t n, const double* arrival_us, const pqc_server_request* reqs,
                          double* latency_us, pqc_trace_stats* out) {
   if (!v || !out || (n && (!arrival_us || !reqs))) return PQC_ERR_INVALID_ARG;
@@ -626,14 +658,14 @@ int pqc_server_run_trace(pqc_server* v, size_t n, const double* arrival_us, cons
   std::vector<pqc_server_completion> comp(4096);
   double last = base, progress = v->now_us();
   while (done < n) {
-    if (v->now_us() - progress > 30e6) return PQC_ERR_DEVICE;  // 30 s without a completion
+    if (v->now_us() - progress > 30e6) return abandon_trace(v, PQC_ERR_DEVICE);  // 30 s without a completion
     const double now = v->now_us();
     size_t end = next;
     while (end < n && base + arrival_us[end] <= now) ++end;
     if (end > next) {
       // one submission for everything that has arrived, each stamped now
       int rc = pqc_server_submit(v, rq.data() + next, end - next);
-      if (rc) return rc;
+      if (rc) return abandon_trace(v, rc);
       next = end;
     }
     size_t got = 0;

@@ -156,6 +156,8 @@ def test_mixed_rejects_bad_descriptors(pqc, ctx, path):
     reqs["alg"][4] = 9            # unknown id
     reqs["ct_off"][5] += 4        # unaligned
     reqs["key_off"][6] = 1 << 40  # outside the arena
+    reqs["ss_off"][7] = 2**64 - 1024  # wraps past 2^64 to 1 KiB below the arena
+    reqs["key_off"][8] = 2**64 - 8
     d_reqs = pqc.DeviceBuffer.from_array(reqs)
     ar.buf["ss"].upload(np.full(lay.nbytes["ss"], 0xAA, np.uint8))
     ctx.kem_dec_mixed(n, d_reqs, ar.buf["ct"], ar.buf["sk"], ar.buf["ss"])
@@ -164,7 +166,7 @@ def test_mixed_rejects_bad_descriptors(pqc, ctx, path):
     sso = lay.offsets("ss")
     for i in range(n):
         got = ss[int(sso[i]):int(sso[i]) + 32]
-        if i in (3, 4, 5, 6):
+        if i in (3, 4, 5, 6, 7, 8):
             assert (got == 0xAA).all(), i
         else:
             assert (got == ss_ok[int(sso[i]):int(sso[i]) + 32]).all(), i
