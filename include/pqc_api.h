@@ -208,6 +208,26 @@ int pqc_kem_dec_mixed(pqc_ctx *ctx, size_t n, const pqc_kem_req *reqs, const uin
  * was rejected.  Host-pointer calls check it themselves.                        */
 int pqc_ctx_take_status(pqc_ctx *ctx, void *stream, uint32_t *status);
 
+/* ---- allocation-free decapsulation ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory or call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation): Kyber-512/768/1024 decapsulation of n
+ * ciphertexts with no context, bit-identical to crypto_kem_dec_batch with the
+ * default tuning (the same kernels).  The tile kernel's scratch (PRF rows,
+ * (2K+1) x 128 bytes per request) lives in a library-owned device array of
+ * 256 + PQC_NOALLOC_MAX_BATCH x 1152 bytes (18.9 MB), one per GPU, held until
+ * the process ends; nothing is allocated or freed per call.  n above
+ * PQC_NOALLOC_MAX_BATCH returns PQC_ERR_BATCH_TOO_LARGE.  Device pointers
+ * (8-byte aligned), asynchronous on `stream`.  `status` (device word, 4-byte
+ * aligned, may be NULL) collects the call's status bits (bit 0: a matrix
+ * sampler hit its block cap) by OR; it is not cleared.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams or from different host threads) race and must be
+ * serialised by the caller.                                                   */
+#define PQC_NOALLOC_MAX_BATCH 16384
+int pqc_kyber_dec_batch_noalloc(int alg, size_t n, uint8_t *ss, const uint8_t *ct, const uint8_t *sk,
+                                uint32_t *status, void *stream);
+
 /* ---- graph capture of device-pointer batch calls --------------------------------
  * Between begin and end, device-pointer calls on `stream` (non-NULL) are recorded
  * instead of run; the graph replays them, on the same buffers, with one launch. */

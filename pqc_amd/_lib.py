@@ -48,6 +48,7 @@ SIGNATURES = {
     "crypto_kem_keypair_derand_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
     "crypto_kem_enc_derand_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _P, _U, _P]),
     "crypto_kem_dec_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
+    "pqc_kyber_dec_batch_noalloc": (_I, [_I, _SZ, _P, _P, _P, _P, _P]),
     "crypto_kem_keypair_batch": (_I, [_P, _I, _SZ, _P, _P, _U, _P]),
     "crypto_kem_enc_batch": (_I, [_P, _I, _SZ, _P, _P, _P, _U, _P]),
     "crypto_kem_keypair": (_I, [_P, _P]),

@@ -20,6 +20,9 @@
 // KDF = SHAKE256 (symmetric.h:33).  Comparing compressed coefficients is
 // comparing ciphertext bytes: every width packs 256 coefficients into whole
 // bytes with no unused bits.
+#include <algorithm>
+#include <mutex>
+
 #include "internal.h"
 #include "req_check.h"
 #include "kyber_wave.cuh"
@@ -1248,6 +1251,69 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
   return hip_err(hipGetLastError());
 }
 
+// ---- allocation-free decapsulation ------------------------------------------------
+// pqc_kyber_dec_batch_noalloc runs the same kernels as crypto_kem_dec_batch on
+// the default tuning, with the tile kernel's PRF scratch and the status word in
+// `scratch`, a file-scope device array (one copy per GPU, held until the process
+// ends) instead of a context's allocation.  The host never names `scratch`
+// directly (in host code it is only a stand-in symbol): the device address is
+// looked up once per device with hipGetSymbolAddress and that address is what
+// the kernels receive.  Nothing in the scratch must start at zero: every PRF
+// row is written by the same tile before it is read, and the status word only
+// collects bits for callers that pass no status word of their own.
+constexpr size_t kNoAllocMax = PQC_NOALLOC_MAX_BATCH;
+constexpr size_t kNoAllocPrf = kNoAllocMax * kyber::fused::Tile<4>::PRFB;  // K = 4 rows are the largest
+constexpr size_t kNoAllocBytes = 256 + kNoAllocPrf;                        // status word, then the PRF rows
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoAllocDev {
+  uint8_t* ptr = nullptr;  // device address of `scratch` on this device
+  int cus = 0;
+  bool ready = false;
+};
+std::mutex g_noalloc_mu;
+NoAllocDev g_noalloc[kMaxDevices];
+
+// Set-up on the first call on a device, with that device current.
+int noalloc_device(NoAllocDev* out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return PQC_ERR_NO_DEVICE;
+  if (dev < 0 || dev >= kMaxDevices) return PQC_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocDev& d = g_noalloc[dev];
+  if (!d.ready) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PQC_ERR_DEVICE;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      return PQC_ERR_DEVICE;
+    d.ptr = static_cast<uint8_t*>(p);
+    d.cus = cus;
+    d.ready = true;
+  }
+  *out = d;
+  return PQC_OK;
+}
+
+template <int K>
+int noalloc_dec(const NoAllocDev& d, size_t n, uint8_t* ss, uint8_t* ct, const uint8_t* sk, uint32_t* status,
+                hipStream_t s) {
+  using namespace kyber::fused;
+  if (n <= (size_t)kLowLatDefaultDec) {
+    hipLaunchKernelGGL((k_lowlat<K, true>), dim3((unsigned)n), dim3(256), 0, s, (int)n, ss, ct, sk,
+                       (const uint8_t*)nullptr, status);
+  } else {
+    const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
+    const int grid = std::max(1, std::min(ntiles, d.cus * 4));  // the default fused_waves_per_cu
+    const DenseSrc src{ss, ct, sk, nullptr, d.ptr + 256};
+    hipLaunchKernelGGL((k_fused<K, true>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, src, status);
+  }
+  return hip_err(hipGetLastError());
+}
+}  // namespace
+
 int kyber_poly_transform(int op, size_t n, int16_t* polys, hipStream_t s) {
   if (op != 0 && op != 1) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
@@ -1266,3 +1332,23 @@ int keccak_batch(int op, size_t n, uint8_t* out, size_t outlen, const uint8_t* i
 }
 
 }  // namespace pqc
+
+extern "C" int pqc_kyber_dec_batch_noalloc(int alg, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk,
+                                           uint32_t* status, void* stream) {
+  if (alg != PQC_ALG_KYBER512 && alg != PQC_ALG_KYBER768 && alg != PQC_ALG_KYBER1024) return PQC_ERR_UNSUPPORTED;
+  if (n > pqc::kNoAllocMax) return PQC_ERR_BATCH_TOO_LARGE;
+  if (n == 0) return PQC_OK;
+  if (!ss || !ct || !sk || (((uintptr_t)ss | (uintptr_t)ct | (uintptr_t)sk) & 7) || ((uintptr_t)status & 3))
+    return PQC_ERR_INVALID_ARG;
+  pqc::NoAllocDev d;
+  const int rc = pqc::noalloc_device(&d);
+  if (rc) return rc;
+  uint32_t* st = status ? status : reinterpret_cast<uint32_t*>(d.ptr);
+  uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decapsulation kernels
+  hipStream_t s = (hipStream_t)stream;
+  switch (alg) {
+    case PQC_ALG_KYBER512: return pqc::noalloc_dec<2>(d, n, ss, ctm, sk, st, s);
+    case PQC_ALG_KYBER768: return pqc::noalloc_dec<3>(d, n, ss, ctm, sk, st, s);
+    default: return pqc::noalloc_dec<4>(d, n, ss, ctm, sk, st, s);
+  }
+}
