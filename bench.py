@@ -56,6 +56,57 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def usable_cores():
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_threads(args):
+    """Threads for the CPU baselines: every core this process may run on, unless
+    --cpu-threads names a number."""
+    return args.cpu_threads if args.cpu_threads > 0 else usable_cores()
+
+
+def host_cpu_info():
+    """The host the CPU baselines ran on: logical CPUs, the ones this process may
+    use, and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"host_cpus": os.cpu_count(), "usable_cpus": usable_cores(), "cpu_model": model}
+
+
+def timed_cpu(run, total, first, min_s=1.0):
+    """Times the CPU oracle for at least `min_s` seconds.  run(m) processes items
+    0..m-1 (and checks them); a first pass over `first` items sets the rate, then
+    one pass over enough items (up to `total`), repeated while the total time is
+    under min_s.  Returns (items processed, seconds) of the timed passes."""
+    t0 = time.perf_counter()
+    run(min(first, total))
+    dt0 = max(time.perf_counter() - t0, 1e-6)
+    m = int(min(total, max(first, first / dt0 * min_s * 1.1)))
+    done, dt = 0, 0.0
+    while dt < min_s:
+        t0 = time.perf_counter()
+        run(m)
+        dt += time.perf_counter() - t0
+        done += m
+    return done, dt
+
+
+def cpu_entry(value, unit, thr, sample, single=None):
+    out = {"value": round(value, 1), "unit": unit, "cores": thr, "kind": "port", "sample": sample}
+    out.update(host_cpu_info())
+    if single is not None:
+        out["single_core_value"] = round(single, 1)
+    return out
+
+
 def timed(fn, stream, steps, warmup, pqc):
     for _ in range(warmup):
         fn()
@@ -173,19 +224,20 @@ def small_batch_latency(pqc, ctx, stream, ct, sk, ss, ss_want, reps=50):
 
 
 def cpu_baseline(ct_h, sk_h, ss_gpu, threads):
+    """The oracle's decapsulation of the same batch on `threads` host threads and
+    on one, each timed for at least a second; outputs checked equal to the GPU's."""
     from tests import oracle_lib as O
-    n1 = min(4096, len(ct_h))
+
+    def run(thr):
+        def f(m):
+            ss = O.kyber_batch("dec", K, [np.ascontiguousarray(ct_h[:m]), np.ascontiguousarray(sk_h[:m])],
+                               nthreads=thr)
+            assert (ss == ss_gpu[:m]).all(), "CPU oracle and GPU disagree"
+        return f
+
     O.kyber_batch("dec", K, [ct_h[:64], sk_h[:64]], nthreads=threads)  # warm
-    t0 = time.perf_counter()
-    ss = O.kyber_batch("dec", K, [np.ascontiguousarray(ct_h[:n1]), np.ascontiguousarray(sk_h[:n1])], nthreads=threads)
-    dt = time.perf_counter() - t0
-    assert (ss == ss_gpu[:n1]).all(), "CPU oracle and GPU disagree"
-    # single core on a smaller sample
-    n2 = min(512, len(ct_h))
-    t0 = time.perf_counter()
-    ss1 = O.kyber_batch("dec", K, [np.ascontiguousarray(ct_h[:n2]), np.ascontiguousarray(sk_h[:n2])], nthreads=1)
-    dt1 = time.perf_counter() - t0
-    assert (ss1 == ss_gpu[:n2]).all()
+    n1, dt = timed_cpu(run(threads), len(ct_h), 256 * threads)
+    n2, dt1 = timed_cpu(run(1), len(ct_h), 512)
     return n1 / dt, n1, n2 / dt1, n2
 
 
@@ -196,7 +248,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads for the CPU baselines (0 = every core this process may use)")
     ap.add_argument("--only", choices=["fused", "plain"], default=None,
                     help="time one path only (profiling runs)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -205,6 +258,8 @@ def main():
                     default="kyber768")
     ap.add_argument("--rate", type=float, default=2e6, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=0.5, help="serving: trace length, seconds")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="kyber768: skip the other configurations' figures (config.secondary)")
     ap.add_argument("--fixed-rate", type=float, default=4e6,
                     help="kyber768: Poisson arrival rate of the fixed-rate latency phase, decaps/s (0 = skip)")
     args = ap.parse_args()
@@ -337,22 +392,67 @@ def main():
     if not args.no_cpu:
         ct_h = ct.download((n, 1088))
         sk_h = sk.download((n, 2400))
-        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        thr = cpu_threads(args)
         r_all, s_all, r_one, s_one = cpu_baseline(ct_h, sk_h, a, thr)
-        out["cpu_baseline"] = {
-            "value": round(r_all, 1),
-            "unit": "decaps/s",
-            "cores": thr,
-            "kind": "port",
-            "sample": "%d Kyber-768 decaps of the same batch on %d threads (and %d on 1 thread: %.1f/s); "
-                      "outputs checked equal to the GPU's" % (s_all, thr, s_one, r_one),
-            "single_core_value": round(r_one, 1),
-        }
+        out["cpu_baseline"] = cpu_entry(
+            r_all, "decaps/s", thr,
+            "%d Kyber-768 decaps of the same batch on %d threads (and %d on 1 thread: %.1f/s), each leg timed for "
+            "at least 1 s; outputs checked equal to the GPU's" % (s_all, thr, s_one, r_one), single=r_one)
         out["config"]["gpu_over_cpu"] = round(rate_f / r_all, 1)
     ctx.close()
     if args.fixed_rate > 0:
         out["config"]["fixed_rate"] = fixed_rate_latency(args.fixed_rate, 0.25)
+    if not args.no_secondary:
+        out["config"]["secondary"] = secondary_workloads(args)
     print(json.dumps(out), flush=True)
+
+
+def _compact(line, keys):
+    """The figures of one workload's line that the headline line carries."""
+    c, r, cpu = line["config"], line.get("roofline") or {}, line.get("cpu_baseline") or {}
+    out = {"metric": line["metric"], "value": line["value"], "unit": line["unit"],
+           "ms_per_step": line["ms_per_step"], "steps": line["steps"], "workload": c["workload"]}
+    if r:
+        out["roofline"] = {k: r[k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac", "kernel_ms",
+                                             "traffic") if k in r}
+    if cpu:
+        out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "cpu_model") if k in cpu}
+        out["gpu_over_cpu"] = c.get("gpu_over_cpu")
+    for k in keys:
+        out[k] = c.get(k)
+    return out
+
+
+def secondary_workloads(args):
+    """The other configurations, each run as its own workload would run it (fewer
+    steps), reduced to their main figures: FrodoKEM-976 shared-key decapsulation
+    (with the int8-MFMA utilisation against its 40% target, and the pipelined
+    rate), NewHope-1024 CPA triples fused vs plain, the graph-captured mixed
+    Kyber batch, and a 0.1 s slice of the 2 M requests/s serving trace."""
+    import copy
+    sa = copy.copy(args)
+    sa.steps, sa.warmup, sa.batch, sa.calibrate_copy_mb = min(args.steps, 10), min(args.warmup, 2), BATCH, 0
+    sec = {}
+    t0 = time.perf_counter()
+    fr = run_frodo(sa)
+    sec["frodo976"] = _compact(fr, ("batch", "decaps_equal_encaps", "pipelined", "batch_latency_ms_p50"))
+    sec["frodo976"]["mfma_frac_target"] = 0.40
+    log("secondary frodo976: %.1f s" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    nh = run_newhope(sa)
+    sec["newhope1024"] = _compact(nh, ("batch", "plain_triples_per_s", "fused_over_plain", "fused_equals_plain",
+                                       "messages_recovered"))
+    log("secondary newhope1024: %.1f s" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    mx = run_mixed(sa)
+    sec["kyber_mixed"] = _compact(mx, ("batch", "counts_512_768_1024", "tuning", "autotune", "fused_over_plain",
+                                       "bitexact_fused_plain"))
+    log("secondary kyber_mixed: %.1f s" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    sv = run_serving(sa, duration=0.1, with_plain=False)
+    sec["serving"] = _compact(sv, ("offered_rate", "requests", "p50_us", "p99_us", "mean_batch", "errors"))
+    log("secondary serving: %.1f s" % (time.perf_counter() - t0))
+    return sec
 
 
 def fixed_rate_latency(rate, duration, keys=1024):
@@ -368,21 +468,21 @@ def fixed_rate_latency(rate, duration, keys=1024):
     ops = np.full(n, 2, np.int32)
     kid = rng.integers(0, keys, n).astype(np.uint32)
     res = {"rate": rate, "requests": n, "slots": 2, "max_wait_us": 25}
-    for plain in (False, True):
+    for plain, graph in ((False, False), (False, True), (True, False)):
         # two slots (one batch on the GPU while the next fills) and batches closed
         # after 25 us, which kept batches near 600 requests on every box measured
         # (50 us let some settle near 1000, doubling p50).  Batches of up to 8192
         # decapsulations run the one-workgroup-per-request kernel.
         srv = Server(["kyber768"], keys_per_alg=keys, key_seed=0, max_batch=65536, slots=2,
-                     arena_bytes=128 << 20, max_wait_us=25, plain=plain)
+                     arena_bytes=128 << 20, max_wait_us=25, plain=plain, graph=graph)
         ws = RequestSet.random(algs, ops, kid, rng)
         srv.run_trace(arrivals, ws.reqs)  # warm-up: the same trace once
         rs = RequestSet.random(algs, ops, kid, rng)
         st, _ = srv.run_trace(arrivals, rs.reqs)
         srv.close()
-        log("fixed rate %.1fM/s plain=%s: %.0f decaps/s, p50 %.0f us, p99 %.0f us, mean batch %.0f" %
-            (rate / 1e6, plain, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch))
-        res["plain" if plain else "fused"] = {"completed_per_s": round(st.ops_per_s, 1), "p50_us": round(st.p50_us, 1),
+        log("fixed rate %.1fM/s plain=%s graph=%s: %.0f decaps/s, p50 %.0f us, p99 %.0f us, mean batch %.0f" %
+            (rate / 1e6, plain, graph, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch))
+        res["plain" if plain else "fused_graph" if graph else "fused"] = {"completed_per_s": round(st.ops_per_s, 1), "p50_us": round(st.p50_us, 1),
                                               "p99_us": round(st.p99_us, 1), "mean_batch": round(st.mean_batch, 1),
                                               "errors": int(st.errors)}
     return res
@@ -395,6 +495,10 @@ MFMA_I8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
 
 
 def bench_frodo(args):
+    print(json.dumps(run_frodo(args)), flush=True)
+
+
+def run_frodo(args):
     import pqc_amd as pqc
     n = FRODO_BATCH if args.batch == BATCH else args.batch
     ctx = pqc.Context(["frodo976"], max_batch=n)
@@ -416,6 +520,7 @@ def bench_frodo(args):
     ms = wall * 1e3 / args.steps
     log("frodo decaps: %.3f ms/step" % ms)
     ok = bool((ss.download((n, 24)) == ss_enc.download((n, 24))).all())
+    pipe = frodo_pipelined(pqc, ctx, stream, pk[0], sk[0], n, args, rng)
     # the shared-A product alone (its decapsulation form, comparing against ct),
     # on the same stream and the same batch
     gemm = lambda: ctx.frodo_gemm(n, ct, stream=stream)
@@ -444,6 +549,7 @@ def bench_frodo(args):
             "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
             "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
             "decaps_equal_encaps": ok,
+            "pipelined": pipe,
         },
         "roofline": {
             "bound": "mfma",
@@ -459,21 +565,55 @@ def bench_frodo(args):
         },
         "cpu_baseline": None,
     }
+    pipe["over_single_batch"] = round(pipe["decaps_per_s"] / out["value"], 3)
     if not args.no_cpu:
         from tests import oracle_lib as O
         ct_h = ct.download((n, 15744))
-        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
-        m = min(n, 8 * thr)
-        t0 = time.perf_counter()
-        ss_cpu = O.frodo_dec_batch(np.ascontiguousarray(ct_h[:m]), sk[0].tobytes(), nthreads=thr)
-        dt = time.perf_counter() - t0
-        assert (ss_cpu == ss.download((n, 24))[:m]).all(), "CPU oracle and GPU disagree"
-        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "decaps/s", "cores": thr, "kind": "port",
-                               "sample": "%d FrodoKEM-976 decaps of the same batch on %d threads, sharing A; "
-                                         "outputs checked equal to the GPU's" % (m, thr)}
+        ss_h = ss.download((n, 24))
+        thr = cpu_threads(args)
+
+        def run(m):
+            ss_cpu = O.frodo_dec_batch(np.ascontiguousarray(ct_h[:m]), sk[0].tobytes(), nthreads=thr)
+            assert (ss_cpu == ss_h[:m]).all(), "CPU oracle and GPU disagree"
+
+        m, dt = timed_cpu(run, n, 2 * thr)
+        out["cpu_baseline"] = cpu_entry(m / dt, "decaps/s", thr,
+                                        "%d FrodoKEM-976 decaps of the same batch on %d threads, sharing A, timed for "
+                                        "at least 1 s; outputs checked equal to the GPU's" % (m, thr))
         out["config"]["gpu_over_cpu"] = round(out["value"] / (m / dt), 1)
-    print(json.dumps(out), flush=True)
     ctx.close()
+    return out
+
+
+def frodo_pipelined(pqc, ctx, stream, pk, sk, n, args, rng, batches=4):
+    """Decapsulation of `batches` consecutive n-ciphertext batches under the same
+    key as one call: the library runs them as n-request chunks on four streams, so
+    one chunk's serial SHAKE256 streams run beside the others' and beside their
+    GEMMs.  Reports decaps/s against the single-batch rate, and checks that the
+    first batch's shared secrets equal those of the same ciphertexts decapsulated
+    as a single batch (one stream)."""
+    pn = batches * n
+    big = pqc.Context(["frodo976"], max_batch=pn)
+    try:
+        mu = pqc.DeviceBuffer.from_array(rng.integers(0, 256, (pn, 24), dtype=np.uint8))
+        ct, ss_enc, ss = pqc.DeviceBuffer(pn * 15744), pqc.DeviceBuffer(pn * 24), pqc.DeviceBuffer(pn * 24)
+        big.frodo_load_key(pk, False, host=True)
+        big.frodo_enc(pn, ct, ss_enc, mu, stream=stream)
+        big.frodo_load_key(sk, True, host=True)
+        wall, per = timed(lambda: big.frodo_dec(pn, ss, ct, stream=stream), stream, args.steps, args.warmup, pqc)
+        ms = wall * 1e3 / args.steps
+        ss_h = ss.download((pn, 24))
+        ok = bool((ss_h == ss_enc.download((pn, 24))).all())
+        one = pqc.DeviceBuffer(n * 24)
+        ctx.frodo_dec(n, one, ct, stream=stream)  # the same first n ciphertexts, unsplit
+        stream.synchronize()
+        ok = ok and bool((one.download((n, 24)) == ss_h[:n]).all())
+    finally:
+        big.close()
+    log("frodo pipelined: %d decaps in %.3f ms" % (pn, ms))
+    return {"batch_per_call": pn, "chunk": n, "streams": min(batches, 4), "ms_per_call": round(ms, 4),
+            "decaps_per_s": round(pn / (ms / 1e3), 1),
+            "call_latency_ms_p50": round(float(np.percentile(per, 50)), 4), "bitexact": ok}
 
 
 # ---- NewHope-1024 CPA: 16384 keygen + encaps + decaps triples ----------------------
@@ -481,6 +621,10 @@ NEWHOPE_BATCH = 16384
 
 
 def bench_newhope(args):
+    print(json.dumps(run_newhope(args)), flush=True)
+
+
+def run_newhope(args):
     import pqc_amd as pqc
     n = NEWHOPE_BATCH if args.batch == BATCH else args.batch
     ctx = pqc.Context(["newhope1024cpa", "newhope1024", "newhope512cpa", "newhope512"], max_batch=n)
@@ -565,19 +709,21 @@ def bench_newhope(args):
     }
     if not args.no_cpu:
         from tests import oracle_lib as O
-        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
-        m = min(n, 256 * thr)
-        s_h, m_h, c_h = seed.download((n, 32))[:m], msg.download((n, 32))[:m], coin.download((n, 32))[:m]
-        t0 = time.perf_counter()
-        opk, osk, oct_, om2 = O.newhope_triples(s_h, m_h, c_h, nthreads=thr)
-        dt = time.perf_counter() - t0
-        assert (oct_ == ct.download((n, 2176))[:m]).all(), "CPU oracle and GPU disagree"
-        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "triples/s", "cores": thr, "kind": "port",
-                               "sample": "%d NewHope-1024 triples of the same batch on %d threads; "
-                                         "ciphertexts checked equal to the GPU's" % (m, thr)}
+        thr = cpu_threads(args)
+        s_h, m_h, c_h = seed.download((n, 32)), msg.download((n, 32)), coin.download((n, 32))
+        ct_h = ct.download((n, 2176))
+
+        def run(m):
+            opk, osk, oct_, om2 = O.newhope_triples(s_h[:m], m_h[:m], c_h[:m], nthreads=thr)
+            assert (oct_ == ct_h[:m]).all(), "CPU oracle and GPU disagree"
+
+        m, dt = timed_cpu(run, n, 64 * thr)
+        out["cpu_baseline"] = cpu_entry(m / dt, "triples/s", thr,
+                                        "%d NewHope-1024 triples of the same batch on %d threads, timed for at least "
+                                        "1 s; ciphertexts checked equal to the GPU's" % (m, thr))
         out["config"]["gpu_over_cpu"] = round(out["value"] / (m / dt), 1)
-    print(json.dumps(out), flush=True)
     ctx.close()
+    return out
 
 
 def newhope_cca_rates(pqc, ctx, stream, n, rng, steps, alg="newhope1024"):
@@ -621,6 +767,10 @@ def newhope512_rates(pqc, ctx, stream, n, rng, steps):
 # Keccak-f per decapsulation: G 1 + A^T 4 K^2 + PRF 2K+1 + H(ct) ceil((ct+1)/136) + KDF 1
 MIXED_KECCAK_PER_DECAPS = {1: 1 + 16 + 5 + 6 + 1, 2: 1 + 36 + 7 + 9 + 1, 3: 1 + 64 + 9 + 12 + 1}
 def bench_mixed(args):
+    print(json.dumps(run_mixed(args)), flush=True)
+
+
+def run_mixed(args):
     import pqc_amd as pqc
     from pqc_amd import tuner as T
     n = args.batch
@@ -641,13 +791,15 @@ def bench_mixed(args):
     log("inputs ready: %d requests (Kyber-512/768/1024 = %d/%d/%d), one key pair each" %
         (n, lay.count[1], lay.count[2], lay.count[3]))
 
-    def enc_dec(s):
-        ctx.kem_enc_mixed(n, ar.enc_reqs, ar.buf["ct"], ar.buf["pk"], ar.buf["ss"], ar.buf["coins"], stream=s)
-        ctx.kem_dec_mixed(n, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ss_dec, stream=s)
+    def enc_dec_n(m, s):
+        ctx.kem_enc_mixed(m, ar.enc_reqs, ar.buf["ct"], ar.buf["pk"], ar.buf["ss"], ar.buf["coins"], stream=s)
+        ctx.kem_dec_mixed(m, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ss_dec, stream=s)
 
-    # the autotuner picks grid waves x tile order for this batch
-    tuned = T.autotune(ctx, T.request("mixed_kyber", n, k=3, op=2, budget=0, reps=2), enc_dec, stream)
-    log("tuned: %s %.3f ms" % (tuned.best.as_dict(), tuned.best_ms))
+    def enc_dec(s):
+        enc_dec_n(n, s)
+
+    tuned, autotune = learned_autotune(ctx, T, n, enc_dec_n, stream)
+    log("tuned (model-guided): %s %.3f ms; %s" % (tuned, autotune["model_pick_ms"], autotune))
     graph = pqc.Graph.capture(stream, lambda: enc_dec(stream))
     wall, per = timed(lambda: graph.launch(stream), stream, args.steps, args.warmup, pqc)
     ms = wall * 1e3 / args.steps
@@ -696,7 +848,8 @@ def bench_mixed(args):
         "data": "synthetic: seeded key pairs (one per request) and coins, made on the GPU",
         "config": {"workload": "kyber_mixed_enc_dec_graph", "batch": n,
                    "counts_512_768_1024": [lay.count[1], lay.count[2], lay.count[3]],
-                   "tuning": tuned.best.as_dict(),
+                   "tuning": tuned,
+                   "autotune": autotune,
                    "plain_ms_per_step": round(ms_p, 4),
                    "fused_over_plain": round(ms_p / ms, 3),
                    "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
@@ -713,28 +866,72 @@ def bench_mixed(args):
     if not args.no_cpu:
         # the oracle on the host cores: encaps + decaps of a sample of each parameter set
         from tests import oracle_lib as O
-        thr = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        thr = cpu_threads(args)
         pk_h, sk_h, co_h = ar.download("pk"), ar.download("sk"), ar.download("coins")
         ct_h, ssd_h = ar.download("ct"), ss_dec.download((lay.nbytes["ss"],))
-        done, dt = 0, 0.0
-        for a in (1, 2, 3):
-            m = min(lay.count[a], 192 * thr // 4)
-            pk_a, co_a = (np.ascontiguousarray(lay.view(x, kd, a)[:m]) for x, kd in ((pk_h, "pk"), (co_h, "coins")))
-            sk_a, ct_a = np.ascontiguousarray(lay.view(sk_h, "sk", a)[:m]), np.ascontiguousarray(lay.view(ct_h, "ct", a)[:m])
-            t0 = time.perf_counter()
-            oct_, oss = O.kyber_batch("enc", a + 1, [pk_a, co_a], nthreads=thr)
-            oss2 = O.kyber_batch("dec", a + 1, [ct_a, sk_a], nthreads=thr)
-            dt += time.perf_counter() - t0
-            assert (oct_ == ct_a).all() and (oss2 == lay.view(ssd_h, "ss", a)[:m]).all(), "CPU oracle and GPU disagree"
-            done += m
-        out["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "requests/s (encaps + decaps each)",
-                               "cores": thr, "kind": "port",
-                               "sample": "%d requests (first of each parameter set's slice, 1:2:1 weights not applied) "
-                                         "encapsulated and decapsulated on %d threads; outputs checked equal to the "
-                                         "GPU's" % (done, thr)}
+        views = {a: [np.ascontiguousarray(lay.view(x, kd, a)) for x, kd in
+                     ((pk_h, "pk"), (co_h, "coins"), (sk_h, "sk"), (ct_h, "ct"), (ssd_h, "ss"))] for a in (1, 2, 3)}
+
+        def run(m):
+            # m requests in the batch's 1:2:1 proportions: m/4 Kyber-512, m/2 Kyber-768, m/4 Kyber-1024
+            for a, share in ((1, 4), (2, 2), (3, 4)):
+                q = max(1, m // share)
+                pk_a, co_a, sk_a, ct_a, ss_a = (v[:q] for v in views[a])
+                oct_, oss = O.kyber_batch("enc", a + 1, [pk_a, co_a], nthreads=thr)
+                oss2 = O.kyber_batch("dec", a + 1, [ct_a, sk_a], nthreads=thr)
+                assert (oct_ == ct_a).all() and (oss2 == ss_a).all(), "CPU oracle and GPU disagree"
+
+        done, dt = timed_cpu(run, min(lay.count[1], lay.count[3]) * 4, 64 * thr)
+        out["cpu_baseline"] = cpu_entry(done / dt, "requests/s (encaps + decaps each)", thr,
+                                        "%d requests in the batch's 1:2:1 proportions encapsulated and decapsulated "
+                                        "on %d threads, timed for at least 1 s; outputs checked equal to the GPU's"
+                                        % (done, thr))
         out["config"]["gpu_over_cpu"] = round(out["value"] / (done / dt), 1)
-    print(json.dumps(out), flush=True)
     ctx.close()
+    return out
+
+
+def learned_autotune(ctx, T, n, run_n, stream, budget=4):
+    """Grid waves x tile order for the mixed batch of n, chosen by the learned cost
+    model.  The model (trained on the GPU by the library's own kernels) learns from
+    exhaustive sweeps of the first n/4, n/2 and 3n/4 requests of the same batch;
+    then pqc_autotune ranks the 16 candidates for n with it and measures only the
+    best-ranked `budget`, installing the fastest of those.  An exhaustive sweep at
+    n follows as the reference: the pick's regret is its time in that sweep over
+    the sweep's best.  The model's pick is what the timed graph runs."""
+    train = [n // 4, n // 2, 3 * n // 4]
+    default = T.get_tuning(ctx).as_dict()
+    for m in train:
+        T.autotune(ctx, T.request("mixed_kyber", m, k=3, op=2, budget=0, reps=2), lambda s, m=m: run_n(m, s), stream)
+    x, y = T.samples(ctx)
+    model = T.CostModel.of_context(ctx)
+    loss = model.fit(x, y, steps=3000, lr=3e-3)
+    T.put_tuning(ctx, **default)
+    req = T.request("mixed_kyber", n, k=3, op=2, budget=budget, use_model=True, reps=2)
+    guided = T.autotune(ctx, req, lambda s: run_n(n, s), stream)
+    pick = guided.best.as_dict()
+    before = len(T.samples(ctx)[1])
+    sweep = T.autotune(ctx, T.request("mixed_kyber", n, k=3, op=2, budget=0, reps=2), lambda s: run_n(n, s), stream)
+    sx, sy = T.samples(ctx)
+    sx, sy = sx[before:], np.exp(sy[before:])
+    def in_sweep(tuning_dict):
+        t = T.Tuning()
+        for k, v in tuning_dict.items():
+            setattr(t, k, v)
+        f = T.features(req, t)
+        row = [i for i in range(len(sy)) if np.allclose(sx[i], f)]
+        return float(sy[row[0]]) if row else None
+
+    pick_in_sweep = in_sweep(pick) or float(guided.best_ms)
+    T.put_tuning(ctx, **pick)
+    return pick, {"train_batches": train, "train_samples": int(len(y)), "model_loss": round(float(loss), 5),
+                  "model_trials": int(guided.measured), "sweep_trials": int(sweep.measured),
+                  "trials_saved": int(sweep.measured - guided.measured),
+                  "model_pick": pick, "model_pick_ms": round(float(guided.best_ms), 4),
+                  "model_pick_ms_in_sweep": round(pick_in_sweep, 4),
+                  "sweep_best": sweep.best.as_dict(), "sweep_best_ms": round(float(sweep.best_ms), 4),
+                  "regret": round(pick_in_sweep / float(sweep.best_ms), 4),
+                  "default": default, "default_ms_in_sweep": in_sweep(default)}
 
 
 # ---- serving loop: Poisson arrivals, mixed Kyber / Frodo, 1024 keys per set ------------
@@ -751,34 +948,40 @@ def serving_trace(n, rate, seed, keys):
 
 
 def bench_serving(args):
+    print(json.dumps(run_serving(args)), flush=True)
+
+
+def run_serving(args, duration=None, with_plain=True):
     from pqc_amd.server import RequestSet, Server
-    n = int(args.rate * args.duration)
+    n = int(args.rate * (duration or args.duration))
     keys = 1024
     arrivals, algs, ops, kid, rng = serving_trace(n, args.rate, 0, keys)
     algs_names = ["kyber512", "kyber768", "kyber1024", "frodo976"]
     results = {}
     slots = 4
-    for plain in (False, True):
+    for plain, graph in ((False, False), (False, True)) + (((True, False),) if with_plain else ()):
         # the plain path cannot hold the offered rate: replay a prefix of the trace
         m = n if not plain else min(n, int(0.2 * n))
         t0 = time.perf_counter()
         srv = Server(algs_names, keys_per_alg=keys, key_seed=0, max_batch=16384, slots=slots,
-                     arena_bytes=96 << 20, max_wait_us=200, plain=plain)
+                     arena_bytes=96 << 20, max_wait_us=200, plain=plain, graph=graph)
         log("server (plain=%s) up in %.1f s: %d key pairs per set" % (plain, time.perf_counter() - t0, keys))
         # untimed warm-up: a short trace at the same rate (first launches, pinned pages)
         wm = min(m, 50000)
         ws = RequestSet.random(algs[:wm], ops[:wm], kid[:wm], rng)
         srv.run_trace(arrivals[:wm], ws.reqs)
-        rs = RequestSet.random(algs[:m], ops[:m], kid[:m], rng)
+        rs = RequestSet.random(algs[:m], ops[:m], kid[:m], np.random.default_rng(1234))  # same inputs every mode
         st, lat = srv.run_trace(arrivals[:m], rs.reqs)
-        results[plain] = (st, m)
-        log("plain=%s: %d requests, %.0f ops/s, p50 %.0f us, p99 %.0f us, mean batch %.0f, errors %d" %
-            (plain, m, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch, st.errors))
-        if not plain and not args.no_cpu:
-            cpu = serving_cpu_baseline(srv, rs, algs, ops, kid, args.cpu_threads)
+        results[(plain, graph)] = (st, m, rs)
+        log("plain=%s graph=%s: %d requests, %.0f ops/s, p50 %.0f us, p99 %.0f us, mean batch %.0f, errors %d" %
+            (plain, graph, m, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch, st.errors))
+        if not plain and not graph and not args.no_cpu:
+            cpu = serving_cpu_baseline(srv, rs, algs, ops, kid, cpu_threads(args))
         srv.close()
-    st, m = results[False]
-    sp, mp = results[True]
+    st, m, rs_e = results[(False, False)]
+    sg, mg, rs_g = results[(False, True)]
+    sp, mp, _ = results.get((True, False), (None, 0, None))
+    graph_same = bool((rs_e.outbuf == rs_g.outbuf).all())
     out = {
         "metric": "KEM serving loop at Poisson 2M req/s (90% Kyber 1:2:1 / 10% Frodo-976, 1024 keys per set): "
                   "ops/s and p50/p99 latency, fused vs plain",
@@ -797,22 +1000,27 @@ def bench_serving(args):
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "p50_us": round(st.p50_us, 1), "p90_us": round(st.p90_us, 1), "p99_us": round(st.p99_us, 1),
                    "p999_us": round(st.p999_us, 1), "mean_batch": round(st.mean_batch, 1), "errors": int(st.errors),
-                   "plain": {"requests": mp, "ops_per_s": round(sp.ops_per_s, 1), "p50_us": round(sp.p50_us, 1),
-                             "p99_us": round(sp.p99_us, 1), "mean_batch": round(sp.mean_batch, 1)}},
+                   "graph": {"ops_per_s": round(sg.ops_per_s, 1), "p50_us": round(sg.p50_us, 1),
+                             "p99_us": round(sg.p99_us, 1), "mean_batch": round(sg.mean_batch, 1),
+                             "errors": int(sg.errors), "bytes_equal_eager": graph_same},
+                   "plain": None if sp is None else {
+                       "requests": mp, "ops_per_s": round(sp.ops_per_s, 1), "p50_us": round(sp.p50_us, 1),
+                       "p99_us": round(sp.p99_us, 1), "mean_batch": round(sp.mean_batch, 1),
+                       "errors": int(sp.errors)}},
         "roofline": None,
         "cpu_baseline": None if args.no_cpu else cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def serving_cpu_baseline(srv, rs, algs, ops, kid, cpu_threads, sample=3000):
-    """The oracle on the host cores over the first `sample` requests of the same
-    trace (same mix, same keys and inputs), one request per task on a thread pool;
-    every output is checked equal to what the server returned."""
+def serving_cpu_baseline(srv, rs, algs, ops, kid, thr, min_s=1.0):
+    """The oracle on the host cores over the first requests of the same trace
+    (same mix, same keys and inputs), one request per task on a thread pool,
+    enough of them for at least `min_s` seconds; every output is checked equal to
+    what the server returned."""
     from concurrent.futures import ThreadPoolExecutor
     from tests import oracle_lib as O
-    thr = max(1, min(cpu_threads, len(os.sched_getaffinity(0))))
-    m = min(sample, rs.n)
+    m = min(rs.n, 50000)  # enough for well over a second on the host, without copying the whole trace
     keys = {}
     for i in range(m):
         keys.setdefault((int(algs[i]), int(kid[i])), None)
@@ -828,14 +1036,16 @@ def serving_cpu_baseline(srv, rs, algs, ops, kid, cpu_threads, sample=3000):
         return b"".join(O.kyber_enc(a + 1, pk, ins[i])) if op == 1 else O.kyber_dec(a + 1, ins[i], sk)
 
     with ThreadPoolExecutor(thr) as ex:
-        list(ex.map(one, range(min(m, 2 * thr))))  # warm
-        t0 = time.perf_counter()
-        got = list(ex.map(one, range(m)))
-        dt = time.perf_counter() - t0
-    assert all(got[i] == rs.out(i) for i in range(m)), "CPU oracle and server disagree"
-    return {"value": round(m / dt, 1), "unit": "requests/s", "cores": thr, "kind": "port",
-            "sample": "the first %d requests of the trace (same mix, keys and inputs), one request per task "
-                      "on %d threads, no arrival pacing; outputs checked equal to the server's" % (m, thr)}
+        def run(k):
+            got = list(ex.map(one, range(k)))
+            assert all(got[i] == rs.out(i) for i in range(k)), "CPU oracle and server disagree"
+
+        run(min(m, 2 * thr))  # warm
+        done, dt = timed_cpu(run, m, min(m, 64 * thr), min_s)
+    return cpu_entry(done / dt, "requests/s", thr,
+                     "the first requests of the trace (same mix, keys and inputs), %d in all, one request per task on "
+                     "%d threads, no arrival pacing, timed for at least %.0f s; outputs checked equal to the server's"
+                     % (done, thr, min_s))
 
 
 if __name__ == "__main__":

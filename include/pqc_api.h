@@ -249,6 +249,13 @@ void pqc_graph_destroy(pqc_graph *graph);
  * PQC_SERVER_PLAIN the plain per-op kernels.  Results are the same bytes as the
  * batch calls give for the request alone.  The reference has no counterpart.  */
 #define PQC_SERVER_PLAIN 1u
+/* PQC_SERVER_GRAPH: each fused batch (input copies, the mixed Kyber kernels, the
+ * Frodo streams forked off and joined back, output copies) is captured into a
+ * hipGraph and launched with one call.  A slot keeps one executable graph per
+ * batch shape (which request groups are present); the next batch of that shape
+ * updates the executable's parameters in place (hipGraphExecUpdate) instead of
+ * re-instantiating it.  Same bytes as the eager launches. */
+#define PQC_SERVER_GRAPH 2u
 typedef struct pqc_server pqc_server;
 typedef struct pqc_server_config {
   uint32_t alg_mask;     /* Kyber sets and/or FrodoKEM-976 */
@@ -262,7 +269,7 @@ typedef struct pqc_server_config {
   uint64_t hbm_budget;   /* bytes of in-flight request inputs + outputs, 0 = no limit */
   uint64_t arena_bytes;  /* per-slot input and output arena (pinned + device), >= 1 MiB */
   uint32_t max_wait_us;  /* while other batches run, a batch waits at most this long to fill */
-  uint32_t flags;        /* PQC_SERVER_PLAIN */
+  uint32_t flags;        /* PQC_SERVER_PLAIN or PQC_SERVER_GRAPH */
   pqc_tuning tuning;     /* launch parameters for the slots' contexts (all zero: defaults) */
 } pqc_server_config;
 typedef struct pqc_server_request {

@@ -13,6 +13,8 @@
 // int8 digits (exact mod 2^16), |S'| <= 10 fits int8, and every int32
 // accumulation stays below 976 * 10 * 128 < 2^31.  K is padded 976 -> 1024
 // and N 984 -> 1024 with zeros.
+#include <algorithm>
+
 #include "internal.h"
 #include "sponge.cuh"
 
@@ -938,6 +940,24 @@ static FrScratch fr_scratch(pqc_ctx* c) {
   return f;
 }
 
+// The scratch of elements [e0, e0 + m) as a scratch of its own: every region is
+// element-linear (S' and E' fragments: 8192 bytes per element, whole 16-row
+// m-tiles for even e0), so a chunk of a batch runs its kernels on these pointers
+// with element indices relative to e0.
+static FrScratch fr_scratch_at(pqc_ctx* c, size_t e0) {
+  FrScratch f = fr_scratch(c);
+  f.se += e0 * (31232 / 2);
+  f.efrag += e0 * 8192;
+  f.out += e0 * 8 * OUTC;
+  f.mu += e0 * 32;
+  f.kp += e0 * 32;
+  f.fstate += e0 * 50;
+  f.flags += e0 * GEMM_NB;
+  f.raw += e0 * RAWB;
+  f.sfrag += e0 * 8192;
+  return f;
+}
+
 template <int MODE>
 static int launch_gemm(pqc_ctx* c, size_t n, const FrScratch& f, uint8_t* ct, const uint8_t* mu, int mu_stride,
                        hipStream_t s) {
@@ -973,21 +993,61 @@ static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, co
                      f.efrag);
 }
 
+// ---- batches under the loaded key, pipelined over streams ------------------------
+// One request's S'/E'/E'' stream is a serial chain of 231 permutations on a lane
+// pair, so a batch of 4096 keeps only a quarter of the SIMDs busy while k_shake
+// runs.  Batches above kFrPipeChunk are cut into chunks of that many requests,
+// and chunk i runs its whole chain (w-decode, stream, sampling, shared-A GEMM,
+// F) on stream i mod 4: the caller's stream or one of the context's three side
+// streams, forked from and joined back to the caller's stream by events.  The
+// chunks' serial chains then run beside each other and beside other chunks'
+// GEMMs; every chunk has scratch of its own (fr_scratch_at), so the bytes are
+// those of the unsplit batch.
+constexpr size_t kFrPipeChunk = 4096;
+static_assert(kFrPipeChunk % 32 == 0, "chunks start on whole 256-row GEMM blocks");
+
+template <class F>
+static int frodo_pipelined(pqc_ctx* c, size_t n, hipStream_t s, F chunk) {
+  if (n <= kFrPipeChunk || !c->fr_side[0]) return chunk((size_t)0, n, s);
+  const size_t nchunks = (n + kFrPipeChunk - 1) / kFrPipeChunk;
+  const int nside = (int)std::min<size_t>(kFrSideStreams, nchunks - 1);
+  if (hipEventRecord(c->fr_fork, s) != hipSuccess) return PQC_ERR_DEVICE;
+  for (int i = 0; i < nside; ++i)
+    if (hipStreamWaitEvent(c->fr_side[i], c->fr_fork, 0) != hipSuccess) return PQC_ERR_DEVICE;
+  int rc = PQC_OK;
+  for (size_t k = 0; k < nchunks && !rc; ++k) {
+    const int lane = (int)(k % (size_t)(nside + 1));
+    hipStream_t t = lane == 0 ? s : c->fr_side[lane - 1];
+    const size_t e0 = k * kFrPipeChunk;
+    rc = chunk(e0, std::min(kFrPipeChunk, n - e0), t);
+  }
+  // join every side stream back, also after an error, so none is left forked
+  for (int i = 0; i < nside; ++i)
+    if (hipEventRecord(c->fr_join[i], c->fr_side[i]) != hipSuccess ||
+        hipStreamWaitEvent(s, c->fr_join[i], 0) != hipSuccess)
+      rc = rc ? rc : PQC_ERR_DEVICE;
+  return rc;
+}
+
 int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t s) {
   int rc = frodo_need(c);
   if (rc) return rc;
   if (!c->fr_loaded || !c->fr_has_sk) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
-  FrScratch f = fr_scratch(c);
   const FrKeys keys = loaded_key(c);
-  const int ni = (int)n;
-  hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, keys, f.mu);
-  launch_stream<false>(c, f, ni, 1, keys, f.mu, 32, ct, s);
-  rc = launch_gemm<kDecaps>(c, n, f, const_cast<uint8_t*>(ct), f.mu, 32, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, s, ni, ss, ct, (const uint64_t*)f.fstate,
-                     (const uint8_t*)f.flags, (const uint8_t*)f.kp, keys);
-  return hip_err(hipGetLastError());
+  return frodo_pipelined(c, n, s, [&](size_t e0, size_t m, hipStream_t t) {
+    const FrScratch f = fr_scratch_at(c, e0);
+    const int ni = (int)m;
+    const uint8_t* ctc = ct + e0 * CT;
+    uint8_t* ssc = ss + e0 * SS;
+    hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, t, ni, ctc, keys, f.mu);
+    launch_stream<false>(c, f, ni, 1, keys, f.mu, 32, ctc, t);
+    int r = launch_gemm<kDecaps>(c, m, f, const_cast<uint8_t*>(ctc), f.mu, 32, t);
+    if (r) return r;
+    hipLaunchKernelGGL(k_f_final, dim3((ni + 63) / 64), dim3(64), 0, t, ni, ssc, ctc, (const uint64_t*)f.fstate,
+                       (const uint8_t*)f.flags, (const uint8_t*)f.kp, keys);
+    return hip_err(hipGetLastError());
+  });
 }
 
 // ---- many keys ------------------------------------------------------------------------
@@ -1064,14 +1124,19 @@ int frodo_enc(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu,
   if (rc) return rc;
   if (!c->fr_loaded) return PQC_ERR_INVALID_ARG;
   if (n == 0) return PQC_OK;
-  FrScratch f = fr_scratch(c);
-  const int ni = (int)n;
-  launch_stream<false>(c, f, ni, 0, loaded_key(c), mu, MU, nullptr, s);
-  rc = launch_gemm<kEncaps>(c, n, f, ct, mu, MU, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, s, ni, ss, (const uint8_t*)ct,
-                     (const uint8_t*)f.kp);
-  return hip_err(hipGetLastError());
+  const FrKeys keys = loaded_key(c);
+  return frodo_pipelined(c, n, s, [&](size_t e0, size_t m, hipStream_t t) {
+    const FrScratch f = fr_scratch_at(c, e0);
+    const int ni = (int)m;
+    uint8_t* ctc = ct + e0 * CT;
+    const uint8_t* muc = mu + e0 * MU;
+    launch_stream<false>(c, f, ni, 0, keys, muc, MU, nullptr, t);
+    int r = launch_gemm<kEncaps>(c, m, f, ctc, muc, MU, t);
+    if (r) return r;
+    hipLaunchKernelGGL(k_f_full, dim3((2 * ni + 63) / 64), dim3(64), 0, t, ni, ss + e0 * SS, (const uint8_t*)ctc,
+                       (const uint8_t*)f.kp);
+    return hip_err(hipGetLastError());
+  });
 }
 
 int frodo_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* rnd, hipStream_t s) {

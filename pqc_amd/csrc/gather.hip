@@ -1,5 +1,6 @@
-// Record gather for the serving loop's plain-path comparison: copies the key of
-// each request out of a key store into a dense per-request array.
+// Small kernels of the serving loop: the record gather of the plain-path
+// comparison (copies the key of each request out of a key store into a dense
+// per-request array), and a word fill.
 #include "internal.h"
 
 namespace pqc {
@@ -19,6 +20,19 @@ int gather_records(size_t n, uint8_t* dst, size_t bytes, const uint8_t* src, siz
   const size_t total = n * (bytes / 8);
   hipLaunchKernelGGL(k_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, (int)n, (int)(bytes / 8),
                      reinterpret_cast<uint64_t*>(dst), src, src_stride, idx);
+  return hip_err(hipGetLastError());
+}
+
+// Zero `n` 32-bit words with a kernel (a memset node inside a captured graph is
+// avoided: fill kernels replay reliably).
+__global__ void k_zero_words(int n, uint32_t* __restrict__ p) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0u;
+}
+
+int zero_words(uint32_t* p, int n, hipStream_t s) {
+  if (n <= 0) return PQC_OK;
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p);
   return hip_err(hipGetLastError());
 }
 

@@ -6,6 +6,8 @@
 
 #include "../../include/pqc_api.h"
 
+constexpr int kFrSideStreams = 3;
+
 struct pqc_ctx {
   int device = 0;
   unsigned alg_mask = 0;
@@ -50,6 +52,10 @@ struct pqc_ctx {
   uint8_t* fr_pkh = nullptr;
   uint8_t* fr_elem = nullptr;
   int fr_loaded = 0, fr_has_sk = 0;
+  // side streams (default priority) and events that pipeline batches under the
+  // loaded key in chunks (frodo.hip)
+  hipStream_t fr_side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fr_fork = nullptr, fr_join[3] = {nullptr, nullptr, nullptr};
 
   // staging for host-pointer calls: device buffers, and (hostio.cpp) the
   // pinned double buffers with their copy threads
@@ -135,6 +141,7 @@ int frodo_dec_store(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const 
 int frodo_enc_store(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* mu, const uint8_t* sk_store,
                     const uint16_t* a_store, const uint32_t* kid, hipStream_t s);
 // dst[r] = src[idx[r]] for n records of `bytes` (multiple of 8) bytes (gather.hip)
+int zero_words(uint32_t* p, int n, hipStream_t s);
 int gather_records(size_t n, uint8_t* dst, size_t bytes, const uint8_t* src, size_t src_stride, const uint32_t* idx,
                    hipStream_t s);
 // the library's deterministic generator: SHAKE256("pqc-mi355x" || seed || stream || index)

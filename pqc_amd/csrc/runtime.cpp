@@ -444,6 +444,14 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     rc = ctx_alloc(c, &p, (size_t)std::min<size_t>(c->ld, 128) * 976 * 976 * 2);
     if (rc) goto fail;
     c->fr_abuf = (uint16_t*)p;
+    bool ok = hipEventCreateWithFlags(&c->fr_fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kFrSideStreams; ++i)
+      ok = hipStreamCreateWithFlags(&c->fr_side[i], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&c->fr_join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      rc = PQC_ERR_DEVICE;
+      goto fail;
+    }
   }
   rc = ctx_alloc(c, &p, 256);
   if (rc) goto fail;
@@ -480,6 +488,11 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->nh_fork) (void)hipEventDestroy(c->nh_fork);
   if (c->nh_join) (void)hipEventDestroy(c->nh_join);
   if (c->nh_side) (void)hipStreamDestroy(c->nh_side);
+  for (int i = 0; i < kFrSideStreams; ++i) {
+    if (c->fr_side[i]) (void)hipStreamDestroy(c->fr_side[i]);
+    if (c->fr_join[i]) (void)hipEventDestroy(c->fr_join[i]);
+  }
+  if (c->fr_fork) (void)hipEventDestroy(c->fr_fork);
   if (c->stage) (void)hipFree(c->stage);
   pqc::hostio_free(c);
   if (c->status) (void)hipFree(c->status);

@@ -22,6 +22,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -78,6 +79,7 @@ struct Slot {
   std::vector<size_t> out_a, out_b, len_a, len_b;  // where each request's output lies in h_out
   std::vector<size_t> in_at;                       // where each request's input goes in h_in
   uint32_t* h_status = nullptr;  // pinned: the device status words of ctx and ctx_fd after the batch
+  std::map<uint32_t, hipGraphExec_t> graphs;  // graph mode: one executable per batch shape (groups present)
   size_t bytes = 0;
   bool busy = false;
 };
@@ -119,6 +121,9 @@ namespace {
 constexpr size_t kCopyPer = 128;  // requests per copy thread (about 140 KB of Kyber-768 ciphertexts)
 
 void free_slot(Slot& sl) {
+  for (auto& g : sl.graphs)
+    if (g.second) (void)hipGraphExecDestroy(g.second);
+  sl.graphs.clear();
   if (sl.ctx) pqc_ctx_destroy(sl.ctx);
   if (sl.ctx_fd) pqc_ctx_destroy(sl.ctx_fd);
   for (hipStream_t t : {sl.s, sl.s2, sl.s3})
@@ -136,6 +141,29 @@ void free_server(pqc_server* v) {
   for (void* p : {(void*)v->d_pk, (void*)v->d_sk, (void*)v->d_fsk, (void*)v->d_fa})
     if (p) (void)hipFree(p);
   delete v;
+}
+
+// Graph mode: the slot's executable graph for this batch shape takes the new
+// batch's parameters (sizes, counts, grids) by an in-place update; a first batch
+// of a shape, or an update the runtime refuses (a different kernel, say), makes
+// a new executable.  The slot's previous batch has finished before the slot is
+// reused, so no update touches a graph in flight.
+int launch_graph(Slot& sl, uint32_t shape, hipGraph_t g, hipStream_t st) {
+  hipGraphExec_t& ex = sl.graphs[shape];
+  if (ex) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+    if (hipGraphExecUpdate(ex, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(ex);
+      ex = nullptr;
+    }
+  }
+  if (!ex && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+    ex = nullptr;
+    return PQC_ERR_DEVICE;
+  }
+  return hipGraphLaunch(ex, st) == hipSuccess ? PQC_OK : PQC_ERR_DEVICE;
 }
 
 // Launch one slot's batch: H2D inputs, kernels, D2H outputs, event.
@@ -223,83 +251,101 @@ int launch(pqc_server* v, Slot& sl) {
   const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
   if (hipEventRecord(sl.start, st) != hipSuccess) return PQC_ERR_DEVICE;
   sl.h_status[0] = sl.h_status[1] = 0;
-  if (hipMemsetAsync(sl.ctx->status, 0, 4, st) != hipSuccess ||
-      (sl.ctx_fd && hipMemsetAsync(sl.ctx_fd->status, 0, 4, st) != hipSuccess))
-    return PQC_ERR_DEVICE;
-  if (hipMemcpyAsync(sl.d_in, sl.h_in, io, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
-  if (hipMemcpyAsync(sl.d_kid, sl.h_kid, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
-  // fused mode: Frodo encapsulation and decapsulation each run on a stream of their
-  // own once the inputs are in
-  hipStream_t sf = plain ? st : sl.s2, sd = plain ? st : sl.s3;
-  if (!plain && (cnt[6] || cnt[7])) {
-    if (hipEventRecord(sl.in_ready, st) != hipSuccess) return PQC_ERR_DEVICE;
-    if (cnt[6] && hipStreamWaitEvent(sf, sl.in_ready, 0) != hipSuccess) return PQC_ERR_DEVICE;
-    if (cnt[7] && hipStreamWaitEvent(sd, sl.in_ready, 0) != hipSuccess) return PQC_ERR_DEVICE;
-  }
-  int rc = PQC_OK;
-  if (!plain) {
-    if (n_kyber &&
-        hipMemcpyAsync(sl.d_desc, sl.h_desc, n_kyber * sizeof(pqc_kem_req), hipMemcpyHostToDevice, st) != hipSuccess)
+  const bool graph = !plain && (v->cfg.flags & PQC_SERVER_GRAPH) != 0;
+  if (graph && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PQC_ERR_DEVICE;
+  // everything from here to the output copies is stream-ordered work on st (and
+  // the Frodo streams forked off it), recorded into a graph in graph mode
+  auto issue = [&]() -> int {
+    if (pqc::zero_words(sl.ctx->status, 1, st) || (sl.ctx_fd && pqc::zero_words(sl.ctx_fd->status, 1, st)))
       return PQC_ERR_DEVICE;
-    if (n_enc)
-      rc = pqc_kem_enc_mixed(sl.ctx, n_enc, sl.d_desc, sl.d_out, v->arena, v->d_pk, v->pk_bytes, sl.d_out, v->arena,
-                             sl.d_in, v->arena, st);
-    if (!rc && n_kyber > n_enc)
-      rc = pqc_kem_dec_mixed(sl.ctx, n_kyber - n_enc, sl.d_desc + n_enc, sl.d_in, v->arena, v->d_sk, v->sk_bytes,
-                             sl.d_out, v->arena, st);
-  } else {
-    for (int g = 0; g < 6 && !rc; ++g) {
-      if (!cnt[g]) continue;
-      const int kk = g % 3, K = kk + 2;
-      const Set& s = gs[g];
-      if (g < 3) {
-        rc = pqc::gather_records(cnt[g], sl.d_keys, s.pk, v->d_pk + v->pk_base[kk], s.pk, sl.d_kid + grp_first[g], st);
-        if (!rc)
-          rc = pqc::kyber_plain_enc(sl.ctx, K, cnt[g], sl.d_out + out_base[g], sl.d_out + out2_base[g], sl.d_keys,
-                                    sl.d_in + in_base[g], st);
-      } else {
-        rc = pqc::gather_records(cnt[g], sl.d_keys, s.sk, v->d_sk + v->sk_base[kk], s.sk, sl.d_kid + grp_first[g], st);
-        if (!rc)
-          rc = pqc::kyber_plain_dec(sl.ctx, K, cnt[g], sl.d_out + out2_base[g], sl.d_in + in_base[g], sl.d_keys, st);
+    if (hipMemcpyAsync(sl.d_in, sl.h_in, io, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
+    if (hipMemcpyAsync(sl.d_kid, sl.h_kid, n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return PQC_ERR_DEVICE;
+    // fused mode: Frodo encapsulation and decapsulation each run on a stream of their
+    // own once the inputs are in
+    hipStream_t sf = plain ? st : sl.s2, sd = plain ? st : sl.s3;
+    if (!plain && (cnt[6] || cnt[7])) {
+      if (hipEventRecord(sl.in_ready, st) != hipSuccess) return PQC_ERR_DEVICE;
+      if (cnt[6] && hipStreamWaitEvent(sf, sl.in_ready, 0) != hipSuccess) return PQC_ERR_DEVICE;
+      if (cnt[7] && hipStreamWaitEvent(sd, sl.in_ready, 0) != hipSuccess) return PQC_ERR_DEVICE;
+    }
+    int rc = PQC_OK;
+    if (!plain) {
+      if (n_kyber &&
+          hipMemcpyAsync(sl.d_desc, sl.h_desc, n_kyber * sizeof(pqc_kem_req), hipMemcpyHostToDevice, st) != hipSuccess)
+        return PQC_ERR_DEVICE;
+      if (n_enc)
+        rc = pqc_kem_enc_mixed(sl.ctx, n_enc, sl.d_desc, sl.d_out, v->arena, v->d_pk, v->pk_bytes, sl.d_out, v->arena,
+                               sl.d_in, v->arena, st);
+      if (!rc && n_kyber > n_enc)
+        rc = pqc_kem_dec_mixed(sl.ctx, n_kyber - n_enc, sl.d_desc + n_enc, sl.d_in, v->arena, v->d_sk, v->sk_bytes,
+                               sl.d_out, v->arena, st);
+    } else {
+      for (int g = 0; g < 6 && !rc; ++g) {
+        if (!cnt[g]) continue;
+        const int kk = g % 3, K = kk + 2;
+        const Set& s = gs[g];
+        if (g < 3) {
+          rc = pqc::gather_records(cnt[g], sl.d_keys, s.pk, v->d_pk + v->pk_base[kk], s.pk, sl.d_kid + grp_first[g], st);
+          if (!rc)
+            rc = pqc::kyber_plain_enc(sl.ctx, K, cnt[g], sl.d_out + out_base[g], sl.d_out + out2_base[g], sl.d_keys,
+                                      sl.d_in + in_base[g], st);
+        } else {
+          rc = pqc::gather_records(cnt[g], sl.d_keys, s.sk, v->d_sk + v->sk_base[kk], s.sk, sl.d_kid + grp_first[g], st);
+          if (!rc)
+            rc = pqc::kyber_plain_dec(sl.ctx, K, cnt[g], sl.d_out + out2_base[g], sl.d_in + in_base[g], sl.d_keys, st);
+        }
       }
     }
-  }
-  // Frodo groups
-  if (!rc && cnt[6]) {
-    if (!plain) {
-      rc = pqc::frodo_enc_store(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6], sl.d_in + in_base[6],
-                                v->d_fsk, v->d_fa, sl.d_kid + grp_first[6], sf);
-    } else {
-      // public keys gathered out of the secret keys (pk at byte 24 of sk); records
-      // start 8 bytes early to keep 8-byte alignment, so the pk sits at +8.  A is
-      // expanded per request.
-      rc = pqc::gather_records(cnt[6], sl.d_keys, 15632 + 8, v->d_fsk + 24 - 8, 31296, sl.d_kid + grp_first[6], st);
-      if (!rc)
-        rc = pqc::frodo_enc_batch_strided(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6],
-                                          sl.d_keys + 8, 15632 + 8, sl.d_in + in_base[6], st);
+    // Frodo groups
+    if (!rc && cnt[6]) {
+      if (!plain) {
+        rc = pqc::frodo_enc_store(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6], sl.d_in + in_base[6],
+                                  v->d_fsk, v->d_fa, sl.d_kid + grp_first[6], sf);
+      } else {
+        // public keys gathered out of the secret keys (pk at byte 24 of sk); records
+        // start 8 bytes early to keep 8-byte alignment, so the pk sits at +8.  A is
+        // expanded per request.
+        rc = pqc::gather_records(cnt[6], sl.d_keys, 15632 + 8, v->d_fsk + 24 - 8, 31296, sl.d_kid + grp_first[6], st);
+        if (!rc)
+          rc = pqc::frodo_enc_batch_strided(sl.ctx, cnt[6], sl.d_out + out_base[6], sl.d_out + out2_base[6],
+                                            sl.d_keys + 8, 15632 + 8, sl.d_in + in_base[6], st);
+      }
     }
-  }
-  if (!rc && cnt[7]) {
-    if (!plain) {
-      rc = pqc::frodo_dec_store(sl.ctx_fd, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], v->d_fsk,
-                                v->d_fa, sl.d_kid + grp_first[7], sd);
-    } else {
-      rc = pqc::gather_records(cnt[7], sl.d_keys, 31296, v->d_fsk, 31296, sl.d_kid + grp_first[7], st);
-      if (!rc) rc = pqc::frodo_dec_batch(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], sl.d_keys, st);
+    if (!rc && cnt[7]) {
+      if (!plain) {
+        rc = pqc::frodo_dec_store(sl.ctx_fd, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], v->d_fsk,
+                                  v->d_fa, sl.d_kid + grp_first[7], sd);
+      } else {
+        rc = pqc::gather_records(cnt[7], sl.d_keys, 31296, v->d_fsk, 31296, sl.d_kid + grp_first[7], st);
+        if (!rc) rc = pqc::frodo_dec_batch(sl.ctx, cnt[7], sl.d_out + out2_base[7], sl.d_in + in_base[7], sl.d_keys, st);
+      }
     }
+    if (!rc && !plain && cnt[6] &&
+        (hipEventRecord(sl.frodo_done, sf) != hipSuccess || hipStreamWaitEvent(st, sl.frodo_done, 0) != hipSuccess))
+      rc = PQC_ERR_DEVICE;
+    if (!rc && !plain && cnt[7] &&
+        (hipEventRecord(sl.fd_done, sd) != hipSuccess || hipStreamWaitEvent(st, sl.fd_done, 0) != hipSuccess))
+      rc = PQC_ERR_DEVICE;
+    if (rc) return rc;
+    if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
+    // the contexts' status words (cleared before the batch) come back with the outputs
+    if (hipMemcpyAsync(sl.h_status, sl.ctx->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (sl.ctx_fd && hipMemcpyAsync(sl.h_status + 1, sl.ctx_fd->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess))
+      return PQC_ERR_DEVICE;
+    return PQC_OK;
+  };
+  int rc = issue();
+  if (graph) {
+    // the capture always ends, also after an error, so the stream leaves capture mode
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (!rc && (e != hipSuccess || !g)) rc = PQC_ERR_DEVICE;
+    uint32_t shape = 0;
+    for (int k = 0; k < kGroups; ++k) shape |= cnt[k] ? 1u << k : 0u;
+    if (!rc) rc = launch_graph(sl, shape, g, st);
+    if (g) (void)hipGraphDestroy(g);
   }
-  if (!rc && !plain && cnt[6] &&
-      (hipEventRecord(sl.frodo_done, sf) != hipSuccess || hipStreamWaitEvent(st, sl.frodo_done, 0) != hipSuccess))
-    rc = PQC_ERR_DEVICE;
-  if (!rc && !plain && cnt[7] &&
-      (hipEventRecord(sl.fd_done, sd) != hipSuccess || hipStreamWaitEvent(st, sl.fd_done, 0) != hipSuccess))
-    rc = PQC_ERR_DEVICE;
   if (rc) return rc;
-  if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
-  // the contexts' status words (cleared before the batch) come back with the outputs
-  if (hipMemcpyAsync(sl.h_status, sl.ctx->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      (sl.ctx_fd && hipMemcpyAsync(sl.h_status + 1, sl.ctx_fd->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess))
-    return PQC_ERR_DEVICE;
   if (hipEventRecord(sl.done, st) != hipSuccess) return PQC_ERR_DEVICE;
   return PQC_OK;
 }

@@ -10,6 +10,7 @@ from .kem import alg_id, sizes
 from .tuner import Tuning
 
 PLAIN = 1
+GRAPH = 2
 
 REQ_DTYPE = np.dtype([("alg", "<i4"), ("op", "<i4"), ("key_id", "<u4"), ("reserved", "<u4"), ("in", "<u8"),
                       ("out", "<u8"), ("tag", "<u8")])
@@ -56,7 +57,7 @@ def in_out_sizes(alg, op):
 
 class Server:
     def __init__(self, algs, keys_per_alg=1024, key_seed=0, max_batch=8192, slots=3, hbm_budget=0,
-                 arena_bytes=64 << 20, max_wait_us=200, plain=False, tuning=None):
+                 arena_bytes=64 << 20, max_wait_us=200, plain=False, tuning=None, graph=False):
         cfg = Config()
         for a in algs:
             cfg.alg_mask |= 1 << alg_id(a)
@@ -67,7 +68,7 @@ class Server:
         cfg.hbm_budget = hbm_budget
         cfg.arena_bytes = arena_bytes
         cfg.max_wait_us = max_wait_us
-        cfg.flags = PLAIN if plain else 0
+        cfg.flags = (PLAIN if plain else 0) | (GRAPH if graph else 0)
         if tuning is not None:
             cfg.tuning = tuning
         h = C.c_void_p()

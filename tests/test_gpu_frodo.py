@@ -157,3 +157,47 @@ def test_nist_dropins_frodo976(ctx):
     assert pqc_amd.crypto_kem_dec(ct, sk, "frodo976") == ss
     assert O.frodo_dec(ct, sk) == ss
     pqc_amd.randombytes_system()
+
+
+def test_pipelined_chunks_equal_unsplit_and_oracle(keys):
+    """Batches above 4096 under one key run in 4096-request chunks on four
+    streams.  9000 requests (chunks 4096, 4096, 808; the last partial) must
+    give the bytes of 4096-request batches over the same inputs, which run
+    unsplit on one stream, and the oracle's on samples from every chunk."""
+    import pqc_amd
+    n = 9000
+    say("frodo pipelined n", n)
+    pk, sk = keys
+    rng = np.random.default_rng(23)
+    mu = rng.integers(0, 256, (n, 24), dtype=np.uint8)
+    big = pqc_amd.Context(["frodo976"], max_batch=n)
+    try:
+        big.frodo_load_key(np.frombuffer(pk, np.uint8).copy(), False, host=True)
+        d_mu = pqc_amd.DeviceBuffer.from_array(mu)
+        d_ct, d_ss = pqc_amd.DeviceBuffer(n * CT), pqc_amd.DeviceBuffer(n * 24)
+        big.frodo_enc(n, d_ct, d_ss, d_mu)
+        ct, ss = d_ct.download((n, CT)), d_ss.download((n, 24))
+        bad = ct.copy()
+        bad[::13, 777] ^= 0x01
+        big.frodo_load_key(np.frombuffer(sk, np.uint8).copy(), True, host=True)
+        d_bad, d_ss2 = pqc_amd.DeviceBuffer.from_array(bad), pqc_amd.DeviceBuffer(n * 24)
+        big.frodo_dec(n, d_ss2, d_bad)
+        ss2 = d_ss2.download((n, 24))
+    finally:
+        big.close()
+    # unsplit 4096-request batches over the same inputs
+    one = pqc_amd.Context(["frodo976"], max_batch=4096)
+    try:
+        for lo in (0, 4096, n - 4096):
+            ct1, ss1 = one.frodo_enc_host(pk, mu[lo:lo + 4096])
+            assert (ct1 == ct[lo:lo + 4096]).all() and (ss1 == ss[lo:lo + 4096]).all(), lo
+            one.frodo_load_key(np.frombuffer(sk, np.uint8).copy(), True, host=True)
+            assert (one.frodo_dec_host(sk, bad[lo:lo + 4096]) == ss2[lo:lo + 4096]).all(), lo
+    finally:
+        one.close()
+    same = (ss2 == ss).all(axis=1)
+    assert (~same[::13]).all() and same[np.arange(n) % 13 != 0].all()
+    idx = np.array([0, 4095, 4096, 8191, 8192, 8999])
+    oct_, oss = O.frodo_enc_batch(pk, mu[idx], nthreads=8)
+    assert (ct[idx] == oct_).all() and (ss[idx] == oss).all()
+    assert (ss2[idx] == O.frodo_dec_batch(bad[idx], sk, nthreads=8)).all()

@@ -162,3 +162,84 @@ def test_admission_holds_the_budget():
     with pytest.raises(pqc_amd.PQCError):
         s2.submit(big.reqs)
     s2.close()
+
+
+def _oracle_out(srv, a, op, k, inp):
+    pk, sk = srv.key(a, k)
+    if op == 2:
+        return O.frodo_dec(inp, sk.tobytes()) if a == FRODO else O.kyber_dec(K_OF[a], inp, sk.tobytes())
+    ct, ss = O.frodo_enc(pk.tobytes(), inp) if a == FRODO else O.kyber_enc(K_OF[a], pk.tobytes(), inp)
+    return ct + ss
+
+
+def test_graph_mode_equals_eager_and_oracle():
+    """PQC_SERVER_GRAPH: every batch is captured into a graph and launched once;
+    a slot's graph of one batch shape is updated in place for the next batch of
+    that shape.  Over a trace of many batches (so every graph is replayed many
+    times, with the eager host work between replays) the bytes must equal the
+    eager launches' on every request, and the oracle on a sample."""
+    from pqc_amd.server import RequestSet, Server
+    n = 4000
+    say("server graph vs eager, n", n)
+    rng = np.random.default_rng(11)
+    algs = rng.choice([1, 2, 2, 3, 1, 2, 2, 3, 2, 5], size=n).astype(np.int32)
+    ops = rng.choice([1, 2], size=n).astype(np.int32)
+    kid = rng.integers(0, 16, size=n).astype(np.uint32)
+    arrivals = trace(n, 200000, 12)
+    outs, sets = [], []
+    for graph in (False, True):
+        s = Server(ALGS, keys_per_alg=16, key_seed=13, max_batch=128, slots=3, arena_bytes=32 << 20,
+                   max_wait_us=50, graph=graph)
+        rs = RequestSet.random(algs, ops, kid, np.random.default_rng(14))
+        st, lat = s.run_trace(arrivals, rs.reqs)
+        say(" graph=%s: %.0f ops/s p50 %.0f us p99 %.0f us mean batch %.1f errors %d" %
+            (graph, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch, st.errors))
+        assert st.completed == n and st.errors == 0
+        assert s.stats().batches >= 32  # at most 128 requests per batch
+        outs.append(rs.outbuf.copy())
+        sets.append(rs)
+        if graph:
+            for i in list(range(0, n, 97))[:40]:
+                a, op, k = int(algs[i]), int(ops[i]), int(kid[i])
+                inp = rs.inbuf[rs.in_off[i]:rs.in_off[i + 1]].tobytes()
+                assert rs.out(i) == _oracle_out(s, a, op, k, inp), i
+        s.close()
+    assert (sets[0].inbuf == sets[1].inbuf).all()
+    assert (outs[0] == outs[1]).all()
+
+
+def test_config_shape_slice_of_2m_trace():
+    """The serving configuration's shape: a 0.05 s slice of the Poisson trace at
+    2 M requests/s (90% Kyber-512/768/1024 at 1:2:1, 10% FrodoKEM-976, encaps and
+    decaps 50/50) against 1024 device-resident key pairs per set.  Fused (graph
+    mode) and plain servers must complete every request without error and give
+    the same bytes; 50 requests are checked against the oracle."""
+    from pqc_amd.server import RequestSet, Server
+    n = 100000  # 0.05 s at 2 M/s
+    say("server config-shape slice, n", n)
+    rng = np.random.default_rng(21)
+    arrivals = trace(n, 2e6, 22)
+    kyber = rng.choice(np.array([1, 2, 2, 3], np.int32), size=n)
+    algs = np.where(rng.random(n) < 0.9, kyber, FRODO).astype(np.int32)
+    ops = rng.choice(np.array([1, 2], np.int32), size=n)
+    kid = rng.integers(0, 1024, size=n).astype(np.uint32)
+    outs = []
+    for plain in (False, True):
+        s = Server(ALGS, keys_per_alg=1024, key_seed=0, max_batch=16384, slots=4, arena_bytes=96 << 20,
+                   max_wait_us=200, plain=plain, graph=not plain)
+        rs = RequestSet.random(algs, ops, kid, np.random.default_rng(23))
+        st, lat = s.run_trace(arrivals, rs.reqs)
+        say(" plain=%s: %.0f ops/s p50 %.0f us p99 %.0f us mean batch %.1f errors %d" %
+            (plain, st.ops_per_s, st.p50_us, st.p99_us, st.mean_batch, st.errors))
+        assert st.completed == n and st.errors == 0
+        outs.append(rs.outbuf.copy())
+        if not plain:
+            pick = np.concatenate([np.flatnonzero(algs == a)[:12] for a in (1, 2, 3)] +
+                                  [np.flatnonzero(algs == FRODO)[:14]])
+            assert len(pick) == 50
+            for i in pick:
+                a, op, k = int(algs[i]), int(ops[i]), int(kid[i])
+                inp = rs.inbuf[rs.in_off[i]:rs.in_off[i + 1]].tobytes()
+                assert rs.out(i) == _oracle_out(s, a, op, k, inp), i
+        s.close()
+    assert (outs[0] == outs[1]).all()

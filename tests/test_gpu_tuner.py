@@ -128,7 +128,10 @@ def test_trained_model_beats_untuned_and_blind_choice(pqc):
         (n, best, ms[pick], regret_model, ms[0], regret_default, regret_blind))
     assert regret_model <= 1.06
     assert regret_model < regret_blind
-    assert regret_model <= regret_default + 0.01
+    # the held-out case is one where the library default is measurably suboptimal,
+    # and the model's pick (made without timing anything) gains at least 5% of it
+    assert regret_default >= 1.05, "the default is near-optimal here: the case shows no gain to learn"
+    assert regret_model <= 0.95 * regret_default
 
     # end to end: pqc_autotune with the model and a budget of one measurement
     T.put_tuning(ctx, **{k: v for k, v in default.as_dict().items()})
