@@ -83,7 +83,10 @@ typedef struct pqc_tuning {
                                  workgroup per request (latency) instead of 64-request tiles
                                  (throughput): 0 = library defaults (8192 decapsulations, 4096
                                  encapsulations), -1 = never, else 1..65536 for both */
-  int32_t reserved[2];        /* 0 */
+  int32_t frodo_chunk;        /* FrodoKEM-976 batches under one key above this many requests are cut
+                                 into chunks of this many, run on four streams: 0 = library default
+                                 (no split up to 65536), else a multiple of 32 in 32..65536 */
+  int32_t reserved;           /* 0 */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
 int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */

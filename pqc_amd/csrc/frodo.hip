@@ -993,21 +993,25 @@ static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, co
                      f.efrag);
 }
 
-// ---- batches under the loaded key, pipelined over streams ------------------------
+// ---- batches under the loaded key, optionally cut into chunks over streams ------
 // One request's S'/E'/E'' stream is a serial chain of 231 permutations on a lane
-// pair, so a batch of 4096 keeps only a quarter of the SIMDs busy while k_shake
-// runs.  Batches above kFrPipeChunk are cut into chunks of that many requests,
-// and chunk i runs its whole chain (w-decode, stream, sampling, shared-A GEMM,
-// F) on stream i mod 4: the caller's stream or one of the context's three side
-// streams, forked from and joined back to the caller's stream by events.  The
-// chunks' serial chains then run beside each other and beside other chunks'
-// GEMMs; every chunk has scratch of its own (fr_scratch_at), so the bytes are
-// those of the unsplit batch.
-constexpr size_t kFrPipeChunk = 4096;
-static_assert(kFrPipeChunk % 32 == 0, "chunks start on whole 256-row GEMM blocks");
+// pair (plus a pair for the F prefix), so a batch of 4096 keeps only a quarter of
+// the SIMDs busy while k_shake runs; 16384 requests put one wave on every SIMD.
+// Batches above tune.frodo_chunk requests are cut into chunks of that many, and
+// chunk i runs its whole chain (w-decode, stream, sampling, shared-A GEMM, F) on
+// stream i mod 4: the caller's stream or one of the context's three side
+// streams, forked from and joined back to the caller's stream by events.  Every
+// chunk has scratch of its own (fr_scratch_at), so the bytes are those of the
+// unsplit batch.  Measured on MI355X, four concurrent 4096-request chains slow
+// each other's k_shake to 1.6-3.2 ms (1.1 ms alone), so one chain over the whole
+// batch is faster than chunks of 4096; the default therefore splits only above
+// 65536 requests, and the knob stays for the autotuner.
+constexpr size_t kFrPipeChunkDefault = 65536;
+static_assert(kFrPipeChunkDefault % 32 == 0, "chunks start on whole 256-row GEMM blocks");
 
 template <class F>
 static int frodo_pipelined(pqc_ctx* c, size_t n, hipStream_t s, F chunk) {
+  const size_t kFrPipeChunk = c->tune.frodo_chunk > 0 ? (size_t)c->tune.frodo_chunk : kFrPipeChunkDefault;
   if (n <= kFrPipeChunk || !c->fr_side[0]) return chunk((size_t)0, n, s);
   const size_t nchunks = (n + kFrPipeChunk - 1) / kFrPipeChunk;
   const int nside = (int)std::min<size_t>(kFrSideStreams, nchunks - 1);

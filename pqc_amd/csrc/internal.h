@@ -68,7 +68,7 @@ struct pqc_ctx {
   size_t allocated = 0;  // bytes of device memory owned by the context
   int cu_count = 0;
   int fused_grid_override = 0;
-  pqc_tuning tune = {256, 128, 4, 0, 64, 0, {0, 0}};
+  pqc_tuning tune = {256, 128, 4, 0, 64, 0, 0, 0};
   void* tuner = nullptr;  // autotuner samples and the learned cost model (tuner.cpp)
 };
 

@@ -160,8 +160,8 @@ def test_nist_dropins_frodo976(ctx):
 
 
 def test_pipelined_chunks_equal_unsplit_and_oracle(keys):
-    """Batches above 4096 under one key run in 4096-request chunks on four
-    streams.  9000 requests (chunks 4096, 4096, 808; the last partial) must
+    """With frodo_chunk = 4096, batches above 4096 under one key run in
+    4096-request chunks on four streams.  9000 requests (chunks 4096, 4096, 808; the last partial) must
     give the bytes of 4096-request batches over the same inputs, which run
     unsplit on one stream, and the oracle's on samples from every chunk."""
     import pqc_amd
@@ -170,8 +170,10 @@ def test_pipelined_chunks_equal_unsplit_and_oracle(keys):
     pk, sk = keys
     rng = np.random.default_rng(23)
     mu = rng.integers(0, 256, (n, 24), dtype=np.uint8)
+    from pqc_amd import tuner as T
     big = pqc_amd.Context(["frodo976"], max_batch=n)
     try:
+        T.put_tuning(big, frodo_chunk=4096)  # chunks of 4096 on four streams
         big.frodo_load_key(np.frombuffer(pk, np.uint8).copy(), False, host=True)
         d_mu = pqc_amd.DeviceBuffer.from_array(mu)
         d_ct, d_ss = pqc_amd.DeviceBuffer(n * CT), pqc_amd.DeviceBuffer(n * 24)
