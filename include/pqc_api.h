@@ -86,7 +86,9 @@ typedef struct pqc_tuning {
   int32_t frodo_chunk;        /* FrodoKEM-976 batches under one key above this many requests are cut
                                  into chunks of this many, run on four streams: 0 = library default
                                  (no split up to 65536), else a multiple of 32 in 32..65536 */
-  int32_t reserved;           /* 0 */
+  int32_t fused_dec_tile;     /* requests per tile of the fused Kyber decapsulation: 32 (two waves per
+                                 SIMD, A^T in a global scratch) or 64 (one wave per SIMD, A^T in an LDS
+                                 ring); 0 = library default (32) */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
 int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */

@@ -69,6 +69,9 @@ struct pqc_ctx {
   int cu_count = 0;
   int fused_grid_override = 0;
   pqc_tuning tune = {256, 128, 4, 0, 64, 0, 0, 0};
+  // fused decapsulation in 32-request tiles: per-wave A^T scratch for kat_waves waves
+  int16_t* kat = nullptr;
+  size_t kat_waves = 0;
   void* tuner = nullptr;  // autotuner samples and the learned cost model (tuner.cpp)
 };
 
@@ -142,6 +145,7 @@ int frodo_enc_store(pqc_ctx* c, size_t n, uint8_t* ct, uint8_t* ss, const uint8_
                     const uint16_t* a_store, const uint32_t* kid, hipStream_t s);
 // dst[r] = src[idx[r]] for n records of `bytes` (multiple of 8) bytes (gather.hip)
 int zero_words(uint32_t* p, int n, hipStream_t s);
+size_t tile32_scratch_bytes(int k);  // per wave, A^T scratch of the 32-request decapsulation tiles
 int gather_records(size_t n, uint8_t* dst, size_t bytes, const uint8_t* src, size_t src_stride, const uint32_t* idx,
                    hipStream_t s);
 // the library's deterministic generator: SHAKE256("pqc-mi355x" || seed || stream || index)

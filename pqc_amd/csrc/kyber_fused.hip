@@ -65,7 +65,7 @@ struct Lds {
 // groups and the secret key's 12-bit groups.
 template <int K>
 struct DecIn {
-  uint32_t u[K][3], s[K][3], v[3];
+  uint32_t u[K][3], s[K][2], v[3];
 };
 
 template <int K>
@@ -95,7 +95,7 @@ __device__ __forceinline__ void coop_indcpa_dec(const DecIn<K>& in, const LaneZ&
     for (int r = 0; r < 4; ++r) u.v[r] = decompress_d<P::du>(t[r]);
     ntt(u, z);
     P4 s;
-    poly12_loaded(s, in.s[i]);
+    poly12_loaded(s, in.s[i], ntt_group(l));
     basemul_acc(mp, s, u, z.bm);
   }
   reduce(mp);
@@ -136,17 +136,25 @@ __device__ __forceinline__ void pack_pair(uint8_t* dst, const uint32_t (&t)[4]) 
 template <int K, bool DEC>
 struct EncIn {
   uint32_t prf[2 * K + 1];
-  uint32_t t[K][3];
+  uint32_t t[K][2];
   uint32_t cu[DEC ? K : 1][3], cv[3];
 };
 
-template <int K, bool DEC>
+// PRF_LDS: the PRF rows are in LDS (the per-request kernels); otherwise in the
+// global scratch, read through a global-address-space pointer.
+template <int K, bool DEC, bool PRF_LDS = false>
 __device__ __forceinline__ void enc_load(EncIn<K, DEC>& in, const uint8_t* prf, const uint8_t* pk, const uint8_t* ct) {
   using P = Params<K>;
   const int l = lane_id();
-  const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
+  if constexpr (PRF_LDS) {
+    const uint16_t* prf16 = reinterpret_cast<const uint16_t*>(prf);
 #pragma unroll
-  for (int j = 0; j < 2 * K + 1; ++j) in.prf[j] = prf16[64 * j + l];
+    for (int j = 0; j < 2 * K + 1; ++j) in.prf[j] = prf16[64 * j + l];
+  } else {
+    const global_t<uint16_t>* prf16 = gptr(reinterpret_cast<const uint16_t*>(prf));
+#pragma unroll
+    for (int j = 0; j < 2 * K + 1; ++j) in.prf[j] = prf16[64 * j + l];
+  }
 #pragma unroll
   for (int j = 0; j < K; ++j) load_h12(in.t[j], pk + 384 * j, ntt_group(l));
   if constexpr (DEC) {
@@ -157,12 +165,25 @@ __device__ __forceinline__ void enc_load(EncIn<K, DEC>& in, const uint8_t* prf, 
 }
 
 // Re-encryption of one request (indcpa_enc from sp-sampling on).  DEC: compare
-// with ct and return the lane's mismatch bits.  ENC: write ct.
-template <int K, bool DEC>
-__device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, const EncIn<K, DEC>& in,
-                                                   const uint8_t* msg, uint8_t* ct, const LaneZ& z) {
+// with ct and return the lane's mismatch bits.  ENC: write ct.  The A^T entries
+// come from `at`: at.get(i, j, w0, w1) gives this lane's group of entry (i, j) as
+// two words of two coefficients.
+template <int K>
+struct AtRing {  // the LDS ring of the 64-request tiles: entry e0 + i K + j in slot mod RING
+  const int16_t* ring;
+  int e0;
+  __device__ __forceinline__ void get(int i, int j, uint32_t& w0, uint32_t& w1) const {
+    const int slot = (e0 + i * K + j) % Tile<K>::RING;
+    const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * ntt_group(lane_id()));
+    w0 = e32[0];
+    w1 = e32[1];
+  }
+};
+
+template <int K, bool DEC, class AT>
+__device__ __forceinline__ uint32_t coop_reencrypt(const AT& at, const EncIn<K, DEC>& in, const uint8_t* msg,
+                                                   uint8_t* ct, const LaneZ& z) {
   using P = Params<K>;
-  constexpr int RING = Tile<K>::RING;
   const int l = lane_id();
   const int gn = ntt_group(l);
   P4 sp[K];
@@ -177,9 +198,8 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
     P4 acc = {{0, 0, 0, 0}};
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int slot = (e0 + i * K + j) % RING;
-      const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
-      const uint32_t w0 = e32[0], w1 = e32[1];
+      uint32_t w0, w1;
+      at.get(i, j, w0, w1);
       P4 a = {{(int32_t)(w0 & 0xFFFF), (int32_t)(w0 >> 16), (int32_t)(w1 & 0xFFFF), (int32_t)(w1 >> 16)}};
       basemul_acc(acc, a, sp[j], z.bm);
     }
@@ -204,7 +224,7 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       P4 t;
-      poly12_loaded(t, in.t[j]);
+      poly12_loaded(t, in.t[j], gn);
       basemul_acc(acc, t, sp[j], z.bm);
     }
     reduce(acc);
@@ -238,7 +258,10 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const int16_t* ring, int e0, 
 // count is clamped to 256 after each word's four candidates, so accepts past
 // the 256th land in guard slots 256..259 (kAStride leaves 262).  Returns the
 // accept count, clamped (256 unless the block cap was hit).
-__device__ __forceinline__ int sample_entry(int16_t* dst, const uint64_t (&rho)[4], int x, int y, bool valid) {
+// DST is an LDS pointer (the ring) or a global-address-space one (the scratch of
+// the 32-request tiles).
+template <class DST>
+__device__ __forceinline__ int sample_entry(DST dst, const uint64_t (&rho)[4], int x, int y, bool valid) {
   uint64_t A[25];
 #pragma unroll
   for (int t = 0; t < 25; ++t) A[t] = (t < 4) ? rho[t] : 0;
@@ -350,18 +373,21 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
     }
     // ---- indcpa_dec, wave-cooperative, one request at a time
     const int nt = min(TL::T, n - base);
+    // request t's words were loaded during request t - 1: the copy at the top of
+    // the iteration is the only wait for them, and the prefetch issued after it
+    // is not waited for until the next iteration
     DecIn<K> cur, nxt;
     {
       const ReqP q = lane_req(me, 0);
-      dec_load<K>(cur, q.ct, q.key);
+      dec_load<K>(nxt, q.ct, q.key);
     }
     for (int t = 0; t < nt; ++t) {
+      cur = nxt;
       if (t + 1 < nt) {
         const ReqP q = lane_req(me, t + 1);
         dec_load<K>(nxt, q.ct, q.key);
       }
       coop_indcpa_dec<K>(cur, z, sh.msg[t]);
-      cur = nxt;
     }
     __syncthreads();
   } else {
@@ -388,7 +414,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       m[w] = load_u64(msg + 8 * w);
-      m[4 + w] = DEC ? load_u64(me.key + P::hpk_off + 8 * w) : hct[w];
+      m[4 + w] = DEC ? load_u64_global(me.key + P::hpk_off + 8 * w) : hct[w];
     }
     absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
     uint64_t rc4[4];
@@ -417,7 +443,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
     const int t = (64 * g + lane) / TL::KK;
     const uint8_t* key = reinterpret_cast<const uint8_t*>(__shfl((long long)me.key, t < nt ? t : 0));
 #pragma unroll
-    for (int w = 0; w < 4; ++w) rho[w] = load_u64(key + PK_OFF + P::polyvec_bytes + 8 * w);
+    for (int w = 0; w < 4; ++w) rho[w] = load_u64_global(key + PK_OFF + P::polyvec_bytes + 8 * w);
   };
   uint64_t rho_next[4];
   load_rho(rho_next, 0);
@@ -451,7 +477,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
         enc_load<K, DEC>(nxt, q.prf, q.key + PK_OFF, q.ct);
       }
       const ReqP q = lane_req(me, t);
-      const uint32_t mm = coop_reencrypt<K, DEC>(sh.a, t * TL::KK, cur, sh.msg[t], q.ct, z);
+      const uint32_t mm = coop_reencrypt<K, DEC>(AtRing<K>{sh.a, t * TL::KK}, cur, sh.msg[t], q.ct, z);
       cur = nxt;
       if constexpr (DEC) {
         const uint32_t f = __any(mm != 0) ? 1u : 0u;
@@ -469,7 +495,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
       const uint64_t mask = 0 - (uint64_t)(sh.fail[lane < TL::T ? lane : 0] & 1u);
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
-        const uint64_t zz = load_u64(me.key + P::z_off + 8 * w);
+        const uint64_t zz = load_u64_global(me.key + P::z_off + 8 * w);
         m[w] = kbar[w] ^ (mask & (zz ^ kbar[w]));
         m[4 + w] = hct[w];
       }
@@ -488,6 +514,289 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
     }
   }
   __syncthreads();
+}
+
+// ---- decapsulation in 32-request tiles, two waves per SIMD --------------------------
+// fused_tile runs one wave per SIMD: a batch of 65536 is 1024 tiles, one per SIMD,
+// and its A^T ring takes 40 KB of LDS, so four waves fill a CU's 160 KB.  A lone
+// wave issues a VALU instruction every ~4.8 cycles; two waves per SIMD issue the
+// same Keccak mix at ~3.9 cycles per instruction per SIMD (measured on MI355X).
+// This tile serves 32 requests, so 65536 requests are 2048 waves, two per SIMD,
+// and its A^T entries go to a per-wave global scratch (written and read back by
+// the same wave while the lines are in L2) instead of LDS: ~1.2 KB of LDS per
+// wave.  So that the per-request sponge chains still keep all 64 lanes busy with
+// 32 requests, lanes 0..31 absorb H(ct) of request `lane` while lanes 32..63
+// compute G and the 2K + 1 PRF outputs of request `lane - 32` in the same
+// permutations (H(ct) is the longer chain for every K).  Same bytes as fused_tile.
+template <int K>
+struct Tile32 {
+  static constexpr int T = 32;
+  static constexpr int KK = K * K;
+  static constexpr int NR = (T * KK + 63) / 64;                              // sampling rounds: 2, 5, 8
+  static constexpr int HB = Params<K>::ct_bytes / 8 / kRateSha3_256 + 1;     // H(ct) blocks: 6, 9, 12
+  static constexpr int ENTRIES = 64 * NR;                                    // scratch entries per wave
+  static_assert(HB >= 2 * K + 2, "the H(ct) lanes run the longest chain of the sponge phase");
+};
+constexpr int kAtStride = 264;  // int16 per A^T entry in the scratch: 256 + guard slots, 16-byte multiple
+static_assert(kAtStride >= N + 4 && (kAtStride * 2) % 16 == 0, "entry stride");
+
+// Sampling stage: one row per lane.  A block's candidates go to the row at their
+// entry position minus base8 (the entry's last 8-aligned position before the
+// block), so that whole 16-byte chunks can be copied to the global entry; the
+// row holds the 0..7 earlier coefficients of that chunk, the block's <= 84
+// accepts, up to 3 accepts past the 256th (clamped per word) and the guard slot
+// for rejected candidates.  A 208-byte row (13 x 16 B) spreads the lanes' rows
+// over 16 bank offsets.
+constexpr int kStageRow = 104;   // int16 per lane row
+constexpr int kStageGuard = 103; // rejected candidates land here
+static_assert(7 + 84 + 3 < kStageGuard && kStageRow % 8 == 0 && (kStageRow / 8) % 2 == 1, "stage row");
+
+struct Lds32 {
+  alignas(16) int16_t stage[64][kStageRow];
+  uint8_t msg[32][32];
+  uint32_t fail[32];
+};
+
+// SHAKE128(rho || x || y) rejection-sampled into the global entry `dst` (as
+// sample_entry), through this lane's LDS stage row: after each block the chunks
+// [base8, round_up_8(ctr)) are copied to dst with 16-byte stores, and the last,
+// partial chunk moves to the front of the row for the next block.  So every
+// global store is a whole 16 bytes of one entry (a 2-byte store per candidate
+// would be one L2 request each).  Returns the accept count, clamped at 256.
+__device__ __forceinline__ int sample_entry_staged(global_t<int16_t>* dst, int16_t* row, const uint64_t (&rho)[4],
+                                                   int x, int y, bool valid) {
+  uint64_t A[25];
+#pragma unroll
+  for (int t = 0; t < 25; ++t) A[t] = (t < 4) ? rho[t] : 0;
+  A[4] = (uint64_t)x | ((uint64_t)y << 8) | (0x1Full << 16);
+  A[kRateShake128 - 1] ^= 0x8000000000000000ull;
+  int ctr = 0, base8 = 0;
+  for (int blk = 0; blk < kMaxBlocks; ++blk) {
+    if (blk >= 4 && !__any(valid && ctr < N)) break;
+    keccakf1600(A);
+#pragma unroll
+    for (int w = 0; w < kRateShake128; ++w) {
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
+        const bool ok = val < 19u * Q;
+        const int32_t red = (int32_t)val - (int32_t)(val >> 12) * Q;
+        row[ok ? ctr - base8 : kStageGuard] = (int16_t)red;
+        ctr += ok ? 1 : 0;
+      }
+      ctr = min(ctr, N);
+    }
+    // whole 16-byte chunks of [base8, ctr): at most (7 + 84 + 7) / 8 = 12
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const int nchunks = (ctr - base8 + 7) >> 3;
+#pragma unroll
+    for (int c = 0; c < (7 + 84 + 7) / 8; ++c) {
+      if (c < nchunks) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(row + 8 * c);
+        *(global_t<u32x4>*)(dst + base8 + 8 * c) = v;
+      }
+    }
+    // the partial chunk that holds position ctr starts the row for the next block
+    const int keep = (ctr & ~7) - base8;
+    const u32x4 t = *reinterpret_cast<const u32x4*>(row + keep);
+    *reinterpret_cast<u32x4*>(row) = t;
+    base8 = ctr & ~7;
+  }
+  return ctr;
+}
+
+// This lane's group of the A^T entries of one request, loaded from the scratch
+// ahead of their use (the entries of request t start at entry t K^2).
+template <int K>
+struct AtRegs {
+  uint32_t w[K * K][2];
+  __device__ __forceinline__ void load(const global_t<int16_t>* at, int t) {
+    const global_t<uint32_t>* p =
+        (const global_t<uint32_t>*)(at + (size_t)t * K * K * kAtStride + 4 * ntt_group(lane_id()));
+#pragma unroll
+    for (int e = 0; e < K * K; ++e) {
+      w[e][0] = p[e * (kAtStride / 2)];
+      w[e][1] = p[e * (kAtStride / 2) + 1];
+    }
+  }
+  __device__ __forceinline__ void get(int i, int j, uint32_t& w0, uint32_t& w1) const {
+    w0 = w[i * K + j][0];
+    w1 = w[i * K + j][1];
+  }
+};
+
+template <int K, class Src>
+__device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds32& sh, global_t<int16_t>* at,
+                                           const LaneZ& z, uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  using TL = Tile32<K>;
+  const int lane = threadIdx.x;
+  const int rq = lane & 31;  // the request this lane serves in the sponge phases (lanes l and l + 32 alike)
+  const int nt = min(TL::T, n - base);
+  const bool act = rq < nt;
+  const ReqP me = src.template req<K, true>(act ? base + rq : base);
+  const bool hl = lane < 32;  // H(ct) lane; the others compute G and the PRF outputs
+
+  // ---- indcpa_dec, wave-cooperative, one request at a time; request t's words are
+  // loaded during request t - 1
+  {
+    DecIn<K> cur, nxt;
+    {
+      const ReqP q = lane_req(me, 0);
+      dec_load<K>(nxt, q.ct, q.key);
+    }
+    for (int t = 0; t < nt; ++t) {
+      cur = nxt;
+      if (t + 1 < nt) {
+        const ReqP q = lane_req(me, t + 1);
+        dec_load<K>(nxt, q.ct, q.key);
+      }
+      coop_indcpa_dec<K>(cur, z, sh.msg[t]);
+    }
+  }
+  __syncthreads();
+
+  // ---- one sponge phase: lanes 0..31 H(ct) (SHA3-256, HB blocks); lanes 32..63
+  // step 0: (Kbar, coins) = G(m || H(pk)), steps 1 .. 2K+1: PRF(coins, step - 1)
+  uint64_t hct[4], kbar[4] = {0, 0, 0, 0};
+  {
+    constexpr int W = P::ct_bytes / 8;
+    constexpr int R = kRateSha3_256;
+    static_assert(W >= R, "the first H(ct) block is whole");
+    const global_t<uint64_t>* cw = gptr(reinterpret_cast<const uint64_t*>(me.ct));
+    uint64_t A[25], cur[R], coins[4];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) A[i] = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) coins[w] = 0;
+    if (hl) {
+#pragma unroll
+      for (int w = 0; w < R; ++w) cur[w] = cw[w];
+    } else {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        A[w] = load_u64(sh.msg[rq] + 8 * w);
+        A[4 + w] = load_u64_global(me.key + P::hpk_off + 8 * w);
+      }
+      A[8] ^= 0x06;
+      A[kRateSha3_512 - 1] ^= 0x8000000000000000ull;
+    }
+    for (int st = 0; st < TL::HB; ++st) {
+      if (hl) {
+        const int rem = W - st * R;  // message words left (uniform), >= 0
+#pragma unroll
+        for (int w = 0; w < R; ++w) {
+          if (w < rem) A[w] ^= cur[w];
+          if (w == rem) A[w] ^= 0x06;
+        }
+        if (st == TL::HB - 1) A[R - 1] ^= 0x8000000000000000ull;
+        // the next block's words load during this permutation
+#pragma unroll
+        for (int w = 0; w < R; ++w)
+          if (w < rem - R) cur[w] = cw[(st + 1) * R + w];
+      } else if (st >= 1 && st <= 2 * K + 1) {
+#pragma unroll
+        for (int i = 0; i < 25; ++i) A[i] = i < 4 ? coins[i] : 0;
+        A[4] = (uint64_t)(st - 1) | (0x1Full << 8);
+        A[kRateShake256 - 1] ^= 0x8000000000000000ull;
+      }
+      keccakf1600(A);
+      if (!hl) {
+        if (st == 0) {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) {
+            kbar[w] = A[w];
+            coins[w] = A[4 + w];
+          }
+        } else if (st <= 2 * K + 1 && act) {
+#pragma unroll
+          for (int w = 0; w < 16; ++w) store_u64(me.prf + 128 * (st - 1) + 8 * w, A[w]);
+        }
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) hct[w] = A[w];  // lanes 0..31
+  }
+  __syncthreads();
+
+  // ---- A^T rounds of 64 entries into the scratch, then the re-encryption of every
+  // request whose entries are complete
+  const int ng = (nt * TL::KK + 63) / 64;
+  int t_done = 0;
+  auto load_rho = [&](uint64_t (&rho)[4], int g) {
+    const int t = (64 * g + lane) / TL::KK;
+    const uint8_t* key = reinterpret_cast<const uint8_t*>(__shfl((long long)me.key, t < nt ? t : 0));
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rho[w] = load_u64_global(key + P::sk_cpa_bytes + P::polyvec_bytes + 8 * w);
+  };
+  uint64_t rho_next[4];
+  load_rho(rho_next, 0);
+  for (int g = 0; g < ng; ++g) {
+    {
+      const int E = 64 * g + lane;
+      const int t = E / TL::KK;
+      const int ij = E % TL::KK;
+      const bool valid = t < nt;
+      uint64_t rho[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) rho[w] = rho_next[w];
+      const int got = sample_entry_staged(at + (size_t)E * kAtStride, sh.stage[lane], rho, ij / K, ij % K, valid);
+      if (valid && got < N) atomicOr(status, 1u);
+    }
+    __syncthreads();  // the round's entries are in the scratch for every lane
+    if (g + 1 < ng) load_rho(rho_next, g + 1);
+    const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
+    if (t_done < t_end) {
+      EncIn<K, true> cur, nxt;
+      {
+        const ReqP q = lane_req(me, t_done);
+        enc_load<K, true>(nxt, q.prf, q.key + P::sk_cpa_bytes, q.ct);
+      }
+      for (int t = t_done; t < t_end; ++t) {
+        cur = nxt;
+        if (t + 1 < t_end) {
+          const ReqP q = lane_req(me, t + 1);
+          enc_load<K, true>(nxt, q.prf, q.key + P::sk_cpa_bytes, q.ct);
+        }
+        AtRegs<K> a;
+        a.load(at, t);
+        const uint32_t mm = coop_reencrypt<K, true>(a, cur, sh.msg[t], nullptr, z);
+        const uint32_t f = __any(mm != 0) ? 1u : 0u;
+        if (lane == 0) sh.fail[t] = f;
+      }
+      t_done = t_end;
+    }
+  }
+  __syncthreads();
+
+  // ---- KDF(Kbar or z, H(ct)) on lanes 0..31; Kbar comes from lane + 32
+  {
+    uint64_t m[8], A[25];
+    const uint64_t mask = 0 - (uint64_t)(sh.fail[rq] & 1u);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint64_t kb = (uint64_t)__shfl_xor((long long)kbar[w], 32);
+      const uint64_t zz = load_u64_global(me.key + P::z_off + 8 * w);
+      m[w] = kb ^ (mask & (zz ^ kb));
+      m[4 + w] = hct[w];
+    }
+    absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+    if (hl && act) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) store_u64(me.ss + 8 * w, A[w]);
+    }
+  }
+  __syncthreads();
+}
+
+template <int K>
+__global__ __launch_bounds__(64, 2) void k_fused_dec32(int n, int ntiles, DenseSrc src, int16_t* at_scratch,
+                                                       uint32_t* __restrict__ status) {
+  __shared__ Lds32 sh;
+  LaneZ z;
+  load_lane_z(z, &c_lane_tw);
+  global_t<int16_t>* at = (global_t<int16_t>*)(at_scratch + (size_t)blockIdx.x * Tile32<K>::ENTRIES * kAtStride);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) dec_tile32<K>(src, tile * 32, n, sh, at, z, status);
 }
 
 // ---- key generation -----------------------------------------------------------------
@@ -784,8 +1093,8 @@ __device__ __forceinline__ void lowlat_dec_body(uint8_t* ss, const uint8_t* c, c
   // ---- phase 2
   if (wave == 0) {
     EncIn<K, true> in;
-    enc_load<K, true>(in, sh.prf, pk, c);
-    const uint32_t mm = coop_reencrypt<K, true>(sh.a, 0, in, sh.msg, nullptr, z);
+    enc_load<K, true, true>(in, sh.prf, pk, c);
+    const uint32_t mm = coop_reencrypt<K, true>(AtRing<K>{sh.a, 0}, in, sh.msg, nullptr, z);
     const uint32_t f = __any(mm != 0) ? 1u : 0u;
     if (l == 0) sh.fail = f;
   } else if (wave == 2 && l < 2) {
@@ -843,8 +1152,8 @@ __device__ __forceinline__ void lowlat_enc_body(uint8_t* ss, uint8_t* c, const u
     lowlat_g_prf<K>(sh, h, l);
     wave_sync();
     EncIn<K, false> in;
-    enc_load<K, false>(in, sh.prf, pk, nullptr);
-    (void)coop_reencrypt<K, false>(sh.a, 0, in, sh.msg, sh.ct, z);
+    enc_load<K, false, true>(in, sh.prf, pk, nullptr);
+    (void)coop_reencrypt<K, false>(AtRing<K>{sh.a, 0}, in, sh.msg, sh.ct, z);
   }
   __syncthreads();
   // ---- phase 3
@@ -1152,6 +1461,19 @@ static int launch_lowlat(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const u
 }
 
 template <int K>
+static int launch_dec32(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const uint8_t* sk, hipStream_t s) {
+  const int ntiles = (int)((n + Tile32<K>::T - 1) / Tile32<K>::T);
+  // two waves per SIMD at the default tuning (fused_waves_per_cu 4 -> 8 waves per CU)
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * 2 * c->tune.fused_waves_per_cu;
+  grid = std::min(grid, ntiles);
+  grid = std::min<long long>(grid, (long long)c->kat_waves);  // one scratch region per wave
+  if (grid < 1) return PQC_ERR_INVALID_ARG;
+  const DenseSrc src{ss, ct, sk, nullptr, c->kprf};
+  hipLaunchKernelGGL((k_fused_dec32<K>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, src, c->kat, c->status);
+  return hip_err(hipGetLastError());
+}
+
+template <int K>
 static int launch_fused_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
   if (n == 0) return PQC_OK;
   const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
@@ -1175,8 +1497,22 @@ int kyber_fused_keypair(pqc_ctx* c, int k, size_t n, uint8_t* pk, uint8_t* sk, c
   }
 }
 
+size_t tile32_scratch_bytes(int k) {
+  using namespace kyber::fused;
+  const int entries = k == 2 ? Tile32<2>::ENTRIES : k == 3 ? Tile32<3>::ENTRIES : Tile32<4>::ENTRIES;
+  return (size_t)entries * kAtStride * sizeof(int16_t);
+}
+
 int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct, const uint8_t* sk, hipStream_t s) {
   uint8_t* ctm = const_cast<uint8_t*>(ct);  // read-only in the decaps instantiation
+  if (n > 0 && !kyber::fused::use_lowlat(c, n, true) && c->tune.fused_dec_tile != 64 && c->kat) {
+    switch (k) {
+      case 2: return kyber::fused::launch_dec32<2>(c, n, ss, ctm, sk, s);
+      case 3: return kyber::fused::launch_dec32<3>(c, n, ss, ctm, sk, s);
+      case 4: return kyber::fused::launch_dec32<4>(c, n, ss, ctm, sk, s);
+      default: return PQC_ERR_UNSUPPORTED;
+    }
+  }
   if (kyber::fused::use_lowlat(c, n, true)) {
     switch (k) {
       case 2: return kyber::fused::launch_lowlat<2, true>(c, n, ss, ctm, sk, nullptr, s);

@@ -184,14 +184,34 @@ __device__ __forceinline__ void load_poly12(P4& p, const uint8_t* base, int g) {
 
 // The same unpackings split into a load (issued early, e.g. for the next
 // request while the current one computes) and the arithmetic on the loaded words.
+//
+// The loads are written so that nothing but the load itself happens where they
+// are issued: the words are kept as loaded and taken apart only where they are
+// used.  (A load whose value is masked, selected or split at the load site makes
+// the wave wait for it right there, which turns a prefetch into a stall.)  They
+// go through global-address-space pointers: global loads count in vmcnt only,
+// while flat loads also count in lgkmcnt, so every wait for an LDS result would
+// also wait for the prefetches in flight.
+template <class T>
+using global_t = __attribute__((address_space(1))) T;
+
+template <class T>
+__device__ __forceinline__ const global_t<T>* gptr(const T* p) {
+  return (const global_t<T>*)p;
+}
+
+// Words 0..2 of group g's bits.  Word 1 (2) is only needed when the group's bits
+// reach it; otherwise the load reads word 0 again, so no dword outside the ones
+// holding the group's bytes is ever read, and no data-dependent select is needed
+// (unpack_loaded never takes a bit from a word the group does not reach).
 template <int D>
 __device__ __forceinline__ void load_group(uint32_t (&w)[3], const uint8_t* base, int g) {
   const uint32_t bit = (uint32_t)(4 * D) * (uint32_t)g;
-  const uint32_t* p = reinterpret_cast<const uint32_t*>(base) + (bit >> 5);
+  const global_t<uint32_t>* p = gptr(reinterpret_cast<const uint32_t*>(base)) + (bit >> 5);
   const int s = (int)(bit & 31);
   w[0] = p[0];
-  w[1] = (s + 4 * D > 32) ? p[1] : 0u;  // as unpack_group: never a dword no bit is taken from
-  w[2] = (s + 4 * D > 64) ? p[2] : 0u;
+  w[1] = p[(s + 4 * D > 32) ? 1 : 0];
+  w[2] = p[(s + 4 * D > 64) ? 2 : 0];
 }
 
 template <int D>
@@ -203,18 +223,19 @@ __device__ __forceinline__ void unpack_loaded(uint32_t (&t)[4], const uint32_t (
   for (int i = 0; i < 4; ++i) t[i] = (uint32_t)(v >> (D * i)) & ((1u << D) - 1);
 }
 
-__device__ __forceinline__ void load_h12(uint32_t (&h)[3], const uint8_t* base, int g) {
-  const uint16_t* p = reinterpret_cast<const uint16_t*>(base + 6 * g);
+// 12-bit group g (6 bytes at 6g, poly_frombytes) as the two aligned dwords that
+// hold it: bytes 6g .. 6g + 5 start at byte 0 or 2 of the first (base is 4-byte
+// aligned, and 6g + 5 < 384 keeps both dwords inside the polynomial).
+__device__ __forceinline__ void load_h12(uint32_t (&h)[2], const uint8_t* base, int g) {
+  const global_t<uint32_t>* p = gptr(reinterpret_cast<const uint32_t*>(base + ((6 * g) & ~3)));
   h[0] = p[0];
   h[1] = p[1];
-  h[2] = p[2];
 }
 
-__device__ __forceinline__ void poly12_loaded(P4& p, const uint32_t (&h)[3]) {
-  p.v[0] = (int32_t)(h[0] & 0xFFF);
-  p.v[1] = (int32_t)((h[0] >> 12) | ((h[1] & 0xFF) << 4));
-  p.v[2] = (int32_t)((h[1] >> 8) | ((h[2] & 0xF) << 8));
-  p.v[3] = (int32_t)(h[2] >> 4);
+__device__ __forceinline__ void poly12_loaded(P4& p, const uint32_t (&h)[2], int g) {
+  const uint64_t v = (((uint64_t)h[1] << 32) | h[0]) >> (8 * ((6 * g) & 3));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) p.v[r] = (int32_t)((uint32_t)(v >> (12 * r)) & 0xFFF);
 }
 
 // CBD (eta = 2) of the group's 16 noise bits (cbd.cu:42-63): coefficient c uses

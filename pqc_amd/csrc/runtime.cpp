@@ -395,6 +395,12 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     rc = ctx_alloc(c, &p, c->kprf_per_elem * c->ld);
     if (rc) goto fail;
     c->kprf = (uint8_t*)p;
+    // A^T scratch of the 32-request decapsulation tiles: one region per wave of the
+    // largest grid (two waves per SIMD) the batch size can use
+    c->kat_waves = std::min<size_t>((size_t)c->cu_count * 8, (max_batch + 31) / 32);
+    rc = ctx_alloc(c, &p, pqc::tile32_scratch_bytes(kmax) * c->kat_waves);
+    if (rc) goto fail;
+    c->kat = (int16_t*)p;
     rc = ctx_alloc(c, &p, (size_t)3 * c->ld * sizeof(uint32_t) + 256);
     if (rc) goto fail;
     c->kmix_order = (uint32_t*)p;
@@ -474,6 +480,7 @@ void pqc_ctx_destroy(pqc_ctx* c) {
   if (c->kpolys) (void)hipFree(c->kpolys);
   if (c->kbytes) (void)hipFree(c->kbytes);
   if (c->kprf) (void)hipFree(c->kprf);
+  if (c->kat) (void)hipFree(c->kat);
   if (c->kmix_order) (void)hipFree(c->kmix_order);
   if (c->fr_a16) (void)hipFree(c->fr_a16);
   if (c->fr_abuf) (void)hipFree(c->fr_abuf);
@@ -521,7 +528,7 @@ int pqc_ctx_put_tuning(pqc_ctx* c, const pqc_tuning* t) {
       t->fused_waves_per_cu > 4 || t->mixed_order < 0 || t->mixed_order > 3 ||
       (t->frodo_stream_block != 64 && t->frodo_stream_block != 128 && t->frodo_stream_block != 256) ||
       t->lowlat_max < -1 || t->lowlat_max > 65536 || t->frodo_chunk < 0 || t->frodo_chunk > 65536 ||
-      t->frodo_chunk % 32 != 0)
+      t->frodo_chunk % 32 != 0 || (t->fused_dec_tile != 0 && t->fused_dec_tile != 32 && t->fused_dec_tile != 64))
     return PQC_ERR_INVALID_ARG;
   c->tune = *t;
   return PQC_OK;

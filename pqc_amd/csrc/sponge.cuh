@@ -21,6 +21,12 @@ __device__ __forceinline__ uint64_t load_u64(const uint8_t* p) {
   return *reinterpret_cast<const uint64_t*>(p);
 }
 
+// The same load from global memory, as a global (not flat) load: it counts in
+// vmcnt only, so waits for LDS results do not also wait for it.
+__device__ __forceinline__ uint64_t load_u64_global(const uint8_t* p) {
+  return *(const __attribute__((address_space(1))) uint64_t*)p;
+}
+
 __device__ __forceinline__ void store_u64(uint8_t* p, uint64_t v) {
   *reinterpret_cast<uint64_t*>(p) = v;
 }

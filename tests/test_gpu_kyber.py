@@ -291,3 +291,36 @@ def test_lowlat_decaps_equals_tile_and_oracle(pqc, k, n):
         assert (low == want).all()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 95, 1000, 9001])
+def test_dec_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
+    """The fused decapsulation in 32-request tiles (two waves per SIMD, A^T in a
+    global scratch; the default) against the 64-request tiles, the plain per-op
+    kernels and the oracle, on ragged batch sizes (partial tiles, partial
+    sampling rounds) and with every third ciphertext tampered (implicit
+    rejection).  lowlat_max = -1 keeps small batches on the tile kernels."""
+    from pqc_amd import tuner as T
+    say("dec tile32 k", k, "n", n)
+    alg = ALG[k]
+    ctx = pqc.Context([alg], max_batch=max(n, 64))
+    try:
+        kc = O.drbg_array(n, 64, 0, 5000 + 10 * k + n % 97)
+        ec = O.drbg_array(n, 32, 0, 5001 + 10 * k + n % 97)
+        pk, sk = ctx.keypair_host(alg, kc)
+        ct, ss = ctx.enc_host(alg, pk, ec)
+        ct[::3, (7 * n) % ct.shape[1]] ^= 0x10
+        T.put_tuning(ctx, lowlat_max=-1, fused_dec_tile=32)
+        t32 = ctx.dec_host(alg, ct, sk)
+        T.put_tuning(ctx, fused_dec_tile=64)
+        t64 = ctx.dec_host(alg, ct, sk)
+        plain = ctx.dec_host(alg, ct, sk, plain=True)
+        assert (t32 == t64).all() and (t32 == plain).all()
+        honest = np.arange(n) % 3 != 0
+        assert (t32[honest] == ss[honest]).all() and (t32[~honest] != ss[~honest]).any(axis=1).all()
+        idx = np.unique(np.array([0, n // 2, n - 1]))
+        want = O.kyber_batch("dec", k, [np.ascontiguousarray(ct[idx]), np.ascontiguousarray(sk[idx])])
+        assert (t32[idx] == want).all()
+    finally:
+        ctx.close()
