@@ -40,10 +40,12 @@ BYTES_PER_DECAPS = 1088 + 2400 + 32  # compulsory: ct in, sk in, ss out
 # measures the sustained issue rate on the box (pqc_valu_issue_rate).
 VALU_PEAK_TOPS = 256 * 4 * 64 / 2 * 2.4e9 / 1e12
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per fused launch at batch 65536 from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs, each scaled by the same counter's reading of a 256 MiB
-# device copy (profiles/kyber768_decaps_b65536.md): 456.9 MB read + 60.8 MB written.
-MEASURED_TRAFFIC_BYTES_B65536 = 517.7e6
+# Bytes between L2 and memory per fused decapsulation launch at batch 65536
+# (k_fused_dec32<3>) from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
+# by the same counter's reading of a 256 MiB device copy
+# (profiles/kyber768_decaps_b65536.md): 794.8 MB read + 522.5 MB written.  Most of
+# it is the A^T scratch of the 32-request tiles, written and read back per wave.
+MEASURED_TRAFFIC_BYTES_B65536 = 1317.3e6
 # HBM bytes per Frodo-976 decapsulation GEMM launch at batch 4096, same method
 # (profiles/frodo976_decaps_b4096.md): 192.3 MB read + 0.07 MB written.
 MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 192.4e6
@@ -149,15 +151,17 @@ def timed_run(fn, stream, launches, warmup, pqc):
 
 
 VALU_OPS = ("v_xor_b32", "v_add_u32", "v_alignbit_b32", "v_mul_lo_u32", "v_bitop3_b32", "v_lshlrev_b32",
-            "v_mad_u32_u24")
+            "v_mad_u32_u24", "v_sub_u32", "v_and_b32", "v_ashrrev_i32", "v_fma_f32", "v_mul_f32", "v_mul_u32_u24",
+            "v_mad_i32_i24", "v_bfe_u32", "v_perm_b32")
 
 
 def valu_issue_check(pqc):
-    """Sustained cycles per wave64 instruction per SIMD of seven instruction kinds
-    at 8 waves per SIMD and at 1 (pqc_valu_issue_rate), and the simple-op rate the
-    8-wave figures imply against VALU_PEAK_TOPS."""
+    """Sustained cycles per wave64 instruction per SIMD of sixteen instruction
+    kinds at 8, 2 and 1 waves per SIMD (pqc_valu_issue_rate), the simple-op rate
+    the 8-wave figures imply against VALU_PEAK_TOPS, and the Keccak-f throughput
+    of the three state layouts at 1 and 2 waves per SIMD (pqc_keccak_rate)."""
     out = {}
-    for wps in (8, 1):
+    for wps in (8, 2, 1):
         row = {}
         for op, name in enumerate(VALU_OPS):
             cyc = ctypes.c_double()
@@ -168,6 +172,15 @@ def valu_issue_check(pqc):
         out["cycles_%d_waves_per_simd" % wps] = row
     best = min(out["cycles_8_waves_per_simd"]["v_xor_b32"], out["cycles_8_waves_per_simd"]["v_add_u32"])
     out["simple_op_rate_T"] = round(256 * 4 * 64 / best * 2.4e9 / 1e12, 2)
+    kr = {}
+    for form, name in enumerate(("single_lane", "lane_pair", "lane_pair_interleaved")):
+        for wps in (1, 2):
+            r = ctypes.c_double()
+            rc = pqc.lib().pqc_keccak_rate(form, wps, ctypes.byref(r), None)
+            if rc:
+                raise pqc.PQCError(rc, "pqc_keccak_rate")
+            kr["%s_%d_waves_per_simd" % (name, wps)] = round(r.value / 1e9, 3)
+    out["keccak_G_state_perms_per_s"] = kr
     log("VALU issue (cycles per wave64 instruction per SIMD): %s" % out)
     return out
 
@@ -373,7 +386,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "k_fused<3,true> (fused Kyber-768 decaps)",
+            "kernel": "k_fused_dec32<3> (fused Kyber-768 decaps, 32-request tiles, two waves per SIMD)",
             "achieved": round(achieved, 3),
             "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "Tops/s (int32)",

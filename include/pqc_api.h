@@ -534,11 +534,20 @@ int pqc_frodo976_gemm(pqc_ctx *ctx, size_t n, const uint8_t *ct, void *stream);
 /* VALU issue-rate microbenchmark (no reference counterpart): sustained cycles
  * per wave64 instruction per SIMD of one instruction kind, op 0 = v_xor_b32,
  * 1 = v_add_u32, 2 = v_alignbit_b32, 3 = v_mul_lo_u32, 4 = v_bitop3_b32,
- * 5 = v_lshlrev_b32, 6 = v_mad_u32_u24, eight independent chains per wave, over
- * a full grid at waves_per_simd (1..8) waves per SIMD on the current device
- * (best of three launches, synchronous on the stream).  It checks on the device
- * the 32-bit vector peak the VALU-bound kernels are priced against.            */
+ * 5 = v_lshlrev_b32, 6 = v_mad_u32_u24, 7 = v_sub_u32, 8 = v_and_b32,
+ * 9 = v_ashrrev_i32, 10 = v_fma_f32, 11 = v_mul_f32, 12 = v_mul_u32_u24,
+ * 13 = v_mad_i32_i24, 14 = v_bfe_u32, 15 = v_perm_b32, eight independent chains
+ * per wave, over a full grid at waves_per_simd (1..8) waves per SIMD on the
+ * current device (best of three launches, synchronous on the stream).  It checks
+ * on the device the 32-bit vector peak the VALU-bound kernels are priced
+ * against, and which instruction kinds issue at the full rate.               */
 int pqc_valu_issue_rate(int op, int waves_per_simd, double *cycles_per_instr, void *stream);
+/* Keccak-f[1600] throughput microbenchmark (no reference counterpart): state
+ * permutations per second of the whole device for the three state layouts the
+ * kernels use (form 0 one state per lane, 1 one state per lane pair with DPP
+ * swaps, 2 the bit-interleaved lane pair) at waves_per_simd (1..8) waves per
+ * SIMD.  Synchronous on the stream.                                          */
+int pqc_keccak_rate(int form, int waves_per_simd, double *state_perms_per_s, void *stream);
 
 const char *pqc_version(void);
 const char *pqc_strerror(int code);

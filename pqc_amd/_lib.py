@@ -79,6 +79,7 @@ SIGNATURES = {
     "pqc_newhope1024_poly_sample": (_I, [_SZ, _P, _P, _SZ, _I, _P]),
     "pqc_frodo976_gemm": (_I, [_P, _SZ, _P, _P]),
     "pqc_valu_issue_rate": (_I, [_I, _I, _P, _P]),
+    "pqc_keccak_rate": (_I, [_I, _I, C.POINTER(C.c_double), _P]),
     "pqc_newhope1024_cpa_keypair_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_enc_batch": (_I, [_P, _SZ, _P, _P, _P, _P, _U, _P]),
     "pqc_newhope1024_cpa_dec_batch": (_I, [_P, _SZ, _P, _P, _P, _U, _P]),

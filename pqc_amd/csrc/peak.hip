@@ -7,6 +7,7 @@
 // Register-only work: no memory is touched besides one word per lane written
 // at the end.
 #include "internal.h"
+#include "keccak.cuh"
 
 namespace pqc {
 namespace {
@@ -37,6 +38,15 @@ __global__ __launch_bounds__(kBlock) void k_valu_rate(uint32_t* out, uint32_t se
       if constexpr (OP == 4) PQC_CHAINS8("v_bitop3_b32", ", %9 bitop3:0x96");
       if constexpr (OP == 5) PQC_CHAINS8("v_lshlrev_b32", "");
       if constexpr (OP == 6) PQC_CHAINS8("v_mad_u32_u24", ", %9");
+      if constexpr (OP == 7) PQC_CHAINS8("v_sub_u32", "");
+      if constexpr (OP == 8) PQC_CHAINS8("v_and_b32", "");
+      if constexpr (OP == 9) PQC_CHAINS8("v_ashrrev_i32", "");
+      if constexpr (OP == 10) PQC_CHAINS8("v_fma_f32", ", %9");
+      if constexpr (OP == 11) PQC_CHAINS8("v_mul_f32", "");
+      if constexpr (OP == 12) PQC_CHAINS8("v_mul_u32_u24", "");
+      if constexpr (OP == 13) PQC_CHAINS8("v_mad_i32_i24", ", %9");
+      if constexpr (OP == 14) PQC_CHAINS8("v_bfe_u32", ", 16");
+      if constexpr (OP == 15) PQC_CHAINS8("v_perm_b32", ", %9");
     }
   }
   out[(size_t)blockIdx.x * kBlock + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
@@ -48,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_valu_rate(uint32_t* out, uint32_t se
 
 extern "C" int pqc_valu_issue_rate(int op, int waves_per_simd, double* cycles_per_instr, void* stream) {
   using namespace pqc;
-  if (!cycles_per_instr || op < 0 || op > 6 || waves_per_simd < 1 || waves_per_simd > 8) return PQC_ERR_INVALID_ARG;
+  if (!cycles_per_instr || op < 0 || op > 15 || waves_per_simd < 1 || waves_per_simd > 8) return PQC_ERR_INVALID_ARG;
   int dev = 0, cus = 0, khz = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -73,6 +83,15 @@ extern "C" int pqc_valu_issue_rate(int op, int waves_per_simd, double* cycles_pe
       if (op == 4) launch(k_valu_rate<4>);
       if (op == 5) launch(k_valu_rate<5>);
       if (op == 6) launch(k_valu_rate<6>);
+      if (op == 7) launch(k_valu_rate<7>);
+      if (op == 8) launch(k_valu_rate<8>);
+      if (op == 9) launch(k_valu_rate<9>);
+      if (op == 10) launch(k_valu_rate<10>);
+      if (op == 11) launch(k_valu_rate<11>);
+      if (op == 12) launch(k_valu_rate<12>);
+      if (op == 13) launch(k_valu_rate<13>);
+      if (op == 14) launch(k_valu_rate<14>);
+      if (op == 15) launch(k_valu_rate<15>);
       if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
           hipEventSynchronize(e1) != hipSuccess)
         rc = PQC_ERR_DEVICE;
@@ -88,5 +107,80 @@ extern "C" int pqc_valu_issue_rate(int op, int waves_per_simd, double* cycles_pe
   // wave64 instructions per SIMD (4 SIMDs per CU) over the elapsed cycles
   const double per_simd = (double)waves_per_simd * kIters * kInstrPerIter;
   *cycles_per_instr = (double)best * 1e-3 * (double)khz * 1e3 / per_simd;
+  return PQC_OK;
+}
+
+// Keccak-f[1600] throughput of the three state layouts the kernels use, on a
+// full grid of `waves_per_simd` waves per SIMD: form 0 one state per lane
+// (keccakf1600), 1 one state per lane pair with DPP swaps (keccakf1600_pair),
+// 2 the bit-interleaved lane pair (keccakf1600_pair_il).  Reports state
+// permutations per second for the whole GPU (a lane pair is one state).
+namespace pqc {
+namespace {
+constexpr int kKIters = 400;
+
+template <int FORM>
+__global__ __launch_bounds__(64) void k_keccak_rate(uint32_t* out, uint32_t seed) {
+  const int l = threadIdx.x;
+  uint32_t x = 0;
+  if constexpr (FORM == 0) {
+    uint64_t A[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) A[i] = (uint64_t)(l * 7 + i) ^ seed;
+    for (int it = 0; it < kKIters; ++it) keccakf1600(A);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) x ^= (uint32_t)A[i] ^ (uint32_t)(A[i] >> 32);
+  } else {
+    uint32_t A[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) A[i] = (uint32_t)(l * 7 + i) ^ seed;
+    for (int it = 0; it < kKIters; ++it) {
+      if constexpr (FORM == 1) keccakf1600_pair(A, (uint32_t)(l & 1));
+      else keccakf1600_pair_il(A, (uint32_t)(l & 1));
+    }
+#pragma unroll
+    for (int i = 0; i < 25; ++i) x ^= A[i];
+  }
+  out[(size_t)blockIdx.x * 64 + l] = x;
+}
+}  // namespace
+}  // namespace pqc
+
+extern "C" int pqc_keccak_rate(int form, int waves_per_simd, double* state_perms_per_s, void* stream) {
+  using namespace pqc;
+  if (!state_perms_per_s || form < 0 || form > 2 || waves_per_simd < 1 || waves_per_simd > 8)
+    return PQC_ERR_INVALID_ARG;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return PQC_ERR_NO_DEVICE;
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = cus * 4 * waves_per_simd;
+  uint32_t* out = nullptr;
+  if (hipMalloc(&out, (size_t)blocks * 64 * sizeof(uint32_t)) != hipSuccess) return PQC_ERR_NO_MEMORY;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PQC_OK;
+  float best = 0.f;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PQC_ERR_DEVICE;
+  for (int rep = 0; !rc && rep < 3; ++rep) {
+    if (hipEventRecord(e0, s) != hipSuccess) rc = PQC_ERR_DEVICE;
+    if (!rc) {
+      if (form == 0) hipLaunchKernelGGL(k_keccak_rate<0>, dim3(blocks), dim3(64), 0, s, out, 7u);
+      if (form == 1) hipLaunchKernelGGL(k_keccak_rate<1>, dim3(blocks), dim3(64), 0, s, out, 7u);
+      if (form == 2) hipLaunchKernelGGL(k_keccak_rate<2>, dim3(blocks), dim3(64), 0, s, out, 7u);
+      if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess)
+        rc = PQC_ERR_DEVICE;
+    }
+    float ms = 0.f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PQC_ERR_DEVICE;
+    if (!rc && (rep == 0 || ms < best)) best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  if (rc) return rc;
+  const double states = (double)blocks * 64 / (form == 0 ? 1 : 2);
+  *state_perms_per_s = states * kKIters / ((double)best * 1e-3);
   return PQC_OK;
 }
