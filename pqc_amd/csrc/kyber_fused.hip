@@ -26,6 +26,7 @@
 #include "internal.h"
 #include "req_check.h"
 #include "kyber_wave.cuh"
+#include "kyber_wavef.cuh"
 #include "sponge.cuh"
 
 namespace pqc {
@@ -33,6 +34,7 @@ namespace kyber {
 namespace fused {
 
 __constant__ LaneTwiddles c_lane_tw = make_lane_twiddles();
+__constant__ LaneTwiddlesF c_lane_twf = kLaneTwF;
 
 constexpr int kAStride = 262;   // int16 slots per A entry: 256 + guard, odd dword count
 static_assert(kAStride >= N + 4, "guard slots for a word's four candidates past the 256th accept");
@@ -237,6 +239,121 @@ __device__ __forceinline__ uint32_t coop_reencrypt(const AT& at, const EncIn<K, 
     for (int r = 0; r < 4; ++r) {
       const int32_t kq = (int32_t)((0u - ((nib >> r) & 1)) & ((Q + 1) / 2));
       c[r] = compress_d<P::dv>(csubq(barrett(acc.v[r] + e.v[r] + kq)));
+    }
+    if constexpr (DEC) {
+      uint32_t t[4];
+      unpack_loaded<P::dv>(t, in.cv, nat_group(l));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
+    } else {
+      pack_pair<P::dv>(ct + P::polyvec_cbytes, c);
+    }
+  }
+  return mismatch;
+}
+
+// ---- the same two steps on exact FP32 values (kyber_wavef.cuh) ------------------
+// Same inputs, same message and mismatch bits: every polynomial value is the
+// integer form's mod q, and compress_f reads only the residue.
+
+template <int K>
+__device__ __forceinline__ void coop_indcpa_dec_f(const DecIn<K>& in, const LaneZF& z, uint8_t* msg) {
+  using P = Params<K>;
+  const int l = lane_id();
+  F4 u[K];
+  Bz uz[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    uint32_t t[4];
+    unpack_loaded<P::du>(t, in.u[i], nat_group(l));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u[i].v[r] = (float)decompress_d<P::du>(t[r]);  // [0, q)
+    ntt_f<Q>(u[i], z);
+    reduce_f4(u[i]);
+    uz[i] = basemul_twist(u[i], z);
+  }
+  F4 mp = basemul_sum_f<K>(
+      [&](int i) {
+        P4 s;
+        poly12_loaded(s, in.s[i], ntt_group(l));  // 12-bit, unreduced
+        F4 sf = to_f4(s);
+        reduce_f4(sf);
+        return sf;
+      },
+      u, uz);
+  invntt_f(mp, z);
+  uint32_t tv[4];
+  unpack_loaded<P::dv>(tv, in.v, nat_group(l));
+  uint32_t nib = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) nib |= f32::compress_f<1>((float)decompress_d<P::dv>(tv[r]) - mp.v[r]) << r;
+  const uint32_t other = (uint32_t)__shfl_xor((int)nib, 1);
+  if ((l & 1) == 0) msg[l >> 1] = (uint8_t)(nib | (other << 4));
+}
+
+template <int K, bool DEC, class AT>
+__device__ __forceinline__ uint32_t coop_reencrypt_f(const AT& at, const EncIn<K, DEC>& in, const uint8_t* msg,
+                                                     uint8_t* ct, const LaneZF& z) {
+  using P = Params<K>;
+  const int l = lane_id();
+  const int gn = ntt_group(l);
+  F4 sp[K];
+  Bz spz[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    P4 c;
+    cbd_group(c, in.prf[j]);  // [-2, 2]
+    sp[j] = to_f4(c);
+    ntt_f<2>(sp[j], z);
+    reduce_f4(sp[j]);
+    spz[j] = basemul_twist(sp[j], z);
+  }
+  uint32_t mismatch = 0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    F4 acc = basemul_sum_f<K>(
+        [&](int j) {
+          uint32_t w0, w1;
+          at.get(i, j, w0, w1);  // sampler output, [0, 2^14)
+          F4 a = {{(float)(w0 & 0xFFFF), (float)(w0 >> 16), (float)(w1 & 0xFFFF), (float)(w1 >> 16)}};
+          reduce_f4(a);
+          return a;
+        },
+        sp, spz);
+    invntt_f(acc, z);
+    P4 e;
+    cbd_group(e, in.prf[K + i]);
+    uint32_t c[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c[r] = f32::compress_f<P::du>(acc.v[r] + (float)e.v[r]);
+    if constexpr (DEC) {
+      uint32_t t[4];
+      unpack_loaded<P::du>(t, in.cu[i], nat_group(l));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
+    } else {
+      pack_pair<P::du>(ct + 32 * P::du * i, c);
+    }
+  }
+  {
+    F4 acc = basemul_sum_f<K>(
+        [&](int j) {
+          P4 t;
+          poly12_loaded(t, in.t[j], gn);
+          F4 tf = to_f4(t);
+          reduce_f4(tf);
+          return tf;
+        },
+        sp, spz);
+    invntt_f(acc, z);
+    P4 e;
+    cbd_group(e, in.prf[2 * K]);
+    const uint32_t nib = (msg[l >> 1] >> (4 * (l & 1))) & 0xF;
+    uint32_t c[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float kq = ((nib >> r) & 1) ? (float)((Q + 1) / 2) : 0.f;
+      c[r] = f32::compress_f<P::dv>(acc.v[r] + (float)e.v[r] + kq);
     }
     if constexpr (DEC) {
       uint32_t t[4];
@@ -627,7 +744,7 @@ struct AtRegs {
 
 template <int K, class Src>
 __device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds32& sh, global_t<int16_t>* at,
-                                           const LaneZ& z, uint32_t* __restrict__ status) {
+                                           const LaneZF& z, uint32_t* __restrict__ status) {
   using P = Params<K>;
   using TL = Tile32<K>;
   const int lane = threadIdx.x;
@@ -651,7 +768,7 @@ __device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds3
         const ReqP q = lane_req(me, t + 1);
         dec_load<K>(nxt, q.ct, q.key);
       }
-      coop_indcpa_dec<K>(cur, z, sh.msg[t]);
+      coop_indcpa_dec_f<K>(cur, z, sh.msg[t]);
     }
   }
   __syncthreads();
@@ -760,7 +877,7 @@ __device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds3
         }
         AtRegs<K> a;
         a.load(at, t);
-        const uint32_t mm = coop_reencrypt<K, true>(a, cur, sh.msg[t], nullptr, z);
+        const uint32_t mm = coop_reencrypt_f<K, true>(a, cur, sh.msg[t], nullptr, z);
         const uint32_t f = __any(mm != 0) ? 1u : 0u;
         if (lane == 0) sh.fail[t] = f;
       }
@@ -793,8 +910,8 @@ template <int K>
 __global__ __launch_bounds__(64, 2) void k_fused_dec32(int n, int ntiles, DenseSrc src, int16_t* at_scratch,
                                                        uint32_t* __restrict__ status) {
   __shared__ Lds32 sh;
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
   global_t<int16_t>* at = (global_t<int16_t>*)(at_scratch + (size_t)blockIdx.x * Tile32<K>::ENTRIES * kAtStride);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) dec_tile32<K>(src, tile * 32, n, sh, at, z, status);
 }

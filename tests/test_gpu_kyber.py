@@ -324,3 +324,33 @@ def test_dec_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
         assert (t32[idx] == want).all()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_dec_tile32_random_bytes_vs_tile64_and_oracle(pqc, k):
+    """Decapsulation of random-byte ciphertexts under random-byte secret keys:
+    12-bit key coefficients up to 4095 (never reduced by a key generation),
+    matrix entries from arbitrary seeds (sampler outputs up to 14833) and
+    decompressed values over their whole range.  The 32-request tiles compute
+    on exact FP32 values mod q and the 64-request tiles on int16 Montgomery
+    arithmetic; both must give the oracle's bytes for every request."""
+    from pqc_amd import tuner as T
+    n = 200
+    say("dec tile32 random bytes k", k, "n", n)
+    alg = ALG[k]
+    ctx = pqc.Context([alg], max_batch=n)
+    try:
+        ctb, skb = {2: (736, 1632), 3: (1088, 2400), 4: (1568, 3168)}[k]
+        rng = np.random.default_rng(900 + k)
+        ct = rng.integers(0, 256, (n, ctb), dtype=np.uint8)
+        sk = rng.integers(0, 256, (n, skb), dtype=np.uint8)
+        sk[: n // 2, : 384 * k] = 0xFF  # every secret coefficient 4095
+        T.put_tuning(ctx, lowlat_max=-1, fused_dec_tile=32)
+        t32 = ctx.dec_host(alg, ct, sk)
+        T.put_tuning(ctx, fused_dec_tile=64)
+        t64 = ctx.dec_host(alg, ct, sk)
+        assert (t32 == t64).all()
+        want = O.kyber_batch("dec", k, [np.ascontiguousarray(ct), np.ascontiguousarray(sk)])
+        assert (t32 == want).all()
+    finally:
+        ctx.close()
