@@ -38,12 +38,15 @@ __device__ __forceinline__ void swap_s1(P4& p) {
     }
   } else {
     // DPP row shifts inside a 16-lane row: row_shl:n reads lane i+n, row_shr:n reads lane i-n.
+    // A lane whose bit is 0 reads i+n (same row), one whose bit is 1 reads i-n;
+    // the lanes whose source falls outside the row discard the value, so
+    // bound_ctrl (read 0 there) spares initialising the destination.
     constexpr int SHL = 0x100 | MASK, SHR = 0x110 | MASK;
     const bool hi = (lane_id() & MASK) != 0;
 #pragma unroll
     for (int s0 = 0; s0 < 2; ++s0) {
-      const int32_t from_up = __builtin_amdgcn_update_dpp(0, p.v[s0], SHL, 0xF, 0xF, false);
-      const int32_t from_dn = __builtin_amdgcn_update_dpp(0, p.v[2 | s0], SHR, 0xF, 0xF, false);
+      const int32_t from_up = __builtin_amdgcn_mov_dpp(p.v[s0], SHL, 0xF, 0xF, true);
+      const int32_t from_dn = __builtin_amdgcn_mov_dpp(p.v[2 | s0], SHR, 0xF, 0xF, true);
       const int32_t nlo = hi ? from_dn : p.v[s0];
       const int32_t nhi = hi ? p.v[2 | s0] : from_up;
       p.v[s0] = nlo;
