@@ -82,38 +82,6 @@ __device__ __forceinline__ void dec_load(DecIn<K>& in, const uint8_t* ct, const 
   load_group<P::dv>(in.v, ct + P::polyvec_cbytes, nat_group(l));
 }
 
-// indcpa_dec for one request; the message is written to msg (LDS, 32 bytes).
-template <int K>
-__device__ __forceinline__ void coop_indcpa_dec(const DecIn<K>& in, const LaneZ& z, uint8_t* msg) {
-  using P = Params<K>;
-  const int l = lane_id();
-  P4 mp = {{0, 0, 0, 0}};
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    uint32_t t[4];
-    unpack_loaded<P::du>(t, in.u[i], nat_group(l));
-    P4 u;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) u.v[r] = decompress_d<P::du>(t[r]);
-    ntt(u, z);
-    P4 s;
-    poly12_loaded(s, in.s[i], ntt_group(l));
-    basemul_acc(mp, s, u, z.bm);
-  }
-  reduce(mp);
-  invntt(mp, z);
-  uint32_t tv[4];
-  unpack_loaded<P::dv>(tv, in.v, nat_group(l));
-  uint32_t nib = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int32_t x = barrett(decompress_d<P::dv>(tv[r]) - mp.v[r]);
-    nib |= compress_d<1>(csubq(x)) << r;
-  }
-  const uint32_t other = (uint32_t)__shfl_xor((int)nib, 1);
-  if ((l & 1) == 0) msg[l >> 1] = (uint8_t)(nib | (other << 4));
-}
-
 // Pack the lane's 4 D-bit values with its odd neighbour's: even lanes write D bytes.
 template <int D>
 __device__ __forceinline__ void pack_pair(uint8_t* dst, const uint32_t (&t)[4]) {
@@ -166,10 +134,8 @@ __device__ __forceinline__ void enc_load(EncIn<K, DEC>& in, const uint8_t* prf, 
   }
 }
 
-// Re-encryption of one request (indcpa_enc from sp-sampling on).  DEC: compare
-// with ct and return the lane's mismatch bits.  ENC: write ct.  The A^T entries
-// come from `at`: at.get(i, j, w0, w1) gives this lane's group of entry (i, j) as
-// two words of two coefficients.
+// Sources of the A^T entries for the re-encryption: at.get(i, j, w0, w1) gives
+// this lane's group of entry (i, j) as two words of two coefficients.
 template <int K>
 struct AtRing {  // the LDS ring of the 64-request tiles: entry e0 + i K + j in slot mod RING
   const int16_t* ring;
@@ -182,80 +148,15 @@ struct AtRing {  // the LDS ring of the 64-request tiles: entry e0 + i K + j in 
   }
 };
 
-template <int K, bool DEC, class AT>
-__device__ __forceinline__ uint32_t coop_reencrypt(const AT& at, const EncIn<K, DEC>& in, const uint8_t* msg,
-                                                   uint8_t* ct, const LaneZ& z) {
-  using P = Params<K>;
-  const int l = lane_id();
-  const int gn = ntt_group(l);
-  P4 sp[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    cbd_group(sp[j], in.prf[j]);
-    ntt(sp[j], z);
-  }
-  uint32_t mismatch = 0;
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    P4 acc = {{0, 0, 0, 0}};
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      uint32_t w0, w1;
-      at.get(i, j, w0, w1);
-      P4 a = {{(int32_t)(w0 & 0xFFFF), (int32_t)(w0 >> 16), (int32_t)(w1 & 0xFFFF), (int32_t)(w1 >> 16)}};
-      basemul_acc(acc, a, sp[j], z.bm);
-    }
-    reduce(acc);
-    invntt(acc, z);
-    P4 e;
-    cbd_group(e, in.prf[K + i]);
-    uint32_t c[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) c[r] = compress_d<P::du>(csubq(barrett(acc.v[r] + e.v[r])));
-    if constexpr (DEC) {
-      uint32_t t[4];
-      unpack_loaded<P::du>(t, in.cu[i], nat_group(l));
-#pragma unroll
-      for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
-    } else {
-      pack_pair<P::du>(ct + 32 * P::du * i, c);
-    }
-  }
-  {
-    P4 acc = {{0, 0, 0, 0}};
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      P4 t;
-      poly12_loaded(t, in.t[j], gn);
-      basemul_acc(acc, t, sp[j], z.bm);
-    }
-    reduce(acc);
-    invntt(acc, z);
-    P4 e;
-    cbd_group(e, in.prf[2 * K]);
-    const uint32_t nib = (msg[l >> 1] >> (4 * (l & 1))) & 0xF;
-    uint32_t c[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int32_t kq = (int32_t)((0u - ((nib >> r) & 1)) & ((Q + 1) / 2));
-      c[r] = compress_d<P::dv>(csubq(barrett(acc.v[r] + e.v[r] + kq)));
-    }
-    if constexpr (DEC) {
-      uint32_t t[4];
-      unpack_loaded<P::dv>(t, in.cv, nat_group(l));
-#pragma unroll
-      for (int r = 0; r < 4; ++r) mismatch |= c[r] ^ t[r];
-    } else {
-      pack_pair<P::dv>(ct + P::polyvec_cbytes, c);
-    }
-  }
-  return mismatch;
-}
+// ---- indcpa_dec and the re-encryption on exact FP32 values (kyber_wavef.cuh) -----
+// Every polynomial value equals the reference's int16 value mod q (the
+// reference's arithmetic never leaves the int16 range, so its values are exact
+// residues too), and the outputs leave through compress_f, which reads only the
+// residue: the message bits, the ciphertext and the mismatch bits are the
+// reference's.
 
-// ---- the same two steps on exact FP32 values (kyber_wavef.cuh) ------------------
-// Same inputs, same message and mismatch bits: every polynomial value is the
-// integer form's mod q, and compress_f reads only the residue.
-
+// indcpa_dec (indcpa.cu:691-733, poly_tomsg) for one request; the message is written to
+// msg (LDS, 32 bytes).
 template <int K>
 __device__ __forceinline__ void coop_indcpa_dec_f(const DecIn<K>& in, const LaneZF& z, uint8_t* msg) {
   using P = Params<K>;
@@ -291,6 +192,8 @@ __device__ __forceinline__ void coop_indcpa_dec_f(const DecIn<K>& in, const Lane
   if ((l & 1) == 0) msg[l >> 1] = (uint8_t)(nib | (other << 4));
 }
 
+// Re-encryption of one request (indcpa_enc from sp-sampling on).  DEC: compare
+// with ct and return the lane's mismatch bits.  ENC: write ct.
 template <int K, bool DEC, class AT>
 __device__ __forceinline__ uint32_t coop_reencrypt_f(const AT& at, const EncIn<K, DEC>& in, const uint8_t* msg,
                                                      uint8_t* ct, const LaneZF& z) {
@@ -468,7 +371,7 @@ __device__ __forceinline__ ReqP lane_req(const ReqP& me, int t) {
 // The cooperative phases load request t + 1's inputs while request t computes
 // (one wave per SIMD: nothing else hides the loads' latency).
 template <int K, bool DEC, class Src>
-__device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<Tile<K>::RING>& sh, const LaneZ& z,
+__device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<Tile<K>::RING>& sh, const LaneZF& z,
                                            uint32_t* __restrict__ status) {
   using P = Params<K>;
   using TL = Tile<K>;
@@ -504,7 +407,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
         const ReqP q = lane_req(me, t + 1);
         dec_load<K>(nxt, q.ct, q.key);
       }
-      coop_indcpa_dec<K>(cur, z, sh.msg[t]);
+      coop_indcpa_dec_f<K>(cur, z, sh.msg[t]);
     }
     __syncthreads();
   } else {
@@ -594,7 +497,7 @@ __device__ __forceinline__ void fused_tile(const Src& src, int base, int n, Lds<
         enc_load<K, DEC>(nxt, q.prf, q.key + PK_OFF, q.ct);
       }
       const ReqP q = lane_req(me, t);
-      const uint32_t mm = coop_reencrypt<K, DEC>(AtRing<K>{sh.a, t * TL::KK}, cur, sh.msg[t], q.ct, z);
+      const uint32_t mm = coop_reencrypt_f<K, DEC>(AtRing<K>{sh.a, t * TL::KK}, cur, sh.msg[t], q.ct, z);
       cur = nxt;
       if constexpr (DEC) {
         const uint32_t f = __any(mm != 0) ? 1u : 0u;
@@ -1087,8 +990,8 @@ __global__ __launch_bounds__(64) void k_fused_keypair(int n, int ntiles, KeySrc 
 template <int K, bool DEC>
 __global__ __launch_bounds__(64) void k_fused(int n, int ntiles, DenseSrc src, uint32_t* __restrict__ status) {
   __shared__ Lds<Tile<K>::RING> sh;
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) fused_tile<K, DEC>(src, tile * 64, n, sh, z, status);
 }
 
@@ -1182,7 +1085,7 @@ __device__ __forceinline__ void lowlat_kdf(uint8_t* ss, const uint32_t (&kb)[8],
 // One decapsulation by the four waves of the workgroup (wave = threadIdx.x / 64)
 template <int K>
 __device__ __forceinline__ void lowlat_dec_body(uint8_t* ss, const uint8_t* c, const uint8_t* key, LowLds<K>& sh,
-                                                const LaneZ& z, int wave, int l, uint32_t* status) {
+                                                const LaneZF& z, int wave, int l, uint32_t* status) {
   using P = Params<K>;
   constexpr int HW = P::ct_bytes / 8;         // H(ct) message words
   constexpr int HB = HW / kRateSha3_256 + 1;  // H(ct) blocks, padding included
@@ -1194,7 +1097,7 @@ __device__ __forceinline__ void lowlat_dec_body(uint8_t* ss, const uint8_t* c, c
     {
       DecIn<K> in;
       dec_load<K>(in, c, key);
-      coop_indcpa_dec<K>(in, z, sh.msg);
+      coop_indcpa_dec_f<K>(in, z, sh.msg);
     }
     wave_sync();
     uint64_t h[4];
@@ -1211,7 +1114,7 @@ __device__ __forceinline__ void lowlat_dec_body(uint8_t* ss, const uint8_t* c, c
   if (wave == 0) {
     EncIn<K, true> in;
     enc_load<K, true, true>(in, sh.prf, pk, c);
-    const uint32_t mm = coop_reencrypt<K, true>(AtRing<K>{sh.a, 0}, in, sh.msg, nullptr, z);
+    const uint32_t mm = coop_reencrypt_f<K, true>(AtRing<K>{sh.a, 0}, in, sh.msg, nullptr, z);
     const uint32_t f = __any(mm != 0) ? 1u : 0u;
     if (l == 0) sh.fail = f;
   } else if (wave == 2 && l < 2) {
@@ -1241,7 +1144,7 @@ __device__ __forceinline__ void lowlat_dec_body(uint8_t* ss, const uint8_t* c, c
 //   phase 4  one lane: KDF(Kbar || H(ct))
 template <int K>
 __device__ __forceinline__ void lowlat_enc_body(uint8_t* ss, uint8_t* c, const uint8_t* pk, const uint8_t* coins,
-                                                LowLds<K>& sh, const LaneZ& z, int wave, int l, uint32_t* status) {
+                                                LowLds<K>& sh, const LaneZF& z, int wave, int l, uint32_t* status) {
   using P = Params<K>;
   // ---- phase 1
   if (wave == 0 && l == 0) {
@@ -1270,7 +1173,7 @@ __device__ __forceinline__ void lowlat_enc_body(uint8_t* ss, uint8_t* c, const u
     wave_sync();
     EncIn<K, false> in;
     enc_load<K, false, true>(in, sh.prf, pk, nullptr);
-    (void)coop_reencrypt<K, false>(AtRing<K>{sh.a, 0}, in, sh.msg, sh.ct, z);
+    (void)coop_reencrypt_f<K, false>(AtRing<K>{sh.a, 0}, in, sh.msg, sh.ct, z);
   }
   __syncthreads();
   // ---- phase 3
@@ -1307,8 +1210,8 @@ __global__ __launch_bounds__(256) void k_lowlat(int n, uint8_t* __restrict__ ss,
   if (x >= n) return;  // whole workgroup
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int l = threadIdx.x & 63;
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
   if constexpr (DEC)
     lowlat_dec_body<K>(ss + (size_t)x * 32, ct + (size_t)x * P::ct_bytes, key + (size_t)x * P::sk_bytes, sh, z, wave,
                        l, status);
@@ -1342,8 +1245,8 @@ __global__ __launch_bounds__(256) void k_lowlat_mixed(int n, unsigned alg_mask, 
   }
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int l = threadIdx.x & 63;
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
   uint8_t* s_ = ss + d.ss_off;
   uint8_t* c_ = ct + d.ct_off;
   const uint8_t* k_ = key + d.key_off;
@@ -1467,8 +1370,8 @@ template <bool DEC>
 __global__ __launch_bounds__(64) void k_fused_mixed(MixedSrc src, const uint32_t* __restrict__ counts, int order,
                                                     uint32_t* __restrict__ status) {
   __shared__ LdsMixed sh;
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
   const int n2 = (int)counts[0], n3 = (int)counts[1], n4 = (int)counts[2];
   const int s4 = mixed_tile_req(order, 4), s3 = mixed_tile_req(order, 3), s2 = mixed_tile_req(order, 2);
   const int t4 = (n4 + s4 - 1) / s4, t3 = (n3 + s3 - 1) / s3, t2 = (n2 + s2 - 1) / s2;
