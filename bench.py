@@ -840,7 +840,7 @@ def run_mixed(args):
         off, cnt = lay.region["ss"][a]
         return _Slice(ss_dec, off, cnt * 32)
 
-    # the dominant kernel, k_fused_mixed<decaps>, timed alone on the stream (with the
+    # the dominant kernel, k_fused_mixed_dec32, timed alone on the stream (with the
     # descriptor classification it needs: two small kernels of ~40 us together)
     dec_ms = timed_run(lambda: ctx.kem_dec_mixed(n, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ss_dec, stream=stream),
                        stream, 10, 2, pqc)
@@ -875,7 +875,7 @@ def run_mixed(args):
                    "batch_latency_ms_p50": round(float(np.percentile(per, 50)), 4),
                    "batch_latency_ms_p99": round(float(np.percentile(per, 99)), 4),
                    "bitexact_fused_plain": bitexact},
-        "roofline": {"bound": "valu", "kernel": "k_fused_mixed<decaps> (persistent, all three parameter sets)",
+        "roofline": {"bound": "valu", "kernel": "k_fused_mixed_dec32 (mixed decaps, 32-request tiles of all three parameter sets, two waves per SIMD)",
                      "achieved": round(dec_ops / dec_ms / 1e9, 3), "peak": VALU_PEAK_TOPS, "unit": "T int32 ops/s",
                      "frac": round(dec_ops / dec_ms / 1e9 / VALU_PEAK_TOPS, 4),
                      "work_per_launch": "Keccak-f per decaps 29 / 54 / 87 (Kyber-512 / 768 / 1024) x 7440 ops, "

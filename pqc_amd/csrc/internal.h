@@ -72,6 +72,7 @@ struct pqc_ctx {
   // fused decapsulation in 32-request tiles: per-wave A^T scratch for kat_waves waves
   int16_t* kat = nullptr;
   size_t kat_waves = 0;
+  size_t kat_region = 0;  // int16 per wave region (sized for the largest K held)
   void* tuner = nullptr;  // autotuner samples and the learned cost model (tuner.cpp)
 };
 

@@ -819,6 +819,29 @@ __global__ __launch_bounds__(64, 2) void k_fused_dec32(int n, int ntiles, DenseS
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) dec_tile32<K>(src, tile * 32, n, sh, at, z, status);
 }
 
+// Mixed batches, decapsulation: the bucketed requests (k_mixed_classify) in
+// 32-request tiles at two waves per SIMD, K = 4 tiles first (the longest), then
+// K = 3, then K = 2; each wave's A^T scratch region holds `region` int16 (sized
+// for the largest K the context holds).
+__global__ __launch_bounds__(64, 2) void k_fused_mixed_dec32(MixedSrc src, const uint32_t* __restrict__ counts,
+                                                            int16_t* at_scratch, size_t region,
+                                                            uint32_t* __restrict__ status) {
+  __shared__ Lds32 sh;
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
+  global_t<int16_t>* at = (global_t<int16_t>*)(at_scratch + (size_t)blockIdx.x * region);
+  const int n2 = (int)counts[0], n3 = (int)counts[1], n4 = (int)counts[2];
+  const int t4 = (n4 + 31) / 32, t3 = (n3 + 31) / 32, t2 = (n2 + 31) / 32;
+  for (int tile = blockIdx.x; tile < t4 + t3 + t2; tile += gridDim.x) {
+    if (tile < t4)
+      dec_tile32<4>(src, tile * 32, n4, sh, at, z, status);
+    else if (tile < t4 + t3)
+      dec_tile32<3>(src, (tile - t4) * 32, n3, sh, at, z, status);
+    else
+      dec_tile32<2>(src, (tile - t4 - t3) * 32, n2, sh, at, z, status);
+  }
+}
+
 // ---- key generation -----------------------------------------------------------------
 // indcpa_keypair (Kyber_GPU_Batched/indcpa.cu:474-564) + the KEM tail: (rho, sigma)
 // = G(d); s_j = CBD(PRF(sigma, j)), e_i = CBD(PRF(sigma, K + i)), both through
@@ -1595,6 +1618,16 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
                      c->kmix_order, c->kmix_counts, c->status);
   const MixedSrc src{reqs, c->kmix_order, cap, ss, ct, key, coins, c->kprf, c->kprf_per_elem};
   const int max_tiles = (int)((n + 31) / 32) + 2;  // half tiles at most; buckets round up separately
+  if (dec && c->kat && c->tune.fused_dec_tile != 64) {
+    // 32-request tiles, two waves per SIMD, one A^T scratch region per wave
+    int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * 2 * c->tune.fused_waves_per_cu;
+    grid = std::min(grid, max_tiles);
+    grid = std::min<long long>(grid, (long long)c->kat_waves);
+    if (grid < 1) return PQC_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_fused_mixed_dec32, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
+                       c->kat, c->kat_region, c->status);
+    return hip_err(hipGetLastError());
+  }
   int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;
   if (grid > max_tiles) grid = max_tiles;
   if (grid < 1) grid = 1;

@@ -398,6 +398,7 @@ int pqc_ctx_create(pqc_ctx** out, unsigned alg_mask, size_t max_batch, size_t hb
     // A^T scratch of the 32-request decapsulation tiles: one region per wave of the
     // largest grid (two waves per SIMD) the batch size can use
     c->kat_waves = std::min<size_t>((size_t)c->cu_count * 8, (max_batch + 31) / 32);
+    c->kat_region = pqc::tile32_scratch_bytes(kmax) / sizeof(int16_t);
     rc = ctx_alloc(c, &p, pqc::tile32_scratch_bytes(kmax) * c->kat_waves);
     if (rc) goto fail;
     c->kat = (int16_t*)p;
