@@ -18,6 +18,7 @@
 //   mulc_wide(x, z) |x| <= kMulcWideMax                        -> |result| <= 1665
 //   reduce_f(x)     |x| <= kReduceMax                          -> |result| <= 1665
 //   compress_f<D>   |c| <= kCompressMax(D)                     -> the integer compress_d of (c mod q)
+//   canon_f(x)      |x| <= kCanonMax                           -> x mod q in [0, q), as an integer
 #pragma once
 #include <stdint.h>
 
@@ -77,6 +78,19 @@ template <int D>
 PQC_F32_HD inline uint32_t compress_f(float c) {
   constexpr float kd = (float)(1 << D) / kQf;
   return bits_of(__builtin_fmaf(c, kd, kMagic)) & ((1u << D) - 1);
+}
+
+// Canonical residue of x in [0, q), as an integer, for |x| <= kCanonMax (the
+// values that are serialised: poly_tobytes packs csubq'd residues).  k =
+// floor(x / q) is the rounding of x / q - 1/2 + 1/(2q): x / q is at least 1/q
+// away from the next integer above unless it is one, so the shifted value never
+// sits on a tie, and the float error (below 2^-20) stays far inside that 1/(2q)
+// margin.  x + M - k q = M + r is exact and r sits in the low significand bits.
+constexpr int kCanonMax = 1 << 16;
+PQC_F32_HD inline uint32_t canon_f(float x) {
+  constexpr float kBias = -0.5f + 0.5f / kQf;
+  const float k = (__builtin_fmaf(x, kQinv, kBias) + kMagic) - kMagic;
+  return bits_of(__builtin_fmaf(k, -kQf, x + kMagic)) & 0xFFFu;
 }
 
 // ---- schedules: which layers need the wide product or a reduction ------------

@@ -871,44 +871,51 @@ __device__ __forceinline__ void key_load(KeyIn<K>& in, const uint8_t* prf) {
   for (int j = 0; j < 2 * K; ++j) in.prf[j] = prf16[64 * j + lane_id()];
 }
 
-// one request's s and t, written into sk_cpa, pk and sk's copy of pk
+// one request's s and t, written into sk_cpa, pk and sk's copy of pk, on the
+// exact FP32 arithmetic (kyber_wavef.cuh).  The reference computes
+// t = frommont(A o s) + e (basemul carries R^-1, frommont multiplies by R^2 R^-1)
+// and packs csubq'd residues; here A o s carries no factor, and canon_f gives
+// the same residues in [0, q).
 template <int K>
 __device__ __forceinline__ void coop_keypair(const int16_t* ring, int e0, const KeyIn<K>& in, uint8_t* pk, uint8_t* sk,
-                                             const LaneZ& z) {
+                                             const LaneZF& z) {
   using P = Params<K>;
   constexpr int RING = Tile<K>::RING;
   const int gn = ntt_group(lane_id());
-  P4 sv[K];
+  F4 sv[K];
+  Bz svz[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    cbd_group(sv[j], in.prf[j]);
-    ntt(sv[j], z);
-    reduce(sv[j]);
     P4 c;
+    cbd_group(c, in.prf[j]);  // [-2, 2]
+    sv[j] = to_f4(c);
+    ntt_f<2>(sv[j], z);
+    reduce_f4(sv[j]);
+    svz[j] = basemul_twist(sv[j], z);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) c.v[r] = csubq(sv[j].v[r]);
+    for (int r = 0; r < 4; ++r) c.v[r] = (int32_t)f32::canon_f(sv[j].v[r]);
     store_poly12(sk + 384 * j, gn, c);
   }
-  const int32_t f = (int32_t)((1ull << 32) % Q);  // frommont: x 2^32 (poly.cu:235-247)
 #pragma unroll
   for (int i = 0; i < K; ++i) {
-    P4 acc = {{0, 0, 0, 0}};
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const int slot = (e0 + i * K + j) % RING;
-      const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
-      const uint32_t w0 = e32[0], w1 = e32[1];
-      P4 a = {{(int32_t)(w0 & 0xFFFF), (int32_t)(w0 >> 16), (int32_t)(w1 & 0xFFFF), (int32_t)(w1 >> 16)}};
-      basemul_acc(acc, a, sv[j], z.bm);
-    }
-    reduce(acc);
+    F4 acc = basemul_sum_f<K>(
+        [&](int j) {
+          const int slot = (e0 + i * K + j) % RING;
+          const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
+          const uint32_t w0 = e32[0], w1 = e32[1];  // sampler output, [0, 2^14)
+          F4 a = {{(float)(w0 & 0xFFFF), (float)(w0 >> 16), (float)(w1 & 0xFFFF), (float)(w1 >> 16)}};
+          reduce_f4(a);
+          return a;
+        },
+        sv, svz);
     P4 e;
     cbd_group(e, in.prf[K + i]);
-    ntt(e, z);
-    reduce(e);
+    F4 ef = to_f4(e);
+    ntt_f<2>(ef, z);
+    static_assert(f32::fwd_out_bound(2) + f32::kOut <= f32::kCanonMax, "t = A o s + e within canon_f's range");
     P4 c;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) c.v[r] = csubq(barrett(fqmul(acc.v[r], f) + e.v[r]));
+    for (int r = 0; r < 4; ++r) c.v[r] = (int32_t)f32::canon_f(acc.v[r] + ef.v[r]);
     store_poly12(pk + 384 * i, gn, c);
     store_poly12(sk + P::sk_cpa_bytes + 384 * i, gn, c);
   }
@@ -923,7 +930,7 @@ struct KeySrc {
 
 template <int K>
 __device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, int n, Lds<Tile<K>::RING>& sh,
-                                                   const LaneZ& z, uint32_t* __restrict__ status) {
+                                                   const LaneZF& z, uint32_t* __restrict__ status) {
   using P = Params<K>;
   using TL = Tile<K>;
   const int lane = threadIdx.x;
@@ -1005,8 +1012,8 @@ __device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, 
 template <int K>
 __global__ __launch_bounds__(64) void k_fused_keypair(int n, int ntiles, KeySrc src, uint32_t* __restrict__ status) {
   __shared__ Lds<Tile<K>::RING> sh;
-  LaneZ z;
-  load_lane_z(z, &c_lane_tw);
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) fused_keypair_tile<K>(src, tile * 64, n, sh, z, status);
 }
 

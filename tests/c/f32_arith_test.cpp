@@ -8,6 +8,7 @@
 //     x z mod q and within 1665 of 0;
 //   - mulc_wide, the same for every |x| <= kMulcWideMax;
 //   - reduce_f, for every |x| <= 2^24;
+//   - canon_f, for every |x| <= kCanonMax: the residue in [0, q);
 //   - compress_f<D>, D = 1, 3, 4, 5, 10, 11, for every representative within
 //     kCompressMax(D): equal to the integer compress of the residue, written
 //     as in the reference (((x << D) + q / 2) / q) & (2^D - 1));
@@ -82,6 +83,13 @@ static void check_reduce() {
     if (!(is_int(r) && std::fabs(r) <= kOut && modq((int64_t)r - x) == 0)) {
       CHECK(false, "reduce_f(%d) = %f", x, r);
     }
+  }
+}
+
+static void check_canon() {
+  for (int x = -kCanonMax; x <= kCanonMax; ++x) {
+    const uint32_t got = canon_f((float)x);
+    CHECK(got == (uint32_t)modq(x), "canon_f(%d) = %u", x, got);
   }
 }
 
@@ -204,6 +212,7 @@ static void check_schedules() {
 int main() {
   check_mulc();
   check_reduce();
+  check_canon();
   check_compress<1>();
   check_compress<3>();
   check_compress<4>();
