@@ -43,9 +43,9 @@ HBM_PEAK_GBPS = 8000.0
 # Bytes between L2 and memory per fused decapsulation launch at batch 65536
 # (k_fused_dec32<3>) from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
 # by the same counter's reading of a 256 MiB device copy
-# (profiles/kyber768_decaps_b65536.md): 794.8 MB read + 522.5 MB written.  Most of
+# (profiles/kyber768_decaps_b65536.md): 786.8 MB read + 515.1 MB written.  Most of
 # it is the A^T scratch of the 32-request tiles, written and read back per wave.
-MEASURED_TRAFFIC_BYTES_B65536 = 1317.3e6
+MEASURED_TRAFFIC_BYTES_B65536 = 1301.9e6
 # HBM bytes per Frodo-976 decapsulation GEMM launch at batch 4096, same method
 # (profiles/frodo976_decaps_b4096.md): 192.3 MB read + 0.07 MB written.
 MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 192.4e6
