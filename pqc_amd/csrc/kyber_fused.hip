@@ -217,7 +217,7 @@ __device__ __forceinline__ uint32_t coop_reencrypt_f(const AT& at, const EncIn<K
     F4 acc = basemul_sum_f<K>(
         [&](int j) {
           uint32_t w0, w1;
-          at.get(i, j, w0, w1);  // sampler output, [0, 2^14)
+          at.get(i, j, w0, w1);  // accepted candidates, [0, 19q)
           F4 a = {{(float)(w0 & 0xFFFF), (float)(w0 >> 16), (float)(w1 & 0xFFFF), (float)(w1 >> 16)}};
           reduce_f4(a);
           return a;
@@ -273,39 +273,40 @@ __device__ __forceinline__ uint32_t coop_reencrypt_f(const AT& at, const EncIn<K
 // ---- per-lane sponge pieces ---------------------------------------------------
 
 // SHAKE128(rho || i || j) rejection-sampled into an LDS entry (gen_matrix_n,
-// indcpa.cu:339-403, rej_uniform indcpa.cu:290-322).  Every candidate is
-// written: accepted ones to slot ctr, rejected ones to the guard slot 256.  The
-// count is clamped to 256 after each word's four candidates, so accepts past
-// the 256th land in guard slots 256..259 (kAStride leaves 262).  Returns the
-// accept count, clamped (256 unless the block cap was hit).
-// DST is an LDS pointer (the ring) or a global-address-space one (the scratch of
-// the 32-request tiles).
-template <class DST>
-__device__ __forceinline__ int sample_entry(DST dst, const uint64_t (&rho)[4], int x, int y, bool valid) {
+// indcpa.cu:339-403, rej_uniform indcpa.cu:290-322).  The entry keeps the
+// accepted 16-bit candidates themselves (< 19q): the reference stores
+// v - (v >> 12) q, the same residue, and every consumer reduces the entry mod q
+// when it converts it to FP32.  Every candidate is written without a branch:
+// at the running position p, which advances past accepted ones only: a
+// rejected candidate is overwritten by the next one, and whatever lies at or
+// past the final count is never read.  p is clamped to slot 256 after each
+// word's four candidates, so writes past the 256th land in slots 256..259
+// (kAStride leaves 262).  Returns the accept count, clamped (256 unless the
+// block cap was hit).
+__device__ __forceinline__ int sample_entry(int16_t* dst, const uint64_t (&rho)[4], int x, int y, bool valid) {
   uint64_t A[25];
 #pragma unroll
   for (int t = 0; t < 25; ++t) A[t] = (t < 4) ? rho[t] : 0;
   A[4] = (uint64_t)x | ((uint64_t)y << 8) | (0x1Full << 16);
   A[kRateShake128 - 1] ^= 0x8000000000000000ull;
-  int ctr = 0;
+  int16_t* p = dst;
+  int16_t* const end = dst + N;
   for (int blk = 0; blk < kMaxBlocks; ++blk) {
-    if (blk >= 4 && !__any(valid && ctr < N)) break;
+    if (blk >= 4 && !__any(valid && p < end)) break;
     keccakf1600(A);
 #pragma unroll
     for (int w = 0; w < kRateShake128; ++w) {
+      const uint32_t half[2] = {(uint32_t)A[w], (uint32_t)(A[w] >> 32)};
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
-        const bool ok = val < 19u * Q;
-        const int32_t red = (int32_t)val - (int32_t)(val >> 12) * Q;
-        const int slot = ok ? ctr : N;
-        dst[slot] = (int16_t)red;
-        ctr += ok ? 1 : 0;
+        const uint32_t val = (h & 1) ? (half[h >> 1] >> 16) : (half[h >> 1] & 0xFFFFu);
+        *p = (int16_t)val;
+        p += val < 19u * Q ? 1 : 0;
       }
-      ctr = min(ctr, N);
+      p = p < end ? p : end;
     }
   }
-  return ctr;
+  return (int)(p - dst);
 }
 
 // Where the requests of a tile live.  DenseSrc: element-contiguous arrays of
@@ -563,13 +564,12 @@ static_assert(kAtStride >= N + 4 && (kAtStride * 2) % 16 == 0, "entry stride");
 // Sampling stage: one row per lane.  A block's candidates go to the row at their
 // entry position minus base8 (the entry's last 8-aligned position before the
 // block), so that whole 16-byte chunks can be copied to the global entry; the
-// row holds the 0..7 earlier coefficients of that chunk, the block's <= 84
-// accepts, up to 3 accepts past the 256th (clamped per word) and the guard slot
-// for rejected candidates.  A 208-byte row (13 x 16 B) spreads the lanes' rows
-// over 16 bank offsets.
-constexpr int kStageRow = 104;   // int16 per lane row
-constexpr int kStageGuard = 103; // rejected candidates land here
-static_assert(7 + 84 + 3 < kStageGuard && kStageRow % 8 == 0 && (kStageRow / 8) % 2 == 1, "stage row");
+// row holds the 0..7 earlier coefficients of that chunk and the block's <= 84
+// candidates written at the running position (the last write at most at index
+// 7 + 83), and the chunk copies read up to the next multiple of 8.  A 208-byte
+// row (13 x 16 B) spreads the lanes' rows over 16 bank offsets.
+constexpr int kStageRow = 104;  // int16 per lane row
+static_assert(7 + 84 + 7 <= kStageRow && kStageRow % 8 == 0 && (kStageRow / 8) % 2 == 1, "stage row");
 
 struct Lds32 {
   alignas(16) int16_t stage[64][kStageRow];
@@ -578,7 +578,8 @@ struct Lds32 {
 };
 
 // SHAKE128(rho || x || y) rejection-sampled into the global entry `dst` (as
-// sample_entry), through this lane's LDS stage row: after each block the chunks
+// sample_entry: the accepted candidates themselves), through this lane's LDS
+// stage row: after each block the chunks
 // [base8, round_up_8(ctr)) are copied to dst with 16-byte stores, and the last,
 // partial chunk moves to the front of the row for the next block.  So every
 // global store is a whole 16 bytes of one entry (a 2-byte store per candidate
@@ -594,18 +595,20 @@ __device__ __forceinline__ int sample_entry_staged(global_t<int16_t>* dst, int16
   for (int blk = 0; blk < kMaxBlocks; ++blk) {
     if (blk >= 4 && !__any(valid && ctr < N)) break;
     keccakf1600(A);
+    int16_t* p = row + (ctr - base8);
+    int16_t* const pend = row + (N - base8);  // position N: the clamp
 #pragma unroll
     for (int w = 0; w < kRateShake128; ++w) {
+      const uint32_t half[2] = {(uint32_t)A[w], (uint32_t)(A[w] >> 32)};
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        const uint32_t val = (uint32_t)(A[w] >> (16 * h)) & 0xFFFFu;
-        const bool ok = val < 19u * Q;
-        const int32_t red = (int32_t)val - (int32_t)(val >> 12) * Q;
-        row[ok ? ctr - base8 : kStageGuard] = (int16_t)red;
-        ctr += ok ? 1 : 0;
+        const uint32_t val = (h & 1) ? (half[h >> 1] >> 16) : (half[h >> 1] & 0xFFFFu);
+        *p = (int16_t)val;  // overwritten by the next candidate unless accepted
+        p += val < 19u * Q ? 1 : 0;
       }
-      ctr = min(ctr, N);
+      p = p < pend ? p : pend;
     }
+    ctr = base8 + (int)(p - row);
     // whole 16-byte chunks of [base8, ctr): at most (7 + 84 + 7) / 8 = 12
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const int nchunks = (ctr - base8 + 7) >> 3;
@@ -902,7 +905,7 @@ __device__ __forceinline__ void coop_keypair(const int16_t* ring, int e0, const 
         [&](int j) {
           const int slot = (e0 + i * K + j) % RING;
           const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
-          const uint32_t w0 = e32[0], w1 = e32[1];  // sampler output, [0, 2^14)
+          const uint32_t w0 = e32[0], w1 = e32[1];  // accepted candidates, [0, 19q)
           F4 a = {{(float)(w0 & 0xFFFF), (float)(w0 >> 16), (float)(w1 & 0xFFFF), (float)(w1 >> 16)}};
           reduce_f4(a);
           return a;

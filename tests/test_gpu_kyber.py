@@ -330,7 +330,7 @@ def test_dec_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
 def test_dec_tile32_random_bytes_vs_tile64_and_oracle(pqc, k):
     """Decapsulation of random-byte ciphertexts under random-byte secret keys:
     12-bit key coefficients up to 4095 (never reduced by a key generation),
-    matrix entries from arbitrary seeds (sampler outputs up to 14833) and
+    matrix entries from arbitrary seeds (accepted candidates up to 19q - 1) and
     decompressed values over their whole range.  The 32-request tiles compute
     on exact FP32 values mod q and the 64-request tiles on int16 Montgomery
     arithmetic; both must give the oracle's bytes for every request."""
