@@ -343,8 +343,10 @@ int pqc_server_run_trace(pqc_server *srv, size_t n, const double *arrival_us, co
  * (sizes of params.h:37-39).
  *
  * Every call runs on a library-owned context of the current device.  Calls
- * from several host threads are safe: each thread uses its own context (made
- * on its first call, freed when the thread exits).                           */
+ * from several host threads are safe: each call checks a context out of a
+ * per-device pool for its whole duration, so no two calls share staging or
+ * pinned buffers or a status word (the pool grows to the largest number of
+ * concurrent callers and is kept until the process exits).                  */
 #ifndef PQC_NIST_ALG
 #define PQC_NIST_ALG PQC_ALG_KYBER768
 #endif
