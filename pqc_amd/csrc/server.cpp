@@ -72,13 +72,18 @@ struct Slot {
   hipStream_t s3 = nullptr;   // fused mode: Frodo decapsulation, overlapping both
   hipEvent_t done = nullptr, in_ready = nullptr, frodo_done = nullptr, fd_done = nullptr;
   hipEvent_t start = nullptr;  // timing pair with `done`
+  // eager fused batches with Kyber and Frodo requests: the Kyber outputs (a prefix
+  // of the output arena) and a status snapshot come back before the Frodo chains
+  // end, and the Kyber requests complete at `kyber_done`
+  hipEvent_t kyber_done = nullptr;
+  bool kyber_pending = false, kyber_retired = false;
   uint8_t *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr, *d_keys = nullptr;
   pqc_kem_req *h_desc = nullptr, *d_desc = nullptr;  // Kyber encaps first, then decaps
   uint32_t *h_kid = nullptr, *d_kid = nullptr;       // key index per request, in batch order
   std::vector<Pending> reqs;
   std::vector<size_t> out_a, out_b, len_a, len_b;  // where each request's output lies in h_out
   std::vector<size_t> in_at;                       // where each request's input goes in h_in
-  uint32_t* h_status = nullptr;  // pinned: the device status words of ctx and ctx_fd after the batch
+  uint32_t* h_status = nullptr;  // pinned: status of ctx and ctx_fd after the batch; [2]: ctx after the Kyber part
   std::map<uint32_t, hipGraphExec_t> graphs;  // graph mode: one executable per batch shape (groups present)
   size_t bytes = 0;
   bool busy = false;
@@ -128,7 +133,7 @@ void free_slot(Slot& sl) {
   if (sl.ctx_fd) pqc_ctx_destroy(sl.ctx_fd);
   for (hipStream_t t : {sl.s, sl.s2, sl.s3})
     if (t) (void)hipStreamDestroy(t);
-  for (hipEvent_t e : {sl.done, sl.in_ready, sl.frodo_done, sl.fd_done, sl.start})
+  for (hipEvent_t e : {sl.done, sl.in_ready, sl.frodo_done, sl.fd_done, sl.start, sl.kyber_done})
     if (e) (void)hipEventDestroy(e);
   for (void* p : {(void*)sl.h_in, (void*)sl.h_out, (void*)sl.h_desc, (void*)sl.h_kid, (void*)sl.h_status})
     if (p) (void)hipHostFree(p);
@@ -250,8 +255,10 @@ int launch(pqc_server* v, Slot& sl) {
   const size_t n = sl.reqs.size();
   const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
   if (hipEventRecord(sl.start, st) != hipSuccess) return PQC_ERR_DEVICE;
-  sl.h_status[0] = sl.h_status[1] = 0;
+  sl.h_status[0] = sl.h_status[1] = sl.h_status[2] = 0;
   const bool graph = !plain && (v->cfg.flags & PQC_SERVER_GRAPH) != 0;
+  const bool early = !plain && !graph && n_kyber && (cnt[6] || cnt[7]);
+  const size_t kyber_end = out_base[6];  // the Kyber groups' outputs come first
   if (graph && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PQC_ERR_DEVICE;
   // everything from here to the output copies is stream-ordered work on st (and
   // the Frodo streams forked off it), recorded into a graph in graph mode
@@ -296,6 +303,13 @@ int launch(pqc_server* v, Slot& sl) {
         }
       }
     }
+    // eager fused mode: the Kyber outputs and status come back now, ahead of the Frodo chains
+    if (!rc && early) {
+      if (hipMemcpyAsync(sl.h_out, sl.d_out, kyber_end, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(sl.h_status + 2, sl.ctx->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipEventRecord(sl.kyber_done, st) != hipSuccess)
+        return PQC_ERR_DEVICE;
+    }
     // Frodo groups
     if (!rc && cnt[6]) {
       if (!plain) {
@@ -327,7 +341,9 @@ int launch(pqc_server* v, Slot& sl) {
         (hipEventRecord(sl.fd_done, sd) != hipSuccess || hipStreamWaitEvent(st, sl.fd_done, 0) != hipSuccess))
       rc = PQC_ERR_DEVICE;
     if (rc) return rc;
-    if (hipMemcpyAsync(sl.h_out, sl.d_out, oo, hipMemcpyDeviceToHost, st) != hipSuccess) return PQC_ERR_DEVICE;
+    const size_t o0 = early ? kyber_end : 0;
+    if (hipMemcpyAsync(sl.h_out + o0, sl.d_out + o0, oo - o0, hipMemcpyDeviceToHost, st) != hipSuccess)
+      return PQC_ERR_DEVICE;
     // the contexts' status words (cleared before the batch) come back with the outputs
     if (hipMemcpyAsync(sl.h_status, sl.ctx->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         (sl.ctx_fd && hipMemcpyAsync(sl.h_status + 1, sl.ctx_fd->status, 4, hipMemcpyDeviceToHost, st) != hipSuccess))
@@ -347,6 +363,8 @@ int launch(pqc_server* v, Slot& sl) {
   }
   if (rc) return rc;
   if (hipEventRecord(sl.done, st) != hipSuccess) return PQC_ERR_DEVICE;
+  sl.kyber_pending = early;
+  sl.kyber_retired = false;
   return PQC_OK;
 }
 
@@ -356,6 +374,40 @@ size_t req_bytes(const Pending& p) { return in_bytes(p.set, p.r.op) + out_bytes(
 size_t key_bytes(const Pending& p) {
   if (p.set.alg == PQC_ALG_FRODO976) return p.r.op == 1 ? 15632 + 8 : 31296;
   return p.r.op == 1 ? p.set.pk : p.set.sk;
+}
+
+// Complete a part of the slot's requests (kAll, kKyberOnly at kyber_done, kFrodoOnly
+// after the Kyber ones completed).  With status PQC_OK their outputs go to the
+// callers' buffers first, copied by the pool without holding the lock.
+enum RetirePart { kAll, kKyberOnly, kFrodoOnly };
+void retire_part(pqc_server* v, Slot& sl, std::unique_lock<std::mutex>& lk, int status, RetirePart part) {
+  auto mine = [&](const Pending& p) {
+    const bool frodo = p.r.alg == PQC_ALG_FRODO976;
+    return part == kAll || (part == kKyberOnly ? !frodo : frodo);
+  };
+  if (status == PQC_OK) {
+    const double r0 = v->now_us();
+    lk.unlock();
+    v->copy_pool.parallel_for(sl.reqs.size(), kCopyPer, [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        const Pending& p = sl.reqs[i];
+        if (!mine(p)) continue;
+        memcpy(p.r.out, sl.h_out + sl.out_a[i], sl.len_a[i]);
+        if (sl.len_b[i]) memcpy(p.r.out + sl.len_a[i], sl.h_out + sl.out_b[i], sl.len_b[i]);
+      }
+    });
+    lk.lock();
+    v->retire_us += v->now_us() - r0;
+  }
+  const double t = v->now_us();
+  size_t done = 0;
+  for (const Pending& p : sl.reqs) {
+    if (!mine(p)) continue;
+    v->completions.push_back(pqc_server_completion{p.r.tag, status, (uint32_t)sl.reqs.size(), p.t_submit, t});
+    ++done;
+  }
+  v->n_completed += done;
+  v->cv_done.notify_all();
 }
 
 void worker_loop(pqc_server* v) {
@@ -372,30 +424,29 @@ void worker_loop(pqc_server* v) {
     // 2. retire finished slots
     for (auto& sl : v->slots) {
       if (!sl.busy) continue;
+      if (sl.kyber_pending) {
+        const hipError_t qk = hipEventQuery(sl.kyber_done);
+        if (qk != hipErrorNotReady) {
+          const int kst = qk != hipSuccess ? PQC_ERR_DEVICE : sl.h_status[2] ? PQC_ERR_DEVICE : PQC_OK;
+          retire_part(v, sl, lk, kst, kKyberOnly);
+          sl.kyber_pending = false;
+          sl.kyber_retired = true;
+          progress = true;
+        }
+      }
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
       // a status bit raised on the device (a rejected descriptor, a sampler at its
       // block cap) fails the whole batch: its outputs are not trusted
       const int status = q != hipSuccess ? PQC_ERR_DEVICE : (sl.h_status[0] | sl.h_status[1]) ? PQC_ERR_DEVICE : PQC_OK;
-      if (status == PQC_OK) {  // the outputs to the callers' buffers, without holding the lock
+      if (status == PQC_OK) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, sl.start, sl.done) == hipSuccess) v->device_us += 1e3 * ms;
-        const double r0 = v->now_us();
-        lk.unlock();
-        v->copy_pool.parallel_for(sl.reqs.size(), kCopyPer, [&](size_t b, size_t e) {
-          for (size_t i = b; i < e; ++i) {
-            const Pending& p = sl.reqs[i];
-            memcpy(p.r.out, sl.h_out + sl.out_a[i], sl.len_a[i]);
-            if (sl.len_b[i]) memcpy(p.r.out + sl.len_a[i], sl.h_out + sl.out_b[i], sl.len_b[i]);
-          }
-        });
-        lk.lock();
-        v->retire_us += v->now_us() - r0;
       }
-      const double t = v->now_us();
-      for (const Pending& p : sl.reqs)
-        v->completions.push_back(pqc_server_completion{p.r.tag, status, (uint32_t)sl.reqs.size(), p.t_submit, t});
-      v->n_completed += sl.reqs.size();
+      // every request not completed at kyber_done (all of them otherwise; also when
+      // kyber_done was still pending at its query above and done is reached now)
+      retire_part(v, sl, lk, status, sl.kyber_retired ? kFrodoOnly : kAll);
+      sl.kyber_pending = sl.kyber_retired = false;
       v->inflight_bytes -= sl.bytes;
       sl.reqs.clear();
       sl.bytes = 0;
@@ -554,7 +605,8 @@ int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
         hipEventCreate(&sl.start) != hipSuccess ||
         hipEventCreateWithFlags(&sl.in_ready, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&sl.frodo_done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.fd_done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&sl.fd_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.kyber_done, hipEventDisableTiming) != hipSuccess)
       rc = PQC_ERR_DEVICE;
     const size_t nb = cfg->max_batch;
     if (!rc && (hipHostMalloc(&sl.h_in, v->arena) != hipSuccess || hipHostMalloc(&sl.h_out, v->arena) != hipSuccess ||
