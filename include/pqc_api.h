@@ -254,12 +254,17 @@ void pqc_graph_destroy(pqc_graph *graph);
  * PQC_SERVER_PLAIN the plain per-op kernels.  Results are the same bytes as the
  * batch calls give for the request alone.  The reference has no counterpart.  */
 #define PQC_SERVER_PLAIN 1u
-/* PQC_SERVER_GRAPH: each fused batch (input copies, the mixed Kyber kernels, the
- * Frodo streams forked off and joined back, output copies) is captured into a
- * hipGraph and launched with one call.  A slot keeps one executable graph per
- * batch shape (which request groups are present); the next batch of that shape
- * updates the executable's parameters in place (hipGraphExecUpdate) instead of
- * re-instantiating it.  Same bytes as the eager launches. */
+/* PQC_SERVER_GRAPH: each fused Kyber batch (input copies, the mixed Kyber
+ * kernels, output copies) is captured into a hipGraph and launched with one
+ * call.  A slot keeps one executable graph per batch shape (which request
+ * groups are present); the next batch of that shape updates the executable's
+ * parameters in place (hipGraphExecUpdate) instead of re-instantiating it.
+ * With Kyber and Frodo both enabled, the slots form two pools: half of them (at
+ * least one) take only the Frodo requests, launched eagerly, and the others
+ * only the Kyber requests, as graphs, so a Kyber batch never waits for a slot
+ * whose Frodo chains still run.  Same bytes as the eager launches.  Without
+ * this flag, a batch holds both kinds and its Kyber requests complete as soon
+ * as its Kyber kernels finish. */
 #define PQC_SERVER_GRAPH 2u
 typedef struct pqc_server pqc_server;
 typedef struct pqc_server_config {

@@ -64,6 +64,13 @@ struct Pending {
   double t_submit;
 };
 
+// Slot classes.  With Kyber and Frodo both enabled in graph mode the slots form
+// two pools: a Frodo batch is bounded by its two serial SHAKE256 chains (several
+// milliseconds whatever its size), a Kyber batch by its much shorter kernels, so
+// Kyber batches keep their own slots instead of waiting for a slot whose Frodo
+// chains still run.
+constexpr int kAnyClass = 0, kKyberClass = 1, kFrodoClass = 2;
+
 struct Slot {
   pqc_ctx* ctx = nullptr;
   pqc_ctx* ctx_fd = nullptr;  // fused mode: Frodo decapsulation, with scratch of its own
@@ -87,6 +94,7 @@ struct Slot {
   std::map<uint32_t, hipGraphExec_t> graphs;  // graph mode: one executable per batch shape (groups present)
   size_t bytes = 0;
   bool busy = false;
+  int cls = kAnyClass;  // which requests the slot takes (split pools: Kyber or Frodo only)
 };
 
 }  // namespace
@@ -107,7 +115,9 @@ struct pqc_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Pending> submitted;
-  std::deque<Pending> pending;
+  std::deque<Pending> pending;    // every request, or the Kyber ones with split pools
+  std::deque<Pending> pending_f;  // split pools: the Frodo requests
+  bool split = false;
   std::deque<pqc_server_completion> completions;
   size_t inflight_bytes = 0;
   uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_batched_reqs = 0;
@@ -256,7 +266,10 @@ int launch(pqc_server* v, Slot& sl) {
   const size_t n_kyber = n_enc + cnt[3] + cnt[4] + cnt[5];
   if (hipEventRecord(sl.start, st) != hipSuccess) return PQC_ERR_DEVICE;
   sl.h_status[0] = sl.h_status[1] = sl.h_status[2] = 0;
-  const bool graph = !plain && (v->cfg.flags & PQC_SERVER_GRAPH) != 0;
+  // graph mode captures the Kyber batches; with split pools the Frodo batches stay
+  // eager (their launch sequence is short next to their serial chains, and capture
+  // plus update cost more than it saves there)
+  const bool graph = !plain && (v->cfg.flags & PQC_SERVER_GRAPH) != 0 && sl.cls != kFrodoClass;
   const bool early = !plain && !graph && n_kyber && (cnt[6] || cnt[7]);
   const size_t kyber_end = out_base[6];  // the Kyber groups' outputs come first
   if (graph && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PQC_ERR_DEVICE;
@@ -415,9 +428,10 @@ void worker_loop(pqc_server* v) {
   const bool plain = (v->cfg.flags & PQC_SERVER_PLAIN) != 0;
   std::unique_lock<std::mutex> lk(v->mu);
   while (!v->stop) {
-    // 1. move submissions in
+    // 1. move submissions in (with split pools, Frodo requests queue apart)
     while (!v->submitted.empty()) {
-      v->pending.push_back(v->submitted.front());
+      const Pending& p = v->submitted.front();
+      (v->split && p.r.alg == PQC_ALG_FRODO976 ? v->pending_f : v->pending).push_back(p);
       v->submitted.pop_front();
     }
     bool progress = false;
@@ -454,16 +468,25 @@ void worker_loop(pqc_server* v) {
       progress = true;
       v->cv_done.notify_all();
     }
-    // 3. launch into free slots
+    // 3. launch into free slots; a class whose next batch is still filling (or
+    // does not fit) is held for this pass
+    bool hold[3] = {false, false, false};
     for (auto& sl : v->slots) {
-      if (sl.busy || v->pending.empty()) continue;
-      const double waited = v->now_us() - v->pending.front().t_submit;
-      bool any_busy = false;
-      for (auto& o : v->slots) any_busy |= o.busy;
-      if (any_busy && v->pending.size() < v->cfg.max_batch && waited < v->cfg.max_wait_us) break;
+      std::deque<Pending>& q = sl.cls == kFrodoClass ? v->pending_f : v->pending;
+      if (sl.busy || q.empty() || hold[sl.cls]) continue;
+      const double waited = v->now_us() - q.front().t_submit;
+      bool any_busy = false, class_busy = false;
+      for (auto& o : v->slots) {
+        any_busy |= o.busy;
+        class_busy |= o.busy && o.cls == sl.cls;
+      }
+      if (class_busy && q.size() < v->cfg.max_batch && waited < v->cfg.max_wait_us) {
+        hold[sl.cls] = true;
+        continue;
+      }
       size_t in_sz = 0, out_sz = 0, key_sz = 0;
-      while (!v->pending.empty() && sl.reqs.size() < v->cfg.max_batch) {
-        const Pending& p = v->pending.front();
+      while (!q.empty() && sl.reqs.size() < v->cfg.max_batch) {
+        const Pending& p = q.front();
         const size_t ib = align8(in_bytes(p.set, p.r.op)) + 8, ob = align8(out_bytes(p.set, p.r.op)) + 16;
         const size_t rb = req_bytes(p);
         if (in_sz + ib > v->arena || out_sz + ob > v->arena) break;
@@ -475,9 +498,12 @@ void worker_loop(pqc_server* v) {
         key_sz += key_bytes(p);
         sl.bytes += rb;
         sl.reqs.push_back(p);
-        v->pending.pop_front();
+        q.pop_front();
       }
-      if (sl.reqs.empty()) break;
+      if (sl.reqs.empty()) {
+        hold[sl.cls] = true;
+        continue;
+      }
       v->inflight_bytes += sl.bytes;
       lk.unlock();
       const double l0 = v->now_us();
@@ -505,7 +531,7 @@ void worker_loop(pqc_server* v) {
       bool any_busy = false;
       for (auto& o : v->slots) any_busy |= o.busy;
       // poll the GPU every few microseconds while batches run, else sleep until work
-      if (any_busy || !v->pending.empty())
+      if (any_busy || !v->pending.empty() || !v->pending_f.empty())
         v->cv_work.wait_for(lk, std::chrono::microseconds(5));
       else
         v->cv_work.wait_for(lk, std::chrono::milliseconds(2));
@@ -592,8 +618,16 @@ int pqc_server_create(pqc_server** out, const pqc_server_config* cfg) {
     if (gctx) pqc_ctx_destroy(gctx);
   }
   (void)hipStreamDestroy(s0);
-  // ---- slots
+  // ---- slots: split pools when Kyber and Frodo share a graph-mode server (half the
+  // slots, at least one, take the Frodo requests).  Eager mode keeps one pool: its
+  // many small Kyber launches compete with the Frodo pool's on the scheduler
+  // thread, which measured a less stable p99 than one pool whose batches complete
+  // their Kyber part early.
   v->slots.resize(cfg->slots);
+  v->split = !(cfg->flags & PQC_SERVER_PLAIN) && (cfg->flags & PQC_SERVER_GRAPH) && v->kyber_mask &&
+             (cfg->alg_mask & PQC_ALG_MASK(PQC_ALG_FRODO976)) && cfg->slots >= 2;
+  for (size_t i = 0; i < v->slots.size(); ++i)
+    v->slots[i].cls = !v->split ? kAnyClass : i < v->slots.size() / 2 ? kFrodoClass : kKyberClass;
   for (auto& sl : v->slots) {
     if (rc) break;
     rc = pqc_ctx_create(&sl.ctx, cfg->alg_mask, cfg->max_batch, 0);
