@@ -386,9 +386,10 @@ __device__ __forceinline__ uint32_t encode_mu(const uint8_t* mu, int k, int i) {
 // The MFMA takes [A | B] as its first operand, so the output is the transposed
 // tile: lane l holds row 16 mt + (l & 15), columns 16 nt + 4 (l >> 4) + r,
 // i.e. four consecutive uint16 of a ciphertext row -- the epilogue reads and
-// writes 8-byte words.  Its loads (E' | E'' 16 bytes per m-tile, and for
-// decapsulation the ciphertext words) are issued at step 13, right after the
-// last DMA, and land during the last MFMA steps.
+// writes 8-byte words.  [E' | E''] (16 bytes per m-tile) is the lo digit's
+// starting value, loaded first.  Decapsulation's ciphertext words are loaded
+// during steps 10..13, one m-tile per step: issued at once, every workgroup's
+// burst of HBM misses queued ahead of its next DMAs and stalled the step after.
 //
 // Epilogue per mode:
 //   products: [B'' | V] (uint16) to the output buffer;
@@ -431,15 +432,19 @@ template <int KS>
 __device__ __forceinline__ void gemm_issue(const GemmArgs& g) {
 #if defined(__HIP_DEVICE_COMPILE__)  // the LDS-DMA builtin exists only in the device pass
   constexpr int st = KS % 3;
+  // a wave-uniform fragment address plus the lane's 32-bit offset (the SGPR-base form)
+  const uint32_t lo = (uint32_t)g.lane * 16u;
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
-    __builtin_amdgcn_global_load_lds(g.sfrag + ((size_t)(g.mt0 + mi) * KSTEPS + KS) * 64 + g.lane,
-                                     &ring_a<st>()[g.wave * 4 + mi][0], 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(
+        reinterpret_cast<const uint8_t*>(g.sfrag + ((size_t)(g.mt0 + mi) * KSTEPS + KS) * 64) + lo,
+        &ring_a<st>()[g.wave * 4 + mi][0], 16, 0, 0);
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int f = 2 * g.wave + q, ni = f >> 1, d = f & 1;
-    __builtin_amdgcn_global_load_lds(g.bfrag + ((size_t)(d * NT + g.nb * 4 + ni) * KSTEPS + KS) * 64 + g.lane,
-                                     &ring_b<st>()[f][0], 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(
+        reinterpret_cast<const uint8_t*>(g.bfrag + ((size_t)(d * NT + g.nb * 4 + ni) * KSTEPS + KS) * 64) + lo,
+        &ring_b<st>()[f][0], 16, 0, 0);
   }
 #endif
 }
@@ -468,37 +473,43 @@ __device__ __forceinline__ uint8_t* ct_word(uint8_t* ct, int row, int n0) {
   return c;  // padding columns: a valid address, never compared or stored
 }
 
-template <int MODE>
+// the ciphertext words of m-tile MI (decapsulation): EpiRegs::kLoads / 4 loads
+template <int MODE, int MI>
 __device__ __forceinline__ void epi_issue(EpiRegs<MODE>& ep, const GemmArgs& g, uint8_t* ct, int nrows) {
   if constexpr (MODE == kDecaps) {
+    const int row = min(16 * (g.mt0 + MI) + (g.lane & 15), nrows - 1);
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) {
-      const int row = min(16 * (g.mt0 + mi) + (g.lane & 15), nrows - 1);
-#pragma unroll
-      for (int ni = 0; ni < 4; ++ni) {
-        const int n0 = 16 * (4 * g.nb + ni) + 4 * (g.lane >> 4);
-        ep.c[mi][ni] = *reinterpret_cast<const uint2*>(ct_word<MODE>(ct, row, n0));
-      }
+    for (int ni = 0; ni < 4; ++ni) {
+      const int n0 = 16 * (4 * g.nb + ni) + 4 * (g.lane >> 4);
+      ep.c[MI][ni] = *reinterpret_cast<const uint2*>(ct_word<MODE>(ct, row, n0));
     }
   }
+}
+
+// the steps that issue the ciphertext words (m-tile ks - kEpiStep0 at step ks)
+constexpr int kEpiStep0 = 10;
+template <int MODE>
+constexpr int epi_loads_at(int ks) {
+  return ks >= kEpiStep0 && ks < kEpiStep0 + 4 ? EpiRegs<MODE>::kLoads / 4 : 0;
 }
 
 template <int MODE, int KS>
 __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g, EpiRegs<MODE>& ep, uint8_t* ct,
                                           int nrows) {
-  // this wave's DMAs of step KS done; what may stay in flight: the DMAs of step
-  // KS + 1 (issued before) and, from step 14 on, the epilogue loads (issued at 13)
-  constexpr int E = EpiRegs<MODE>::kLoads;
-  if constexpr (KS <= 13) wait_vm_lgkm0<DMA_PER_STEP>();
-  else if constexpr (KS == 14) wait_vm_lgkm0<DMA_PER_STEP + E>();
-  else wait_vm_lgkm0<E>();
+  // this wave's DMAs of step KS done; what may stay in flight is what was issued
+  // after them, at steps KS - 2 and KS - 1: the DMAs of step KS + 1 and the
+  // ciphertext words of those steps
+  constexpr int allowed =
+      (KS + 1 < KSTEPS ? DMA_PER_STEP : 0) + epi_loads_at<MODE>(KS - 2) + epi_loads_at<MODE>(KS - 1);
+  wait_vm_lgkm0<allowed>();
   asm volatile("s_barrier" ::: "memory");  // a workgroup barrier no memory access moves across
   if constexpr (KS + 2 < KSTEPS) gemm_issue<KS + 2>(g);
-  if constexpr (KS == 13) {
-    // the explicit vmcnt values above count the epilogue loads as the newest:
-    // keep the scheduler from moving them ahead of this step's DMAs
+  if constexpr (KS >= kEpiStep0 && KS < kEpiStep0 + 4) {
+    // the vmcnt values above count these loads after this step's DMAs: keep the
+    // scheduler from moving them across
     __builtin_amdgcn_sched_barrier(0);
-    epi_issue<MODE>(ep, g, ct, nrows);
+    epi_issue<MODE, KS - kEpiStep0>(ep, g, ct, nrows);
+    __builtin_amdgcn_sched_barrier(0);
   }
   constexpr int st = KS % 3;
   if constexpr (KS + 1 < KSTEPS) {
