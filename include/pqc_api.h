@@ -89,6 +89,7 @@ typedef struct pqc_tuning {
   int32_t fused_dec_tile;     /* requests per tile of the fused Kyber decapsulation: 32 (two waves per
                                  SIMD, A^T in a global scratch) or 64 (one wave per SIMD, A^T in an LDS
                                  ring); 0 = library default (32) */
+  int32_t fused_enc_tile;     /* the same for the fused Kyber encapsulation */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
 int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */
@@ -127,7 +128,7 @@ int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *
 /* ---- autotuner ----------------------------------------------------------------
  * Times the candidate configurations of one workload family (the knobs that
  * family uses) with HIP events and keeps every measurement as a sample
- * (13 features, log ms) in the context.  `run` launches the workload once,
+ * (14 features, log ms) in the context.  `run` launches the workload once,
  * asynchronously, on the given stream and returns 0 or a PQC_ERR code.
  * use_model = 0: candidates in a fixed order (the configuration in use first)
  * up to `budget` (0 = all).  use_model = 1: once the context's learned cost
@@ -137,7 +138,8 @@ int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *
  * installed in the context and returned.                                       */
 #define PQC_TUNE_PLAIN_KYBER 0 /* plain_block */
 #define PQC_TUNE_FUSED_KYBER 1 /* fused_waves_per_cu */
-#define PQC_TUNE_MIXED_KYBER 2 /* fused_waves_per_cu x mixed_order */
+#define PQC_TUNE_MIXED_KYBER 2 /* fused_waves_per_cu x tile layout: 32-request tiles both ways, or
+                                  64-request tiles (fused_enc_tile = fused_dec_tile = 64) in mixed_order 0, 1, 3 */
 #define PQC_TUNE_FRODO 3       /* frodo_stream_block */
 #define PQC_TUNE_NEWHOPE 4     /* newhope_block */
 typedef int (*pqc_workload_fn)(void *user, void *stream);
@@ -159,9 +161,9 @@ typedef struct pqc_tune_result {
 } pqc_tune_result;
 int pqc_autotune(pqc_ctx *ctx, const pqc_tune_request *req, pqc_workload_fn run, void *user, void *stream,
                  pqc_tune_result *out);
-/* features of (request, configuration) as the cost model sees them; returns 13 */
+/* features of (request, configuration) as the cost model sees them; returns 14 */
 int pqc_tune_features(const pqc_tune_request *req, const pqc_tuning *t, float *out, int count);
-/* copies up to cap samples (13 features each, log ms) and returns the count */
+/* copies up to cap samples (14 features each, log ms) and returns the count */
 int pqc_tune_samples(pqc_ctx *ctx, float *x, float *y, int cap);
 
 /* ---- the learned cost model (an MLP trained on the GPU) -------------------------

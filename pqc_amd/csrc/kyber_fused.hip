@@ -648,6 +648,70 @@ struct AtRegs {
   }
 };
 
+// The A^T rounds of a 32-request tile: round g samples entries 64 g .. 64 g + 63
+// (entry E = request E / K^2, position E % K^2) into the scratch, then re-encrypts
+// every request whose entries are all sampled (request t's words load during
+// request t - 1).  Decapsulation compares with the ciphertext and sets sh.fail[t];
+// encapsulation writes the ciphertext.
+template <int K, bool DEC>
+__device__ __forceinline__ void at_reencrypt32(const ReqP& me, int nt, Lds32& sh, global_t<int16_t>* at,
+                                               const LaneZF& z, uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  using TL = Tile32<K>;
+  constexpr int PK_OFF = DEC ? P::sk_cpa_bytes : 0;  // public key inside the key buffer
+  const int lane = threadIdx.x;
+  const int ng = (nt * TL::KK + 63) / 64;
+  int t_done = 0;
+  auto load_rho = [&](uint64_t (&rho)[4], int g) {
+    const int t = (64 * g + lane) / TL::KK;
+    const uint8_t* key = reinterpret_cast<const uint8_t*>(__shfl((long long)me.key, t < nt ? t : 0));
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rho[w] = load_u64_global(key + PK_OFF + P::polyvec_bytes + 8 * w);
+  };
+  uint64_t rho_next[4];
+  load_rho(rho_next, 0);
+  for (int g = 0; g < ng; ++g) {
+    {
+      const int E = 64 * g + lane;
+      const int t = E / TL::KK;
+      const int ij = E % TL::KK;
+      const bool valid = t < nt;
+      uint64_t rho[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) rho[w] = rho_next[w];
+      const int got = sample_entry_staged(at + (size_t)E * kAtStride, sh.stage[lane], rho, ij / K, ij % K, valid);
+      if (valid && got < N) atomicOr(status, 1u);
+    }
+    __syncthreads();  // the round's entries are in the scratch for every lane
+    if (g + 1 < ng) load_rho(rho_next, g + 1);
+    const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
+    if (t_done < t_end) {
+      EncIn<K, DEC> cur, nxt;
+      {
+        const ReqP q = lane_req(me, t_done);
+        enc_load<K, DEC>(nxt, q.prf, q.key + PK_OFF, q.ct);
+      }
+      for (int t = t_done; t < t_end; ++t) {
+        cur = nxt;
+        if (t + 1 < t_end) {
+          const ReqP q = lane_req(me, t + 1);
+          enc_load<K, DEC>(nxt, q.prf, q.key + PK_OFF, q.ct);
+        }
+        AtRegs<K> a;
+        a.load(at, t);
+        if constexpr (DEC) {
+          const uint32_t mm = coop_reencrypt_f<K, true>(a, cur, sh.msg[t], nullptr, z);
+          const uint32_t f = __any(mm != 0) ? 1u : 0u;
+          if (lane == 0) sh.fail[t] = f;
+        } else {
+          coop_reencrypt_f<K, false>(a, cur, sh.msg[t], lane_req(me, t).ct, z);
+        }
+      }
+      t_done = t_end;
+    }
+  }
+}
+
 template <int K, class Src>
 __device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds32& sh, global_t<int16_t>* at,
                                            const LaneZF& z, uint32_t* __restrict__ status) {
@@ -744,52 +808,7 @@ __device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds3
 
   // ---- A^T rounds of 64 entries into the scratch, then the re-encryption of every
   // request whose entries are complete
-  const int ng = (nt * TL::KK + 63) / 64;
-  int t_done = 0;
-  auto load_rho = [&](uint64_t (&rho)[4], int g) {
-    const int t = (64 * g + lane) / TL::KK;
-    const uint8_t* key = reinterpret_cast<const uint8_t*>(__shfl((long long)me.key, t < nt ? t : 0));
-#pragma unroll
-    for (int w = 0; w < 4; ++w) rho[w] = load_u64_global(key + P::sk_cpa_bytes + P::polyvec_bytes + 8 * w);
-  };
-  uint64_t rho_next[4];
-  load_rho(rho_next, 0);
-  for (int g = 0; g < ng; ++g) {
-    {
-      const int E = 64 * g + lane;
-      const int t = E / TL::KK;
-      const int ij = E % TL::KK;
-      const bool valid = t < nt;
-      uint64_t rho[4];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) rho[w] = rho_next[w];
-      const int got = sample_entry_staged(at + (size_t)E * kAtStride, sh.stage[lane], rho, ij / K, ij % K, valid);
-      if (valid && got < N) atomicOr(status, 1u);
-    }
-    __syncthreads();  // the round's entries are in the scratch for every lane
-    if (g + 1 < ng) load_rho(rho_next, g + 1);
-    const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
-    if (t_done < t_end) {
-      EncIn<K, true> cur, nxt;
-      {
-        const ReqP q = lane_req(me, t_done);
-        enc_load<K, true>(nxt, q.prf, q.key + P::sk_cpa_bytes, q.ct);
-      }
-      for (int t = t_done; t < t_end; ++t) {
-        cur = nxt;
-        if (t + 1 < t_end) {
-          const ReqP q = lane_req(me, t + 1);
-          enc_load<K, true>(nxt, q.prf, q.key + P::sk_cpa_bytes, q.ct);
-        }
-        AtRegs<K> a;
-        a.load(at, t);
-        const uint32_t mm = coop_reencrypt_f<K, true>(a, cur, sh.msg[t], nullptr, z);
-        const uint32_t f = __any(mm != 0) ? 1u : 0u;
-        if (lane == 0) sh.fail[t] = f;
-      }
-      t_done = t_end;
-    }
-  }
+  at_reencrypt32<K, true>(me, nt, sh, at, z, status);
   __syncthreads();
 
   // ---- KDF(Kbar or z, H(ct)) on lanes 0..31; Kbar comes from lane + 32
@@ -812,6 +831,123 @@ __device__ __forceinline__ void dec_tile32(const Src& src, int base, int n, Lds3
   __syncthreads();
 }
 
+// Encapsulation in the same 32-request tiles.  The first sponge phase runs
+// H(pk) (the longest chain) on lanes 0..31 while lanes 32..63 compute
+// m = H(coins) ("don't release system RNG output") in its first permutation;
+// then both lanes of a pair compute G(m || H(pk)) and split the 2K + 1 PRF
+// outputs (lane l the even nonces, lane l + 32 the odd ones); the A^T rounds
+// and re-encryptions write the ciphertexts; H(ct) and the KDF end it.  Same
+// bytes as fused_tile<K, false>.
+template <int K>
+struct EncTile32 {
+  static constexpr int HB = Params<K>::pk_bytes / 8 / kRateSha3_256 + 1;  // H(pk) blocks: 6, 9, 12
+  static_assert(Params<K>::pk_bytes / 8 >= kRateSha3_256, "the first H(pk) block is whole");
+};
+
+template <int K, class Src>
+__device__ __forceinline__ void enc_tile32(const Src& src, int base, int n, Lds32& sh, global_t<int16_t>* at,
+                                           const LaneZF& z, uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  using TL = Tile32<K>;
+  const int lane = threadIdx.x;
+  const int rq = lane & 31;  // the request this lane serves in the sponge phases (lanes l and l + 32 alike)
+  const int nt = min(TL::T, n - base);
+  const bool act = rq < nt;
+  const ReqP me = src.template req<K, false>(act ? base + rq : base);
+  const bool hl = lane < 32;  // H(pk) lane; the others compute m = H(coins)
+
+  // ---- lanes 0..31: H(pk) (SHA3-256, HB blocks); lanes 32..63: m = H(coins)
+  uint64_t hpk[4];
+  {
+    constexpr int W = P::pk_bytes / 8;
+    constexpr int R = kRateSha3_256;
+    const global_t<uint64_t>* pw = gptr(reinterpret_cast<const uint64_t*>(me.key));
+    uint64_t A[25], cur[R];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) A[i] = 0;
+    if (hl) {
+#pragma unroll
+      for (int w = 0; w < R; ++w) cur[w] = pw[w];
+    } else {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) A[w] = load_u64_global(me.coins + 8 * w);
+      A[4] ^= 0x06;
+      A[R - 1] ^= 0x8000000000000000ull;
+    }
+    for (int st = 0; st < EncTile32<K>::HB; ++st) {
+      if (hl) {
+        const int rem = W - st * R;  // message words left (uniform), >= 0
+#pragma unroll
+        for (int w = 0; w < R; ++w) {
+          if (w < rem) A[w] ^= cur[w];
+          if (w == rem) A[w] ^= 0x06;
+        }
+        if (st == EncTile32<K>::HB - 1) A[R - 1] ^= 0x8000000000000000ull;
+        // the next block's words load during this permutation
+#pragma unroll
+        for (int w = 0; w < R; ++w)
+          if (w < rem - R) cur[w] = pw[(st + 1) * R + w];
+      }
+      keccakf1600(A);
+      if (!hl && st == 0 && act) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) store_u64(sh.msg[rq] + 8 * w, A[w]);
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) hpk[w] = A[w];  // lanes 0..31
+  }
+  __syncthreads();
+
+  // ---- (Kbar, coins) = G(m || H(pk)) on both lanes of a pair, then PRF(coins,
+  // 2 i + (lane >= 32)) for i = 0 .. K (lane l + 32's last one, nonce 2K + 1, unused)
+  uint64_t kbar[4];
+  {
+    uint64_t m[8], A[25], coins[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      m[w] = load_u64(sh.msg[rq] + 8 * w);
+      m[4 + w] = (uint64_t)__shfl((long long)hpk[w], rq);
+    }
+    absorb_fixed<kRateSha3_512, 8>(A, m, 0x06);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      kbar[w] = A[w];
+      coins[w] = A[4 + w];
+    }
+    for (int i = 0; i <= K; ++i) {
+      const int nonce = 2 * i + (hl ? 0 : 1);
+      absorb_fixed<kRateShake256, 4>(A, coins, (uint64_t)nonce | (0x1Full << 8));
+      if (act && nonce <= 2 * K) {
+#pragma unroll
+        for (int w = 0; w < 16; ++w) store_u64(me.prf + 128 * nonce + 8 * w, A[w]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- A^T rounds and the re-encryptions, which write the ciphertexts
+  at_reencrypt32<K, false>(me, nt, sh, at, z, status);
+  __syncthreads();
+
+  // ---- KDF(Kbar, H(ct)); both lanes of a pair compute it, lanes 0..31 store
+  {
+    uint64_t m[8], A[25];
+    sponge_absorb_words_prefetch<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(me.ct), P::ct_bytes / 8, 0x06);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      m[w] = kbar[w];
+      m[4 + w] = A[w];
+    }
+    absorb_fixed<kRateShake256, 8>(A, m, 0x1F);
+    if (hl && act) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) store_u64(me.ss + 8 * w, A[w]);
+    }
+  }
+  __syncthreads();
+}
+
 template <int K>
 __global__ __launch_bounds__(64, 2) void k_fused_dec32(int n, int ntiles, DenseSrc src, int16_t* at_scratch,
                                                        uint32_t* __restrict__ status) {
@@ -822,13 +958,23 @@ __global__ __launch_bounds__(64, 2) void k_fused_dec32(int n, int ntiles, DenseS
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) dec_tile32<K>(src, tile * 32, n, sh, at, z, status);
 }
 
-// Mixed batches, decapsulation: the bucketed requests (k_mixed_classify) in
-// 32-request tiles at two waves per SIMD, K = 4 tiles first (the longest), then
-// K = 3, then K = 2; each wave's A^T scratch region holds `region` int16 (sized
-// for the largest K the context holds).
-__global__ __launch_bounds__(64, 2) void k_fused_mixed_dec32(MixedSrc src, const uint32_t* __restrict__ counts,
-                                                            int16_t* at_scratch, size_t region,
-                                                            uint32_t* __restrict__ status) {
+template <int K>
+__global__ __launch_bounds__(64, 2) void k_fused_enc32(int n, int ntiles, DenseSrc src, int16_t* at_scratch,
+                                                       uint32_t* __restrict__ status) {
+  __shared__ Lds32 sh;
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
+  global_t<int16_t>* at = (global_t<int16_t>*)(at_scratch + (size_t)blockIdx.x * Tile32<K>::ENTRIES * kAtStride);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) enc_tile32<K>(src, tile * 32, n, sh, at, z, status);
+}
+
+// Mixed batches: the bucketed requests (k_mixed_classify) in 32-request tiles at
+// two waves per SIMD, K = 4 tiles first (the longest), then K = 3, then K = 2;
+// each wave's A^T scratch region holds `region` int16 (sized for the largest K
+// the context holds).
+template <bool DEC>
+__device__ __forceinline__ void mixed32_body(const MixedSrc& src, const uint32_t* __restrict__ counts,
+                                             int16_t* at_scratch, size_t region, uint32_t* __restrict__ status) {
   __shared__ Lds32 sh;
   LaneZF z;
   load_lane_zf(z, &c_lane_twf);
@@ -836,13 +982,29 @@ __global__ __launch_bounds__(64, 2) void k_fused_mixed_dec32(MixedSrc src, const
   const int n2 = (int)counts[0], n3 = (int)counts[1], n4 = (int)counts[2];
   const int t4 = (n4 + 31) / 32, t3 = (n3 + 31) / 32, t2 = (n2 + 31) / 32;
   for (int tile = blockIdx.x; tile < t4 + t3 + t2; tile += gridDim.x) {
-    if (tile < t4)
-      dec_tile32<4>(src, tile * 32, n4, sh, at, z, status);
-    else if (tile < t4 + t3)
-      dec_tile32<3>(src, (tile - t4) * 32, n3, sh, at, z, status);
-    else
-      dec_tile32<2>(src, (tile - t4 - t3) * 32, n2, sh, at, z, status);
+    if (tile < t4) {
+      if constexpr (DEC) dec_tile32<4>(src, tile * 32, n4, sh, at, z, status);
+      else enc_tile32<4>(src, tile * 32, n4, sh, at, z, status);
+    } else if (tile < t4 + t3) {
+      if constexpr (DEC) dec_tile32<3>(src, (tile - t4) * 32, n3, sh, at, z, status);
+      else enc_tile32<3>(src, (tile - t4) * 32, n3, sh, at, z, status);
+    } else {
+      if constexpr (DEC) dec_tile32<2>(src, (tile - t4 - t3) * 32, n2, sh, at, z, status);
+      else enc_tile32<2>(src, (tile - t4 - t3) * 32, n2, sh, at, z, status);
+    }
   }
+}
+
+__global__ __launch_bounds__(64, 2) void k_fused_mixed_dec32(MixedSrc src, const uint32_t* __restrict__ counts,
+                                                            int16_t* at_scratch, size_t region,
+                                                            uint32_t* __restrict__ status) {
+  mixed32_body<true>(src, counts, at_scratch, region, status);
+}
+
+__global__ __launch_bounds__(64, 2) void k_fused_mixed_enc32(MixedSrc src, const uint32_t* __restrict__ counts,
+                                                            int16_t* at_scratch, size_t region,
+                                                            uint32_t* __restrict__ status) {
+  mixed32_body<false>(src, counts, at_scratch, region, status);
 }
 
 // ---- key generation -----------------------------------------------------------------
@@ -1527,6 +1689,19 @@ static int launch_dec32(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
 }
 
 template <int K>
+static int launch_enc32(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const uint8_t* pk, const uint8_t* coins,
+                        hipStream_t s) {
+  const int ntiles = (int)((n + Tile32<K>::T - 1) / Tile32<K>::T);
+  int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * 2 * c->tune.fused_waves_per_cu;
+  grid = std::min(grid, ntiles);
+  grid = std::min<long long>(grid, (long long)c->kat_waves);  // one scratch region per wave
+  if (grid < 1) return PQC_ERR_INVALID_ARG;
+  const DenseSrc src{ss, ct, pk, coins, c->kprf};
+  hipLaunchKernelGGL((k_fused_enc32<K>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, src, c->kat, c->status);
+  return hip_err(hipGetLastError());
+}
+
+template <int K>
 static int launch_fused_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
   if (n == 0) return PQC_OK;
   const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
@@ -1584,6 +1759,14 @@ int kyber_fused_dec(pqc_ctx* c, int k, size_t n, uint8_t* ss, const uint8_t* ct,
 
 int kyber_fused_enc(pqc_ctx* c, int k, size_t n, uint8_t* ct, uint8_t* ss, const uint8_t* pk, const uint8_t* coins,
                     hipStream_t s) {
+  if (n > 0 && !kyber::fused::use_lowlat(c, n, false) && c->tune.fused_enc_tile != 64 && c->kat) {
+    switch (k) {
+      case 2: return kyber::fused::launch_enc32<2>(c, n, ss, ct, pk, coins, s);
+      case 3: return kyber::fused::launch_enc32<3>(c, n, ss, ct, pk, coins, s);
+      case 4: return kyber::fused::launch_enc32<4>(c, n, ss, ct, pk, coins, s);
+      default: return PQC_ERR_UNSUPPORTED;
+    }
+  }
   if (kyber::fused::use_lowlat(c, n, false)) {
     switch (k) {
       case 2: return kyber::fused::launch_lowlat<2, false>(c, n, ss, ct, pk, coins, s);
@@ -1628,14 +1811,18 @@ int kyber_fused_mixed(pqc_ctx* c, int dec, size_t n, const pqc_kem_req* reqs, ui
                      c->kmix_order, c->kmix_counts, c->status);
   const MixedSrc src{reqs, c->kmix_order, cap, ss, ct, key, coins, c->kprf, c->kprf_per_elem};
   const int max_tiles = (int)((n + 31) / 32) + 2;  // half tiles at most; buckets round up separately
-  if (dec && c->kat && c->tune.fused_dec_tile != 64) {
+  if (c->kat && (dec ? c->tune.fused_dec_tile : c->tune.fused_enc_tile) != 64) {
     // 32-request tiles, two waves per SIMD, one A^T scratch region per wave
     int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * 2 * c->tune.fused_waves_per_cu;
     grid = std::min(grid, max_tiles);
     grid = std::min<long long>(grid, (long long)c->kat_waves);
     if (grid < 1) return PQC_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(k_fused_mixed_dec32, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
-                       c->kat, c->kat_region, c->status);
+    if (dec)
+      hipLaunchKernelGGL(k_fused_mixed_dec32, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
+                         c->kat, c->kat_region, c->status);
+    else
+      hipLaunchKernelGGL(k_fused_mixed_enc32, dim3(grid), dim3(64), 0, s, src, (const uint32_t*)c->kmix_counts,
+                         c->kat, c->kat_region, c->status);
     return hip_err(hipGetLastError());
   }
   int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;

@@ -30,7 +30,18 @@ struct Samples {
 
 static float log2f_safe(double v) { return (float)std::log2(v > 1.0 ? v : 1.0); }
 
-// one-hot family (5), K / 4, op / 2, log2(n) / 17, log2(block) / 10, waves / 4, order one-hot (4)
+// The mixed family's tile layouts: 32-request tiles for both operations (the
+// tile order does not apply), or 64-request tiles in tile order 0, 1 or 3.
+constexpr int kMixedOrders[3] = {0, 1, 3};
+static int mixed_layout(const pqc_tuning& t) {
+  if (t.fused_enc_tile != 64 && t.fused_dec_tile != 64) return 0;
+  for (int i = 0; i < 3; ++i)
+    if (t.mixed_order == kMixedOrders[i]) return 1 + i;
+  return 3;  // order 2 is not a candidate: nearest slot
+}
+
+// one-hot family (5), K / 4, op / 2, log2(n) / 17, log2(block) / 10, waves / 4,
+// mixed tile layout one-hot (4)
 static void features(const pqc_tune_request& r, const pqc_tuning& t, float* f) {
   for (int i = 0; i < kFeatures; ++i) f[i] = 0.f;
   if (r.family >= 0 && r.family < 5) f[r.family] = 1.f;
@@ -47,7 +58,7 @@ static void features(const pqc_tune_request& r, const pqc_tuning& t, float* f) {
   f[8] = block ? log2f_safe(block) / 10.f : 0.f;
   const bool fused = r.family == PQC_TUNE_FUSED_KYBER || r.family == PQC_TUNE_MIXED_KYBER;
   f[9] = fused ? (float)t.fused_waves_per_cu / 4.f : 0.f;
-  if (r.family == PQC_TUNE_MIXED_KYBER) f[10 + t.mixed_order] = 1.f;
+  if (r.family == PQC_TUNE_MIXED_KYBER) f[10 + mixed_layout(t)] = 1.f;
 }
 
 static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_tuning& base) {
@@ -77,10 +88,11 @@ static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_t
       break;
     case PQC_TUNE_MIXED_KYBER:
       for (int w = 1; w <= 4; ++w)
-        for (int o = 0; o < 4; ++o) {
+        for (int l = 0; l < 4; ++l) {
           pqc_tuning t = base;
           t.fused_waves_per_cu = w;
-          t.mixed_order = o;
+          t.mixed_order = l == 0 ? 0 : kMixedOrders[l - 1];
+          t.fused_enc_tile = t.fused_dec_tile = l == 0 ? 32 : 64;
           add(t);
         }
       break;
@@ -96,10 +108,14 @@ static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_t
   return out;
 }
 
+// 0 and 32 both mean 32-request tiles
+static int tile_of(int v) { return v == 64 ? 64 : 32; }
+
 static bool same(const pqc_tuning& a, const pqc_tuning& b) {
   return a.plain_block == b.plain_block && a.newhope_block == b.newhope_block &&
          a.fused_waves_per_cu == b.fused_waves_per_cu && a.mixed_order == b.mixed_order &&
-         a.frodo_stream_block == b.frodo_stream_block;
+         a.frodo_stream_block == b.frodo_stream_block && tile_of(a.fused_enc_tile) == tile_of(b.fused_enc_tile) &&
+         tile_of(a.fused_dec_tile) == tile_of(b.fused_dec_tile);
 }
 
 // median of reps timed runs (one untimed warm-up first)

@@ -327,6 +327,66 @@ def test_dec_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
 
 
 @pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 95, 1000, 9001])
+def test_enc_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
+    """The fused encapsulation in 32-request tiles (two waves per SIMD, H(pk) on
+    half the lanes beside H(coins), G and the PRF outputs split over lane pairs,
+    A^T in a global scratch; the default above lowlat_max) against the 64-request
+    tiles, the plain per-op kernels and the oracle, on ragged batch sizes, with
+    the same coins: identical ciphertexts and shared secrets, which decapsulate
+    back to the same secrets."""
+    from pqc_amd import tuner as T
+    say("enc tile32 k", k, "n", n)
+    alg = ALG[k]
+    ctx = pqc.Context([alg], max_batch=max(n, 64))
+    try:
+        kc = O.drbg_array(n, 64, 0, 6000 + 10 * k + n % 97)
+        ec = O.drbg_array(n, 32, 0, 6001 + 10 * k + n % 97)
+        pk, sk = ctx.keypair_host(alg, kc)
+        T.put_tuning(ctx, lowlat_max=-1, fused_enc_tile=32)
+        ct32, ss32 = ctx.enc_host(alg, pk, ec)
+        T.put_tuning(ctx, fused_enc_tile=64)
+        ct64, ss64 = ctx.enc_host(alg, pk, ec)
+        ctp, ssp = ctx.enc_host(alg, pk, ec, plain=True)
+        assert (ct32 == ct64).all() and (ss32 == ss64).all()
+        assert (ct32 == ctp).all() and (ss32 == ssp).all()
+        assert (ctx.dec_host(alg, ct32, sk) == ss32).all()
+        idx = np.unique(np.array([0, n // 2, n - 1]))
+        wct, wss = O.kyber_batch("enc", k, [np.ascontiguousarray(pk[idx]), np.ascontiguousarray(ec[idx])])
+        assert (ct32[idx] == wct).all() and (ss32[idx] == wss).all()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_enc_tile32_random_pk_vs_tile64_and_oracle(pqc, k):
+    """Encapsulation under random-byte public keys: 12-bit key coefficients up to
+    4095 (never reduced by a key generation) and matrix entries from arbitrary
+    seeds.  The 32-request tiles, the 64-request tiles and the oracle must give
+    the same ciphertexts and shared secrets."""
+    from pqc_amd import tuner as T
+    n = 200
+    say("enc tile32 random pk k", k, "n", n)
+    alg = ALG[k]
+    ctx = pqc.Context([alg], max_batch=n)
+    try:
+        pkb = {2: 800, 3: 1184, 4: 1568}[k]
+        rng = np.random.default_rng(950 + k)
+        pk = rng.integers(0, 256, (n, pkb), dtype=np.uint8)
+        pk[: n // 2, : 384 * k] = 0xFF  # every key coefficient 4095
+        ec = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+        T.put_tuning(ctx, lowlat_max=-1, fused_enc_tile=32)
+        ct32, ss32 = ctx.enc_host(alg, pk, ec)
+        T.put_tuning(ctx, fused_enc_tile=64)
+        ct64, ss64 = ctx.enc_host(alg, pk, ec)
+        assert (ct32 == ct64).all() and (ss32 == ss64).all()
+        wct, wss = O.kyber_batch("enc", k, [np.ascontiguousarray(pk), np.ascontiguousarray(ec)])
+        assert (ct32 == wct).all() and (ss32 == wss).all()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
 def test_dec_tile32_random_bytes_vs_tile64_and_oracle(pqc, k):
     """Decapsulation of random-byte ciphertexts under random-byte secret keys:
     12-bit key coefficients up to 4095 (never reduced by a key generation),

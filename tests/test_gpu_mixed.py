@@ -207,10 +207,11 @@ def test_graph_replay_mixed_dec(pqc, ctx, path):
 def test_mixed_tile_orders_agree(pqc, ctx, n):
     """Every tile order of the 64-request tiles (0: K 4,3,2; 1: K 2,3,4; 2:
     round-robin; 3: K 4,3,2 with 32-request tiles for K = 4 and K = 2), and the
-    decapsulation in 32-request tiles at two waves per SIMD (the default,
-    fused_dec_tile 0; the order then applies to encapsulation only), give the
-    same ciphertexts and shared secrets, through the tile kernels (lowlat_max =
-    -1), at ragged and full sizes; tampered ciphertexts included."""
+    encapsulation and decapsulation in 32-request tiles at two waves per SIMD
+    (the defaults, fused_enc_tile and fused_dec_tile 0; the order then applies
+    to the 64-request tiles only), alone and mixed with the other tile size,
+    give the same ciphertexts and shared secrets, through the tile kernels
+    (lowlat_max = -1), at ragged and full sizes; tampered ciphertexts included."""
     from pqc_amd import tuner as T
     say("mixed tile orders n", n)
     lay = pqc.MixedLayout(mixed_algs(n, 30 + n), np.random.default_rng(31))
@@ -219,9 +220,10 @@ def test_mixed_tile_orders_agree(pqc, ctx, n):
     offs = lay.offsets("ct")
     ref = None
     try:
-        for order, dec_tile in ((0, 0), (0, 64), (1, 64), (2, 64), (3, 64), (3, 0)):
-            say(" order", order, "dec tile", dec_tile)
-            T.put_tuning(ctx, lowlat_max=-1, mixed_order=order, fused_dec_tile=dec_tile)
+        for order, dec_tile, enc_tile in ((0, 0, 0), (0, 64, 64), (1, 64, 64), (2, 64, 64), (3, 64, 64),
+                                          (3, 0, 64), (1, 64, 0)):
+            say(" order", order, "dec tile", dec_tile, "enc tile", enc_tile)
+            T.put_tuning(ctx, lowlat_max=-1, mixed_order=order, fused_dec_tile=dec_tile, fused_enc_tile=enc_tile)
             ar.buf["ct"].upload(np.zeros(lay.nbytes["ct"], np.uint8))
             ctx.kem_enc_mixed(n, ar.enc_reqs, ar.buf["ct"], ar.buf["pk"], ar.buf["ss"], ar.buf["coins"])
             ct, ss = ar.download("ct"), ar.download("ss")
@@ -237,9 +239,9 @@ def test_mixed_tile_orders_agree(pqc, ctx, n):
                 ref = got
             else:
                 for a, b in zip(ref, got):
-                    assert (a == b).all(), (order, dec_tile)
+                    assert (a == b).all(), (order, dec_tile, enc_tile)
     finally:
-        T.put_tuning(ctx, lowlat_max=0, mixed_order=0, fused_dec_tile=0)
+        T.put_tuning(ctx, lowlat_max=0, mixed_order=0, fused_dec_tile=0, fused_enc_tile=0)
     sso = lay.offsets("ss")
     same = np.array([(ref[2][int(o):int(o) + 32] == ref[1][int(o):int(o) + 32]).all() for o in sso])
     assert (same == (np.arange(n) % 5 != 0)).all()
