@@ -90,6 +90,7 @@ typedef struct pqc_tuning {
                                  SIMD, A^T in a global scratch) or 64 (one wave per SIMD, A^T in an LDS
                                  ring); 0 = library default (32) */
   int32_t fused_enc_tile;     /* the same for the fused Kyber encapsulation */
+  int32_t fused_key_tile;     /* the same for the fused Kyber key generation */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
 int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */

@@ -68,7 +68,7 @@ struct pqc_ctx {
   size_t allocated = 0;  // bytes of device memory owned by the context
   int cu_count = 0;
   int fused_grid_override = 0;
-  pqc_tuning tune = {256, 128, 4, 0, 64, 0, 0, 0, 0};
+  pqc_tuning tune = {256, 128, 4, 0, 64, 0, 0, 0, 0, 0};
   // fused decapsulation in 32-request tiles: per-wave A^T scratch for kat_waves waves
   int16_t* kat = nullptr;
   size_t kat_waves = 0;

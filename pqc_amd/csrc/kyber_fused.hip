@@ -1041,11 +1041,10 @@ __device__ __forceinline__ void key_load(KeyIn<K>& in, const uint8_t* prf) {
 // t = frommont(A o s) + e (basemul carries R^-1, frommont multiplies by R^2 R^-1)
 // and packs csubq'd residues; here A o s carries no factor, and canon_f gives
 // the same residues in [0, q).
-template <int K>
-__device__ __forceinline__ void coop_keypair(const int16_t* ring, int e0, const KeyIn<K>& in, uint8_t* pk, uint8_t* sk,
+template <int K, class AT>
+__device__ __forceinline__ void coop_keypair(const AT& at, const KeyIn<K>& in, uint8_t* pk, uint8_t* sk,
                                              const LaneZF& z) {
   using P = Params<K>;
-  constexpr int RING = Tile<K>::RING;
   const int gn = ntt_group(lane_id());
   F4 sv[K];
   Bz svz[K];
@@ -1065,9 +1064,8 @@ __device__ __forceinline__ void coop_keypair(const int16_t* ring, int e0, const 
   for (int i = 0; i < K; ++i) {
     F4 acc = basemul_sum_f<K>(
         [&](int j) {
-          const int slot = (e0 + i * K + j) % RING;
-          const uint32_t* e32 = reinterpret_cast<const uint32_t*>(ring + slot * kAStride + 4 * gn);
-          const uint32_t w0 = e32[0], w1 = e32[1];  // accepted candidates, [0, 19q)
+          uint32_t w0, w1;
+          at.get(i, j, w0, w1);  // A[i][j]: accepted candidates, [0, 19q)
           F4 a = {{(float)(w0 & 0xFFFF), (float)(w0 >> 16), (float)(w1 & 0xFFFF), (float)(w1 >> 16)}};
           reduce_f4(a);
           return a;
@@ -1152,7 +1150,7 @@ __device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, 
     __syncthreads();
     for (int t = t_done; t < t_end; ++t) {
       if (t + 1 < t_end) key_load<K>(nxt, src.prf + (size_t)(base + t + 1) * TL::PRFB);
-      coop_keypair<K>(sh.a, t * TL::KK, cur, src.pk + (size_t)(base + t) * P::pk_bytes,
+      coop_keypair<K>(AtRing<K>{sh.a, t * TL::KK}, cur, src.pk + (size_t)(base + t) * P::pk_bytes,
                       src.sk + (size_t)(base + t) * P::sk_bytes, z);
       cur = nxt;
     }
@@ -1172,6 +1170,112 @@ __device__ __forceinline__ void fused_keypair_tile(const KeySrc& src, int base, 
     }
   }
   __syncthreads();
+}
+
+// Key generation in the 32-request tiles of decapsulation and encapsulation
+// (two waves per SIMD, A through the per-wave global scratch): both lanes of a
+// pair compute G(d) and split the 2K PRF outputs (lane l the even nonces, lane
+// l + 32 the odd ones); the rounds of A entries, each followed by the key
+// computation of the requests it completes; H(pk) on lanes 0..31.  Same bytes
+// as fused_keypair_tile.
+template <int K>
+__device__ __forceinline__ void keypair_tile32(const KeySrc& src, int base, int n, Lds32& sh, global_t<int16_t>* at,
+                                               const LaneZF& z, uint32_t* __restrict__ status) {
+  using P = Params<K>;
+  using TL = Tile32<K>;
+  constexpr size_t PRFB = Tile<K>::PRFB;
+  const int lane = threadIdx.x;
+  const int rq = lane & 31;  // the request this lane serves in the sponge phases (lanes l and l + 32 alike)
+  const int nt = min(TL::T, n - base);
+  const bool act = rq < nt;
+  const bool hl = lane < 32;
+  const int rc = act ? base + rq : base;  // clamped request for address math
+  uint8_t* pk = src.pk + (size_t)rc * P::pk_bytes;
+  uint8_t* sk = src.sk + (size_t)rc * P::sk_bytes;
+  uint8_t* prf = src.prf + (size_t)rc * PRFB;
+  const uint8_t* coins = src.coins + (size_t)rc * 64;
+  // ---- (rho, sigma) = G(d) on both lanes of a pair; PRF(sigma, 2 i + (lane >= 32)),
+  // i = 0 .. K - 1 -> scratch; rho -> LDS and both pk copies
+  {
+    uint64_t d[4], A[25], sg[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) d[w] = load_u64_global(coins + 8 * w);
+    absorb_fixed<kRateSha3_512, 4>(A, d, 0x06);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      sg[w] = A[4 + w];
+      if (hl) store_u64(sh.msg[rq] + 8 * w, A[w]);
+      if (hl && act) {
+        store_u64(pk + P::polyvec_bytes + 8 * w, A[w]);
+        store_u64(sk + P::sk_cpa_bytes + P::polyvec_bytes + 8 * w, A[w]);
+      }
+    }
+    for (int i = 0; i < K; ++i) {
+      const int nonce = 2 * i + (hl ? 0 : 1);
+      absorb_fixed<kRateShake256, 4>(A, sg, (uint64_t)nonce | (0x1Full << 8));
+      if (act) {
+#pragma unroll
+        for (int w = 0; w < 16; ++w) store_u64(prf + 128 * nonce + 8 * w, A[w]);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- rounds of 64 A entries into the scratch (entry E: request E / K^2, A[i][j]
+  // at position i K + j), then t and s of every request the round completes
+  const int ng = (nt * TL::KK + 63) / 64;
+  int t_done = 0;
+  for (int g = 0; g < ng; ++g) {
+    {
+      const int E = 64 * g + lane;
+      const int t = E / TL::KK;
+      const int ij = E % TL::KK;
+      const bool valid = t < nt;
+      uint64_t rho[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) rho[w] = load_u64(sh.msg[valid ? t : 0] + 8 * w);
+      const int got = sample_entry_staged(at + (size_t)E * kAtStride, sh.stage[lane], rho, ij % K, ij / K, valid);
+      if (valid && got < N) atomicOr(status, 1u);
+    }
+    __syncthreads();  // the round's entries are in the scratch for every lane
+    const int t_end = min(nt, (64 * (g + 1)) / TL::KK);
+    if (t_done < t_end) {
+      KeyIn<K> cur, nxt;
+      key_load<K>(nxt, src.prf + (size_t)(base + t_done) * PRFB);
+      for (int t = t_done; t < t_end; ++t) {
+        cur = nxt;
+        if (t + 1 < t_end) key_load<K>(nxt, src.prf + (size_t)(base + t + 1) * PRFB);
+        AtRegs<K> a;
+        a.load(at, t);
+        coop_keypair<K>(a, cur, src.pk + (size_t)(base + t) * P::pk_bytes, src.sk + (size_t)(base + t) * P::sk_bytes,
+                        z);
+      }
+      t_done = t_end;
+    }
+  }
+  __syncthreads();
+  // ---- H(pk) over the packed key (written by the whole wave), z; lanes 0..31 store
+  {
+    uint64_t A[25];
+    sponge_absorb_words_prefetch<kRateSha3_256>(A, reinterpret_cast<const uint64_t*>(pk), P::pk_bytes / 8, 0x06);
+    if (hl && act) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        store_u64(sk + P::hpk_off + 8 * w, A[w]);
+        store_u64(sk + P::z_off + 8 * w, load_u64_global(coins + 32 + 8 * w));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int K>
+__global__ __launch_bounds__(64, 2) void k_fused_keypair32(int n, int ntiles, KeySrc src, int16_t* at_scratch,
+                                                           uint32_t* __restrict__ status) {
+  __shared__ Lds32 sh;
+  LaneZF z;
+  load_lane_zf(z, &c_lane_twf);
+  global_t<int16_t>* at = (global_t<int16_t>*)(at_scratch + (size_t)blockIdx.x * Tile32<K>::ENTRIES * kAtStride);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) keypair_tile32<K>(src, tile * 32, n, sh, at, z, status);
 }
 
 template <int K>
@@ -1720,6 +1824,17 @@ static int launch_enc32(pqc_ctx* c, size_t n, uint8_t* ss, uint8_t* ct, const ui
 template <int K>
 static int launch_fused_keypair(pqc_ctx* c, size_t n, uint8_t* pk, uint8_t* sk, const uint8_t* coins, hipStream_t s) {
   if (n == 0) return PQC_OK;
+  if (c->tune.fused_key_tile != 64 && c->kat) {
+    // 32-request tiles, two waves per SIMD, one scratch region per wave
+    const int ntiles = (int)((n + Tile32<K>::T - 1) / Tile32<K>::T);
+    int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * 2 * c->tune.fused_waves_per_cu;
+    grid = std::min(grid, ntiles);
+    grid = std::min<long long>(grid, (long long)c->kat_waves);
+    if (grid < 1) return PQC_ERR_INVALID_ARG;
+    const KeySrc src{pk, sk, coins, c->kprf};
+    hipLaunchKernelGGL((k_fused_keypair32<K>), dim3(grid), dim3(64), 0, s, (int)n, ntiles, src, c->kat, c->status);
+    return hip_err(hipGetLastError());
+  }
   const int ntiles = (int)((n + Tile<K>::T - 1) / Tile<K>::T);
   int grid = c->fused_grid_override > 0 ? c->fused_grid_override : c->cu_count * c->tune.fused_waves_per_cu;
   if (grid > ntiles) grid = ntiles;

@@ -530,7 +530,8 @@ int pqc_ctx_put_tuning(pqc_ctx* c, const pqc_tuning* t) {
       (t->frodo_stream_block != 64 && t->frodo_stream_block != 128 && t->frodo_stream_block != 256) ||
       t->lowlat_max < -1 || t->lowlat_max > 65536 || t->frodo_chunk < 0 || t->frodo_chunk > 65536 ||
       t->frodo_chunk % 32 != 0 || (t->fused_dec_tile != 0 && t->fused_dec_tile != 32 && t->fused_dec_tile != 64) ||
-      (t->fused_enc_tile != 0 && t->fused_enc_tile != 32 && t->fused_enc_tile != 64))
+      (t->fused_enc_tile != 0 && t->fused_enc_tile != 32 && t->fused_enc_tile != 64) ||
+      (t->fused_key_tile != 0 && t->fused_key_tile != 32 && t->fused_key_tile != 64))
     return PQC_ERR_INVALID_ARG;
   c->tune = *t;
   return PQC_OK;

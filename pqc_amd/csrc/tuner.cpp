@@ -115,7 +115,7 @@ static bool same(const pqc_tuning& a, const pqc_tuning& b) {
   return a.plain_block == b.plain_block && a.newhope_block == b.newhope_block &&
          a.fused_waves_per_cu == b.fused_waves_per_cu && a.mixed_order == b.mixed_order &&
          a.frodo_stream_block == b.frodo_stream_block && tile_of(a.fused_enc_tile) == tile_of(b.fused_enc_tile) &&
-         tile_of(a.fused_dec_tile) == tile_of(b.fused_dec_tile);
+         tile_of(a.fused_dec_tile) == tile_of(b.fused_dec_tile) && tile_of(a.fused_key_tile) == tile_of(b.fused_key_tile);
 }
 
 // median of reps timed runs (one untimed warm-up first)

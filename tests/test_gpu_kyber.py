@@ -135,6 +135,37 @@ def test_ragged_fused_keypair_vs_oracle(ctx, k, n):
     assert (gpk == pk).all() and (gsk == sk).all()
 
 
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 95, 1000, 9001])
+def test_keypair_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
+    """The fused key generation in 32-request tiles (two waves per SIMD, G on
+    both lanes of a pair and the PRF outputs split between them, A through the
+    per-wave scratch; the default) against the 64-request tiles, the plain
+    per-op kernels and the oracle, on ragged batch sizes; the keys then
+    round-trip through encapsulation and decapsulation."""
+    from pqc_amd import tuner as T
+    say("keypair tile32 k", k, "n", n)
+    alg = ALG[k]
+    ctx = pqc.Context([alg], max_batch=max(n, 64))
+    try:
+        kc = O.drbg_array(n, 64, 0, 7000 + 10 * k + n % 97)
+        T.put_tuning(ctx, fused_key_tile=32)
+        pk32, sk32 = ctx.keypair_host(alg, kc)
+        T.put_tuning(ctx, fused_key_tile=64)
+        pk64, sk64 = ctx.keypair_host(alg, kc)
+        pkp, skp = ctx.keypair_host(alg, kc, plain=True)
+        assert (pk32 == pk64).all() and (sk32 == sk64).all()
+        assert (pk32 == pkp).all() and (sk32 == skp).all()
+        idx = np.unique(np.array([0, n // 2, n - 1]))
+        wpk, wsk = O.kyber_batch("keypair", k, [np.ascontiguousarray(kc[idx])])
+        assert (pk32[idx] == wpk).all() and (sk32[idx] == wsk).all()
+        ec = O.drbg_array(n, 32, 0, 7001 + 10 * k + n % 97)
+        ct, ss = ctx.enc_host(alg, pk32, ec)
+        assert (ctx.dec_host(alg, ct, sk32) == ss).all()
+    finally:
+        ctx.close()
+
+
 def test_batch_of_one_equals_batch_of_n(ctx):
     say("batch-of-1 vs batch-of-n")
     coins, ecoins, pk, sk, ct, ss = oracle_triples(3, 40, 900)
