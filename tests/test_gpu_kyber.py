@@ -166,6 +166,44 @@ def test_keypair_tile32_equals_tile64_plain_and_oracle(pqc, k, n):
         ctx.close()
 
 
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_tile32_many_tiles_per_wave(pqc, k):
+    """The 32-request tiles of key generation, encapsulation and decapsulation
+    with a grid of 5 waves, so that each wave runs six or seven tiles one after
+    another (the last one partial) through the same A^T scratch region, LDS
+    staging rows and message slots: the bytes must equal the default grid's,
+    where each wave runs one tile, and the oracle's."""
+    say("tile32 many tiles per wave k", k)
+    alg = ALG[k]
+    n = 1000
+    ctx = pqc.Context([alg], max_batch=n)
+    try:
+        from pqc_amd import tuner as T
+        T.put_tuning(ctx, lowlat_max=-1)
+        kc = O.drbg_array(n, 64, 0, 7100 + k)
+        ec = O.drbg_array(n, 32, 0, 7101 + k)
+        pk, sk = ctx.keypair_host(alg, kc)
+        ct, ss = ctx.enc_host(alg, pk, ec)
+        ct[::4, 17] ^= 0x01
+        dec = ctx.dec_host(alg, ct, sk)
+        ctx.set_tuning(grid=5)
+        pk5, sk5 = ctx.keypair_host(alg, kc)
+        ct5, ss5 = ctx.enc_host(alg, pk, ec)
+        ct5[::4, 17] ^= 0x01
+        dec5 = ctx.dec_host(alg, ct5, sk)
+        assert (pk5 == pk).all() and (sk5 == sk).all()
+        assert (ct5 == ct).all() and (ss5 == ss).all() and (dec5 == dec).all()
+        honest = np.arange(n) % 4 != 0
+        assert (dec[honest] == ss[honest]).all() and (dec[~honest] != ss[~honest]).any(axis=1).all()
+        idx = np.array([0, 31, 32, 499, 999])
+        wpk, wsk = O.kyber_batch("keypair", k, [np.ascontiguousarray(kc[idx])])
+        assert (pk[idx] == wpk).all() and (sk[idx] == wsk).all()
+        want = O.kyber_batch("dec", k, [np.ascontiguousarray(ct[idx]), np.ascontiguousarray(sk[idx])])
+        assert (dec[idx] == want).all()
+    finally:
+        ctx.close()
+
+
 def test_batch_of_one_equals_batch_of_n(ctx):
     say("batch-of-1 vs batch-of-n")
     coins, ecoins, pk, sk, ct, ss = oracle_triples(3, 40, 900)

@@ -203,6 +203,43 @@ def test_graph_replay_mixed_dec(pqc, ctx, path):
     assert ctx.take_status() == 0
 
 
+def test_mixed_tile32_many_tiles_per_wave(pqc, ctx):
+    """Mixed encapsulation and decapsulation in 32-request tiles on a grid of 3
+    waves, each running many tiles of all three parameter sets in turn through
+    one A^T scratch region: the same bytes as the default grid."""
+    from pqc_amd import tuner as T
+    n = 700
+    say("mixed tile32 grid 3 n", n)
+    lay = pqc.MixedLayout(mixed_algs(n, 77), np.random.default_rng(78))
+    ar = pqc.DeviceArenas(lay)
+    fill_keys_on_gpu(pqc, ctx, lay, ar, 79)
+    offs = lay.offsets("ct")
+    outs = []
+    try:
+        T.put_tuning(ctx, lowlat_max=-1, mixed_order=0, fused_dec_tile=0, fused_enc_tile=0)
+        for grid in (0, 3):
+            ctx.set_tuning(grid=grid)
+            ar.buf["ct"].upload(np.zeros(lay.nbytes["ct"], np.uint8))
+            ctx.kem_enc_mixed(n, ar.enc_reqs, ar.buf["ct"], ar.buf["pk"], ar.buf["ss"], ar.buf["coins"])
+            ct, ss = ar.download("ct"), ar.download("ss")
+            bad = ct.copy()
+            for i in range(0, n, 3):
+                bad[int(offs[i]) + 5] ^= 0x08
+            ar.buf["ct"].upload(bad)
+            ar.buf["ss"].upload(np.zeros(lay.nbytes["ss"], np.uint8))
+            ctx.kem_dec_mixed(n, ar.dec_reqs, ar.buf["ct"], ar.buf["sk"], ar.buf["ss"])
+            outs.append((ct, ss, ar.download("ss")))
+            assert ctx.take_status() == 0
+    finally:
+        ctx.set_tuning(grid=0)
+        T.put_tuning(ctx, lowlat_max=0, mixed_order=0, fused_dec_tile=0, fused_enc_tile=0)
+    for a, b in zip(outs[0], outs[1]):
+        assert (a == b).all()
+    sso = lay.offsets("ss")
+    same = np.array([(outs[0][2][int(o):int(o) + 32] == outs[0][1][int(o):int(o) + 32]).all() for o in sso])
+    assert (same == (np.arange(n) % 3 != 0)).all()
+
+
 @pytest.mark.parametrize("n", [5, 65, 3000, 65536])
 def test_mixed_tile_orders_agree(pqc, ctx, n):
     """Every tile order of the 64-request tiles (0: K 4,3,2; 1: K 2,3,4; 2:
