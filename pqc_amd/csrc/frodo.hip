@@ -186,22 +186,68 @@ __global__ void k_prep_frags(int8_t* frag, const uint16_t* A, const uint8_t* pk)
 }
 
 // ---- decapsulation: W = B'S, mu' = decode(C - W) -------------------------------
-// One wave per request; lane (i, j) computes W[i][j] (frodo_mul_bs_kernel), the
-// 8 lanes of row i assemble 24 bits of mu' (frodo_key_decode_kernel).
-__global__ __launch_bounds__(64) void k_w_decode(int n, const uint8_t* ct, FrKeys keys, uint8_t* mu) {
-  const int r = blockIdx.x;
-  if (r >= n) return;
-  const int l = threadIdx.x, i = l >> 3, j = l & 7;
-  const uint8_t* bp = ct + (size_t)r * CT + 2 * N * i;
-  const uint8_t* st = keys.sk_of(r) + SK_S + 2 * N * j;
-  uint32_t acc = 0;
-  for (int k = 0; k < N; k += 4) {
-    const uint64_t bw = load_u64(bp + 2 * k), sw = load_u64(st + 2 * k);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc += bswap16((uint32_t)(bw >> (16 * q))) * ((uint32_t)(sw >> (16 * q)) & 0xFFFF);
+// frodo_mul_bs_kernel + frodo_key_decode_kernel, one wave per request.  The
+// request's S^T (8 rows of 976 int16) is staged in LDS as k-pairs: entry kp holds
+// (S[2kp][j], S[2kp + 1][j]) for j = 0..7, padded to kWdStride words so that the
+// 8 entries a wave reads at once fall in different banks.  Lane (i, c) takes the
+// 16-byte chunks c, c + 8, c + 16, ... of B' row i (the 8 lanes of a row read
+// 128 contiguous bytes per load) and keeps W[i][0..7] as packed 16-bit pairs
+// (even and odd k): only W mod 2^16 is used, so the 16-bit wrap is exact.  The 8
+// lanes of a row then sum their partial rows, and lane (i, j) decodes W[i][j].
+// A workgroup takes requests r, r + G, ...; under one key for all requests the
+// S^T staging happens once per workgroup, else once per request.
+constexpr int kWdChunks = N / 8;  // 16-byte chunks (8 coefficients, 4 k-pairs) per row
+constexpr int kWdStride = 10;     // words per k-pair entry: 8 + 2 pad
+static_assert(N % 8 == 0, "whole chunks");
+
+__device__ __forceinline__ void w_stage(uint32_t* spair, const uint8_t* sk, int l) {
+  // S^T row j, chunk q: k-pairs 4q .. 4q + 3 (S^T sits 8-byte aligned in sk)
+  const uint8_t* st = sk + SK_S;
+  for (int t = l; t < NB * kWdChunks; t += 64) {
+    const int j = t / kWdChunks, q = t - j * kWdChunks;
+    const uint8_t* src = st + 2 * N * j + 16 * q;
+    const uint2 lo = *reinterpret_cast<const uint2*>(src), hi = *reinterpret_cast<const uint2*>(src + 8);
+    uint32_t* e = spair + 4 * q * kWdStride + j;
+    e[0] = lo.x;
+    e[kWdStride] = lo.y;
+    e[2 * kWdStride] = hi.x;
+    e[3 * kWdStride] = hi.y;
   }
-  const uint32_t c = bswap16(*reinterpret_cast<const uint16_t*>(ct + (size_t)r * CT + C1 + 2 * (8 * i + j)));
-  const uint32_t m = (c - acc) & 0xFFFF;
+}
+
+__device__ __forceinline__ void w_decode_one(const uint32_t* spair, int r, const uint8_t* ct, uint8_t* mu, int l) {
+  const int i = l >> 3, c = l & 7;
+  typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+  u16x2 acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = u16x2{0, 0};
+  const uint8_t* bp = ct + (size_t)r * CT + 2 * N * i;
+  for (int q = c; q < kWdChunks; q += 8) {
+    const uint4 b = *reinterpret_cast<const uint4*>(bp + 16 * q);
+    const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      // two big-endian B' coefficients -> two native uint16
+      const u16x2 b2 = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(0u, bw[p], 0x02030001u));
+      const uint32_t* e = spair + (4 * q + p) * kWdStride;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[j] += b2 * __builtin_bit_cast(u16x2, e[j]);
+    }
+  }
+  uint32_t w[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    w[j] = (uint32_t)acc[j].x + (uint32_t)acc[j].y;
+#pragma unroll
+    for (int sft = 1; sft < 8; sft <<= 1) w[j] += (uint32_t)__shfl_xor((int)w[j], sft);
+  }
+  // lane (i, j): W[i][j]
+  uint32_t wij = w[0];
+#pragma unroll
+  for (int j = 1; j < NB; ++j) wij = c == j ? w[j] : wij;
+  const int j = c;
+  const uint32_t cv = bswap16(*reinterpret_cast<const uint16_t*>(ct + (size_t)r * CT + C1 + 2 * (8 * i + j)));
+  const uint32_t m = (cv - wij) & 0xFFFF;
   uint32_t word = ((m + (1u << 12)) >> 13 & 7) << (3 * j);
   word |= (uint32_t)__shfl_xor((int)word, 1);
   word |= (uint32_t)__shfl_xor((int)word, 2);
@@ -213,6 +259,22 @@ __global__ __launch_bounds__(64) void k_w_decode(int n, const uint8_t* ct, FrKey
     o[2] = (uint8_t)(word >> 16);
   }
 }
+
+__global__ __launch_bounds__(64) void k_w_decode(int n, const uint8_t* ct, FrKeys keys, uint8_t* mu) {
+  __shared__ uint32_t spair[(N / 2) * kWdStride];
+  const int l = threadIdx.x;
+  const bool one_key = keys.sk_stride == 0 && keys.kid == nullptr;  // uniform
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    if (!one_key || r == (int)blockIdx.x) {
+      __syncthreads();  // the previous request's LDS reads are done
+      w_stage(spair, keys.sk_of(r), l);
+      __syncthreads();
+    }
+    w_decode_one(spair, r, ct, mu, l);
+  }
+}
+
+static inline int w_decode_grid(const pqc_ctx* c, int n) { return std::max(1, std::min(n, c->cu_count * 8)); }
 
 // ---- G2 + the S'/E'/E'' SHAKE256 stream (one lane pair per request) ----------------
 // Pairs [0, npad): (seedSE || k) = SHAKE256(pkh || mu, 48), then the stream
@@ -1055,7 +1117,7 @@ int frodo_dec(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, hipStream_t 
     const int ni = (int)m;
     const uint8_t* ctc = ct + e0 * CT;
     uint8_t* ssc = ss + e0 * SS;
-    hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, t, ni, ctc, keys, f.mu);
+    hipLaunchKernelGGL(k_w_decode, dim3(w_decode_grid(c, ni)), dim3(64), 0, t, ni, ctc, keys, f.mu);
     launch_stream<false>(c, f, ni, 1, keys, f.mu, 32, ctc, t);
     int r = launch_gemm<kDecaps>(c, m, f, const_cast<uint8_t*>(ctc), f.mu, 32, t);
     if (r) return r;
@@ -1073,7 +1135,7 @@ int frodo_dec_keyed(pqc_ctx* c, size_t n, uint8_t* ss, const uint8_t* ct, const 
   if (n == 0) return PQC_OK;
   FrScratch f = fr_scratch(c);
   const int ni = (int)n;
-  hipLaunchKernelGGL(k_w_decode, dim3(ni), dim3(64), 0, s, ni, ct, keys, f.mu);
+  hipLaunchKernelGGL(k_w_decode, dim3(w_decode_grid(c, ni)), dim3(64), 0, s, ni, ct, keys, f.mu);
   launch_stream<true>(c, f, ni, 1, keys, f.mu, 32, ct, s);
   hipLaunchKernelGGL(k_mk_product<kDecaps>, dim3(ni, 2), dim3(256), 0, s, ni, (const int8_t*)f.sfrag,
                      (const int8_t*)f.efrag, keys, const_cast<uint8_t*>(ct), (const uint8_t*)f.mu, 32, f.flags);
