@@ -921,17 +921,19 @@ def learned_autotune(ctx, T, n, run_n, stream, budget=4):
     the sweep's best.  The model's pick is what the timed graph runs."""
     train = [n // 4, n // 2, 3 * n // 4]
     default = T.get_tuning(ctx).as_dict()
+    # medians of several runs: each run times eager launches from Python, and one
+    # slow launch in two runs made a worse configuration look better
     for m in train:
-        T.autotune(ctx, T.request("mixed_kyber", m, k=3, op=2, budget=0, reps=2), lambda s, m=m: run_n(m, s), stream)
+        T.autotune(ctx, T.request("mixed_kyber", m, k=3, op=2, budget=0, reps=3), lambda s, m=m: run_n(m, s), stream)
     x, y = T.samples(ctx)
     model = T.CostModel.of_context(ctx)
     loss = model.fit(x, y, steps=3000, lr=3e-3)
     T.put_tuning(ctx, **default)
-    req = T.request("mixed_kyber", n, k=3, op=2, budget=budget, use_model=True, reps=2)
+    req = T.request("mixed_kyber", n, k=3, op=2, budget=budget, use_model=True, reps=5)
     guided = T.autotune(ctx, req, lambda s: run_n(n, s), stream)
     pick = guided.best.as_dict()
     before = len(T.samples(ctx)[1])
-    sweep = T.autotune(ctx, T.request("mixed_kyber", n, k=3, op=2, budget=0, reps=2), lambda s: run_n(n, s), stream)
+    sweep = T.autotune(ctx, T.request("mixed_kyber", n, k=3, op=2, budget=0, reps=5), lambda s: run_n(n, s), stream)
     sx, sy = T.samples(ctx)
     sx, sy = sx[before:], np.exp(sy[before:])
     def in_sweep(tuning_dict):

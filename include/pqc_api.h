@@ -134,8 +134,10 @@ int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *
  * use_model = 0: candidates in a fixed order (the configuration in use first)
  * up to `budget` (0 = all).  use_model = 1: once the context's learned cost
  * model has been fitted (after the first use_model call), it ranks all
- * candidates and only the best-ranked `budget` are measured; the model is then
- * refitted on the GPU with every sample.  The best measured configuration is
+ * candidates and only the best-ranked `budget` are measured (with a budget of
+ * two or more, the configuration in use is always one of them, so the result is
+ * never measured slower than it); the model is then refitted on the GPU with
+ * every sample.  The best measured configuration is
  * installed in the context and returned.                                       */
 #define PQC_TUNE_PLAIN_KYBER 0 /* plain_block */
 #define PQC_TUNE_FUSED_KYBER 1 /* fused_waves_per_cu */

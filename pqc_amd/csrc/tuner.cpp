@@ -215,6 +215,15 @@ int pqc_autotune(pqc_ctx* c, const pqc_tune_request* r, pqc_workload_fn run, voi
     int rc = pqc_costmodel_predict(S->model, (int)cand.size(), fx.data(), pred.data());
     if (rc) return rc;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return pred[a] < pred[b]; });
+    // with two or more measurements, the configuration in use is one of them
+    // (the last), so a wrong ranking never installs something measured slower
+    const size_t b = r->budget == 0 ? cand.size() : std::min<size_t>(r->budget, cand.size());
+    if (b >= 2)
+      for (size_t i = 0; i < cand.size(); ++i)
+        if (same(cand[order[i]], start)) {
+          if (i >= b) std::rotate(order.begin() + b - 1, order.begin() + i, order.begin() + i + 1);
+          break;
+        }
   } else {
     // the configuration in use first, the rest in enumeration order
     for (size_t i = 0; i < cand.size(); ++i)
