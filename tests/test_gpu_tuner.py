@@ -139,6 +139,25 @@ def test_trained_model_beats_untuned_and_blind_choice(pqc):
     say("model-guided, 1 measurement: %s %.3f ms (predicted %.3f)" % (res.best.as_dict(), res.best_ms,
                                                                         res.predicted_ms))
     assert res.measured == 1 and res.best_ms <= 1.08 * best
+
+    # with two model-guided measurements, the configuration in use is always one of
+    # them: start from the block size the model ranks last, so the guarantee (not the
+    # ranking) puts it in the measured set, and the pick is never slower than it
+    req2 = T.request("plain_kyber", n, k=3, op=2, budget=2, use_model=True, reps=3)
+    blocks = list(range(64, 1025, 64))
+    fb = np.stack([T.features(req2, T.put_tuning(ctx, plain_block=b)) for b in blocks])
+    worst = blocks[int(np.argmax(model.predict(fb)))]
+    start = T.put_tuning(ctx, plain_block=worst)
+    before = len(T.samples(ctx)[1])
+    res2 = T.autotune(ctx, req2, plain(n), st)
+    x2, y2 = T.samples(ctx)
+    fs = T.features(req2, start)
+    hit = [i for i in range(before, len(y2)) if np.array_equal(x2[i], fs)]
+    say("model-guided, 2 measurements from block %d: pick %d, %.3f ms (start measured: %s)" %
+        (worst, res2.best.plain_block, res2.best_ms, ["%.3f ms" % np.exp(y2[i]) for i in hit]))
+    assert res2.measured == 2 and len(y2) - before == 2
+    assert len(hit) == 1
+    assert res2.best_ms <= float(np.exp(y2[hit[0]])) * 1.0001
     # tuning changes no result
     ctx.dec("kyber768", n, ss2, ct, sk, plain=True, stream=st)
     st.synchronize()
