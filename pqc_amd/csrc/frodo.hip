@@ -438,20 +438,25 @@ __device__ __forceinline__ uint32_t encode_mu(const uint8_t* mu, int k, int i) {
 
 // ---- the shared-A GEMM on int8 MFMA ----------------------------------------------
 // Workgroup = 4 waves, tile = 256 rows (16 m-tiles, 4 per wave) x 64 columns
-// (4 n-tiles) x 2 digits.  Each 64-deep k-step's 16 S' and 8 [A | B] fragments
-// go global -> LDS by global_load_lds_dwordx4 (each wave moves 4 + 2) into a
-// 3-slot ring, two steps ahead of the MFMAs; every slot is its own __shared__
-// object so the compiler's LDS-DMA tracking never makes an LDS read wait for an
-// unrelated slot.  Per step: this wave's DMAs of the step done (explicit vmcnt),
-// LDS reads of the previous step returned (lgkmcnt 0), one s_barrier.
+// (4 n-tiles) x 2 digits.  Per 64-deep k-step, the 8 [A | B] fragments, which
+// all four waves read, go global -> LDS by global_load_lds_dwordx4 (2 per wave)
+// into a 3-slot ring; a wave's own 4 S' fragments, which no other wave reads, go
+// straight into a 3-slot register ring by plain 16-byte loads.  Both are issued
+// two steps ahead of the MFMAs.  (S' through LDS cost a DMA write and an LDS
+// read of 16 KB per workgroup and step for data no other wave uses: 3-5% of the
+// launch.)  Every LDS slot is its own __shared__ object so the compiler's LDS-DMA
+// tracking never makes an LDS read wait for an unrelated slot.  Per step: this
+// wave's loads of the step done (explicit vmcnt), LDS reads of the previous step
+// returned (lgkmcnt 0), one s_barrier.
 //
 // The MFMA takes [A | B] as its first operand, so the output is the transposed
 // tile: lane l holds row 16 mt + (l & 15), columns 16 nt + 4 (l >> 4) + r,
 // i.e. four consecutive uint16 of a ciphertext row -- the epilogue reads and
 // writes 8-byte words.  [E' | E''] (16 bytes per m-tile) is the lo digit's
 // starting value, loaded first.  Decapsulation's ciphertext words are loaded
-// during steps 10..13, one m-tile per step: issued at once, every workgroup's
-// burst of HBM misses queued ahead of its next DMAs and stalled the step after.
+// during steps 12..15, one m-tile per step, when the S' register ring drains
+// (earlier, the 32 VGPRs they hold spill); issued at once, every workgroup's
+// burst of HBM misses queued ahead of its next loads and stalled the step after.
 //
 // Epilogue per mode:
 //   products: [B'' | V] (uint16) to the output buffer;
@@ -468,15 +473,8 @@ constexpr int GEMM_NB = NT / 4;  // 16 column blocks
 constexpr int GEMM_NTILES = (N + NB + 15) / 16;  // 62 n-tiles carry data
 enum { kProducts = 0, kEncaps = 1, kDecaps = 2 };
 
-__shared__ v4i g_sa0[16][64], g_sa1[16][64], g_sa2[16][64];  // S' fragments of one k-step
-__shared__ v4i g_sb0[8][64], g_sb1[8][64], g_sb2[8][64];     // [A | B] fragments of one k-step
+__shared__ v4i g_sb0[8][64], g_sb1[8][64], g_sb2[8][64];  // [A | B] fragments of one k-step
 
-template <int ST>
-__device__ __forceinline__ v4i (*ring_a())[64] {
-  if constexpr (ST == 0) return g_sa0;
-  else if constexpr (ST == 1) return g_sa1;
-  else return g_sa2;
-}
 template <int ST>
 __device__ __forceinline__ v4i (*ring_b())[64] {
   if constexpr (ST == 0) return g_sb0;
@@ -490,17 +488,21 @@ struct GemmArgs {
   int mt0, nb, wave, lane;
 };
 
+// this wave's S' fragments (4 m-tiles) of three k-steps
+struct SRing {
+  v4i a[3][4];
+};
+
 template <int KS>
-__device__ __forceinline__ void gemm_issue(const GemmArgs& g) {
+__device__ __forceinline__ void gemm_issue(const GemmArgs& g, SRing& sr) {
 #if defined(__HIP_DEVICE_COMPILE__)  // the LDS-DMA builtin exists only in the device pass
   constexpr int st = KS % 3;
   // a wave-uniform fragment address plus the lane's 32-bit offset (the SGPR-base form)
   const uint32_t lo = (uint32_t)g.lane * 16u;
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
-    __builtin_amdgcn_global_load_lds(
-        reinterpret_cast<const uint8_t*>(g.sfrag + ((size_t)(g.mt0 + mi) * KSTEPS + KS) * 64) + lo,
-        &ring_a<st>()[g.wave * 4 + mi][0], 16, 0, 0);
+    sr.a[st][mi] = *reinterpret_cast<const v4i*>(
+        reinterpret_cast<const uint8_t*>(g.sfrag + ((size_t)(g.mt0 + mi) * KSTEPS + KS) * 64) + lo);
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int f = 2 * g.wave + q, ni = f >> 1, d = f & 1;
@@ -518,7 +520,7 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
   __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | 0x0070);
 }
 
-constexpr int DMA_PER_STEP = 6;  // global_load_lds issued per wave per k-step
+constexpr int DMA_PER_STEP = 6;  // loads issued per wave per k-step: 4 S' (registers) + 2 [A | B] (LDS DMA)
 
 template <int MODE>
 struct EpiRegs {
@@ -549,7 +551,8 @@ __device__ __forceinline__ void epi_issue(EpiRegs<MODE>& ep, const GemmArgs& g, 
 }
 
 // the steps that issue the ciphertext words (m-tile ks - kEpiStep0 at step ks)
-constexpr int kEpiStep0 = 10;
+constexpr int kEpiStep0 = 12;
+static_assert(kEpiStep0 + 4 <= KSTEPS, "every m-tile's words are loaded inside the k loop");
 template <int MODE>
 constexpr int epi_loads_at(int ks) {
   return ks >= kEpiStep0 && ks < kEpiStep0 + 4 ? EpiRegs<MODE>::kLoads / 4 : 0;
@@ -557,15 +560,17 @@ constexpr int epi_loads_at(int ks) {
 
 template <int MODE, int KS>
 __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g, EpiRegs<MODE>& ep, uint8_t* ct,
-                                          int nrows) {
-  // this wave's DMAs of step KS done; what may stay in flight is what was issued
-  // after them, at steps KS - 2 and KS - 1: the DMAs of step KS + 1 and the
-  // ciphertext words of those steps
+                                          int nrows, SRing& sr) {
+  // this wave's loads of step KS done; what may stay in flight is what was issued
+  // after them, at steps KS - 2 and KS - 1: the loads of step KS + 1 and the
+  // ciphertext words of those steps.  (Within a step the S' loads and the DMAs
+  // may issue in either order; the s_barrier asm below keeps each step's loads
+  // in its own step.)
   constexpr int allowed =
       (KS + 1 < KSTEPS ? DMA_PER_STEP : 0) + epi_loads_at<MODE>(KS - 2) + epi_loads_at<MODE>(KS - 1);
   wait_vm_lgkm0<allowed>();
   asm volatile("s_barrier" ::: "memory");  // a workgroup barrier no memory access moves across
-  if constexpr (KS + 2 < KSTEPS) gemm_issue<KS + 2>(g);
+  if constexpr (KS + 2 < KSTEPS) gemm_issue<KS + 2>(g, sr);
   if constexpr (KS >= kEpiStep0 && KS < kEpiStep0 + 4) {
     // the vmcnt values above count these loads after this step's DMAs: keep the
     // scheduler from moving them across
@@ -577,7 +582,7 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
   if constexpr (KS + 1 < KSTEPS) {
     v4i a[4];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) a[mi] = ring_a<st>()[g.wave * 4 + mi][g.lane];
+    for (int mi = 0; mi < 4; ++mi) a[mi] = sr.a[st][mi];
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) {
       if (4 * g.nb + ni >= GEMM_NTILES) continue;  // uniform: padding n-tiles
@@ -593,7 +598,7 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
     // the 16-deep k tail (frag_k): 32-deep MFMAs on the first 8 bytes of each lane
     long a[4];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) a[mi] = *reinterpret_cast<const long*>(&ring_a<st>()[g.wave * 4 + mi][g.lane]);
+    for (int mi = 0; mi < 4; ++mi) a[mi] = (long)(uint32_t)sr.a[st][mi][0] | ((long)sr.a[st][mi][1] << 32);
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) {
       if (4 * g.nb + ni >= GEMM_NTILES) continue;
@@ -606,7 +611,7 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
       }
     }
   }
-  if constexpr (KS + 1 < KSTEPS) gemm_step<MODE, KS + 1>(acc, g, ep, ct, nrows);
+  if constexpr (KS + 1 < KSTEPS) gemm_step<MODE, KS + 1>(acc, g, ep, ct, nrows, sr);
 }
 
 __device__ __forceinline__ uint32_t bswap16x2(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x02030001u); }
@@ -644,8 +649,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* 
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
     e[mi] = *reinterpret_cast<const uint4*>(efrag + (((size_t)(g.mt0 + mi) * 16 + nb) * 64 + lane) * 16);
-  gemm_issue<0>(g);
-  gemm_issue<1>(g);
+  SRing sr;
+  gemm_issue<0>(g, sr);
+  // the explicit vmcnt values count the loads in step order: step 1's plain loads
+  // must not move above step 0's LDS DMAs
+  asm volatile("" ::: "memory");
+  gemm_issue<1>(g, sr);
   v4i acc[4][4][2];
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
@@ -657,7 +666,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(int mblocks, int n, const v4i* 
         acc[mi][ni][1][r] = 0;
       }
   EpiRegs<MODE> ep;
-  gemm_step<MODE, 0>(acc, g, ep, ct, nrows);
+  gemm_step<MODE, 0>(acc, g, ep, ct, nrows, sr);
 
   // epilogue: value = acc_lo + 256 acc_hi (mod 2^16) (acc_lo started at E); lane
   // holds row 16 mt + (l & 15), columns n0 .. n0 + 3 with n0 = 16 nt + 4 (l >> 4)
