@@ -47,8 +47,8 @@ HBM_PEAK_GBPS = 8000.0
 # it is the A^T scratch of the 32-request tiles, written and read back per wave.
 MEASURED_TRAFFIC_BYTES_B65536 = 1301.9e6
 # HBM bytes per Frodo-976 decapsulation GEMM launch at batch 4096, same method
-# (profiles/frodo976_decaps_b4096.md): 185.7 MB read + 0.08 MB written.
-MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 185.8e6
+# (profiles/frodo976_decaps_b4096.md): 186.2 MB read + 0.07 MB written.
+MEASURED_TRAFFIC_FRODO_GEMM_B4096 = 186.3e6
 # HBM bytes per NewHope-1024 fused encryption launch at batch 16384, same method
 # (profiles/newhope1024_cpa_triples_b16384.md): 35.86 MB read + 35.65 MB written.
 MEASURED_TRAFFIC_NEWHOPE_ENC_B16384 = 71.5e6
