@@ -368,6 +368,65 @@ __global__ __launch_bounds__(256) void k_shake(int n, int npad, int with_f, FrKe
   }
 }
 
+// The same G2, stream and F prefix with one lane per request (the throughput form):
+// a lane pair finishes a request's chain sooner, but costs 2 x 107 against 180 VALU
+// instructions per round, so calls that fill every SIMD several waves deep run
+// faster on single lanes.  The stream is written as plain little-endian words.
+// Lanes [0, nlane) are requests; [nlane, nlane + n) the F prefixes (nlane is a
+// multiple of 64, so no wave mixes the two).
+__global__ __launch_bounds__(256) void k_shake_lane(int n, int nlane, int with_f, FrKeys keys, const uint8_t* mu,
+                                                   int mu_stride, uint8_t* raw, uint8_t* kout, const uint8_t* ct,
+                                                   uint64_t* fstate) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t S[25];
+#pragma unroll
+  for (int w = 0; w < 25; ++w) S[w] = 0;
+  if (t < nlane) {
+    const int r = t;
+    if (r >= n) return;
+    const uint8_t* ph = keys.pkh_of(r);
+    const uint8_t* pm = mu + (size_t)r * mu_stride;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      S[w] = load_u64(ph + 8 * w);
+      S[3 + w] = load_u64(pm + 8 * w);
+    }
+    S[6] ^= 0x1Full;
+    S[kRateShake256 - 1] ^= 0x8000000000000000ull;
+    keccakf1600(S);
+#pragma unroll
+    for (int w = 0; w < 3; ++w) store_u64(kout + (size_t)r * 32 + 8 * w, S[3 + w]);
+    // 0x96 || seedSE, padded
+    const uint64_t sd0 = S[0], sd1 = S[1], sd2 = S[2];
+#pragma unroll
+    for (int w = 0; w < 25; ++w) S[w] = 0;
+    S[0] = 0x96 | (sd0 << 8);
+    S[1] = (sd0 >> 56) | (sd1 << 8);
+    S[2] = (sd1 >> 56) | (sd2 << 8);
+    S[3] = (sd2 >> 56) | (0x1Full << 8);
+    S[kRateShake256 - 1] ^= 0x8000000000000000ull;
+    uint64_t* out = reinterpret_cast<uint64_t*>(raw + (size_t)r * RAWB);
+    for (int blk = 0; blk < 231; ++blk) {
+      keccakf1600(S);
+#pragma unroll
+      for (int w = 0; w < kRateShake256; ++w) out[w] = S[w];
+      out += kRateShake256;
+    }
+  } else if (with_f) {
+    const int r = t - nlane;
+    if (r >= n) return;
+    const uint64_t* p = reinterpret_cast<const uint64_t*>(ct + (size_t)r * CT);
+    for (int blk = 0; blk < (CT + SS) / 136; ++blk) {  // 115 full blocks
+#pragma unroll
+      for (int w = 0; w < kRateShake256; ++w) S[w] ^= p[w];
+      keccakf1600(S);
+      p += kRateShake256;
+    }
+#pragma unroll
+    for (int w = 0; w < 25; ++w) fstate[(size_t)r * 25 + w] = S[w];
+  }
+}
+
 // CDF sampling of the raw stream (frodo_sample_n, frodo976.cu:693-715; kem.c:94-98):
 // one thread per 8 consecutive samples (16 stream bytes).  Sample g of request r
 // is S' row g / 976, column g % 976 for g < 7808, then E' likewise, then E''
@@ -391,15 +450,22 @@ __device__ __forceinline__ uint32_t il_join(uint32_t e, uint32_t o, int h) {
 }
 static_assert(NSAMP % 8 == 0 && (N * NB) % 8 == 0, "sample groups");
 
-template <bool ROWS>
+// IL: the stream as k_shake writes it (each word as its even-bit then odd-bit
+// halves); otherwise plain little-endian words (k_shake_lane).
+template <bool ROWS, bool IL>
 __global__ __launch_bounds__(256) void k_sample(int n, const uint8_t* raw, int8_t* sfrag, int8_t* efrag) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * kSampleThreads) return;
   const int r = t / kSampleThreads, g = 8 * (t - r * kSampleThreads);
-  // two stream words, each as (even-bit word, odd-bit word)
+  // two stream words
   const uint4 v = *reinterpret_cast<const uint4*>(raw + (size_t)r * RAWB + 2 * g);
-  const uint32_t w0 = il_join(v.x, v.y, 0), w1 = il_join(v.x, v.y, 1);
-  const uint32_t w2 = il_join(v.z, v.w, 0), w3 = il_join(v.z, v.w, 1);
+  uint32_t w0 = v.x, w1 = v.y, w2 = v.z, w3 = v.w;
+  if constexpr (IL) {
+    w0 = il_join(v.x, v.y, 0);
+    w1 = il_join(v.x, v.y, 1);
+    w2 = il_join(v.z, v.w, 0);
+    w3 = il_join(v.z, v.w, 1);
+  }
   const uint32_t lo = bytes2(sample2(w0)) | (bytes2(sample2(w1)) << 16);
   const uint32_t hi = bytes2(sample2(w2)) | (bytes2(sample2(w3)) << 16);
   if constexpr (ROWS) {
@@ -1055,6 +1121,7 @@ static int launch_gemm(pqc_ctx* c, size_t n, const FrScratch& f, uint8_t* ct, co
 static inline int pairs_pad(int n) { return (n + 31) / 32 * 32; }
 
 constexpr int kFrChunk = 128;  // per-element-key requests per A expansion (A buffer: 128 x 1.9 MB)
+constexpr int kFrLaneMin = 49152;  // calls from this size run the stream on one lane per request
 
 // the loaded key as a key source for every request (strides 0)
 static FrKeys loaded_key(pqc_ctx* c) {
@@ -1066,13 +1133,21 @@ static FrKeys loaded_key(pqc_ctx* c) {
 template <bool ROWS>
 static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, const FrKeys& keys, const uint8_t* mu,
                           int mu_stride, const uint8_t* ct, hipStream_t s) {
-  const int npad = pairs_pad(ni), sb = c->tune.frodo_stream_block;
+  const int sb = c->tune.frodo_stream_block, nt = ni * kSampleThreads;
+  if (!ROWS && ni >= kFrLaneMin) {  // shared-key calls; the keyed paths run at most a server batch
+    const int nlane = (ni + 63) / 64 * 64, lanes = nlane + (with_f ? ni : 0);
+    hipLaunchKernelGGL(k_shake_lane, dim3((lanes + sb - 1) / sb), dim3(sb), 0, s, ni, nlane, with_f, keys, mu,
+                       mu_stride, f.raw, f.kp, ct, with_f ? reinterpret_cast<uint64_t*>(f.fstate) : nullptr);
+    hipLaunchKernelGGL((k_sample<false, false>), dim3((nt + 255) / 256), dim3(256), 0, s, ni, (const uint8_t*)f.raw,
+                       f.sfrag, f.efrag);
+    return;
+  }
+  const int npad = pairs_pad(ni);
   const int pairs = npad + (with_f ? ni : 0);
   hipLaunchKernelGGL(k_shake, dim3((2 * pairs + sb - 1) / sb), dim3(sb), 0, s, ni, npad, with_f, keys, mu, mu_stride,
                      f.raw, f.kp, ct, with_f ? f.fstate : (uint32_t*)nullptr);
-  const int nt = ni * kSampleThreads;
-  hipLaunchKernelGGL(k_sample<ROWS>, dim3((nt + 255) / 256), dim3(256), 0, s, ni, (const uint8_t*)f.raw, f.sfrag,
-                     f.efrag);
+  hipLaunchKernelGGL((k_sample<ROWS, true>), dim3((nt + 255) / 256), dim3(256), 0, s, ni, (const uint8_t*)f.raw,
+                     f.sfrag, f.efrag);
 }
 
 // ---- batches under the loaded key, optionally cut into chunks over streams ------

@@ -203,3 +203,53 @@ def test_pipelined_chunks_equal_unsplit_and_oracle(keys):
     oct_, oss = O.frodo_enc_batch(pk, mu[idx], nthreads=8)
     assert (ct[idx] == oct_).all() and (ss[idx] == oss).all()
     assert (ss2[idx] == O.frodo_dec_batch(bad[idx], sk, nthreads=8)).all()
+
+
+@pytest.mark.gpu
+def test_lane_stream_large_call_equals_pair_stream_and_oracle(keys):
+    """A shared-key call of 50 000 requests runs the noise stream and the F prefix
+    with one lane per request (and samples plain stream words); 4096-request
+    calls over the same inputs run them on lane pairs.  Encapsulation and
+    decapsulation (with tampered ciphertexts) must give the same bytes both ways,
+    and the oracle's on samples across the batch."""
+    import pqc_amd
+    n = 50000
+    say("frodo lane stream n", n)
+    pk, sk = keys
+    rng = np.random.default_rng(29)
+    mu = rng.integers(0, 256, (n, 24), dtype=np.uint8)
+    big = pqc_amd.Context(["frodo976"], max_batch=n)
+    try:
+        big.frodo_load_key(np.frombuffer(pk, np.uint8).copy(), False, host=True)
+        d_mu = pqc_amd.DeviceBuffer.from_array(mu)
+        d_ct, d_ss = pqc_amd.DeviceBuffer(n * CT), pqc_amd.DeviceBuffer(n * 24)
+        big.frodo_enc(n, d_ct, d_ss, d_mu)
+        ct, ss = d_ct.download((n, CT)), d_ss.download((n, 24))
+        del d_ct
+        bad = ct.copy()
+        bad[::11, 3] ^= 0x10          # in B'
+        bad[5::17, CT - 30] ^= 0x02   # in C
+        big.frodo_load_key(np.frombuffer(sk, np.uint8).copy(), True, host=True)
+        d_bad, d_ss2 = pqc_amd.DeviceBuffer.from_array(bad), pqc_amd.DeviceBuffer(n * 24)
+        big.frodo_dec(n, d_ss2, d_bad)
+        ss2 = d_ss2.download((n, 24))
+    finally:
+        big.close()
+    one = pqc_amd.Context(["frodo976"], max_batch=4096)
+    try:
+        for lo in (0, 24576, n - 4096):
+            ct1, ss1 = one.frodo_enc_host(pk, mu[lo:lo + 4096])
+            assert (ct1 == ct[lo:lo + 4096]).all() and (ss1 == ss[lo:lo + 4096]).all(), lo
+            one.frodo_load_key(np.frombuffer(sk, np.uint8).copy(), True, host=True)
+            assert (one.frodo_dec_host(sk, bad[lo:lo + 4096]) == ss2[lo:lo + 4096]).all(), lo
+    finally:
+        one.close()
+    tampered = np.zeros(n, bool)
+    tampered[::11] = True
+    tampered[5::17] = True
+    same = (ss2 == ss).all(axis=1)
+    assert (~same[tampered]).all() and same[~tampered].all()
+    idx = np.array([0, 5, 63, 64, 12345, 33000, 49151, 49152, n - 1])
+    oct_, oss = O.frodo_enc_batch(pk, mu[idx], nthreads=8)
+    assert (ct[idx] == oct_).all() and (ss[idx] == oss).all()
+    assert (ss2[idx] == O.frodo_dec_batch(bad[idx], sk, nthreads=8)).all()
