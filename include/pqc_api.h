@@ -91,6 +91,10 @@ typedef struct pqc_tuning {
                                  ring); 0 = library default (32) */
   int32_t fused_enc_tile;     /* the same for the fused Kyber encapsulation */
   int32_t fused_key_tile;     /* the same for the fused Kyber key generation */
+  int32_t frodo_lane_min;     /* Keccak lane layout of the FrodoKEM-976 noise stream and F prefix in
+                                 shared-key calls: calls of at least this many requests run one state per
+                                 lane (throughput), smaller ones one state per lane pair (latency);
+                                 0 = library default (49152), -1 = lane pairs always, else >= 1 */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);
 int pqc_ctx_put_tuning(pqc_ctx *ctx, const pqc_tuning *t); /* PQC_ERR_INVALID_ARG outside the ranges */
@@ -143,7 +147,7 @@ int crypto_kem_enc_batch(pqc_ctx *ctx, int alg, size_t n, uint8_t *ct, uint8_t *
 #define PQC_TUNE_FUSED_KYBER 1 /* fused_waves_per_cu */
 #define PQC_TUNE_MIXED_KYBER 2 /* fused_waves_per_cu x tile layout: 32-request tiles both ways, or
                                   64-request tiles (fused_enc_tile = fused_dec_tile = 64) in mixed_order 0, 1, 3 */
-#define PQC_TUNE_FRODO 3       /* frodo_stream_block */
+#define PQC_TUNE_FRODO 3       /* frodo_stream_block x Keccak lane layout (frodo_lane_min) */
 #define PQC_TUNE_NEWHOPE 4     /* newhope_block */
 typedef int (*pqc_workload_fn)(void *user, void *stream);
 typedef struct pqc_tune_request {

@@ -1121,7 +1121,6 @@ static int launch_gemm(pqc_ctx* c, size_t n, const FrScratch& f, uint8_t* ct, co
 static inline int pairs_pad(int n) { return (n + 31) / 32 * 32; }
 
 constexpr int kFrChunk = 128;  // per-element-key requests per A expansion (A buffer: 128 x 1.9 MB)
-constexpr int kFrLaneMin = 49152;  // calls from this size run the stream on one lane per request
 
 // the loaded key as a key source for every request (strides 0)
 static FrKeys loaded_key(pqc_ctx* c) {
@@ -1134,7 +1133,7 @@ template <bool ROWS>
 static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, const FrKeys& keys, const uint8_t* mu,
                           int mu_stride, const uint8_t* ct, hipStream_t s) {
   const int sb = c->tune.frodo_stream_block, nt = ni * kSampleThreads;
-  if (!ROWS && ni >= kFrLaneMin) {  // shared-key calls; the keyed paths run at most a server batch
+  if (!ROWS && frodo_lanes(c->tune, (uint64_t)ni)) {  // shared-key calls; the keyed paths run at most a server batch
     const int nlane = (ni + 63) / 64 * 64, lanes = nlane + (with_f ? ni : 0);
     hipLaunchKernelGGL(k_shake_lane, dim3((lanes + sb - 1) / sb), dim3(sb), 0, s, ni, nlane, with_f, keys, mu,
                        mu_stride, f.raw, f.kp, ct, with_f ? reinterpret_cast<uint64_t*>(f.fstate) : nullptr);

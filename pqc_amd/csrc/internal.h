@@ -8,6 +8,14 @@
 
 constexpr int kFrSideStreams = 3;
 
+// FrodoKEM-976 shared-key calls of at least this many requests run the noise
+// stream with one Keccak state per lane (tuning.frodo_lane_min = 0)
+constexpr int kFrodoLaneMinDefault = 49152;
+inline bool frodo_lanes(const pqc_tuning& t, uint64_t n) {
+  const int m = t.frodo_lane_min == 0 ? kFrodoLaneMinDefault : t.frodo_lane_min;
+  return m > 0 && n >= (uint64_t)m;
+}
+
 struct pqc_ctx {
   int device = 0;
   unsigned alg_mask = 0;
@@ -68,7 +76,7 @@ struct pqc_ctx {
   size_t allocated = 0;  // bytes of device memory owned by the context
   int cu_count = 0;
   int fused_grid_override = 0;
-  pqc_tuning tune = {256, 128, 4, 0, 64, 0, 0, 0, 0, 0};
+  pqc_tuning tune = {256, 128, 4, 0, 64, 0, 0, 0, 0, 0, 0};
   // fused decapsulation in 32-request tiles: per-wave A^T scratch for kat_waves waves
   int16_t* kat = nullptr;
   size_t kat_waves = 0;

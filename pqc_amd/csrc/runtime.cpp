@@ -531,7 +531,7 @@ int pqc_ctx_put_tuning(pqc_ctx* c, const pqc_tuning* t) {
       t->lowlat_max < -1 || t->lowlat_max > 65536 || t->frodo_chunk < 0 || t->frodo_chunk > 65536 ||
       t->frodo_chunk % 32 != 0 || (t->fused_dec_tile != 0 && t->fused_dec_tile != 32 && t->fused_dec_tile != 64) ||
       (t->fused_enc_tile != 0 && t->fused_enc_tile != 32 && t->fused_enc_tile != 64) ||
-      (t->fused_key_tile != 0 && t->fused_key_tile != 32 && t->fused_key_tile != 64))
+      (t->fused_key_tile != 0 && t->fused_key_tile != 32 && t->fused_key_tile != 64) || t->frodo_lane_min < -1)
     return PQC_ERR_INVALID_ARG;
   c->tune = *t;
   return PQC_OK;

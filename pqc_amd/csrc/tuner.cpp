@@ -41,7 +41,8 @@ static int mixed_layout(const pqc_tuning& t) {
 }
 
 // one-hot family (5), K / 4, op / 2, log2(n) / 17, log2(block) / 10, waves / 4,
-// mixed tile layout one-hot (4)
+// mixed tile layout one-hot (4); for the Frodo family, slot 10 is the stream's
+// Keccak lane layout at this n (1 = one state per lane)
 static void features(const pqc_tune_request& r, const pqc_tuning& t, float* f) {
   for (int i = 0; i < kFeatures; ++i) f[i] = 0.f;
   if (r.family >= 0 && r.family < 5) f[r.family] = 1.f;
@@ -59,6 +60,7 @@ static void features(const pqc_tune_request& r, const pqc_tuning& t, float* f) {
   const bool fused = r.family == PQC_TUNE_FUSED_KYBER || r.family == PQC_TUNE_MIXED_KYBER;
   f[9] = fused ? (float)t.fused_waves_per_cu / 4.f : 0.f;
   if (r.family == PQC_TUNE_MIXED_KYBER) f[10 + mixed_layout(t)] = 1.f;
+  if (r.family == PQC_TUNE_FRODO) f[10] = frodo_lanes(t, r.n) ? 1.f : 0.f;
 }
 
 static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_tuning& base) {
@@ -97,11 +99,15 @@ static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_t
         }
       break;
     case PQC_TUNE_FRODO:
-      for (int b : {64, 128, 256}) {
-        pqc_tuning t = base;
-        t.frodo_stream_block = b;
-        add(t);
-      }
+      // block size x Keccak lane layout at this n (lane pairs or one state per lane);
+      // the layout in use keeps its threshold, so the configuration in use is a candidate
+      for (int lanes = 0; lanes < 2; ++lanes)
+        for (int b : {64, 128, 256}) {
+          pqc_tuning t = base;
+          t.frodo_stream_block = b;
+          if (frodo_lanes(base, r.n) != (lanes != 0)) t.frodo_lane_min = lanes ? 1 : -1;
+          add(t);
+        }
       break;
     default: break;
   }
@@ -110,12 +116,15 @@ static std::vector<pqc_tuning> candidates(const pqc_tune_request& r, const pqc_t
 
 // 0 and 32 both mean 32-request tiles
 static int tile_of(int v) { return v == 64 ? 64 : 32; }
+// 0 means the library's threshold
+static int lane_min_of(int v) { return v == 0 ? kFrodoLaneMinDefault : v; }
 
 static bool same(const pqc_tuning& a, const pqc_tuning& b) {
   return a.plain_block == b.plain_block && a.newhope_block == b.newhope_block &&
          a.fused_waves_per_cu == b.fused_waves_per_cu && a.mixed_order == b.mixed_order &&
          a.frodo_stream_block == b.frodo_stream_block && tile_of(a.fused_enc_tile) == tile_of(b.fused_enc_tile) &&
-         tile_of(a.fused_dec_tile) == tile_of(b.fused_dec_tile) && tile_of(a.fused_key_tile) == tile_of(b.fused_key_tile);
+         tile_of(a.fused_dec_tile) == tile_of(b.fused_dec_tile) && tile_of(a.fused_key_tile) == tile_of(b.fused_key_tile) &&
+         lane_min_of(a.frodo_lane_min) == lane_min_of(b.frodo_lane_min);
 }
 
 // median of reps timed runs (one untimed warm-up first)

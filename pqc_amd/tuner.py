@@ -16,7 +16,7 @@ class Tuning(C.Structure):
     _fields_ = [("plain_block", C.c_int32), ("newhope_block", C.c_int32), ("fused_waves_per_cu", C.c_int32),
                 ("mixed_order", C.c_int32), ("frodo_stream_block", C.c_int32), ("lowlat_max", C.c_int32),
                 ("frodo_chunk", C.c_int32), ("fused_dec_tile", C.c_int32),
-                ("fused_enc_tile", C.c_int32), ("fused_key_tile", C.c_int32)]
+                ("fused_enc_tile", C.c_int32), ("fused_key_tile", C.c_int32), ("frodo_lane_min", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

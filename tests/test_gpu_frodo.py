@@ -253,3 +253,39 @@ def test_lane_stream_large_call_equals_pair_stream_and_oracle(keys):
     oct_, oss = O.frodo_enc_batch(pk, mu[idx], nthreads=8)
     assert (ct[idx] == oct_).all() and (ss[idx] == oss).all()
     assert (ss2[idx] == O.frodo_dec_batch(bad[idx], sk, nthreads=8)).all()
+
+
+@pytest.mark.gpu
+def test_autotuner_chooses_stream_lane_layout(keys):
+    """The Frodo tuning family measures the noise stream's Keccak lane layout
+    (lane pairs or one state per lane) beside the block size: at 4096 requests
+    it must pick lane pairs (one wave per SIMD would idle three quarters of the
+    GPU on single lanes), at 65 536 single lanes (lane pairs spend 2 x 107
+    against 180 instructions per round once every SIMD holds several waves).
+    Decapsulation under the pick equals the default's."""
+    import pqc_amd
+    from pqc_amd import tuner as T
+    pk, sk = keys
+    for n, want_lanes in ((4096, False), (65536, True)):
+        say("frodo lane layout autotune n", n)
+        rng = np.random.default_rng(31)
+        ctx = pqc_amd.Context(["frodo976"], max_batch=n)
+        try:
+            st = pqc_amd.Stream()
+            ctx.frodo_load_key(np.frombuffer(pk, np.uint8).copy(), False, host=True)
+            d_mu = pqc_amd.DeviceBuffer.from_array(rng.integers(0, 256, (n, 24), dtype=np.uint8))
+            d_ct, d_ss, d_ss2 = (pqc_amd.DeviceBuffer(n * b) for b in (CT, 24, 24))
+            ctx.frodo_enc(n, d_ct, d_ss, d_mu)
+            ctx.frodo_load_key(np.frombuffer(sk, np.uint8).copy(), True, host=True)
+            res = T.autotune(ctx, T.request("frodo", n, op=2, budget=0, reps=3),
+                             lambda s: ctx.frodo_dec(n, d_ss2, d_ct, stream=s), st)
+            pick = res.best
+            lanes = pick.frodo_lane_min > 0 and n >= pick.frodo_lane_min or (pick.frodo_lane_min == 0 and n >= 49152)
+            say("  pick", pick.frodo_stream_block, pick.frodo_lane_min, "%.3f ms" % res.best_ms, "lanes", lanes)
+            assert res.measured == 6
+            assert lanes == want_lanes
+            ctx.frodo_dec(n, d_ss2, d_ct, stream=st)
+            st.synchronize()
+            assert (d_ss2.download((n, 24)) == d_ss.download((n, 24))).all()
+        finally:
+            ctx.close()
