@@ -92,8 +92,8 @@ typedef struct pqc_tuning {
   int32_t fused_enc_tile;     /* the same for the fused Kyber encapsulation */
   int32_t fused_key_tile;     /* the same for the fused Kyber key generation */
   int32_t frodo_lane_min;     /* Keccak lane layout of the FrodoKEM-976 noise stream and F prefix in
-                                 shared-key calls: calls of at least this many requests run one state per
-                                 lane (throughput), smaller ones one state per lane pair (latency);
+                                 shared-key decapsulation: calls of at least this many requests run one state
+                                 per lane (throughput), smaller ones one state per lane pair (latency);
                                  0 = library default (49152), -1 = lane pairs always, else >= 1 */
 } pqc_tuning;
 int pqc_ctx_get_tuning(const pqc_ctx *ctx, pqc_tuning *t);

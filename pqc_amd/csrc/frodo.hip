@@ -1133,7 +1133,10 @@ template <bool ROWS>
 static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, const FrKeys& keys, const uint8_t* mu,
                           int mu_stride, const uint8_t* ct, hipStream_t s) {
   const int sb = c->tune.frodo_stream_block, nt = ni * kSampleThreads;
-  if (!ROWS && frodo_lanes(c->tune, (uint64_t)ni)) {  // shared-key calls; the keyed paths run at most a server batch
+  // shared-key decapsulation: its F-prefix lanes share the SIMDs with the stream
+  // lanes (encapsulation alone would run one wave per SIMD: 4.4 against 3.4 ms on
+  // pairs at 32768, even at 65536); the keyed paths run at most a server batch
+  if (!ROWS && with_f && frodo_lanes(c->tune, (uint64_t)ni)) {
     const int nlane = (ni + 63) / 64 * 64, lanes = nlane + (with_f ? ni : 0);
     hipLaunchKernelGGL(k_shake_lane, dim3((lanes + sb - 1) / sb), dim3(sb), 0, s, ni, nlane, with_f, keys, mu,
                        mu_stride, f.raw, f.kp, ct, with_f ? reinterpret_cast<uint64_t*>(f.fstate) : nullptr);

@@ -207,11 +207,12 @@ def test_pipelined_chunks_equal_unsplit_and_oracle(keys):
 
 @pytest.mark.gpu
 def test_lane_stream_large_call_equals_pair_stream_and_oracle(keys):
-    """A shared-key call of 50 000 requests runs the noise stream and the F prefix
-    with one lane per request (and samples plain stream words); 4096-request
-    calls over the same inputs run them on lane pairs.  Encapsulation and
-    decapsulation (with tampered ciphertexts) must give the same bytes both ways,
-    and the oracle's on samples across the batch."""
+    """A shared-key decapsulation call of 50 000 requests runs the noise stream and
+    the F prefix with one lane per request (and samples plain stream words);
+    4096-request calls over the same inputs run them on lane pairs.  Decapsulation
+    (with tampered ciphertexts) must give the same bytes both ways, and the
+    oracle's on samples across the batch; encapsulation (lane pairs at every
+    size) likewise."""
     import pqc_amd
     n = 50000
     say("frodo lane stream n", n)
