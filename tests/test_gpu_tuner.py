@@ -163,3 +163,29 @@ def test_trained_model_beats_untuned_and_blind_choice(pqc):
     st.synchronize()
     assert (ss2.download((n, 32)) == ss.download((n, 32))).all()
     ctx.close()
+
+
+def test_put_tuning_rejects_out_of_range_knobs(pqc):
+    """Every knob outside its documented range is rejected (PQC_ERR_INVALID_ARG)
+    and leaves the context's tuning as it was; the range ends are accepted."""
+    from pqc_amd import tuner as T
+    from pqc_amd._lib import PQCError
+    ctx = pqc.Context(["kyber768", "frodo976"], max_batch=64)
+    try:
+        before = T.get_tuning(ctx).as_dict()
+        bad = [("plain_block", 96), ("plain_block", 1088), ("newhope_block", 0), ("fused_waves_per_cu", 0),
+               ("fused_waves_per_cu", 5), ("mixed_order", 4), ("frodo_stream_block", 96), ("lowlat_max", -2),
+               ("lowlat_max", 65537), ("frodo_chunk", 33), ("frodo_chunk", 65568), ("fused_dec_tile", 48),
+               ("fused_enc_tile", 16), ("fused_key_tile", 128), ("frodo_lane_min", -2)]
+        for knob, v in bad:
+            with pytest.raises(PQCError):
+                T.put_tuning(ctx, **{knob: v})
+            assert T.get_tuning(ctx).as_dict() == before, knob
+        good = [("plain_block", 64), ("plain_block", 1024), ("fused_waves_per_cu", 1), ("mixed_order", 3),
+                ("frodo_stream_block", 256), ("lowlat_max", -1), ("lowlat_max", 65536), ("frodo_chunk", 65536),
+                ("fused_dec_tile", 64), ("frodo_lane_min", -1), ("frodo_lane_min", 1), ("frodo_lane_min", 0)]
+        for knob, v in good:
+            T.put_tuning(ctx, **{knob: v})
+            assert getattr(T.get_tuning(ctx), knob) == v, knob
+    finally:
+        ctx.close()
