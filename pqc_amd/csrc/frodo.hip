@@ -586,7 +586,7 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
   __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | 0x0070);
 }
 
-constexpr int DMA_PER_STEP = 6;  // loads issued per wave per k-step: 4 S' (registers) + 2 [A | B] (LDS DMA)
+constexpr int LOADS_PER_STEP = 6;  // loads issued per wave per k-step: 4 S' (registers) + 2 [A | B] (LDS DMA)
 
 template <int MODE>
 struct EpiRegs {
@@ -633,7 +633,7 @@ __device__ __forceinline__ void gemm_step(v4i (&acc)[4][4][2], const GemmArgs& g
   // may issue in either order; the s_barrier asm below keeps each step's loads
   // in its own step.)
   constexpr int allowed =
-      (KS + 1 < KSTEPS ? DMA_PER_STEP : 0) + epi_loads_at<MODE>(KS - 2) + epi_loads_at<MODE>(KS - 1);
+      (KS + 1 < KSTEPS ? LOADS_PER_STEP : 0) + epi_loads_at<MODE>(KS - 2) + epi_loads_at<MODE>(KS - 1);
   wait_vm_lgkm0<allowed>();
   asm volatile("s_barrier" ::: "memory");  // a workgroup barrier no memory access moves across
   if constexpr (KS + 2 < KSTEPS) gemm_issue<KS + 2>(g, sr);
@@ -1134,8 +1134,8 @@ static void launch_stream(pqc_ctx* c, const FrScratch& f, int ni, int with_f, co
                           int mu_stride, const uint8_t* ct, hipStream_t s) {
   const int sb = c->tune.frodo_stream_block, nt = ni * kSampleThreads;
   // shared-key decapsulation: its F-prefix lanes share the SIMDs with the stream
-  // lanes (encapsulation alone would run one wave per SIMD: 4.4 against 3.4 ms on
-  // pairs at 32768, even at 65536); the keyed paths run at most a server batch
+  // lanes.  Encapsulation on single lanes runs one wave per SIMD (4.4 against 3.4 ms
+  // on pairs at 32768, no faster at 65536); the keyed paths run at most a server batch
   if (!ROWS && with_f && frodo_lanes(c->tune, (uint64_t)ni)) {
     const int nlane = (ni + 63) / 64 * 64, lanes = nlane + (with_f ? ni : 0);
     hipLaunchKernelGGL(k_shake_lane, dim3((lanes + sb - 1) / sb), dim3(sb), 0, s, ni, nlane, with_f, keys, mu,
