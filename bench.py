@@ -287,6 +287,8 @@ def main():
                     default="kyber768")
     ap.add_argument("--rate", type=float, default=2e6, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=0.5, help="serving: trace length, seconds")
+    ap.add_argument("--slots", type=int, default=4,
+                    help="serving: server slots (with Kyber and Frodo in graph mode, half take the Frodo requests)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="kyber768: skip the other configurations' figures (config.secondary)")
     ap.add_argument("--fixed-rate", type=float, default=4e6,
@@ -1004,7 +1006,7 @@ def run_serving(args, duration=None, with_plain=True):
     arrivals, algs, ops, kid, rng = serving_trace(n, args.rate, 0, keys)
     algs_names = ["kyber512", "kyber768", "kyber1024", "frodo976"]
     results = {}
-    slots = 4
+    slots = args.slots
     for plain, graph in ((False, False), (False, True)) + (((True, False),) if with_plain else ()):
         # the plain path cannot hold the offered rate: replay a prefix of the trace
         m = n if not plain else min(n, int(0.2 * n))
