@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -79,3 +80,25 @@ def test_deterministic_randombytes_matches_drbg_definition():
     want = hashlib.shake_256(b"pqc-mi355x" + struct.pack("<QQQ", 5, 0xFFFFFFFF, 0)).digest(48)
     assert bytes(out) == want
     L.lib().pqc_randombytes_system()
+
+
+def test_tuner_features_encode_the_frodo_lane_layout():
+    """pqc_tune_features (host code, no device): for the Frodo family, slot 10 is 1
+    exactly when the stream runs one Keccak state per lane at the request's n
+    (tuning.frodo_lane_min: 0 = the library's 49152, -1 = lane pairs always);
+    other families leave it to their own layout."""
+    from pqc_amd import tuner as T
+    t = T.Tuning(plain_block=256, newhope_block=128, fused_waves_per_cu=4, frodo_stream_block=64)
+    cases = ((0, 4096, 0), (0, 49151, 0), (0, 49152, 1), (0, 65536, 1), (-1, 65536, 0), (1, 1, 1), (1, 4096, 1),
+             (8192, 8191, 0), (8192, 8192, 1))
+    for lane_min, n, want in cases:
+        t.frodo_lane_min = lane_min
+        f = T.features(T.request("frodo", n, op=2), t)
+        assert len(f) == T.N_FEATURES == 14
+        assert f[T.FAMILIES["frodo"]] == 1.0 and f[:5].sum() == 1.0
+        assert f[10] == want, (lane_min, n)
+        assert f[11:].sum() == 0.0
+        assert abs(f[8] - np.log2(64) / 10) < 1e-6  # the stream's block size
+    t.frodo_lane_min = 1
+    f = T.features(T.request("plain_kyber", 4096, k=3, op=2), t)
+    assert f[10:].sum() == 0.0
